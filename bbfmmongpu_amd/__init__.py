@@ -1,0 +1,223 @@
+"""bbfmmongpu_amd -- MI355X (gfx950) black-box FMM downward pass.
+
+Python mirror of the library's C ABI (include/bbfmm_amd.h).  The compute
+path is the HIP library libbbfmm_amd_{s,d}.so built in-tree; importing this
+package does not build it, and every entry point raises if it is missing.
+Device buffers are torch tensors on ``cuda:0`` (plumbing only); host buffers
+are numpy arrays in the reference's layouts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._build import lib_path
+
+__all__ = [
+    "load", "Plan", "operators", "weights", "maxlev", "m2l_drv", "BBFMMError",
+    "SCHEME_BA", "SCHEME_SI", "SCHEME_CL", "SCHEME_IJ", "FORM_PLAIN", "FORM_BF16X2", "FORM_FP32MFMA",
+]
+
+SCHEME_BA, SCHEME_SI, SCHEME_CL, SCHEME_IJ = 0, 1, 2, 3
+FORM_PLAIN, FORM_BF16X2, FORM_FP32MFMA = 0, 1, 2
+OPT_SCHEME, OPT_FORM, OPT_TIMING = 1, 2, 3
+STAGES = ["gather", "p2m", "m2m", "m2l", "post_m2l", "l2l", "l2p", "p2p", "scatter", "total"]
+
+_libs: dict = {}
+
+
+class BBFMMError(RuntimeError):
+    pass
+
+
+def dtype(prec: str):
+    return np.float32 if prec == "s" else np.float64
+
+
+def load(prec: str = "s"):
+    """The ctypes handle of libbbfmm_amd_{prec}.so (raises if not built)."""
+    if prec in _libs:
+        return _libs[prec]
+    path = lib_path(prec)
+    if not os.path.exists(path):
+        raise BBFMMError(f"{path} is not built; run __graft_entry__.build() first")
+    L = C.CDLL(path)
+    R = C.c_float if prec == "s" else C.c_double
+    vp, ip, i = C.c_void_p, C.c_int, C.c_int
+    L.bbfmm_amd_plan_create.argtypes = [C.POINTER(vp), i, vp, R, i, i, i, vp, vp, i, vp]
+    L.bbfmm_amd_plan_destroy.argtypes = [vp]
+    L.bbfmm_amd_plan_set.argtypes = [vp, i, i]
+    L.bbfmm_amd_plan_info.argtypes = [vp, vp, vp]
+    L.bbfmm_amd_solve.argtypes = [vp, vp, vp]
+    L.bbfmm_amd_solve_host.argtypes = [vp, vp, vp]
+    L.bbfmm_amd_m2l.argtypes = [vp, vp, vp]
+    L.bbfmm_amd_fetch.argtypes = [vp, i, vp]
+    L.bbfmm_amd_stage_times.argtypes = [vp, vp, i]
+    L.bbfmm_amd_maxlev.argtypes = [i, i]
+    L.bbfmm_amd_weights.argtypes = [i, vp, vp, vp, vp]
+    L.bbfmm_amd_weights.restype = None
+    L.bbfmm_amd_operators.argtypes = [i, i, vp, vp, vp]
+    L.bbfmm_amd_last_error_string.restype = C.c_char_p
+    L.bbfmm_amd_set_default_scheme.argtypes = [i]
+    L.m2l_drv.argtypes = [i, vp, vp, vp, i, i, R, vp, vp, i, vp, vp, i, vp, i]
+    for name in ("anotherFMMCompute",):
+        getattr(L, name).argtypes = [C.POINTER(vp), vp, vp, vp, vp, vp, vp, vp, vp, vp, R, i, i, i, vp]
+        getattr(L, name).restype = None
+    L.anotherFMMInteraction.argtypes = [C.POINTER(vp), vp, vp, vp, vp, i, i, i, R]
+    L.anotherFMMInteraction.restype = None
+    L.anotherDownwardPass.argtypes = [C.POINTER(vp), vp, vp, vp, vp, vp, vp, i, i, i, R, vp]
+    L.anotherDownwardPass.restype = None
+    L.anotherBuildFMMHierachy.argtypes = [R, i, i, i, i, C.POINTER(vp), i, i, vp, vp]
+    L.anotherBuildFMMHierachy.restype = None
+    L.anotherFMMCleanup.argtypes = [C.POINTER(vp)]
+    L.anotherFMMCleanup.restype = None
+    _libs[prec] = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(L, rc):
+    if rc != 0:
+        raise BBFMMError(f"bbfmm_amd error {rc}: {L.bbfmm_amd_last_error_string().decode()}")
+    return rc
+
+
+def last_error(prec: str = "s"):
+    L = load(prec)
+    return L.bbfmm_amd_last_error(), L.bbfmm_amd_last_error_string().decode()
+
+
+def maxlev(N: int, lmod: int = 0) -> int:
+    """Level rule of the reference (bbfmm.c:151-160)."""
+    return load("d").bbfmm_amd_maxlev(N, lmod)
+
+
+def weights(n: int, prec: str = "s"):
+    """(Tkz, Ktable, Kweights, Cweights) of ComputeWeights (bbfmm.c:922-1028)."""
+    dt = dtype(prec)
+    Tkz, Kt = np.zeros(n * n, dt), np.zeros(343, np.int32)
+    Kw, Cw = np.zeros(n ** 3, dt), np.zeros(2 * n * n, dt)
+    load(prec).bbfmm_amd_weights(n, _p(Tkz), _p(Kt), _p(Kw), _p(Cw))
+    return Tkz, Kt, Kw, Cw
+
+
+def operators(n: int, r: int | None = None, prec: str = "s"):
+    """FP64 operator precompute, cast to the build's real: (U, K, sigma)
+    with U n^3 x r and K 316 blocks of r x r, column-major (reference layout)."""
+    n3 = n ** 3
+    r = r or n3 // 2
+    dt = dtype(prec)
+    U = np.zeros(n3 * r, dt)
+    K = np.zeros(316 * r * r, dt)
+    sig = np.zeros(n3)
+    L = load(prec)
+    _check(L, L.bbfmm_amd_operators(n, r, _p(U), _p(K), _p(sig)))
+    return U, K, sig
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class Plan:
+    """Persistent device plan (bbfmm_amd_plan_create)."""
+
+    def __init__(self, pos, n: int, U, K, nrhs: int = 1, lmod: int = 0, L: float = 1.0,
+                 prec: str = "s", stream=None):
+        self.prec = prec
+        self.lib = load(prec)
+        dt = dtype(prec)
+        self.pos = np.ascontiguousarray(pos, dtype=dt).reshape(-1, 3)
+        self.N = self.pos.shape[0]
+        self.n = n
+        self.nrhs = nrhs
+        self._U = np.ascontiguousarray(U, dtype=dt)
+        self._K = np.ascontiguousarray(K, dtype=dt)
+        self.r = self._U.size // n ** 3
+        h = C.c_void_p()
+        _check(self.lib, self.lib.bbfmm_amd_plan_create(
+            C.byref(h), self.N, _p(self.pos), L, n, lmod, self.r, _p(self._K), _p(self._U), nrhs,
+            C.c_void_p(stream)))
+        self.h = h
+        info = np.zeros(6, np.int32)
+        dinfo = np.zeros(2)
+        self.lib.bbfmm_amd_plan_info(self.h, _p(info), _p(dinfo))
+        self.maxlev, self.ncell = int(info[0]), int(info[1])
+        self.m2l_interactions, self.p2p_pairs = float(dinfo[0]), float(dinfo[1])
+
+    def set(self, scheme: int | None = None, form: int | None = None, timing: bool | None = None):
+        if scheme is not None:
+            _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_SCHEME, scheme))
+        if form is not None:
+            _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_FORM, form))
+        if timing is not None:
+            _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_TIMING, int(timing)))
+        return self
+
+    def solve(self, q, phi=None):
+        """q: torch tensor on the GPU (nrhs x N or N), returns phi = q .* potential."""
+        torch = _torch()
+        want = torch.float32 if self.prec == "s" else torch.float64
+        if q.dtype != want or not q.is_contiguous():
+            raise BBFMMError(f"q must be a contiguous {want} tensor")
+        if q.numel() != self.N * self.nrhs:
+            raise BBFMMError("q has the wrong size")
+        if phi is None:
+            phi = torch.empty_like(q)
+        _check(self.lib, self.lib.bbfmm_amd_solve(self.h, C.c_void_p(q.data_ptr()),
+                                                  C.c_void_p(phi.data_ptr())))
+        return phi
+
+    def solve_host(self, q):
+        dt = dtype(self.prec)
+        q = np.ascontiguousarray(q, dtype=dt)
+        if q.size != self.N * self.nrhs:
+            raise BBFMMError("q has the wrong size")
+        phi = np.zeros_like(q)
+        _check(self.lib, self.lib.bbfmm_amd_solve_host(self.h, _p(q), _p(phi)))
+        return phi
+
+    def m2l(self, M, Lout):
+        _check(self.lib, self.lib.bbfmm_amd_m2l(self.h, C.c_void_p(M.data_ptr()),
+                                                C.c_void_p(Lout.data_ptr())))
+        return Lout
+
+    def fetch(self, which: str):
+        dt = dtype(self.prec)
+        idx = {"M": 0, "L": 1, "F": 2, "phi_far": 3}[which]
+        size = {0: self.ncell * self.r, 1: self.ncell * self.r, 2: self.ncell * self.n ** 3,
+                3: self.N}[idx]
+        out = np.zeros(size, dt)
+        _check(self.lib, self.lib.bbfmm_amd_fetch(self.h, idx, _p(out)))
+        return out
+
+    def stage_times(self) -> dict:
+        out = np.zeros(len(STAGES))
+        k = self.lib.bbfmm_amd_stage_times(self.h, _p(out), len(STAGES))
+        return dict(zip(STAGES[:k], out[:k].tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bbfmm_amd_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def m2l_drv(r, Lv, K, M, minlev, maxlev, L0, center, ineigh, neighbors, iinter, interaction,
+            scheme, prec="s", pitch_neighbors=64, pitch_interaction=192) -> int:
+    """The m2l_drv entry point on host arrays (Lv is written in place)."""
+    L = load(prec)
+    return L.m2l_drv(r, _p(Lv), _p(K), _p(M), minlev, maxlev, L0, _p(center), _p(ineigh),
+                     pitch_neighbors, _p(neighbors), _p(iinter), pitch_interaction,
+                     _p(interaction), scheme)

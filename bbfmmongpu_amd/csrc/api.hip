@@ -1,0 +1,927 @@
+// api.hip -- C ABI of the library: error state, the persistent plan, the
+// drop-in entry points of the reference's CUDA translation unit
+// (anotherFMMCompute, anotherFMMInteraction, anotherDownwardPass, m2l_drv,
+// the event timers) and the tree builder.
+#include "common.h"
+#include "kernels.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+// ---------------------------------------------------------------- errors
+namespace bb {
+static thread_local int g_err = BBFMM_OK;
+static thread_local char g_msg[512] = "";
+
+int set_error(int code, const char *fmt, ...)
+{
+  g_err = code;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+void clear_error()
+{
+  g_err = BBFMM_OK;
+  g_msg[0] = 0;
+}
+} // namespace bb
+
+extern "C" int bbfmm_amd_last_error(void) { return bb::g_err; }
+extern "C" const char *bbfmm_amd_last_error_string(void) { return bb::g_msg; }
+
+// m2l_host.h:24
+extern "C" {
+double kernel_exec_time = 0;
+}
+
+static int g_default_scheme = M2L_SCHEME_IJ;
+
+extern "C" int bbfmm_amd_set_default_scheme(int scheme)
+{
+  if (scheme < M2L_SCHEME_BA || scheme > M2L_SCHEME_IJ)
+    return bb::set_error(BBFMM_E_ARG, "unknown M2L scheme %d", scheme);
+  g_default_scheme = scheme;
+  return BBFMM_OK;
+}
+
+static bool g_verbose = getenv("BBFMM_AMD_VERBOSE") != nullptr;
+
+// ---------------------------------------------------------------- plan
+enum { ST_GATHER, ST_P2M, ST_M2M, ST_M2L, ST_POST, ST_L2L, ST_L2P, ST_P2P, ST_SCATTER, ST_TOTAL };
+
+struct bbfmm_amd_plan {
+  int N = 0, n = 0, n3 = 0, r = 0, nrhs = 1, maxlev = 0, ncell = 0, nleaf = 0, leaf0 = 0;
+  real L = 1, homogen = 1;
+  hipStream_t stream = nullptr;
+  int scheme = M2L_SCHEME_BA, form = BBFMM_M2L_PLAIN, timing = 0;
+  std::vector<int> h_levsta;  // [maxlev+2]
+  std::vector<int> h_perm;    // leaf order -> original index
+  double m2l_interactions = 0, p2p_pairs = 0;
+  // device
+  real4 *d_pos = nullptr, *d_center = nullptr, *d_K4 = nullptr;
+  int *d_perm = nullptr, *d_leafsta = nullptr, *d_leafcnt = nullptr, *d_leafof = nullptr;
+  int *d_nbr = nullptr, *d_nnbr = nullptr, *d_iinter = nullptr, *d_inter = nullptr, *d_kidx = nullptr;
+  int *d_Ktable = nullptr, *d_levsta = nullptr;
+  real *d_Ucm = nullptr, *d_Urm = nullptr, *d_Tkz = nullptr, *d_Kw = nullptr, *d_Cw = nullptr;
+  real *d_scale = nullptr;
+  real *d_q = nullptr, *d_phi = nullptr, *d_S = nullptr, *d_M = nullptr, *d_L = nullptr, *d_F = nullptr;
+  real *d_qin = nullptr, *d_phiout = nullptr;  // staging for the host-array entry points
+  real *d_farfield = nullptr;                  // far-field q.*phi of the last solve (vector 0)
+  std::vector<real> h_K;                       // column-major operators (for the grid form)
+  bb::GridM2L *grid = nullptr;
+  int grid_form = -1;
+  hipEvent_t ev[ST_TOTAL + 1] = {};
+  double stage_ms[ST_TOTAL + 1] = {};
+  bool solved = false;
+};
+
+namespace bb {
+
+template <typename T>
+static int dalloc(T **p, size_t count)
+{
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(BBFMM_E_NOMEM, "hipMalloc of %zu bytes failed: %s", sizeof(T) * count,
+                     hipGetErrorString(e));
+  }
+  return BBFMM_OK;
+}
+
+template <typename T>
+static int upload(T **p, const T *h, size_t count, hipStream_t s)
+{
+  BB_TRY(dalloc(p, count));
+  if (count) BB_HIP(hipMemcpyAsync(*p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+  return BBFMM_OK;
+}
+
+static int check_device()
+{
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return set_error(BBFMM_E_NODEVICE, "no GPU visible (%s)", hipGetErrorString(e));
+  }
+  return BBFMM_OK;
+}
+
+static void plan_free_device(bbfmm_amd_plan *p)
+{
+  void *ptrs[] = {p->d_pos, p->d_center, p->d_K4, p->d_perm, p->d_leafsta, p->d_leafcnt, p->d_leafof,
+                  p->d_nbr, p->d_nnbr, p->d_iinter, p->d_inter, p->d_kidx, p->d_Ktable, p->d_levsta,
+                  p->d_Ucm, p->d_Urm, p->d_Tkz, p->d_Kw, p->d_Cw, p->d_scale, p->d_q, p->d_phi,
+                  p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  if (p->grid) grid_m2l_destroy(p->grid);
+  p->grid = nullptr;
+  for (auto &e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+// Upload everything a solve needs, from a tree in the reference's layout.
+// Tkz/Kw/Cw are the caller's Chebyshev weights; K and U are in the
+// reference's column-major file layouts.
+static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, int n, int r,
+                     const real *K, const real *U, const real *Tkz, const real *Kw, const real *Cw,
+                     real homogen, int nrhs, hipStream_t stream)
+{
+  BB_TRY(check_device());
+  if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
+  if (r != 32 && r != 256) return set_error(BBFMM_E_ARG, "SVD rank r=%d not supported (r = 32 or 256)", r);
+  if (r > n * n * n) return set_error(BBFMM_E_ARG, "rank r=%d exceeds n^3=%d", r, n * n * n);
+  if (nrhs < 1) return set_error(BBFMM_E_ARG, "nrhs must be >= 1");
+  if (t->maxlev < kMinLev) return set_error(BBFMM_E_ARG, "maxlev=%d < %d", t->maxlev, kMinLev);
+  const cell *c = t->c;
+  p->N = t->Nf;
+  p->n = n;
+  p->n3 = n * n * n;
+  p->r = r;
+  p->nrhs = nrhs;
+  p->maxlev = t->maxlev;
+  p->ncell = t->ncell;
+  p->leaf0 = t->levsta[t->maxlev];
+  p->nleaf = t->levend[t->maxlev] - p->leaf0 + 1;
+  p->L = t->celeng[0];
+  p->homogen = homogen;
+  p->stream = stream;
+  p->h_levsta.assign(t->levsta, t->levsta + t->maxlev + 1);
+  p->h_levsta.push_back(t->levend[t->maxlev] + 1);
+  const int N = p->N, ncell = p->ncell, nleaf = p->nleaf, n3 = p->n3;
+
+  // particles in leaf order
+  p->h_perm.assign(t->fieldlist, t->fieldlist + N);
+  std::vector<real4> hpos((size_t)N);
+  std::vector<int> leafof((size_t)N), lsta((size_t)nleaf), lcnt((size_t)nleaf);
+  for (int k = 0; k < nleaf; k++) {
+    const int A = p->leaf0 + k;
+    lsta[k] = c->fieldsta[A];
+    lcnt[k] = c->fieldend[A] - c->fieldsta[A] + 1;
+    for (int i = c->fieldsta[A]; i <= c->fieldend[A]; i++) leafof[i] = k;
+  }
+  for (int i = 0; i < N; i++) {
+    const real3 x = pos[p->h_perm[i]];
+    hpos[i].x = x.x;
+    hpos[i].y = x.y;
+    hpos[i].z = x.z;
+    hpos[i].w = 0;
+  }
+  std::vector<real4> hctr((size_t)ncell);
+  for (int A = 0; A < ncell; A++) {
+    hctr[A].x = c->center[A].x;
+    hctr[A].y = c->center[A].y;
+    hctr[A].z = c->center[A].z;
+    hctr[A].w = 0;
+  }
+  // neighbour leaves (local indices), in the reference's list order
+  std::vector<int> nbr((size_t)nleaf * 27, -1), nnbr((size_t)nleaf);
+  double pairs = 0;
+  for (int k = 0; k < nleaf; k++) {
+    const int A = p->leaf0 + k;
+    const int ne = c->ineigh[A];
+    if (ne > 27) return set_error(BBFMM_E_ARG, "leaf %d has %d neighbours (> 27)", A, ne);
+    nnbr[k] = ne;
+    for (int b = 0; b < ne; b++) {
+      const int B = c->neighbors[(size_t)c->pitch_neighbors * A + b] - p->leaf0;
+      nbr[(size_t)k * 27 + b] = B;
+      pairs += (double)lcnt[k] * (double)lcnt[B];
+    }
+  }
+  p->p2p_pairs = pairs;
+  // interaction lists with pitch 192 and their operator indices
+  std::vector<int> inter((size_t)ncell * kPitchInteraction, 0), kidx((size_t)ncell * kPitchInteraction, 0);
+  std::vector<int> iinter(c->iinter, c->iinter + ncell);
+  int Kt[343];
+  ktable(Kt);
+  double ninter = 0;
+  for (int A = 0; A < ncell; A++) {
+    if (iinter[A] > kPitchInteraction) return set_error(BBFMM_E_ARG, "cell %d has %d interactions", A, iinter[A]);
+    for (int i = 0; i < iinter[A]; i++)
+      inter[(size_t)kPitchInteraction * A + i] = c->interaction[(size_t)c->pitch_interaction * A + i];
+    if (A >= p->h_levsta[kMinLev]) ninter += iinter[A];
+  }
+  p->m2l_interactions = ninter;
+  {
+    // operator index per interaction (m2l_aux.c:1135-1191)
+    anotherTree tv = *t;
+    cell cv = *c;
+    cv.interaction = inter.data();
+    cv.pitch_interaction = kPitchInteraction;
+    tv.c = &cv;
+    kindex(&tv, Kt, kidx.data());
+  }
+  // operators: K4 (four consecutive columns of a row), U both ways
+  std::vector<real4> K4((size_t)kNumOps * r * r / 4);
+  for (int k = 0; k < kNumOps; k++)
+    for (int jb = 0; jb < r / 4; jb++)
+      for (int i = 0; i < r; i++) {
+        const real *Kk = K + (size_t)k * r * r;
+        real4 v;
+        v.x = Kk[(size_t)(4 * jb + 0) * r + i];
+        v.y = Kk[(size_t)(4 * jb + 1) * r + i];
+        v.z = Kk[(size_t)(4 * jb + 2) * r + i];
+        v.w = Kk[(size_t)(4 * jb + 3) * r + i];
+        K4[((size_t)k * (r / 4) + jb) * r + i] = v;
+      }
+  p->h_K.assign(K, K + (size_t)kNumOps * r * r);
+  std::vector<real> Urm((size_t)n3 * r);
+  for (int l = 0; l < n3; l++)
+    for (int i = 0; i < r; i++) Urm[(size_t)l * r + i] = U[l + (size_t)n3 * i];
+  std::vector<real> scale((size_t)p->maxlev + 1);
+  for (int l = 0; l <= p->maxlev; l++) scale[l] = std::pow((real)1 / t->celeng[l], homogen);
+
+  BB_TRY(upload(&p->d_pos, hpos.data(), (size_t)N, stream));
+  BB_TRY(upload(&p->d_center, hctr.data(), (size_t)ncell, stream));
+  BB_TRY(upload(&p->d_perm, p->h_perm.data(), (size_t)N, stream));
+  BB_TRY(upload(&p->d_leafsta, lsta.data(), (size_t)nleaf, stream));
+  BB_TRY(upload(&p->d_leafcnt, lcnt.data(), (size_t)nleaf, stream));
+  BB_TRY(upload(&p->d_leafof, leafof.data(), (size_t)N, stream));
+  BB_TRY(upload(&p->d_nbr, nbr.data(), nbr.size(), stream));
+  BB_TRY(upload(&p->d_nnbr, nnbr.data(), nnbr.size(), stream));
+  BB_TRY(upload(&p->d_iinter, iinter.data(), iinter.size(), stream));
+  BB_TRY(upload(&p->d_inter, inter.data(), inter.size(), stream));
+  BB_TRY(upload(&p->d_kidx, kidx.data(), kidx.size(), stream));
+  BB_TRY(upload(&p->d_Ktable, Kt, (size_t)343, stream));
+  BB_TRY(upload(&p->d_levsta, p->h_levsta.data(), p->h_levsta.size(), stream));
+  BB_TRY(upload(&p->d_K4, K4.data(), K4.size(), stream));
+  BB_TRY(upload(&p->d_Ucm, U, (size_t)n3 * r, stream));
+  BB_TRY(upload(&p->d_Urm, Urm.data(), Urm.size(), stream));
+  BB_TRY(upload(&p->d_Tkz, Tkz, (size_t)n * n, stream));
+  BB_TRY(upload(&p->d_Kw, Kw, (size_t)n3, stream));
+  BB_TRY(upload(&p->d_Cw, Cw, (size_t)2 * n * n, stream));
+  BB_TRY(upload(&p->d_scale, scale.data(), scale.size(), stream));
+  const size_t NR = (size_t)N * nrhs;
+  BB_TRY(dalloc(&p->d_q, NR));
+  BB_TRY(dalloc(&p->d_phi, NR));
+  BB_TRY(dalloc(&p->d_farfield, (size_t)N));
+  BB_TRY(dalloc(&p->d_S, (size_t)ncell * n3 * nrhs));
+  BB_TRY(dalloc(&p->d_F, (size_t)ncell * n3 * nrhs));
+  BB_TRY(dalloc(&p->d_M, (size_t)ncell * r * nrhs));
+  BB_TRY(dalloc(&p->d_L, (size_t)ncell * r * nrhs));
+  BB_HIP(hipMemsetAsync(p->d_M, 0, sizeof(real) * (size_t)ncell * r * nrhs, stream));
+  BB_HIP(hipMemsetAsync(p->d_L, 0, sizeof(real) * (size_t)ncell * r * nrhs, stream));
+  BB_HIP(hipMemsetAsync(p->d_S, 0, sizeof(real) * (size_t)ncell * n3 * nrhs, stream));
+  BB_HIP(hipMemsetAsync(p->d_F, 0, sizeof(real) * (size_t)ncell * n3 * nrhs, stream));
+  for (auto &e : p->ev) BB_HIP(hipEventCreate(&e));
+  BB_HIP(hipStreamSynchronize(stream));
+  return BBFMM_OK;
+}
+
+static PassArgs pass_args(const bbfmm_amd_plan *p)
+{
+  PassArgs a;
+  a.N = p->N;
+  a.n = p->n;
+  a.r = p->r;
+  a.ncell = p->ncell;
+  a.maxlev = p->maxlev;
+  a.nleaf = p->nleaf;
+  a.leaf0 = p->leaf0;
+  const real leaflen = p->L / (real)(1 << p->maxlev);
+  a.ihalfL = (real)1 / (leaflen / (real)2);
+  const real pf = (real)2 / (real)p->n;
+  a.prefac3 = pf * pf * pf;
+  a.pos = p->d_pos;
+  a.center = p->d_center;
+  a.leafsta = p->d_leafsta;
+  a.leafcnt = p->d_leafcnt;
+  a.leafof = p->d_leafof;
+  a.levsta = p->d_levsta;
+  a.scale = p->d_scale;
+  a.Tkz = p->d_Tkz;
+  a.Kw = p->d_Kw;
+  a.Cw = p->d_Cw;
+  a.Ucm = p->d_Ucm;
+  a.Urm = p->d_Urm;
+  a.q = p->d_q;
+  a.phi = p->d_phi;
+  a.S = p->d_S;
+  a.M = p->d_M;
+  a.L = p->d_L;
+  a.F = p->d_F;
+  return a;
+}
+
+static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L)
+{
+  M2LArgs m;
+  m.r = p->r;
+  m.ncell = p->ncell;
+  m.nrhs = p->nrhs;
+  m.minlev = kMinLev;
+  m.maxlev = p->maxlev;
+  m.h_levsta = p->h_levsta.data();
+  m.iinter = p->d_iinter;
+  m.inter = p->d_inter;
+  m.kidx = p->d_kidx;
+  m.K4 = p->d_K4;
+  m.Ktable = p->d_Ktable;
+  m.M = M;
+  m.L = L;
+  switch (p->scheme) {
+  case M2L_SCHEME_BA: return launch_m2l_basic(m, p->stream);
+  case M2L_SCHEME_SI: return launch_m2l_sibling(m, p->stream);
+  case M2L_SCHEME_CL:
+  case M2L_SCHEME_IJ: {
+    const int form = p->scheme == M2L_SCHEME_CL ? BBFMM_M2L_PLAIN : p->form;
+    if (!p->grid || p->grid_form != form) {
+      if (p->grid) grid_m2l_destroy(p->grid);
+      p->grid = nullptr;
+      BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, p->nrhs, form, p->h_K.data(), p->stream));
+      p->grid_form = form;
+    }
+    return grid_m2l_run(p->grid, M, L, p->ncell, p->stream);
+  }
+  default: return set_error(BBFMM_E_ARG, "unknown M2L scheme %d", p->scheme);
+  }
+}
+
+// The solve pipeline on device arrays in leaf order (d_q -> d_phi).
+static int solve_sorted(bbfmm_amd_plan *p)
+{
+  hipStream_t s = p->stream;
+  const PassArgs a = pass_args(p);
+  const bool tm = p->timing != 0;
+  if (tm) BB_HIP(hipEventRecord(p->ev[ST_P2M], s));
+  BB_TRY(launch_p2m(a, s, p->nrhs));
+  if (tm) BB_HIP(hipEventRecord(p->ev[ST_M2M], s));
+  BB_TRY(launch_m2m(a, p->h_levsta.data(), s, p->nrhs));
+  if (tm) BB_HIP(hipEventRecord(p->ev[ST_M2L], s));
+  BB_TRY(run_m2l(p, p->d_M, p->d_L));
+  if (tm) BB_HIP(hipEventRecord(p->ev[ST_POST], s));
+  BB_TRY(launch_postm2l(a, p->h_levsta.data(), s, p->nrhs));
+  if (tm) BB_HIP(hipEventRecord(p->ev[ST_L2L], s));
+  BB_TRY(launch_l2l(a, p->h_levsta.data(), s, p->nrhs));
+  if (tm) BB_HIP(hipEventRecord(p->ev[ST_L2P], s));
+  BB_TRY(launch_l2p(a, s, p->nrhs));
+  BB_HIP(hipMemcpyAsync(p->d_farfield, p->d_phi, sizeof(real) * p->N, hipMemcpyDeviceToDevice, s));
+  if (tm) BB_HIP(hipEventRecord(p->ev[ST_P2P], s));
+  P2PArgs pp;
+  pp.N = p->N;
+  pp.nleaf = p->nleaf;
+  pp.nrhs = p->nrhs;
+  pp.pos = p->d_pos;
+  pp.leafsta = p->d_leafsta;
+  pp.leafcnt = p->d_leafcnt;
+  pp.nbr = p->d_nbr;
+  pp.nnbr = p->d_nnbr;
+  pp.q = p->d_q;
+  pp.phi = p->d_phi;
+  BB_TRY(launch_p2p(pp, s));
+  return BBFMM_OK;
+}
+
+static int solve_dev(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev)
+{
+  hipStream_t s = p->stream;
+  const bool tm = p->timing != 0;
+  if (tm) BB_HIP(hipEventRecord(p->ev[ST_GATHER], s));
+  BB_TRY(launch_gather(p->d_perm, q_dev, p->d_q, p->N, p->nrhs, s));
+  BB_TRY(solve_sorted(p));
+  if (tm) BB_HIP(hipEventRecord(p->ev[ST_SCATTER], s));
+  BB_TRY(launch_scatter(p->d_perm, p->d_phi, phi_dev, p->N, p->nrhs, s));
+  if (tm) {
+    BB_HIP(hipEventRecord(p->ev[ST_TOTAL], s));
+    BB_HIP(hipEventSynchronize(p->ev[ST_TOTAL]));
+    for (int k = 0; k < ST_TOTAL; k++) {
+      float ms = 0;
+      BB_HIP(hipEventElapsedTime(&ms, p->ev[k], p->ev[k + 1]));
+      p->stage_ms[k] = ms;
+    }
+    float tot = 0;
+    BB_HIP(hipEventElapsedTime(&tot, p->ev[ST_GATHER], p->ev[ST_TOTAL]));
+    p->stage_ms[ST_TOTAL] = tot;
+  }
+  p->solved = true;
+  return BBFMM_OK;
+}
+
+} // namespace bb
+
+using namespace bb;
+
+// ---------------------------------------------------------------- plan API
+extern "C" int bbfmm_amd_maxlev(int N, int lmod) { return maxlev_rule(N, lmod); }
+
+extern "C" void bbfmm_amd_weights(int n, real *Tkz, int *Ktable, real *Kweights, real *Cweights)
+{
+  weights(n, Tkz, Ktable, Kweights, Cweights);
+}
+
+extern "C" int bbfmm_amd_plan_create(bbfmm_amd_plan **plan, int N, const real3 *pos, real L, int n, int lmod,
+                                     int r, const real *K, const real *U, int nrhs, void *stream)
+{
+  clear_error();
+  if (!plan || (!pos && N > 0) || !K || !U || N < 0) return set_error(BBFMM_E_ARG, "null argument");
+  *plan = nullptr;
+  if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
+  anotherTree *t = (anotherTree *)calloc(1, sizeof(anotherTree));
+  const int maxlev = maxlev_rule(N > 0 ? N : 1, lmod);
+  int rc = build_tree(t, L, maxlev, N, pos);
+  if (rc != BBFMM_OK) {
+    free_tree(t);
+    return set_error(rc, "a particle lies outside the root cell [-L/2, L/2]^3");
+  }
+  std::vector<real> Tkz((size_t)n * n), Kw((size_t)n * n * n), Cw((size_t)2 * n * n);
+  int Kt[343];
+  weights(n, Tkz.data(), Kt, Kw.data(), Cw.data());
+  bbfmm_amd_plan *p = new bbfmm_amd_plan();
+  rc = plan_init(p, t, pos, n, r, K, U, Tkz.data(), Kw.data(), Cw.data(), (real)1, nrhs, (hipStream_t)stream);
+  free_tree(t);
+  if (rc != BBFMM_OK) {
+    plan_free_device(p);
+    delete p;
+    return rc;
+  }
+  *plan = p;
+  return BBFMM_OK;
+}
+
+extern "C" int bbfmm_amd_plan_destroy(bbfmm_amd_plan *p)
+{
+  if (!p) return BBFMM_OK;
+  (void)hipStreamSynchronize(p->stream);
+  plan_free_device(p);
+  delete p;
+  return BBFMM_OK;
+}
+
+extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
+{
+  clear_error();
+  if (!p) return set_error(BBFMM_E_ARG, "null plan");
+  switch (option) {
+  case BBFMM_OPT_SCHEME:
+    if (value < M2L_SCHEME_BA || value > M2L_SCHEME_IJ) return set_error(BBFMM_E_ARG, "unknown M2L scheme %d", value);
+    p->scheme = value;
+    return BBFMM_OK;
+  case BBFMM_OPT_FORM:
+    if (value < BBFMM_M2L_PLAIN || value > BBFMM_M2L_FP32MFMA) return set_error(BBFMM_E_ARG, "unknown M2L form %d", value);
+#ifndef SINGLE
+    if (value != BBFMM_M2L_PLAIN) return set_error(BBFMM_E_ARG, "MFMA forms are fp32-build only");
+#endif
+    p->form = value;
+    return BBFMM_OK;
+  case BBFMM_OPT_TIMING:
+    p->timing = value != 0;
+    return BBFMM_OK;
+  default: return set_error(BBFMM_E_ARG, "unknown option %d", option);
+  }
+}
+
+extern "C" int bbfmm_amd_plan_info(const bbfmm_amd_plan *p, int *out, double *dout)
+{
+  if (!p) return set_error(BBFMM_E_ARG, "null plan");
+  if (out) {
+    out[0] = p->maxlev;
+    out[1] = p->ncell;
+    out[2] = p->N;
+    out[3] = p->r;
+    out[4] = p->n;
+    out[5] = p->nrhs;
+  }
+  if (dout) {
+    dout[0] = p->m2l_interactions;
+    dout[1] = p->p2p_pairs;
+  }
+  return BBFMM_OK;
+}
+
+extern "C" int bbfmm_amd_solve(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev)
+{
+  clear_error();
+  if (!p || !q_dev || !phi_dev) return set_error(BBFMM_E_ARG, "null argument");
+  return solve_dev(p, q_dev, phi_dev);
+}
+
+extern "C" int bbfmm_amd_solve_host(bbfmm_amd_plan *p, const real *q, real *phi)
+{
+  clear_error();
+  if (!p || !q || !phi) return set_error(BBFMM_E_ARG, "null argument");
+  const size_t NR = (size_t)p->N * p->nrhs;
+  if (!p->d_qin) BB_TRY(dalloc(&p->d_qin, NR));
+  if (!p->d_phiout) BB_TRY(dalloc(&p->d_phiout, NR));
+  BB_HIP(hipMemcpyAsync(p->d_qin, q, sizeof(real) * NR, hipMemcpyHostToDevice, p->stream));
+  BB_TRY(solve_dev(p, p->d_qin, p->d_phiout));
+  BB_HIP(hipMemcpyAsync(phi, p->d_phiout, sizeof(real) * NR, hipMemcpyDeviceToHost, p->stream));
+  BB_HIP(hipStreamSynchronize(p->stream));
+  return BBFMM_OK;
+}
+
+extern "C" int bbfmm_amd_m2l(bbfmm_amd_plan *p, const real *M_dev, real *L_dev)
+{
+  clear_error();
+  if (!p || !M_dev || !L_dev) return set_error(BBFMM_E_ARG, "null argument");
+  return run_m2l(p, M_dev, L_dev);
+}
+
+extern "C" int bbfmm_amd_fetch(bbfmm_amd_plan *p, int which, real *dst)
+{
+  clear_error();
+  if (!p || !dst) return set_error(BBFMM_E_ARG, "null argument");
+  if (!p->solved) return set_error(BBFMM_E_ARG, "no solve has run on this plan");
+  BB_HIP(hipStreamSynchronize(p->stream));
+  switch (which) {
+  case 0: BB_HIP(hipMemcpy(dst, p->d_M, sizeof(real) * (size_t)p->ncell * p->r, hipMemcpyDeviceToHost)); break;
+  case 1: BB_HIP(hipMemcpy(dst, p->d_L, sizeof(real) * (size_t)p->ncell * p->r, hipMemcpyDeviceToHost)); break;
+  case 2: {
+    // F after post-M2L: recompute it from L (the solve's F also holds L2L)
+    const PassArgs a = pass_args(p);
+    BB_TRY(launch_postm2l(a, p->h_levsta.data(), p->stream, 1));
+    BB_HIP(hipMemcpyAsync(dst, p->d_F, sizeof(real) * (size_t)p->ncell * p->n3, hipMemcpyDeviceToHost, p->stream));
+    BB_HIP(hipStreamSynchronize(p->stream));
+    break;
+  }
+  case 3: {
+    std::vector<real> tmp((size_t)p->N);
+    BB_HIP(hipMemcpy(tmp.data(), p->d_farfield, sizeof(real) * p->N, hipMemcpyDeviceToHost));
+    for (int i = 0; i < p->N; i++) dst[p->h_perm[i]] = tmp[i];
+    break;
+  }
+  default: return set_error(BBFMM_E_ARG, "unknown buffer %d", which);
+  }
+  return BBFMM_OK;
+}
+
+extern "C" int bbfmm_amd_stage_times(const bbfmm_amd_plan *p, double *out, int nout)
+{
+  if (!p || !out) return 0;
+  const int m = nout < ST_TOTAL + 1 ? nout : ST_TOTAL + 1;
+  for (int k = 0; k < m; k++) out[k] = p->stage_ms[k];
+  return m;
+}
+
+extern "C" int bbfmm_amd_operators(int n, int r, real *U, real *K, double *sigma)
+{
+  clear_error();
+  if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
+  const int n3 = n * n * n;
+  if (r < 1 || r > n3) return set_error(BBFMM_E_ARG, "rank r=%d out of range", r);
+  std::vector<double> Ud((size_t)n3 * r), Kd((size_t)kNumOps * r * r), sg((size_t)n3);
+  BB_TRY(operators_fp64(n, r, Ud.data(), Kd.data(), sg.data()));
+  for (size_t i = 0; i < Ud.size(); i++) U[i] = (real)Ud[i];
+  for (size_t i = 0; i < Kd.size(); i++) K[i] = (real)Kd[i];
+  if (sigma)
+    for (int i = 0; i < n3; i++) sigma[i] = sg[i];
+  return BBFMM_OK;
+}
+
+// ---------------------------------------------------------------- tree builder
+extern "C" void anotherBuildFMMHierachy(real L, int n, int dof, int l, int cutoff, anotherTree **atree, int Nf,
+                                        int Ns, real3 *field, real3 *source)
+{
+  (void)n;
+  (void)dof;
+  (void)cutoff;
+  (void)Ns;
+  (void)source;
+  clear_error();
+  anotherTree *t = (anotherTree *)calloc(1, sizeof(anotherTree));
+  const int rc = build_tree(t, L, l, Nf, field);
+  if (rc != BBFMM_OK) set_error(rc, "a particle lies outside the root cell [-L/2, L/2]^3");
+  *atree = t;
+}
+
+extern "C" void anotherFMMCleanup(anotherTree **atree)
+{
+  if (atree && *atree) {
+    free_tree(*atree);
+    *atree = nullptr;
+  }
+}
+
+// ---------------------------------------------------------------- event timers
+extern "C" void initEventTimer(eventTimerType *t)
+{
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+    set_error(BBFMM_E_HIP, "hipEventCreate failed");
+  t->start = (void *)a;
+  t->stop = (void *)b;
+  t->time = 0.0f;
+}
+
+extern "C" void startEventTimer(eventTimerType *t) { (void)hipEventRecord((hipEvent_t)t->start, 0); }
+
+extern "C" void stopEventTimer(eventTimerType *t)
+{
+  (void)hipEventRecord((hipEvent_t)t->stop, 0);
+  (void)hipEventSynchronize((hipEvent_t)t->stop);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, (hipEvent_t)t->start, (hipEvent_t)t->stop);
+  t->time += ms;
+}
+
+extern "C" void finalizeEventTimer(eventTimerType *t)
+{
+  (void)hipEventDestroy((hipEvent_t)t->start);
+  (void)hipEventDestroy((hipEvent_t)t->stop);
+}
+
+extern "C" void printEventTimer(FILE *fp, char *s, eventTimerType *t)
+{
+  fprintf(fp, "# %s: %s = %14.7e\n", "printEventTimer", s, t->time / 1000);
+}
+
+extern "C" double readEventTimer(eventTimerType *t) { return (double)t->time; }
+extern "C" double getEventTimer(eventTimerType *t) { return (double)(t->time / 1000); }
+extern "C" void setEventTimer(eventTimerType *t, double val) { t->time = (float)val; }
+
+// ---------------------------------------------------------------- drop-in passes
+namespace {
+
+struct DropInPlan {
+  bbfmm_amd_plan *p = nullptr;
+  ~DropInPlan()
+  {
+    if (p) bbfmm_amd_plan_destroy(p);
+  }
+};
+
+int dropin_plan(DropInPlan &d, anotherTree *t, const real3 *field, real *K, real *U, real *Tkz, real *Kw,
+                real *Cw, real homogen, int cutoff, int n)
+{
+  d.p = new bbfmm_amd_plan();
+  const int rc = plan_init(d.p, t, field, n, cutoff, K, U, Tkz, Kw, Cw, homogen, 1, nullptr);
+  if (rc != BBFMM_OK) return rc;
+  d.p->scheme = g_default_scheme;
+  d.p->form = BBFMM_M2L_PLAIN;
+  return BBFMM_OK;
+}
+
+} // namespace
+
+extern "C" void anotherFMMCompute(anotherTree **atree, real3 *field, real3 *source, real *q, real *K, real *U,
+                                  real *Tkz, int *Ktable, real *Kweights, real *Cweights, real homogen, int cutoff,
+                                  int n, int dof, real *phi)
+{
+  (void)source;
+  (void)Ktable;
+  clear_error();
+  if (dof != 1) {
+    set_error(BBFMM_E_ARG, "dof=%d not supported (dof = 1)", dof);
+    return;
+  }
+  DropInPlan d;
+  if (dropin_plan(d, *atree, field, K, U, Tkz, Kweights, Cweights, homogen, cutoff, n) != BBFMM_OK) return;
+  if (bbfmm_amd_solve_host(d.p, q, phi) != BBFMM_OK) return;
+  if (g_verbose) {
+    fprintf(stderr, "# anotherFMMCompute: N = %d, maxlev = %d, scheme = %d\n", d.p->N, d.p->maxlev, d.p->scheme);
+  }
+}
+
+extern "C" void anotherFMMInteraction(anotherTree **atree, real *E, int *Ktable, real *U, real *Kweights, int n,
+                                      int dof, int cutoff, real homogen)
+{
+  (void)Ktable;
+  clear_error();
+  anotherTree *t = *atree;
+  if (dof != 1) {
+    set_error(BBFMM_E_ARG, "dof=%d not supported (dof = 1)", dof);
+    return;
+  }
+  if (!t->proxysval) {
+    set_error(BBFMM_E_ARG, "proxysval is not set (run the upward pass first)");
+    return;
+  }
+  // weights are only needed for the scaling here; Tkz/Cw are unused
+  std::vector<real> Tkz((size_t)n * n), Kw((size_t)n * n * n), Cw((size_t)2 * n * n);
+  int Kt[343];
+  weights(n, Tkz.data(), Kt, Kw.data(), Cw.data());
+  std::vector<real3> nopos((size_t)(t->Nf > 0 ? t->Nf : 1));
+  for (auto &x : nopos) x.x = x.y = x.z = 0;
+  DropInPlan d;
+  if (dropin_plan(d, t, nopos.data(), E, U, Tkz.data(), Kweights, Cw.data(), homogen, cutoff, n) != BBFMM_OK)
+    return;
+  bbfmm_amd_plan *p = d.p;
+  const size_t nm = (size_t)p->ncell * p->r;
+  if (hipMemcpy(p->d_M, t->proxysval, sizeof(real) * nm, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error(BBFMM_E_HIP, "copy of M failed");
+    return;
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, p->stream);
+  if (run_m2l(p, p->d_M, p->d_L) != BBFMM_OK) return;
+  (void)hipEventRecord(e1, p->stream);
+  const PassArgs a = pass_args(p);
+  if (launch_postm2l(a, p->h_levsta.data(), p->stream, 1) != BBFMM_OK) return;
+  float ms = 0;
+  (void)hipEventSynchronize(e1);
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  kernel_exec_time = ms * 1e-3;
+  t->fieldval = (real *)calloc((size_t)p->ncell * p->n3, sizeof(real));
+  if (hipMemcpy(t->fieldval, p->d_F, sizeof(real) * (size_t)p->ncell * p->n3, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error(BBFMM_E_HIP, "copy of F failed");
+    return;
+  }
+  free(t->proxysval);
+  t->proxysval = nullptr;
+  if (g_verbose) {
+    const double flop = (double)p->r * (2.0 * p->r - 1) * p->m2l_interactions;
+    fprintf(stderr, "# anotherFMMInteraction: calc_performance: kernel = %f [Gflop/s]\n",
+            flop / kernel_exec_time * 1e-9);
+  }
+}
+
+extern "C" void anotherDownwardPass(anotherTree **atree, real3 *field, real3 *source, real *Cweights, real *Tkz,
+                                    real *q, real *U, int cutoff, int n, int dof, real homogen, real *phi)
+{
+  (void)source;
+  clear_error();
+  anotherTree *t = *atree;
+  if (dof != 1) {
+    set_error(BBFMM_E_ARG, "dof=%d not supported (dof = 1)", dof);
+    return;
+  }
+  if (!t->fieldval) {
+    set_error(BBFMM_E_ARG, "fieldval is not set (run anotherFMMInteraction first)");
+    return;
+  }
+  std::vector<real> Tk((size_t)n * n), Kw((size_t)n * n * n), Cw((size_t)2 * n * n);
+  int Kt[343];
+  weights(n, Tk.data(), Kt, Kw.data(), Cw.data());
+  const int r = cutoff;
+  std::vector<real> K((size_t)kNumOps * r * r, (real)0);  // M2L is not run here
+  DropInPlan d;
+  if (dropin_plan(d, t, field, K.data(), U, Tkz, Kw.data(), Cweights, homogen, cutoff, n) != BBFMM_OK) return;
+  bbfmm_amd_plan *p = d.p;
+  hipStream_t s = p->stream;
+  if (hipMemcpy(p->d_F, t->fieldval, sizeof(real) * (size_t)p->ncell * p->n3, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error(BBFMM_E_HIP, "copy of F failed");
+    return;
+  }
+  if (dalloc(&p->d_qin, (size_t)p->N) != BBFMM_OK || dalloc(&p->d_phiout, (size_t)p->N) != BBFMM_OK) return;
+  if (hipMemcpy(p->d_qin, q, sizeof(real) * p->N, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error(BBFMM_E_HIP, "copy of q failed");
+    return;
+  }
+  if (launch_gather(p->d_perm, p->d_qin, p->d_q, p->N, 1, s) != BBFMM_OK) return;
+  const PassArgs a = pass_args(p);
+  if (launch_l2l(a, p->h_levsta.data(), s, 1) != BBFMM_OK) return;
+  if (launch_l2p(a, s, 1) != BBFMM_OK) return;
+  P2PArgs pp;
+  pp.N = p->N;
+  pp.nleaf = p->nleaf;
+  pp.nrhs = 1;
+  pp.pos = p->d_pos;
+  pp.leafsta = p->d_leafsta;
+  pp.leafcnt = p->d_leafcnt;
+  pp.nbr = p->d_nbr;
+  pp.nnbr = p->d_nnbr;
+  pp.q = p->d_q;
+  pp.phi = p->d_phi;
+  if (launch_p2p(pp, s) != BBFMM_OK) return;
+  if (launch_scatter(p->d_perm, p->d_phi, p->d_phiout, p->N, 1, s) != BBFMM_OK) return;
+  if (hipMemcpy(phi, p->d_phiout, sizeof(real) * p->N, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error(BBFMM_E_HIP, "copy of phi failed");
+    return;
+  }
+  free(t->fieldval);
+  t->fieldval = nullptr;
+}
+
+// ---------------------------------------------------------------- m2l_drv
+extern "C" int m2l_drv(int r, real *L, real *K, real *M, int minlev, int maxlev, real L0, real3 *center,
+                       int *ineigh, int pitch_neighbors, int *neighbors, int *iinter, int pitch_interaction,
+                       int *interaction, int scheme)
+{
+  (void)ineigh;
+  (void)pitch_neighbors;
+  (void)neighbors;
+  clear_error();
+  if (r != 32 && r != 256) {
+    set_error(BBFMM_E_ARG, "m2l_drv: r=%d not supported (r = 32 or 256)", r);
+    return M2L_EXIT_FAIL;
+  }
+  if (scheme < M2L_SCHEME_BA || scheme > M2L_SCHEME_IJ) {
+    set_error(BBFMM_E_ARG, "m2l_drv: unknown scheme %d", scheme);
+    return M2L_EXIT_FAIL;
+  }
+  if (minlev < kMinLev || maxlev < minlev || maxlev > 9 || pitch_interaction < 189) {
+    set_error(BBFMM_E_ARG, "m2l_drv: invalid level range %d..%d or pitch %d", minlev, maxlev, pitch_interaction);
+    return M2L_EXIT_FAIL;
+  }
+  if (check_device() != BBFMM_OK) return M2L_EXIT_FAIL;
+  bbfmm_amd_plan p;
+  p.r = r;
+  p.nrhs = 1;
+  p.maxlev = maxlev;
+  p.scheme = scheme;
+  p.form = BBFMM_M2L_PLAIN;
+  p.ncell = ((1 << (3 * (maxlev + 1))) - 1) / 7;
+  p.h_levsta.resize(maxlev + 2);
+  for (int l = 0; l <= maxlev + 1; l++) p.h_levsta[l] = ((1 << (3 * l)) - 1) / 7;
+  const int ncell = p.ncell;
+  int rc = BBFMM_OK;
+  std::vector<int> inter((size_t)ncell * kPitchInteraction, 0), kidx((size_t)ncell * kPitchInteraction, 0);
+  std::vector<int> iint(iinter, iinter + ncell);
+  int Kt[343];
+  ktable(Kt);
+  for (int A = 0; A < ncell; A++) {
+    if (iint[A] > kPitchInteraction) {
+      set_error(BBFMM_E_ARG, "m2l_drv: cell %d has %d interactions", A, iint[A]);
+      return M2L_EXIT_FAIL;
+    }
+    for (int i = 0; i < iint[A]; i++) inter[(size_t)kPitchInteraction * A + i] = interaction[(size_t)pitch_interaction * A + i];
+  }
+  {
+    std::vector<int> ls(maxlev + 1), le(maxlev + 1);
+    std::vector<real> cl(maxlev + 1);
+    level_tables(maxlev, L0, ls.data(), le.data(), cl.data());
+    anotherTree tv{};
+    cell cv{};
+    cv.center = center;
+    cv.iinter = iint.data();
+    cv.interaction = inter.data();
+    cv.pitch_interaction = kPitchInteraction;
+    tv.c = &cv;
+    tv.maxlev = maxlev;
+    tv.levsta = ls.data();
+    tv.levend = le.data();
+    tv.celeng = cl.data();
+    kindex(&tv, Kt, kidx.data());
+  }
+  std::vector<real4> K4((size_t)kNumOps * r * r / 4);
+  for (int k = 0; k < kNumOps; k++)
+    for (int jb = 0; jb < r / 4; jb++)
+      for (int i = 0; i < r; i++) {
+        const real *Kk = K + (size_t)k * r * r;
+        real4 v;
+        v.x = Kk[(size_t)(4 * jb + 0) * r + i];
+        v.y = Kk[(size_t)(4 * jb + 1) * r + i];
+        v.z = Kk[(size_t)(4 * jb + 2) * r + i];
+        v.w = Kk[(size_t)(4 * jb + 3) * r + i];
+        K4[((size_t)k * (r / 4) + jb) * r + i] = v;
+      }
+  p.h_K.assign(K, K + (size_t)kNumOps * r * r);
+  real *dL = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  do {
+    if ((rc = upload(&p.d_iinter, iint.data(), iint.size(), nullptr)) != BBFMM_OK) break;
+    if ((rc = upload(&p.d_inter, inter.data(), inter.size(), nullptr)) != BBFMM_OK) break;
+    if ((rc = upload(&p.d_kidx, kidx.data(), kidx.size(), nullptr)) != BBFMM_OK) break;
+    if ((rc = upload(&p.d_Ktable, Kt, (size_t)343, nullptr)) != BBFMM_OK) break;
+    if ((rc = upload(&p.d_K4, K4.data(), K4.size(), nullptr)) != BBFMM_OK) break;
+    if ((rc = upload(&p.d_M, M, (size_t)ncell * r, nullptr)) != BBFMM_OK) break;
+    if ((rc = dalloc(&dL, (size_t)ncell * r)) != BBFMM_OK) break;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      rc = set_error(BBFMM_E_HIP, "hipEventCreate failed");
+      break;
+    }
+    (void)hipEventRecord(e0, nullptr);
+    M2LArgs m;
+    m.r = r;
+    m.ncell = ncell;
+    m.nrhs = 1;
+    m.minlev = minlev;
+    m.maxlev = maxlev;
+    m.h_levsta = p.h_levsta.data();
+    m.iinter = p.d_iinter;
+    m.inter = p.d_inter;
+    m.kidx = p.d_kidx;
+    m.K4 = p.d_K4;
+    m.Ktable = p.d_Ktable;
+    m.M = p.d_M;
+    m.L = dL;
+    if (scheme == M2L_SCHEME_BA) rc = launch_m2l_basic(m, nullptr);
+    else if (scheme == M2L_SCHEME_SI) rc = launch_m2l_sibling(m, nullptr);
+    else {
+      rc = grid_m2l_create(&p.grid, r, maxlev, 1, BBFMM_M2L_PLAIN, p.h_K.data(), nullptr);
+      if (rc == BBFMM_OK) rc = grid_m2l_run(p.grid, p.d_M, dL, ncell, nullptr);
+    }
+    if (rc != BBFMM_OK) break;
+    (void)hipEventRecord(e1, nullptr);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = set_error(BBFMM_E_HIP, "M2L kernel failed");
+      break;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    kernel_exec_time = ms * 1e-3;
+    const size_t off = (size_t)p.h_levsta[minlev] * r;
+    const size_t cnt = (size_t)(p.h_levsta[maxlev + 1] - p.h_levsta[minlev]) * r;
+    if (hipMemcpy(L + off, dL + off, sizeof(real) * cnt, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = set_error(BBFMM_E_HIP, "copy of L failed");
+  } while (0);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (dL) (void)hipFree(dL);
+  plan_free_device(&p);
+  return rc == BBFMM_OK ? M2L_EXIT_SUCCESS : M2L_EXIT_FAIL;
+}

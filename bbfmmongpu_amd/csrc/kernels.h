@@ -1,0 +1,74 @@
+// kernels.h -- argument blocks and launchers of the device kernels.
+#pragma once
+
+#include "common.h"
+
+namespace bb {
+
+// Everything the pass kernels read, by value (device pointers).
+struct PassArgs {
+  int N, n, r, ncell, maxlev, nleaf, leaf0;
+  real ihalfL;          // 1 / (half leaf length)
+  real prefac3;         // (2/n)^3
+  const real4 *pos;     // [N] leaf order
+  const real4 *center;  // [ncell]
+  const int *leafsta;   // [nleaf]
+  const int *leafcnt;   // [nleaf]
+  const int *leafof;    // [N] leaf (local index) of each leaf-ordered particle
+  const int *levsta;    // [maxlev+2]
+  const real *scale;    // [maxlev+1] (1/L_l)^homogen
+  const real *Tkz, *Kw, *Cw;
+  const real *Ucm;      // U column-major (n^3 x r)
+  const real *Urm;      // U row-major
+  const real *q;        // [rhs][N] leaf order
+  real *phi;            // [rhs][N] leaf order
+  real *S, *M, *L, *F;
+};
+
+int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
+int launch_scatter(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
+int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs);
+int launch_m2m(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs);
+int launch_postm2l(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs);
+int launch_l2l(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs);
+int launch_l2p(const PassArgs &a, hipStream_t s, int nrhs);
+
+// ---- M2L ----
+struct M2LArgs {
+  int r, ncell, nrhs, minlev, maxlev;
+  const int *h_levsta;     // host [maxlev+2]
+  // basic / sibling schemes
+  const int *iinter;       // [ncell]
+  const int *inter;        // [ncell*192]
+  const int *kidx;         // [ncell*192]
+  const real4 *K4;         // [316][r/4][r] four consecutive columns of row i
+  const int *Ktable;       // [343] (device)
+  const real *M;           // [rhs][ncell][r]
+  real *L;                 // [rhs][ncell][r]
+};
+int launch_m2l_basic(const M2LArgs &a, hipStream_t s);
+int launch_m2l_sibling(const M2LArgs &a, hipStream_t s);
+
+// ---- dense-grid M2L (ghost-padded per-sibling-class grids) ----
+struct GridM2L;  // defined in m2l_grid.hip
+int grid_m2l_create(GridM2L **g, int r, int maxlev, int nrhs, int form, const real *K_colmajor_host,
+                    hipStream_t s);
+void grid_m2l_destroy(GridM2L *g);
+int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s);
+
+// ---- P2P ----
+struct P2PArgs {
+  int N, nleaf, nrhs;
+  const real4 *pos;        // [N] leaf order (x,y,z,0)
+  const int *leafsta, *leafcnt;
+  const int *nbr;          // [nleaf*27] neighbour leaves (local index), -1 padded
+  const int *nnbr;         // [nleaf]
+  const real *q;           // [rhs][N]
+  real *phi;               // [rhs][N] (accumulated into)
+};
+int launch_p2p(const P2PArgs &a, hipStream_t s);
+
+// ---- FP64 operator precompute ----
+int operators_fp64(int n, int r, double *U, double *K, double *sigma);
+
+} // namespace bb

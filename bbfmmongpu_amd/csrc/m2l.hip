@@ -1,0 +1,188 @@
+// m2l.hip -- list-driven M2L kernels (plain real-precision FMAs):
+//   basic   (M2L_SCHEME_BA): one thread per (field cell, row), walks the
+//            interaction list (the semantics of m2l_kern_basic.cu:218-384);
+//   sibling (M2L_SCHEME_SI): one block per 8-sibling field cluster, the 8
+//            source siblings of each neighbouring cluster staged in LDS
+//            (the blocking of m2l_kern_sibling_blocking.cu / CPU7A).
+// Both write L (not accumulate) for every cell of levels minlev..maxlev.
+//
+// K4 layout: K4[k][j/4][i] = (K_k(i,4j'), K_k(i,4j'+1), K_k(i,4j'+2), K_k(i,4j'+3))
+// so that the lanes of a row group read consecutive 16-byte (fp32) words.
+#include "common.h"
+#include "kernels.h"
+
+namespace bb {
+
+template <int R>
+__global__ void __launch_bounds__(256) k_m2l_basic(int F0, int nF, int ncell, const int *__restrict__ iinter,
+                                                   const int *__restrict__ inter,
+                                                   const int *__restrict__ kidx,
+                                                   const real4 *__restrict__ K4,
+                                                   const real *__restrict__ M, real *__restrict__ L)
+{
+  constexpr int CPB = 256 / R;  // field cells per block
+  const int cl = threadIdx.x / R;
+  const int i = threadIdx.x % R;
+  const int F = F0 + blockIdx.x * CPB + cl;
+  if (F >= F0 + nF) return;  // no barrier below
+  const size_t roff = (size_t)blockIdx.y * ncell * R;
+  const int ni = iinter[F];
+  const int *il = inter + (size_t)kPitchInteraction * F;
+  const int *kl = kidx + (size_t)kPitchInteraction * F;
+  real acc0 = 0, acc1 = 0;
+  for (int t = 0; t < ni; t++) {
+    const int S = il[t];
+    const int k = kl[t];
+    const real4 *Kp = K4 + (size_t)k * (R / 4) * R + i;
+    const real4 *Mp = reinterpret_cast<const real4 *>(M + roff + (size_t)S * R);
+#pragma unroll 8
+    for (int jb = 0; jb < R / 4; jb += 2) {
+      const real4 ka = Kp[(size_t)jb * R], kb = Kp[(size_t)(jb + 1) * R];
+      const real4 ma = Mp[jb], mb = Mp[jb + 1];
+      acc0 += ka.x * ma.x;
+      acc1 += kb.x * mb.x;
+      acc0 += ka.y * ma.y;
+      acc1 += kb.y * mb.y;
+      acc0 += ka.z * ma.z;
+      acc1 += kb.z * mb.z;
+      acc0 += ka.w * ma.w;
+      acc1 += kb.w * mb.w;
+    }
+  }
+  L[roff + (size_t)F * R + i] = acc0 + acc1;
+}
+
+// Morton helpers: local index within a level <-> grid coordinates.  The
+// sibling index is 4*xbit + 2*ybit + zbit at every level (anotherFMMDistribute.c:239-253).
+__device__ __forceinline__ void morton_decode(int m, int level, int &x, int &y, int &z)
+{
+  x = y = z = 0;
+  for (int b = 0; b < level; b++) {
+    const int s = (m >> (3 * b)) & 7;
+    x |= ((s >> 2) & 1) << b;
+    y |= ((s >> 1) & 1) << b;
+    z |= (s & 1) << b;
+  }
+}
+
+__device__ __forceinline__ int morton_encode(int x, int y, int z, int level)
+{
+  int m = 0;
+  for (int b = 0; b < level; b++)
+    m |= ((((x >> b) & 1) << 2) | (((y >> b) & 1) << 1) | ((z >> b) & 1)) << (3 * b);
+  return m;
+}
+
+// Sibling blocking: one block per field cluster (the 8 children of a parent
+// P at level-1).  Threads = R rows, each accumulating the row for all 8
+// field siblings; the 8 source siblings of each of P's 26 neighbours are
+// staged in LDS and every (field, source) pair whose offset is an M2L
+// offset uses K[Ktable[offset]].
+template <int R>
+__global__ void __launch_bounds__(R < 64 ? 64 : R) k_m2l_sibling(int level, int plev0, int lev0, int ncell,
+                                                                 const int *__restrict__ Ktable,
+                                                                 const real4 *__restrict__ K4,
+                                                                 const real *__restrict__ M,
+                                                                 real *__restrict__ L)
+{
+  constexpr int CPB = R < 64 ? 64 / R : 1;  // clusters per block
+  constexpr int BLOCK = R * CPB;
+  __shared__ real4 sm[CPB][8][R / 4];
+  __shared__ int skt[343];
+  const int cl = threadIdx.x / R;
+  const int i = threadIdx.x % R;
+  const int plevel = level - 1;
+  const int npar = 1 << (3 * plevel);
+  const int pidx = blockIdx.x * CPB + cl;  // local parent index
+  const bool active = pidx < npar;
+  const size_t roff = (size_t)blockIdx.y * ncell * R;
+  for (int o = threadIdx.x; o < 343; o += BLOCK) skt[o] = Ktable[o];
+  int px = 0, py = 0, pz = 0;
+  if (active) morton_decode(pidx, plevel, px, py, pz);
+  const int G = 1 << plevel;
+  real acc[8];
+#pragma unroll
+  for (int f = 0; f < 8; f++) acc[f] = 0;
+  for (int d = 0; d < 27; d++) {
+    const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+    const int sx = px + dx, sy = py + dy, sz = pz + dz;
+    const bool ok = active && (dx | dy | dz) != 0 && sx >= 0 && sy >= 0 && sz >= 0 && sx < G && sy < G && sz < G;
+    __syncthreads();
+    if (ok) {
+      const int SC = plev0 + morton_encode(sx, sy, sz, plevel);
+      const real4 *src = reinterpret_cast<const real4 *>(M + roff + (size_t)(8 * SC + 1) * R);
+      for (int o = i; o < 8 * (R / 4); o += R) sm[cl][o / (R / 4)][o % (R / 4)] = src[o];
+    }
+    __syncthreads();
+    if (!ok) continue;
+#pragma unroll
+    for (int f = 0; f < 8; f++) {
+      const int fx = (f >> 2) & 1, fy = (f >> 1) & 1, fz = f & 1;
+      for (int s = 0; s < 8; s++) {
+        const int vx = 2 * dx + ((s >> 2) & 1) - fx;
+        const int vy = 2 * dy + ((s >> 1) & 1) - fy;
+        const int vz = 2 * dz + (s & 1) - fz;
+        const int k = skt[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
+        if (k < 0) continue;
+        const real4 *Kp = K4 + (size_t)k * (R / 4) * R + i;
+        real a = 0;
+#pragma unroll 8
+        for (int jb = 0; jb < R / 4; jb++) {
+          const real4 kv = Kp[(size_t)jb * R];
+          const real4 mv = sm[cl][s][jb];
+          a += kv.x * mv.x + kv.y * mv.y + kv.z * mv.z + kv.w * mv.w;
+        }
+        acc[f] += a;
+      }
+    }
+  }
+  if (active) {
+    const int A0 = lev0 + 8 * pidx;
+#pragma unroll
+    for (int f = 0; f < 8; f++) L[roff + (size_t)(A0 + f) * R + i] = acc[f];
+  }
+}
+
+template <int R>
+static int basic_t(const M2LArgs &a, hipStream_t s)
+{
+  constexpr int CPB = 256 / R;
+  for (int level = a.minlev; level <= a.maxlev; level++) {
+    const int F0 = a.h_levsta[level], nF = a.h_levsta[level + 1] - F0;
+    hipLaunchKernelGGL((k_m2l_basic<R>), dim3((nF + CPB - 1) / CPB, a.nrhs), dim3(256), 0, s, F0, nF,
+                       a.ncell, a.iinter, a.inter, a.kidx, a.K4, a.M, a.L);
+    BB_HIP(hipGetLastError());
+  }
+  return BBFMM_OK;
+}
+
+int launch_m2l_basic(const M2LArgs &a, hipStream_t s)
+{
+  if (a.r == 32) return basic_t<32>(a, s);
+  if (a.r == 256) return basic_t<256>(a, s);
+  return set_error(BBFMM_E_ARG, "M2L: r=%d not supported (r = 32 or 256)", a.r);
+}
+
+template <int R>
+static int sibling_t(const M2LArgs &a, hipStream_t s)
+{
+  constexpr int CPB = R < 64 ? 64 / R : 1;
+  constexpr int BLOCK = R * CPB;
+  for (int level = a.minlev; level <= a.maxlev; level++) {
+    const int npar = 1 << (3 * (level - 1));
+    hipLaunchKernelGGL((k_m2l_sibling<R>), dim3((npar + CPB - 1) / CPB, a.nrhs), dim3(BLOCK), 0, s, level,
+                       a.h_levsta[level - 1], a.h_levsta[level], a.ncell, a.Ktable, a.K4, a.M, a.L);
+    BB_HIP(hipGetLastError());
+  }
+  return BBFMM_OK;
+}
+
+int launch_m2l_sibling(const M2LArgs &a, hipStream_t s)
+{
+  if (a.minlev < 2) return set_error(BBFMM_E_ARG, "M2L sibling scheme needs minlev >= 2");
+  if (a.r == 32) return sibling_t<32>(a, s);
+  if (a.r == 256) return sibling_t<256>(a, s);
+  return set_error(BBFMM_E_ARG, "M2L: r=%d not supported (r = 32 or 256)", a.r);
+}
+
+} // namespace bb

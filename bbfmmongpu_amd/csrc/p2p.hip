@@ -1,0 +1,133 @@
+// p2p.hip -- near-field P2P on the leaf level: for every particle i of leaf A,
+//   phi_i += q_i * sum_{B in neighbours(A)} sum_{j in B, r_ij != 0} q_j / r_ij
+// (the semantics of the reference GPU kernel direct.cu:69-209 and the CPU
+// anotherNearField.c:137-206; the r != 0 test is applied in the self leaf).
+//
+// One 64-lane wavefront per leaf; the neighbour particles are staged in LDS
+// 64 at a time and every lane (target) sweeps them.  NR right-hand sides are
+// processed together so that the geometry (dx, dy, dz, 1/r) is computed once
+// per pair and shared by NR charge vectors.
+// 1/r: fp32 uses the hardware v_rsq_f32 (AMD's own reciprocal square root,
+// about 1 ulp); fp64 uses the device math library's rsqrt.
+#include "common.h"
+#include "kernels.h"
+
+namespace bb {
+
+__device__ __forceinline__ real rsqrt_r(real x)
+{
+#ifdef SINGLE
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return rsqrt(x);
+#endif
+}
+
+template <int NR>
+__global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
+{
+  constexpr int CH = 64;
+  __shared__ real4 sp[CH];
+  __shared__ real sq[NR][CH];
+  __shared__ int nsta[27], ncnt[27], nself[27];
+  const int leaf = blockIdx.x;
+  const int t = threadIdx.x;
+  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
+  const int nn = a.nnbr[leaf];
+  if (t < nn) {
+    const int B = a.nbr[(size_t)leaf * 27 + t];
+    nsta[t] = a.leafsta[B];
+    ncnt[t] = a.leafcnt[B];
+    nself[t] = (B == leaf);
+  }
+  __syncthreads();
+  const size_t N = a.N;
+  for (int tb = 0; tb < cnt; tb += CH) {
+    const int i = tb + t;
+    const bool act = i < cnt;
+    real4 xf;
+    if (act) {
+      xf = a.pos[sta + i];
+    } else {
+      xf.x = xf.y = xf.z = xf.w = 0;
+    }
+    real acc[NR], acc2[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) acc[r] = acc2[r] = 0;
+    for (int b = 0; b < nn; b++) {
+      const int jsta = nsta[b], nj = ncnt[b];
+      const bool self = nself[b] != 0;
+      for (int jb = 0; jb < nj; jb += CH) {
+        const int nk = min(CH, nj - jb);
+        if (t < nk) {
+          sp[t] = a.pos[jsta + jb + t];
+#pragma unroll
+          for (int r = 0; r < NR; r++) sq[r][t] = a.q[(size_t)(rhs0 + r) * N + jsta + jb + t];
+        }
+        __syncthreads();
+        if (!self) {
+          int k = 0;
+          for (; k + 1 < nk; k += 2) {
+            const real4 s0 = sp[k], s1 = sp[k + 1];
+            const real dx0 = s0.x - xf.x, dy0 = s0.y - xf.y, dz0 = s0.z - xf.z;
+            const real dx1 = s1.x - xf.x, dy1 = s1.y - xf.y, dz1 = s1.z - xf.z;
+            const real w0 = rsqrt_r(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
+            const real w1 = rsqrt_r(dx1 * dx1 + dy1 * dy1 + dz1 * dz1);
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+              acc[r] += w0 * sq[r][k];
+              acc2[r] += w1 * sq[r][k + 1];
+            }
+          }
+          if (k < nk) {
+            const real4 s0 = sp[k];
+            const real dx0 = s0.x - xf.x, dy0 = s0.y - xf.y, dz0 = s0.z - xf.z;
+            const real w0 = rsqrt_r(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
+#pragma unroll
+            for (int r = 0; r < NR; r++) acc[r] += w0 * sq[r][k];
+          }
+        } else {
+          for (int k = 0; k < nk; k++) {
+            const real4 s0 = sp[k];
+            const real dx0 = s0.x - xf.x, dy0 = s0.y - xf.y, dz0 = s0.z - xf.z;
+            const real rr = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+            const real w0 = rr != (real)0 ? rsqrt_r(rr) : (real)0;
+#pragma unroll
+            for (int r = 0; r < NR; r++) acc[r] += w0 * sq[r][k];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const size_t o = (size_t)(rhs0 + r) * N + sta + i;
+        a.phi[o] += a.q[o] * (acc[r] + acc2[r]);
+      }
+    }
+  }
+}
+
+int launch_p2p(const P2PArgs &a, hipStream_t s)
+{
+  if (a.nleaf <= 0 || a.N <= 0) return BBFMM_OK;
+  int rhs = 0;
+  while (rhs < a.nrhs) {
+    const int left = a.nrhs - rhs;
+    if (left >= 16) {
+      hipLaunchKernelGGL(k_p2p<16>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+      rhs += 16;
+    } else if (left >= 4) {
+      hipLaunchKernelGGL(k_p2p<4>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+      rhs += 4;
+    } else {
+      hipLaunchKernelGGL(k_p2p<1>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+      rhs += 1;
+    }
+    BB_HIP(hipGetLastError());
+  }
+  return BBFMM_OK;
+}
+
+} // namespace bb

@@ -1,0 +1,376 @@
+// passes.hip -- GPU kernels of the upward and downward passes around M2L:
+// charge gather, P2M, M2M, post-M2L, L2L, L2P and the result scatter.
+//
+// Layouts (device, per right-hand side `rhs`, rhs-major):
+//   pos   [N]          real4 (x, y, z, 0) in leaf order (particles of a leaf contiguous)
+//   q,phi [rhs][N]     leaf order
+//   S, F  [rhs][ncell][n^3]   Chebyshev-node source / field values
+//   M, L  [rhs][ncell][r]     multipole / local coefficients
+// Cells are numbered as the reference numbers them (cell.h:27-28): the
+// children of A are 8A+1..8A+8, so the children of a cell are contiguous.
+//
+// The Chebyshev order is a template parameter (n = 4 or 8, r = n^3/2),
+// the orders the reference's GPU kernels support (m2l_host_basic.cu:82-97).
+#include "common.h"
+#include "kernels.h"
+
+namespace bb {
+
+// S_n(x_m, x) for m = 0..NN-1 (bbfmm.c:1641-1679, Clenshaw recurrence).
+template <int NN>
+__device__ __forceinline__ void cheb_row(const real *__restrict__ Tkz, real x, real *out)
+{
+#pragma unroll
+  for (int m = 0; m < NN; m++) {
+    real b1 = 0, b2 = 0;
+#pragma unroll
+    for (int j = NN - 1; j > 0; j--) {
+      const real b0 = 2 * x * b1 - b2 + Tkz[m * NN + j];
+      b2 = b1;
+      b1 = b0;
+    }
+    out[m] = x * b1 - b2 + (real)0.5 * Tkz[m * NN];
+  }
+}
+
+__global__ void k_gather(int N, const int *__restrict__ perm, const real *__restrict__ src,
+                         real *__restrict__ dst)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t off = (size_t)blockIdx.y * N;
+  if (i < N) dst[off + i] = src[off + perm[i]];
+}
+
+__global__ void k_scatter(int N, const int *__restrict__ perm, const real *__restrict__ src,
+                          real *__restrict__ dst)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t off = (size_t)blockIdx.y * N;
+  if (i < N) dst[off + perm[i]] = src[off + i];
+}
+
+// M = U^T (Kw .* S) for one cell; sw holds Kw .* S in LDS.
+template <int NN, int BLOCK>
+__device__ __forceinline__ void project_u(const real *__restrict__ sw, const real *__restrict__ Urm,
+                                          int r, real *__restrict__ Mout)
+{
+  constexpr int N3 = NN * NN * NN;
+  for (int i = threadIdx.x; i < r; i += BLOCK) {
+    real m = 0;
+#pragma unroll 8
+    for (int l = 0; l < N3; l++) m += Urm[(size_t)l * r + i] * sw[l];
+    Mout[i] = m;
+  }
+}
+
+// P2M at the leaf level (anotherUpwardPass.c:61-330): one block per leaf,
+// one thread per Chebyshev node (NN=8: two nodes per thread).
+template <int NN, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
+{
+  constexpr int N3 = NN * NN * NN;
+  constexpr int NPT = N3 / BLOCK;
+  constexpr int CH = BLOCK;
+  static_assert(N3 % BLOCK == 0, "nodes per thread");
+  __shared__ real sx[NN][CH], sy[NN][CH], sz[NN][CH], sq[CH];
+  __shared__ real sw[N3];
+  const int leaf = blockIdx.x;
+  const int rhs = blockIdx.y;
+  const int A = a.leaf0 + leaf;
+  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
+  const real4 c = a.center[A];
+  const real *q = a.q + (size_t)rhs * a.N;
+  real acc[NPT];
+#pragma unroll
+  for (int p = 0; p < NPT; p++) acc[p] = 0;
+  for (int base = 0; base < cnt; base += CH) {
+    const int nk = min(CH, cnt - base);
+    if ((int)threadIdx.x < nk) {
+      const real4 x = a.pos[sta + base + threadIdx.x];
+      real t[NN];
+      cheb_row<NN>(a.Tkz, a.ihalfL * (x.x - c.x), t);
+#pragma unroll
+      for (int m = 0; m < NN; m++) sx[m][threadIdx.x] = t[m];
+      cheb_row<NN>(a.Tkz, a.ihalfL * (x.y - c.y), t);
+#pragma unroll
+      for (int m = 0; m < NN; m++) sy[m][threadIdx.x] = t[m];
+      cheb_row<NN>(a.Tkz, a.ihalfL * (x.z - c.z), t);
+#pragma unroll
+      for (int m = 0; m < NN; m++) sz[m][threadIdx.x] = t[m];
+      sq[threadIdx.x] = q[sta + base + threadIdx.x];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NPT; p++) {
+      const int l = threadIdx.x + p * BLOCK;
+      const int l1 = l / (NN * NN), l2 = (l / NN) % NN, l3 = l % NN;
+      real s = acc[p];
+      for (int j = 0; j < nk; j++) s += sq[j] * sx[l1][j] * sy[l2][j] * sz[l3][j];
+      acc[p] = s;
+    }
+    __syncthreads();
+  }
+  real *S = a.S + (size_t)rhs * a.ncell * N3 + (size_t)A * N3;
+#pragma unroll
+  for (int p = 0; p < NPT; p++) {
+    const int l = threadIdx.x + p * BLOCK;
+    const real v = a.prefac3 * acc[p];
+    S[l] = v;
+    sw[l] = v * a.Kw[l];
+  }
+  __syncthreads();
+  project_u<NN, BLOCK>(sw, a.Urm, a.r, a.M + (size_t)rhs * a.ncell * a.r + (size_t)A * a.r);
+}
+
+// M2M for one level (anotherUpwardPass.c:345-560): one block per parent,
+// separable interpolation of the eight children's node values (z, y, x).
+template <int NN, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_m2m(PassArgs a, int Afirst)
+{
+  constexpr int N3 = NN * NN * NN, N2 = NN * NN;
+  __shared__ real sc[8 * N3];
+  __shared__ real szz[4 * N3];
+  __shared__ real syy[2 * N3];
+  __shared__ real sw[N3];
+  __shared__ real cw[2 * N2];
+  const int A = Afirst + blockIdx.x;
+  const int rhs = blockIdx.y;
+  real *S = a.S + (size_t)rhs * a.ncell * N3;
+  for (int o = threadIdx.x; o < 8 * N3; o += BLOCK) sc[o] = S[(size_t)(8 * A + 1) * N3 + o];
+  for (int o = threadIdx.x; o < 2 * N2; o += BLOCK) cw[o] = a.Cw[o];
+  __syncthreads();
+  for (int o = threadIdx.x; o < 4 * N3; o += BLOCK) { // z
+    const int pair = o / N3, rem = o % N3, l1 = rem / NN, pz = rem % NN;
+    real v = 0;
+#pragma unroll
+    for (int zb = 0; zb < 2; zb++) {
+      const real *ch = &sc[(2 * pair + zb) * N3 + l1 * NN];
+      const real *w = &cw[zb * N2 + pz * NN];
+#pragma unroll
+      for (int k = 0; k < NN; k++) v += ch[k] * w[k];
+    }
+    szz[o] = v;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < 2 * N3; o += BLOCK) { // y
+    const int xb = o / N3, rem = o % N3, cx = rem / N2, py = (rem / NN) % NN, pz = rem % NN;
+    real v = 0;
+#pragma unroll
+    for (int yb = 0; yb < 2; yb++) {
+      const real *s = &szz[(2 * xb + yb) * N3 + cx * N2 + pz];
+      const real *w = &cw[yb * N2 + py * NN];
+#pragma unroll
+      for (int k = 0; k < NN; k++) v += s[k * NN] * w[k];
+    }
+    syy[o] = v;
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < N3; l += BLOCK) { // x
+    const int px = l / N2, rem = l % N2;
+    real v = 0;
+#pragma unroll
+    for (int xb = 0; xb < 2; xb++) {
+      const real *s = &syy[xb * N3 + rem];
+      const real *w = &cw[xb * N2 + px * NN];
+#pragma unroll
+      for (int k = 0; k < NN; k++) v += s[k * N2] * w[k];
+    }
+    v *= a.prefac3;
+    S[(size_t)A * N3 + l] = v;
+    sw[l] = v * a.Kw[l];
+  }
+  __syncthreads();
+  project_u<NN, BLOCK>(sw, a.Urm, a.r, a.M + (size_t)rhs * a.ncell * a.r + (size_t)A * a.r);
+}
+
+// post-M2L (auxAnotherFMMInteraction.c:398-416, scale per level as
+// anotherFMMInteraction.cu:89-113): F = (1/L_l)^homogen Kw .* (U L).
+template <int NN, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_postm2l(PassArgs a, int Afirst)
+{
+  constexpr int N3 = NN * NN * NN;
+  extern __shared__ unsigned char smem_raw[];
+  real *sl = reinterpret_cast<real *>(smem_raw);
+  const int A = Afirst + blockIdx.x;
+  const int rhs = blockIdx.y;
+  int level = 0;
+  while (level < a.maxlev && A >= a.levsta[level + 1]) level++;
+  const real scale = a.scale[level];
+  const real *Lv = a.L + (size_t)rhs * a.ncell * a.r + (size_t)A * a.r;
+  for (int i = threadIdx.x; i < a.r; i += BLOCK) sl[i] = Lv[i];
+  __syncthreads();
+  real *F = a.F + (size_t)rhs * a.ncell * N3 + (size_t)A * N3;
+  for (int l = threadIdx.x; l < N3; l += BLOCK) {
+    real s = 0;
+#pragma unroll 8
+    for (int i = 0; i < a.r; i++) s += a.Ucm[l + (size_t)N3 * i] * sl[i];
+    F[l] = scale * a.Kw[l] * s;
+  }
+}
+
+// L2L for one level (anotherDownwardPassX.c:40-211): one block per parent,
+// children F += (2/n)^3 * interpolation of the parent's F (x, y, z).
+template <int NN, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_l2l(PassArgs a, int Afirst)
+{
+  constexpr int N3 = NN * NN * NN, N2 = NN * NN;
+  __shared__ real fa[N3];
+  __shared__ real fx[2 * N3];
+  __shared__ real fy[4 * N3];
+  __shared__ real cw[2 * N2];
+  const int A = Afirst + blockIdx.x;
+  const int rhs = blockIdx.y;
+  real *F = a.F + (size_t)rhs * a.ncell * N3;
+  for (int o = threadIdx.x; o < N3; o += BLOCK) fa[o] = F[(size_t)A * N3 + o];
+  for (int o = threadIdx.x; o < 2 * N2; o += BLOCK) cw[o] = a.Cw[o];
+  __syncthreads();
+  for (int o = threadIdx.x; o < 2 * N3; o += BLOCK) { // x
+    const int xb = o / N3, rem = o % N3, cx = rem / N2, yz = rem % N2;
+    real v = 0;
+#pragma unroll
+    for (int k = 0; k < NN; k++) v += fa[k * N2 + yz] * cw[xb * N2 + k * NN + cx];
+    fx[o] = v;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < 4 * N3; o += BLOCK) { // y
+    const int jj = o / N3, rem = o % N3, cx = rem / N2, cy = (rem / NN) % NN, pz = rem % NN;
+    real v = 0;
+#pragma unroll
+    for (int k = 0; k < NN; k++)
+      v += fx[(jj >> 1) * N3 + cx * N2 + k * NN + pz] * cw[(jj & 1) * N2 + k * NN + cy];
+    fy[o] = v;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < 8 * N3; o += BLOCK) { // z, add into the children
+    const int j = o / N3, rem = o % N3, l1 = rem / NN, cz = rem % NN;
+    real v = 0;
+#pragma unroll
+    for (int k = 0; k < NN; k++) v += fy[(j >> 1) * N3 + l1 * NN + k] * cw[(j & 1) * N2 + k * NN + cz];
+    F[(size_t)(8 * A + 1 + j) * N3 + rem] += a.prefac3 * v;
+  }
+}
+
+// L2P (anotherDownwardPassX.c:226-424): one thread per particle,
+// phi = q (2/n)^3 sum_{l} S(x) S(y) S(z) F_l.
+template <int NN>
+__global__ void __launch_bounds__(256) k_l2p(PassArgs a)
+{
+  constexpr int N3 = NN * NN * NN;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs = blockIdx.y;
+  if (i >= a.N) return;
+  const int A = a.leaf0 + a.leafof[i];
+  const real4 x = a.pos[i];
+  const real4 c = a.center[A];
+  real sxv[NN], syv[NN], szv[NN];
+  cheb_row<NN>(a.Tkz, a.ihalfL * (x.x - c.x), sxv);
+  cheb_row<NN>(a.Tkz, a.ihalfL * (x.y - c.y), syv);
+  cheb_row<NN>(a.Tkz, a.ihalfL * (x.z - c.z), szv);
+  const real *FA = a.F + (size_t)rhs * a.ncell * N3 + (size_t)A * N3;
+  real sum = 0;
+#pragma unroll
+  for (int l1 = 0; l1 < NN; l1++)
+#pragma unroll
+    for (int l2 = 0; l2 < NN; l2++) {
+      real inner = 0;
+#pragma unroll
+      for (int l3 = 0; l3 < NN; l3++) inner += szv[l3] * FA[(l1 * NN + l2) * NN + l3];
+      sum += sxv[l1] * syv[l2] * inner;
+    }
+  const size_t o = (size_t)rhs * a.N + i;
+  a.phi[o] = a.q[o] * a.prefac3 * sum;
+}
+
+// ---------------------------------------------------------------- launchers
+int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s)
+{
+  if (N <= 0) return BBFMM_OK;
+  hipLaunchKernelGGL(k_gather, dim3((N + 255) / 256, nrhs), dim3(256), 0, s, N, perm, src, dst);
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
+}
+
+int launch_scatter(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s)
+{
+  if (N <= 0) return BBFMM_OK;
+  hipLaunchKernelGGL(k_scatter, dim3((N + 255) / 256, nrhs), dim3(256), 0, s, N, perm, src, dst);
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
+}
+
+int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs)
+{
+  if (a.nleaf <= 0) return BBFMM_OK;
+  if (a.n == 4)
+    hipLaunchKernelGGL((k_p2m<4, 64>), dim3(a.nleaf, nrhs), dim3(64), 0, s, a);
+  else if (a.n == 8)
+    hipLaunchKernelGGL((k_p2m<8, 256>), dim3(a.nleaf, nrhs), dim3(256), 0, s, a);
+  else
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
+}
+
+int launch_m2m(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs)
+{
+  if (a.n != 4 && a.n != 8)
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+  for (int level = a.maxlev - 1; level >= kMinLev; level--) {
+    const int first = h_levsta[level], count = h_levsta[level + 1] - first;
+    if (a.n == 4)
+      hipLaunchKernelGGL((k_m2m<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, first);
+    else
+      hipLaunchKernelGGL((k_m2m<8, 256>), dim3(count, nrhs), dim3(256), 0, s, a, first);
+    BB_HIP(hipGetLastError());
+  }
+  return BBFMM_OK;
+}
+
+template <int NN, int BLOCK>
+static int postm2l_t(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs)
+{
+  const int first = h_levsta[kMinLev], count = a.ncell - first;
+  if (count <= 0) return BBFMM_OK;
+  hipLaunchKernelGGL((k_postm2l<NN, BLOCK>), dim3(count, nrhs), dim3(BLOCK), sizeof(real) * a.r, s, a,
+                     first);
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
+}
+
+int launch_postm2l(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs)
+{
+  if (a.n == 4) return postm2l_t<4, 64>(a, h_levsta, s, nrhs);
+  if (a.n == 8) return postm2l_t<8, 256>(a, h_levsta, s, nrhs);
+  return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+}
+
+int launch_l2l(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs)
+{
+  // L2L only (levels minlev..maxlev-1)
+  if (a.n != 4 && a.n != 8)
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+  for (int level = kMinLev; level < a.maxlev; level++) {
+    const int first = h_levsta[level], count = h_levsta[level + 1] - first;
+    if (a.n == 4)
+      hipLaunchKernelGGL((k_l2l<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, first);
+    else
+      hipLaunchKernelGGL((k_l2l<8, 256>), dim3(count, nrhs), dim3(256), 0, s, a, first);
+    BB_HIP(hipGetLastError());
+  }
+  return BBFMM_OK;
+}
+
+int launch_l2p(const PassArgs &a, hipStream_t s, int nrhs)
+{
+  if (a.N <= 0) return BBFMM_OK;
+  if (a.n == 4)
+    hipLaunchKernelGGL((k_l2p<4>), dim3((a.N + 255) / 256, nrhs), dim3(256), 0, s, a);
+  else if (a.n == 8)
+    hipLaunchKernelGGL((k_l2p<8>), dim3((a.N + 255) / 256, nrhs), dim3(256), 0, s, a);
+  else
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
+}
+
+} // namespace bb

@@ -1,0 +1,211 @@
+/*
+ * bbfmm_amd.h -- C ABI of the MI355X (gfx950) bbFMM downward-pass library.
+ *
+ * Two shared libraries are built from the same sources, as the reference
+ * builds two executables from one tree (real.h:6-62):
+ *   libbbfmm_amd_s.so   real = float   (define SINGLE before including)
+ *   libbbfmm_amd_d.so   real = double
+ *
+ * Part 1 is the drop-in surface: the entry points the reference's CUDA
+ * translation unit (main.cu -> anotherFMMCompute.cu) exports, with the same
+ * names, argument meaning and data layouts.  Part 2 is a persistent,
+ * device-resident plan used by the batched forms, the benchmark and the
+ * parity tests.  Every solve pass (P2M, M2M, M2L, post-M2L, L2L, L2P, P2P)
+ * runs on the GPU; there is no CPU fallback.  Setup (tree, lists, Chebyshev
+ * weights, operator precompute) runs on the host, as in the reference.
+ */
+#ifndef BBFMM_AMD_H
+#define BBFMM_AMD_H
+
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifdef SINGLE
+typedef float real;
+#else
+typedef double real;
+#endif
+
+/* vec234.h:18-24: real3 is an unaligned triple, real4 is 16-byte aligned */
+typedef struct { real x, y, z; } real3;
+
+/* cell.h:10-22 (non-PBC build) */
+typedef struct {
+  int *neighbors;
+  int *interaction;
+  real3 *center;
+  int *fieldsta, *fieldend, *sourcesta, *sourceend;
+  int *ineigh, *iinter;
+  int pitch_neighbors, pitch_interaction;
+} cell;
+
+/* anotherTree.h:7-20 */
+typedef struct {
+  int ncell;
+  int minlev;
+  int maxlev;
+  int *levsta;
+  int *levend;
+  real *celeng;
+  cell *c;
+  real *fieldval;
+  real *sourceval;
+  real *proxysval;
+  int ncell0;
+  int Nf, Ns, *fieldlist, *sourcelist;
+} anotherTree;
+
+/* m2l_drv.h:4-10 */
+#define M2L_EXIT_SUCCESS 0
+#define M2L_EXIT_FAIL 1
+#define M2L_SCHEME_BA 0
+#define M2L_SCHEME_SI 1
+#define M2L_SCHEME_CL 2
+#define M2L_SCHEME_IJ 3
+
+/* ====================================================================== */
+/* Part 1: drop-in entry points                                           */
+/* ====================================================================== */
+
+/* Replaces anotherFMMCompute (another.h:42-45; anotherFMMCompute.cu:8-45).
+   Runs the whole solve (upward, M2L, post-M2L, L2L/L2P, P2P) on the GPU for
+   the tree built by anotherBuildFMMHierachy; writes phi = q .* potential. */
+void anotherFMMCompute(anotherTree **atree, real3 *field, real3 *source, real *q,
+                       real *K, real *U, real *Tkz, int *Ktable,
+                       real *Kweights, real *Cweights, real homogen,
+                       int cutoff, int n, int dof, real *phi);
+
+/* Replaces anotherFMMInteraction (another.h:47-49; anotherFMMInteraction.cu:29-126).
+   Reads (*atree)->proxysval (M), runs M2L through m2l_drv and post-M2L on the
+   GPU, allocates and fills (*atree)->fieldval, frees proxysval. */
+void anotherFMMInteraction(anotherTree **atree, real *E, int *Ktable,
+                           real *U, real *Kweights, int n, int dof,
+                           int cutoff, real homogen);
+
+/* Replaces the GPU anotherDownwardPass (another.h:52-54; anotherDownwardPass.cu:41-411).
+   L2L and L2P from (*atree)->fieldval (freed on return), then P2P; writes phi. */
+void anotherDownwardPass(anotherTree **atree, real3 *field, real3 *source,
+                         real *Cweights, real *Tkz, real *q, real *U,
+                         int cutoff, int n, int dof, real homogen, real *phi);
+
+/* Replaces m2l_drv (m2l_drv.cu:14-93).  L[cell*r] for cells in levels
+   minlev..maxlev from M[cell*r] and the 316 column-major r x r operators K.
+   scheme: M2L_SCHEME_BA (basic), _SI (sibling), _CL (cluster), _IJ (dense
+   grid).  Returns M2L_EXIT_SUCCESS, or M2L_EXIT_FAIL for an unsupported r
+   or scheme, or a HIP failure (the reference aborts or returns FAIL). */
+int m2l_drv(int r, real *L, real *K, real *M, int minlev, int maxlev,
+            real L0, real3 *center,
+            int *ineigh, int pitch_neighbors, int *neighbors,
+            int *iinter, int pitch_interaction, int *interaction,
+            int scheme);
+
+/* m2l_host.h:24: seconds spent in M2L kernels by the last m2l_drv call */
+extern double kernel_exec_time;
+
+/* Tree builder (another.h:36-39; anotherBuildFMMHierachy.c:24-155) */
+void anotherBuildFMMHierachy(real L, int n, int dof, int l, int cutoff,
+                             anotherTree **atree, int Nf, int Ns,
+                             real3 *field, real3 *source);
+void anotherFMMCleanup(anotherTree **atree);
+
+/* Event timers (eventTimer.h:16-32), on the library's HIP stream */
+typedef struct {
+  float time;    /* accumulated milliseconds */
+  void *start;   /* hipEvent_t */
+  void *stop;    /* hipEvent_t */
+} eventTimerType;
+void initEventTimer(eventTimerType *timer);
+void startEventTimer(eventTimerType *timer);
+void stopEventTimer(eventTimerType *timer);
+void finalizeEventTimer(eventTimerType *timer);
+void printEventTimer(FILE *fp, char *s, eventTimerType *timer);
+double readEventTimer(eventTimerType *timer);   /* milliseconds */
+double getEventTimer(eventTimerType *timer);    /* seconds */
+void setEventTimer(eventTimerType *timer, double val);
+
+/* Error reporting for the void entry points: 0 if the last call on this
+   thread succeeded, else a BBFMM_E_* code; the string describes it. */
+int bbfmm_amd_last_error(void);
+const char *bbfmm_amd_last_error_string(void);
+
+/* Default M2L scheme used by anotherFMMInteraction / anotherFMMCompute (the
+   reference fixes it at compile time with -DSCHEME; default here: IJ). */
+int bbfmm_amd_set_default_scheme(int scheme);
+
+#define BBFMM_OK 0
+#define BBFMM_E_ARG 1        /* invalid argument or unsupported n / r / scheme */
+#define BBFMM_E_HIP 2        /* HIP runtime failure (message has the HIP error) */
+#define BBFMM_E_NOMEM 3      /* allocation failure */
+#define BBFMM_E_NODEVICE 4   /* no GPU visible */
+#define BBFMM_E_RANGE 5      /* particle outside the root cell */
+
+/* ====================================================================== */
+/* Part 2: persistent plan (device-resident tree, operators and buffers)   */
+/* ====================================================================== */
+typedef struct bbfmm_amd_plan bbfmm_amd_plan;
+
+/* M2L arithmetic forms */
+#define BBFMM_M2L_PLAIN 0     /* real-precision FMAs (drop-in parity) */
+#define BBFMM_M2L_BF16X2 1    /* split BF16 hi/lo operands, 3 MFMA products, FP32 accumulate */
+#define BBFMM_M2L_FP32MFMA 2  /* FP32 MFMA */
+
+/* Plan options (bbfmm_amd_plan_set) */
+#define BBFMM_OPT_SCHEME 1    /* M2L_SCHEME_* */
+#define BBFMM_OPT_FORM 2      /* BBFMM_M2L_* */
+#define BBFMM_OPT_TIMING 3    /* 1: record per-stage HIP events on every solve */
+
+/* Create a plan for N particles at host positions pos (N real3, inside the
+   root cell [-L/2,L/2]^3), Chebyshev order n, level modifier lmod (the -d
+   option; maxlev follows bbfmm.c:151-160), and nrhs charge vectors per solve.
+   K (316 r x r column-major) and U (n^3 x r column-major) are host arrays
+   in the reference's file layout.  The tree and lists are built on the host
+   and uploaded with the operators.  stream: hipStream_t (NULL = default). */
+int bbfmm_amd_plan_create(bbfmm_amd_plan **plan, int N, const real3 *pos, real L,
+                          int n, int lmod, int r, const real *K, const real *U,
+                          int nrhs, void *stream);
+int bbfmm_amd_plan_destroy(bbfmm_amd_plan *plan);
+int bbfmm_amd_plan_set(bbfmm_amd_plan *plan, int option, int value);
+
+/* Tree facts: out[0]=maxlev, out[1]=ncell, out[2]=N, out[3]=r, out[4]=n,
+   out[5]=nrhs.  dout[0]=M2L interactions (levels 2..maxlev, per vector),
+   dout[1]=P2P pairs (self pairs included, per vector). */
+int bbfmm_amd_plan_info(const bbfmm_amd_plan *plan, int *out, double *dout);
+
+/* One solve on device memory: q_dev is nrhs x N (row per vector, original
+   particle order); phi_dev receives q .* potential in the same layout.
+   Asynchronous on the plan's stream. */
+int bbfmm_amd_solve(bbfmm_amd_plan *plan, const real *q_dev, real *phi_dev);
+
+/* Same from host arrays (PCIe copies included; synchronous). */
+int bbfmm_amd_solve_host(bbfmm_amd_plan *plan, const real *q, real *phi);
+
+/* M2L alone on the plan's tree: M_dev and L_dev are nrhs x ncell x r
+   device arrays in global cell order; L is written for levels 2..maxlev. */
+int bbfmm_amd_m2l(bbfmm_amd_plan *plan, const real *M_dev, real *L_dev);
+
+/* Copy an intermediate of the last solve to host memory, in global cell
+   order for vector 0: which = 0 M (ncell*r), 1 L (ncell*r),
+   2 F after post-M2L (ncell*n^3), 3 far-field q.*phi (N, original order). */
+int bbfmm_amd_fetch(bbfmm_amd_plan *plan, int which, real *dst);
+
+/* Per-stage kernel times (ms) of the last timed solve: 0 gather, 1 P2M,
+   2 M2M, 3 M2L, 4 post-M2L, 5 L2L, 6 L2P, 7 P2P, 8 scatter, 9 total.
+   Returns the number of entries written (<= nout). */
+int bbfmm_amd_stage_times(const bbfmm_amd_plan *plan, double *out, int nout);
+
+/* Host-side setup helpers (reference layouts) */
+int bbfmm_amd_maxlev(int N, int lmod);                                   /* bbfmm.c:151-160 */
+void bbfmm_amd_weights(int n, real *Tkz, int *Ktable, real *Kweights,
+                       real *Cweights);                                  /* bbfmm.c:922-1028 */
+/* Operator precompute in FP64 (bbfmm.c:1209-1421): U (n^3 x r) and K
+   (316 r x r), column-major, cast to real.  sigma (n^3 values) optional. */
+int bbfmm_amd_operators(int n, int r, real *U, real *K, double *sigma);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BBFMM_AMD_H */
