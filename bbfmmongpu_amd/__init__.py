@@ -151,7 +151,7 @@ class Plan:
         self.maxlev, self.ncell = int(info[0]), int(info[1])
         self.m2l_interactions, self.p2p_pairs = float(dinfo[0]), float(dinfo[1])
 
-    def set(self, scheme: int | None = None, form: int | None = None, timing: bool | None = None):
+    def set(self, scheme: int | None = None, form: int | None = None, timing: int | None = None):
         if scheme is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_SCHEME, scheme))
         if form is not None:

@@ -77,7 +77,11 @@ struct bbfmm_amd_plan {
   std::vector<real> h_K;                       // column-major operators (for the grid form)
   bb::GridM2L *grid = nullptr;
   int grid_form = -1;
-  hipEvent_t ev[ST_TOTAL + 1] = {};
+  // per-stage timing: a ring of `nsets` event sets, one set per solve,
+  // read (and averaged) only when stage times are requested
+  int nsets = 0;
+  std::vector<hipEvent_t> ev;  // nsets * (ST_TOTAL + 1)
+  long long solves = 0, timed_from = 0;
   double stage_ms[ST_TOTAL + 1] = {};
   bool solved = false;
 };
@@ -129,6 +133,46 @@ static void plan_free_device(bbfmm_amd_plan *p)
   p->grid = nullptr;
   for (auto &e : p->ev)
     if (e) (void)hipEventDestroy(e);
+  p->ev.clear();
+  p->nsets = 0;
+}
+
+static int set_timing(bbfmm_amd_plan *p, int nsets)
+{
+  (void)hipStreamSynchronize(p->stream);
+  for (auto &e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  p->ev.assign((size_t)nsets * (ST_TOTAL + 1), nullptr);
+  for (auto &e : p->ev) BB_HIP(hipEventCreate(&e));
+  p->nsets = nsets;
+  p->timed_from = p->solves;
+  return BBFMM_OK;
+}
+
+static int read_timing(bbfmm_amd_plan *p)
+{
+  for (auto &v : p->stage_ms) v = 0;
+  const long long done = p->solves - p->timed_from;
+  const int sets = (int)(done < p->nsets ? done : p->nsets);
+  if (sets <= 0) return BBFMM_OK;
+  BB_HIP(hipStreamSynchronize(p->stream));
+  for (int k = 0; k < sets; k++) {
+    hipEvent_t *e = &p->ev[(size_t)k * (ST_TOTAL + 1)];
+    for (int st = 0; st < ST_TOTAL; st++) {
+      float ms = 0;
+      BB_HIP(hipEventElapsedTime(&ms, e[st], e[st + 1]));
+      p->stage_ms[st] += ms / sets;
+    }
+    float tot = 0;
+    BB_HIP(hipEventElapsedTime(&tot, e[0], e[ST_TOTAL]));
+    p->stage_ms[ST_TOTAL] += tot / sets;
+  }
+  return BBFMM_OK;
+}
+
+static inline hipEvent_t stage_event(bbfmm_amd_plan *p, int st)
+{
+  return p->ev[(size_t)(p->solves % p->nsets) * (ST_TOTAL + 1) + st];
 }
 
 // Upload everything a solve needs, from a tree in the reference's layout.
@@ -274,7 +318,6 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
   BB_HIP(hipMemsetAsync(p->d_L, 0, sizeof(real) * (size_t)ncell * r * nrhs, stream));
   BB_HIP(hipMemsetAsync(p->d_S, 0, sizeof(real) * (size_t)ncell * n3 * nrhs, stream));
   BB_HIP(hipMemsetAsync(p->d_F, 0, sizeof(real) * (size_t)ncell * n3 * nrhs, stream));
-  for (auto &e : p->ev) BB_HIP(hipEventCreate(&e));
   BB_HIP(hipStreamSynchronize(stream));
   return BBFMM_OK;
 }
@@ -353,21 +396,21 @@ static int solve_sorted(bbfmm_amd_plan *p)
 {
   hipStream_t s = p->stream;
   const PassArgs a = pass_args(p);
-  const bool tm = p->timing != 0;
-  if (tm) BB_HIP(hipEventRecord(p->ev[ST_P2M], s));
+  const bool tm = p->nsets > 0;
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2M), s));
   BB_TRY(launch_p2m(a, s, p->nrhs));
-  if (tm) BB_HIP(hipEventRecord(p->ev[ST_M2M], s));
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2M), s));
   BB_TRY(launch_m2m(a, p->h_levsta.data(), s, p->nrhs));
-  if (tm) BB_HIP(hipEventRecord(p->ev[ST_M2L], s));
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2L), s));
   BB_TRY(run_m2l(p, p->d_M, p->d_L));
-  if (tm) BB_HIP(hipEventRecord(p->ev[ST_POST], s));
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_POST), s));
   BB_TRY(launch_postm2l(a, p->h_levsta.data(), s, p->nrhs));
-  if (tm) BB_HIP(hipEventRecord(p->ev[ST_L2L], s));
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2L), s));
   BB_TRY(launch_l2l(a, p->h_levsta.data(), s, p->nrhs));
-  if (tm) BB_HIP(hipEventRecord(p->ev[ST_L2P], s));
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2P), s));
   BB_TRY(launch_l2p(a, s, p->nrhs));
   BB_HIP(hipMemcpyAsync(p->d_farfield, p->d_phi, sizeof(real) * p->N, hipMemcpyDeviceToDevice, s));
-  if (tm) BB_HIP(hipEventRecord(p->ev[ST_P2P], s));
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2P), s));
   P2PArgs pp;
   pp.N = p->N;
   pp.nleaf = p->nleaf;
@@ -386,24 +429,14 @@ static int solve_sorted(bbfmm_amd_plan *p)
 static int solve_dev(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev)
 {
   hipStream_t s = p->stream;
-  const bool tm = p->timing != 0;
-  if (tm) BB_HIP(hipEventRecord(p->ev[ST_GATHER], s));
+  const bool tm = p->nsets > 0;
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_GATHER), s));
   BB_TRY(launch_gather(p->d_perm, q_dev, p->d_q, p->N, p->nrhs, s));
   BB_TRY(solve_sorted(p));
-  if (tm) BB_HIP(hipEventRecord(p->ev[ST_SCATTER], s));
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_SCATTER), s));
   BB_TRY(launch_scatter(p->d_perm, p->d_phi, phi_dev, p->N, p->nrhs, s));
-  if (tm) {
-    BB_HIP(hipEventRecord(p->ev[ST_TOTAL], s));
-    BB_HIP(hipEventSynchronize(p->ev[ST_TOTAL]));
-    for (int k = 0; k < ST_TOTAL; k++) {
-      float ms = 0;
-      BB_HIP(hipEventElapsedTime(&ms, p->ev[k], p->ev[k + 1]));
-      p->stage_ms[k] = ms;
-    }
-    float tot = 0;
-    BB_HIP(hipEventElapsedTime(&tot, p->ev[ST_GATHER], p->ev[ST_TOTAL]));
-    p->stage_ms[ST_TOTAL] = tot;
-  }
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_TOTAL), s));
+  p->solves++;
   p->solved = true;
   return BBFMM_OK;
 }
@@ -475,8 +508,9 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     p->form = value;
     return BBFMM_OK;
   case BBFMM_OPT_TIMING:
-    p->timing = value != 0;
-    return BBFMM_OK;
+    if (value < 0 || value > 4096) return set_error(BBFMM_E_ARG, "timing sets %d out of range", value);
+    p->timing = value;
+    return set_timing(p, value);
   default: return set_error(BBFMM_E_ARG, "unknown option %d", option);
   }
 }
@@ -555,9 +589,10 @@ extern "C" int bbfmm_amd_fetch(bbfmm_amd_plan *p, int which, real *dst)
   return BBFMM_OK;
 }
 
-extern "C" int bbfmm_amd_stage_times(const bbfmm_amd_plan *p, double *out, int nout)
+extern "C" int bbfmm_amd_stage_times(bbfmm_amd_plan *p, double *out, int nout)
 {
   if (!p || !out) return 0;
+  if (read_timing(p) != BBFMM_OK) return 0;
   const int m = nout < ST_TOTAL + 1 ? nout : ST_TOTAL + 1;
   for (int k = 0; k < m; k++) out[k] = p->stage_ms[k];
   return m;

@@ -1,26 +1,501 @@
 // m2l_grid.hip -- dense-grid M2L (schemes CL and IJ).
+//
+// At level l (2^l cells per axis) a field cell f = 2p + s has parent p and
+// sibling class s in {0,1}^3.  Its interaction list is every cell c = f + v
+// whose parent is a neighbour of p (|p(c) - p|_inf <= 1) and which is not a
+// neighbour of f (|v|_inf >= 2) -- 189 offsets per class, with
+//   source class  s' = (s + v) mod 2,   parent shift  d = floor((s + v) / 2),
+// so the sources of all class-s cells for one offset v form the class-s'
+// parent grid shifted by d.  M is scattered into eight per-class parent grids
+// with one ghost layer of zeros on every side; then for every class s and
+// offset v
+//     L_s[:, p] += K_v M_{s'}[:, p + d]      (an r x r by r x ncols product)
+// which is a GEMM per (s, v) accumulated in registers -- the natural MFMA
+// form.  Cells at the domain boundary read ghost zeros, so the result equals
+// the list-driven M2L (anotherInteractionList.c:11-111).
+//
+// Columns are (parent position, right-hand side) pairs, so many charge
+// vectors widen every product.  One launch covers all levels.
+//
+// Forms:
+//   PLAIN   real-precision FMAs on the vector ALUs (drop-in parity);
+//   BF16X2  K and M split into bf16 hi + lo, three MFMA products
+//           (hi*hi + hi*lo + lo*hi) accumulated in fp32 (fp32 build);
+//   FP32    v_mfma_f32_32x32x2_f32 (fp32 build).
+// MFMA forms use r = 32 (one 32 x 32 output tile per 32 columns).
 #include "common.h"
 #include "kernels.h"
 
+#include <vector>
+
 namespace bb {
 
-struct GridM2L {
-  int dummy;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kOffsets = 189;
+constexpr int kMaxLev = 10;
+
+struct GridArgs {
+  int r, nrhs, ncell, minlev, maxlev;
+  int blk0[kMaxLev + 2];      // first block of each level (flattened launch)
+  int lvl_slot0[kMaxLev + 1];  // first slot of each level's 8 class grids
+  int levsta[kMaxLev + 2];
+  const int4 *tab;  // [8][189]: (k, source class, packed shift, 0)
+  const void *Ka;   // operators in the form's layout
+  const void *Mg0;  // grids (hi / real)
+  const void *Mg1;  // grids (lo) for BF16X2
+  real *L;
+  int wave_cols;    // columns per wave
 };
 
-int grid_m2l_create(GridM2L **g, int r, int maxlev, int nrhs, int form, const real *K, hipStream_t s)
+struct GridM2L {
+  int r = 0, maxlev = 0, nrhs = 1, form = 0, ncell_cap = 0;
+  GridArgs args{};
+  int nblocks = 0;
+  size_t nslots = 0;
+  int4 *d_tab = nullptr;
+  void *d_K = nullptr;
+  void *d_Mg0 = nullptr;
+  void *d_Mg1 = nullptr;
+};
+
+__device__ __forceinline__ int morton3(int x, int y, int z, int bits)
 {
-  (void)r; (void)maxlev; (void)nrhs; (void)form; (void)K; (void)s;
-  *g = nullptr;
-  return set_error(BBFMM_E_ARG, "dense-grid M2L is not built yet");
+  int m = 0;
+  for (int b = 0; b < bits; b++)
+    m |= ((((x >> b) & 1) << 2) | (((y >> b) & 1) << 1) | ((z >> b) & 1)) << (3 * b);
+  return m;
 }
 
-void grid_m2l_destroy(GridM2L *g) { delete g; }
+__device__ __forceinline__ void demorton3(int m, int bits, int &x, int &y, int &z)
+{
+  x = y = z = 0;
+  for (int b = 0; b < bits; b++) {
+    const int s = (m >> (3 * b)) & 7;
+    x |= ((s >> 2) & 1) << b;
+    y |= ((s >> 1) & 1) << b;
+    z |= (s & 1) << b;
+  }
+}
+
+// slot of (level, class, parent p) in the ghost-padded grids
+__device__ __forceinline__ int grid_slot(const GridArgs &g, int l, int c, int px, int py, int pz)
+{
+  const int PP = (1 << (l - 1)) + 2;
+  return g.lvl_slot0[l] + ((c * PP + px + 1) * PP + py + 1) * PP + pz + 1;
+}
+
+// ---------------------------------------------------------------- scatter
+// M [rhs][ncell][r] -> per-class grids [slot][rhs][r]; one thread per
+// (cell, rhs, 8 consecutive coefficients).
+template <int FORM>
+__global__ void k_grid_scatter(GridArgs g, const real *__restrict__ M)
+{
+  const int chunks = g.r / 8;
+  const long long total = (long long)(g.levsta[g.maxlev + 1] - g.levsta[g.minlev]) * g.nrhs * chunks;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int ch = (int)(t % chunks);
+  const long long cr = t / chunks;
+  const int rhs = (int)(cr % g.nrhs);
+  const int A = g.levsta[g.minlev] + (int)(cr / g.nrhs);
+  int l = g.minlev;
+  while (A >= g.levsta[l + 1]) l++;
+  int x, y, z;
+  demorton3(A - g.levsta[l], l, x, y, z);
+  const int c = ((x & 1) << 2) | ((y & 1) << 1) | (z & 1);
+  const size_t slot = grid_slot(g, l, c, x >> 1, y >> 1, z >> 1);
+  const real *src = M + ((size_t)rhs * g.ncell + A) * g.r + ch * 8;
+  const size_t dst = (slot * g.nrhs + rhs) * g.r + ch * 8;
+  if (FORM == BBFMM_M2L_BF16X2) {
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const float v = (float)src[e];
+      const __bf16 h = (__bf16)v;
+      hi[e] = h;
+      lo[e] = (__bf16)(v - (float)h);
+    }
+    *reinterpret_cast<bf16x8 *>((__bf16 *)g.Mg0 + dst) = hi;
+    *reinterpret_cast<bf16x8 *>((__bf16 *)g.Mg1 + dst) = lo;
+  } else {
+    real *d = (real *)g.Mg0 + dst;
+#pragma unroll
+    for (int e = 0; e < 8; e++) d[e] = src[e];
+  }
+}
+
+// level / class / column-group of a block of the flattened launch
+__device__ __forceinline__ void decode_block(const GridArgs &g, int b, int &l, int &s, int &grp)
+{
+  l = g.minlev;
+  while (l < g.maxlev && b >= g.blk0[l + 1]) l++;
+  const int rel = b - g.blk0[l];
+  s = rel & 7;
+  grp = rel >> 3;
+}
+
+// ---------------------------------------------------------------- MFMA kernel
+// 256 threads = 4 waves, all of one class s at one level; each wave owns
+// TPW tiles of 32 columns.  Per offset v: A = K_v (bf16 hi/lo or fp32, read
+// in lane order), B = the shifted sources of the wave's columns.
+template <int FORM, int TPW>
+__global__ void __launch_bounds__(256) k_m2l_mfma(GridArgs g)
+{
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  int l, s, grp;
+  decode_block(g, blockIdx.x, l, s, grp);
+  const int P = 1 << (l - 1);
+  const int PP = P + 2;
+  const int ncols = P * P * P * g.nrhs;
+  const int half = lane >> 5;  // which k-half of the operand this lane holds
+  int base[TPW], rhs[TPW], pcol[TPW];
+  bool valid[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; t++) {
+    const int col = ((grp * 4 + wave) * TPW + t) * 32 + (lane & 31);
+    valid[t] = col < ncols;
+    const int c = valid[t] ? col : 0;
+    const int pl = c / g.nrhs;
+    rhs[t] = c - pl * g.nrhs;
+    pcol[t] = pl;
+    const int pz = pl % P, py = (pl / P) % P, px = pl / (P * P);
+    base[t] = ((px + 1) * PP + py + 1) * PP + pz + 1;
+  }
+  f32x16 acc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; t++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[t][e] = 0.0f;
+
+  const int4 *tab = g.tab + s * kOffsets;
+  const int slot0 = g.lvl_slot0[l];
+  for (int v = 0; v < kOffsets; v++) {
+    const int4 e = tab[v];
+    const int k = e.x;
+    const int sc = e.y;
+    const int dx = ((e.z >> 4) & 3) - 1, dy = ((e.z >> 2) & 3) - 1, dz = (e.z & 3) - 1;
+    const int off = slot0 + sc * PP * PP * PP + (dx * PP + dy) * PP + dz;
+    if (FORM == BBFMM_M2L_BF16X2) {
+      // A: [k][kstep][hi/lo][lane] bf16x8
+      const bf16x8 *Ka = (const bf16x8 *)g.Ka + (size_t)k * 4 * 64 + lane;
+      const bf16x8 a0h = Ka[0 * 64], a0l = Ka[1 * 64], a1h = Ka[2 * 64], a1l = Ka[3 * 64];
+      const __bf16 *Mh = (const __bf16 *)g.Mg0;
+      const __bf16 *Ml = (const __bf16 *)g.Mg1;
+#pragma unroll
+      for (int t = 0; t < TPW; t++) {
+        const size_t o = ((size_t)(off + base[t]) * g.nrhs + rhs[t]) * 32 + half * 8;
+        bf16x8 b0h = *(const bf16x8 *)(Mh + o), b0l = *(const bf16x8 *)(Ml + o);
+        bf16x8 b1h = *(const bf16x8 *)(Mh + o + 16), b1l = *(const bf16x8 *)(Ml + o + 16);
+        if (!valid[t]) {
+#pragma unroll
+          for (int q = 0; q < 8; q++) b0h[q] = b0l[q] = b1h[q] = b1l[q] = (__bf16)0.0f;
+        }
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0h, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0l, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, b0h, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1h, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1l, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, b1h, acc[t], 0, 0, 0);
+      }
+    } else {
+      // FP32: lane holds k = 16*half + ks for ks = 0..15; A: [k][lane][16]
+      const f32x4 *Ka = (const f32x4 *)g.Ka + ((size_t)k * 64 + lane) * 4;
+      f32x4 a[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) a[q] = Ka[q];
+      const float *Mf = (const float *)g.Mg0;
+#pragma unroll
+      for (int t = 0; t < TPW; t++) {
+        const size_t o = ((size_t)(off + base[t]) * g.nrhs + rhs[t]) * 32 + half * 16;
+        const f32x4 *bp = (const f32x4 *)(Mf + o);
+        f32x4 b[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          b[q] = bp[q];
+          if (!valid[t]) b[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+          for (int w = 0; w < 4; w++)
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q][w], b[q][w], acc[t], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: lane holds rows 8*(e/4) + 4*half + e%4 of its column
+#pragma unroll
+  for (int t = 0; t < TPW; t++) {
+    if (!valid[t]) continue;
+    const int pl = pcol[t];
+    const int pz = pl % P, py = (pl / P) % P, px = pl / (P * P);
+    const int fx = 2 * px + ((s >> 2) & 1), fy = 2 * py + ((s >> 1) & 1), fz = 2 * pz + (s & 1);
+    const int A = g.levsta[l] + morton3(fx, fy, fz, l);
+    float *Lp = (float *)g.L + ((size_t)rhs[t] * g.ncell + A) * 32 + 4 * half;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      f32x4 o = {acc[t][4 * q], acc[t][4 * q + 1], acc[t][4 * q + 2], acc[t][4 * q + 3]};
+      *reinterpret_cast<f32x4 *>(Lp + 8 * q) = o;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- plain kernel
+// One thread per (column, row); 256 / R columns per block, all of one class.
+template <int R>
+__global__ void __launch_bounds__(256) k_m2l_grid_plain(GridArgs g)
+{
+  constexpr int CPB = 256 / R;
+  int l, s, grp;
+  decode_block(g, blockIdx.x, l, s, grp);
+  const int P = 1 << (l - 1);
+  const int PP = P + 2;
+  const int ncols = P * P * P * g.nrhs;
+  const int i = threadIdx.x % R;
+  const int col = grp * CPB + threadIdx.x / R;
+  if (col >= ncols) return;  // no barrier in this kernel
+  const int pl = col / g.nrhs, rh = col - (col / g.nrhs) * g.nrhs;
+  const int pz = pl % P, py = (pl / P) % P, px = pl / (P * P);
+  const int base = ((px + 1) * PP + py + 1) * PP + pz + 1;
+  const int4 *tab = g.tab + s * kOffsets;
+  const real4 *K4 = (const real4 *)g.Ka;
+  const real *Mg = (const real *)g.Mg0;
+  const int slot0 = g.lvl_slot0[l];
+  real acc0 = 0, acc1 = 0;
+  for (int v = 0; v < kOffsets; v++) {
+    const int4 e = tab[v];
+    const int dx = ((e.z >> 4) & 3) - 1, dy = ((e.z >> 2) & 3) - 1, dz = (e.z & 3) - 1;
+    const size_t slot = (size_t)slot0 + e.y * PP * PP * PP + (dx * PP + dy) * PP + dz + base;
+    const real4 *Mp = (const real4 *)(Mg + (slot * g.nrhs + rh) * R);
+    const real4 *Kp = K4 + (size_t)e.x * (R / 4) * R + i;
+#pragma unroll 8
+    for (int jb = 0; jb < R / 4; jb += 2) {
+      const real4 ka = Kp[(size_t)jb * R], kb = Kp[(size_t)(jb + 1) * R];
+      const real4 ma = Mp[jb], mb = Mp[jb + 1];
+      acc0 += ka.x * ma.x;
+      acc1 += kb.x * mb.x;
+      acc0 += ka.y * ma.y;
+      acc1 += kb.y * mb.y;
+      acc0 += ka.z * ma.z;
+      acc1 += kb.z * mb.z;
+      acc0 += ka.w * ma.w;
+      acc1 += kb.w * mb.w;
+    }
+  }
+  const int fx = 2 * px + ((s >> 2) & 1), fy = 2 * py + ((s >> 1) & 1), fz = 2 * pz + (s & 1);
+  const int A = g.levsta[l] + morton3(fx, fy, fz, l);
+  g.L[((size_t)rh * g.ncell + A) * R + i] = acc0 + acc1;
+}
+
+// ---------------------------------------------------------------- host side
+static void offset_table(std::vector<int4> &tab)
+{
+  int Kt[343];
+  ktable(Kt);
+  tab.assign(8 * kOffsets, int4{0, 0, 0, 0});
+  for (int s = 0; s < 8; s++) {
+    const int sb[3] = {(s >> 2) & 1, (s >> 1) & 1, s & 1};
+    int nv = 0;
+    for (int vx = -2 - sb[0]; vx <= 3 - sb[0]; vx++)
+      for (int vy = -2 - sb[1]; vy <= 3 - sb[1]; vy++)
+        for (int vz = -2 - sb[2]; vz <= 3 - sb[2]; vz++) {
+          if (vx >= -1 && vx <= 1 && vy >= -1 && vy <= 1 && vz >= -1 && vz <= 1) continue;
+          const int v[3] = {vx, vy, vz};
+          int sc = 0, packed = 0;
+          for (int a = 0; a < 3; a++) {
+            const int t = sb[a] + v[a];           // in [-2, 3]
+            const int d = t >= 0 ? t / 2 : -1;    // floor(t / 2) in {-1, 0, 1}
+            sc |= (t - 2 * d) << (2 - a);
+            packed |= (d + 1) << (2 * (2 - a));
+          }
+          tab[s * kOffsets + nv] = int4{Kt[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)], sc, packed, 0};
+          nv++;
+        }
+  }
+}
+
+int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const real *K, hipStream_t s)
+{
+  *out = nullptr;
+  if (maxlev > kMaxLev) return set_error(BBFMM_E_ARG, "dense-grid M2L: maxlev %d > %d", maxlev, kMaxLev);
+  if (r != 32 && r != 256) return set_error(BBFMM_E_ARG, "dense-grid M2L: r=%d not supported", r);
+#ifdef SINGLE
+  if (form != BBFMM_M2L_PLAIN && r != 32)
+    return set_error(BBFMM_E_ARG, "MFMA M2L forms are built for r = 32 (got r=%d)", r);
+#else
+  if (form != BBFMM_M2L_PLAIN) return set_error(BBFMM_E_ARG, "MFMA M2L forms need the fp32 build");
+#endif
+  GridM2L *g = new GridM2L();
+  g->r = r;
+  g->maxlev = maxlev;
+  g->nrhs = nrhs;
+  g->form = form;
+  GridArgs &a = g->args;
+  a.r = r;
+  a.nrhs = nrhs;
+  a.minlev = kMinLev;
+  a.maxlev = maxlev;
+  for (int l = 0; l <= maxlev + 1; l++) a.levsta[l] = ((1 << (3 * l)) - 1) / 7;
+  a.ncell = a.levsta[maxlev + 1];
+  size_t slots = 0;
+  for (int l = kMinLev; l <= maxlev; l++) {
+    a.lvl_slot0[l] = (int)slots;
+    const size_t PP = ((size_t)1 << (l - 1)) + 2;
+    slots += 8 * PP * PP * PP;
+  }
+  g->nslots = slots;
+  int tpw = 1, cpb;
+  if (form == BBFMM_M2L_PLAIN) {
+    cpb = 256 / r;
+  } else {
+    // two tiles per wave once there are enough columns to fill the chip
+    const long long deep = (long long)(1 << (3 * (maxlev - 1))) * nrhs;
+    tpw = deep >= 8 * 1024 ? 2 : 1;
+    cpb = 4 * tpw * 32;
+  }
+  a.wave_cols = tpw * 32;
+  int nb = 0;
+  for (int l = kMinLev; l <= maxlev; l++) {
+    a.blk0[l] = nb;
+    const long long cols = (long long)(1 << (3 * (l - 1))) * nrhs;
+    nb += 8 * (int)((cols + cpb - 1) / cpb);
+  }
+  a.blk0[maxlev + 1] = nb;
+  g->nblocks = nb;
+
+  std::vector<int4> tab;
+  offset_table(tab);
+  int rc = BBFMM_OK;
+  auto fail = [&](hipError_t e, const char *what) {
+    rc = set_error(BBFMM_E_HIP, "dense-grid M2L: %s: %s", what, hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&g->d_tab, sizeof(int4) * tab.size())) != hipSuccess) fail(e, "hipMalloc");
+  if (rc == BBFMM_OK && (e = hipMemcpyAsync(g->d_tab, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+    fail(e, "copy");
+  // operators in the form's layout
+  if (rc == BBFMM_OK) {
+    if (form == BBFMM_M2L_PLAIN) {
+      std::vector<real4> K4((size_t)kNumOps * r * r / 4);
+      for (int k = 0; k < kNumOps; k++)
+        for (int jb = 0; jb < r / 4; jb++)
+          for (int i = 0; i < r; i++) {
+            const real *Kk = K + (size_t)k * r * r;
+            K4[((size_t)k * (r / 4) + jb) * r + i] = real4{Kk[(size_t)(4 * jb) * r + i], Kk[(size_t)(4 * jb + 1) * r + i],
+                                                          Kk[(size_t)(4 * jb + 2) * r + i], Kk[(size_t)(4 * jb + 3) * r + i]};
+          }
+      if ((e = hipMalloc(&g->d_K, sizeof(real4) * K4.size())) != hipSuccess) fail(e, "hipMalloc");
+      else if ((e = hipMemcpyAsync(g->d_K, K4.data(), sizeof(real4) * K4.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+        fail(e, "copy");
+      if (rc == BBFMM_OK && (e = hipMalloc(&g->d_Mg0, sizeof(real) * slots * nrhs * r)) != hipSuccess) fail(e, "hipMalloc");
+      if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_Mg0, 0, sizeof(real) * slots * nrhs * r, s)) != hipSuccess) fail(e, "memset");
+    }
+#ifdef SINGLE
+    else if (form == BBFMM_M2L_BF16X2) {
+      // [k][kstep][hi/lo][lane][8]: lane l holds row l%32, k = 16*kstep + 8*(l/32) + e
+      std::vector<__bf16> Kb((size_t)kNumOps * 2 * 2 * 64 * 8);
+      for (int k = 0; k < kNumOps; k++)
+        for (int ks = 0; ks < 2; ks++)
+          for (int lane = 0; lane < 64; lane++)
+            for (int q = 0; q < 8; q++) {
+              const int row = lane & 31, col = 16 * ks + 8 * (lane >> 5) + q;
+              const float v = K[(size_t)k * 32 * 32 + (size_t)col * 32 + row];
+              const __bf16 h = (__bf16)v;
+              const __bf16 lo = (__bf16)(v - (float)h);
+              Kb[((((size_t)k * 2 + ks) * 2 + 0) * 64 + lane) * 8 + q] = h;
+              Kb[((((size_t)k * 2 + ks) * 2 + 1) * 64 + lane) * 8 + q] = lo;
+            }
+      if ((e = hipMalloc(&g->d_K, sizeof(__bf16) * Kb.size())) != hipSuccess) fail(e, "hipMalloc");
+      else if ((e = hipMemcpyAsync(g->d_K, Kb.data(), sizeof(__bf16) * Kb.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+        fail(e, "copy");
+      const size_t nb2 = sizeof(__bf16) * slots * nrhs * r;
+      if (rc == BBFMM_OK && (e = hipMalloc(&g->d_Mg0, nb2)) != hipSuccess) fail(e, "hipMalloc");
+      if (rc == BBFMM_OK && (e = hipMalloc(&g->d_Mg1, nb2)) != hipSuccess) fail(e, "hipMalloc");
+      if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_Mg0, 0, nb2, s)) != hipSuccess) fail(e, "memset");
+      if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_Mg1, 0, nb2, s)) != hipSuccess) fail(e, "memset");
+    } else {
+      // FP32: [k][lane][16]: lane l holds row l%32, k = 16*(l/32) + ks
+      std::vector<float> Kf((size_t)kNumOps * 64 * 16);
+      for (int k = 0; k < kNumOps; k++)
+        for (int lane = 0; lane < 64; lane++)
+          for (int ks = 0; ks < 16; ks++) {
+            const int row = lane & 31, col = 16 * (lane >> 5) + ks;
+            Kf[((size_t)k * 64 + lane) * 16 + ks] = K[(size_t)k * 32 * 32 + (size_t)col * 32 + row];
+          }
+      if ((e = hipMalloc(&g->d_K, sizeof(float) * Kf.size())) != hipSuccess) fail(e, "hipMalloc");
+      else if ((e = hipMemcpyAsync(g->d_K, Kf.data(), sizeof(float) * Kf.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+        fail(e, "copy");
+      const size_t nb2 = sizeof(float) * slots * nrhs * r;
+      if (rc == BBFMM_OK && (e = hipMalloc(&g->d_Mg0, nb2)) != hipSuccess) fail(e, "hipMalloc");
+      if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_Mg0, 0, nb2, s)) != hipSuccess) fail(e, "memset");
+    }
+#endif
+  }
+  if (rc == BBFMM_OK && (e = hipStreamSynchronize(s)) != hipSuccess) fail(e, "sync");
+  if (rc != BBFMM_OK) {
+    grid_m2l_destroy(g);
+    return rc;
+  }
+  a.tab = g->d_tab;
+  a.Ka = g->d_K;
+  a.Mg0 = g->d_Mg0;
+  a.Mg1 = g->d_Mg1;
+  *out = g;
+  return BBFMM_OK;
+}
+
+void grid_m2l_destroy(GridM2L *g)
+{
+  if (!g) return;
+  if (g->d_tab) (void)hipFree(g->d_tab);
+  if (g->d_K) (void)hipFree(g->d_K);
+  if (g->d_Mg0) (void)hipFree(g->d_Mg0);
+  if (g->d_Mg1) (void)hipFree(g->d_Mg1);
+  delete g;
+}
 
 int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s)
 {
-  (void)g; (void)M; (void)L; (void)ncell; (void)s;
-  return set_error(BBFMM_E_ARG, "dense-grid M2L is not built yet");
+  GridArgs a = g->args;
+  if (ncell != a.ncell) return set_error(BBFMM_E_ARG, "dense-grid M2L: ncell %d != %d", ncell, a.ncell);
+  a.L = L;
+  const long long scat = (long long)(a.levsta[a.maxlev + 1] - a.levsta[a.minlev]) * a.nrhs * (a.r / 8);
+  const int sb = (int)((scat + 255) / 256);
+  switch (g->form) {
+  case BBFMM_M2L_PLAIN:
+    hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_PLAIN>, dim3(sb), dim3(256), 0, s, a, M);
+    BB_HIP(hipGetLastError());
+    if (a.r == 32)
+      hipLaunchKernelGGL(k_m2l_grid_plain<32>, dim3(g->nblocks), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_m2l_grid_plain<256>, dim3(g->nblocks), dim3(256), 0, s, a);
+    BB_HIP(hipGetLastError());
+    return BBFMM_OK;
+#ifdef SINGLE
+  case BBFMM_M2L_BF16X2:
+    hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_BF16X2>, dim3(sb), dim3(256), 0, s, a, M);
+    BB_HIP(hipGetLastError());
+    if (a.wave_cols == 64)
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 1>), dim3(g->nblocks), dim3(256), 0, s, a);
+    BB_HIP(hipGetLastError());
+    return BBFMM_OK;
+  case BBFMM_M2L_FP32MFMA:
+    hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_FP32MFMA>, dim3(sb), dim3(256), 0, s, a, M);
+    BB_HIP(hipGetLastError());
+    if (a.wave_cols == 64)
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP32MFMA, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP32MFMA, 1>), dim3(g->nblocks), dim3(256), 0, s, a);
+    BB_HIP(hipGetLastError());
+    return BBFMM_OK;
+#endif
+  default: return set_error(BBFMM_E_ARG, "dense-grid M2L: unknown form %d", g->form);
+  }
 }
 
 } // namespace bb

@@ -155,7 +155,8 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
 /* Plan options (bbfmm_amd_plan_set) */
 #define BBFMM_OPT_SCHEME 1    /* M2L_SCHEME_* */
 #define BBFMM_OPT_FORM 2      /* BBFMM_M2L_* */
-#define BBFMM_OPT_TIMING 3    /* 1: record per-stage HIP events on every solve */
+#define BBFMM_OPT_TIMING 3    /* k > 0: record per-stage HIP events on every solve into a
+                                 ring of k event sets; 0: off (resets the averages) */
 
 /* Create a plan for N particles at host positions pos (N real3, inside the
    root cell [-L/2,L/2]^3), Chebyshev order n, level modifier lmod (the -d
@@ -191,10 +192,11 @@ int bbfmm_amd_m2l(bbfmm_amd_plan *plan, const real *M_dev, real *L_dev);
    2 F after post-M2L (ncell*n^3), 3 far-field q.*phi (N, original order). */
 int bbfmm_amd_fetch(bbfmm_amd_plan *plan, int which, real *dst);
 
-/* Per-stage kernel times (ms) of the last timed solve: 0 gather, 1 P2M,
+/* Per-stage times (ms, HIP events on the plan's stream) averaged over the
+   last min(solves since timing was set, k) timed solves: 0 gather, 1 P2M,
    2 M2M, 3 M2L, 4 post-M2L, 5 L2L, 6 L2P, 7 P2P, 8 scatter, 9 total.
-   Returns the number of entries written (<= nout). */
-int bbfmm_amd_stage_times(const bbfmm_amd_plan *plan, double *out, int nout);
+   Synchronises the stream.  Returns the number of entries written. */
+int bbfmm_amd_stage_times(bbfmm_amd_plan *plan, double *out, int nout);
 
 /* Host-side setup helpers (reference layouts) */
 int bbfmm_amd_maxlev(int N, int lmod);                                   /* bbfmm.c:151-160 */

@@ -56,7 +56,7 @@ def test_solve_against_oracle(prec, N, scheme, ops4, cuda):
     dt = _dt(prec)
     tree = O.Tree(pos, 0, prec)
     ref = O.fmm(tree, q.astype(dt), 4, U.astype(dt), K.astype(dt), scheme="sibling")
-    plan = B.Plan(pos, 4, U, K, prec=prec).set(scheme=scheme, timing=True)
+    plan = B.Plan(pos, 4, U, K, prec=prec).set(scheme=scheme, timing=1)
     phi = plan.solve_host(q)
     e_phi = O.rel_l2(ref["phi"], phi)
     e_M = O.rel_l2(ref["M"][O.levsta(tree.maxlev)[2] * 32:], plan.fetch("M")[O.levsta(tree.maxlev)[2] * 32:])
@@ -68,3 +68,37 @@ def test_solve_against_oracle(prec, N, scheme, ops4, cuda):
     assert e_F < TOL[prec]
     assert e_far < TOL[prec]
     assert e_phi < TOL[prec]
+
+
+# (scheme, form, tolerance at the L level vs the fp64 oracle on identical inputs)
+GRID_CASES = [
+    ("s", 2, 0, 1e-5), ("d", 2, 0, 1e-12), ("s", 3, 0, 1e-5), ("d", 3, 0, 1e-12),
+    ("s", 3, 2, 1e-5),   # FP32 MFMA
+    ("s", 3, 1, 3e-5),   # split BF16 (hi+lo, 3 products), fp32 accumulate; measured ~4e-6
+]
+
+
+@pytest.mark.parametrize("prec,scheme,form,tol", GRID_CASES)
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_grid_m2l_against_oracle(prec, scheme, form, tol, nrhs, ops4, cuda):
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    N = 65536
+    pos, _q = O.uniform_inputs(N, seed=0)
+    plan = B.Plan(pos, 4, U, K, prec=prec, nrhs=nrhs).set(scheme=scheme, form=form)
+    tree = O.Tree(pos, 0, "d")
+    rng = np.random.default_rng(11)
+    M = rng.standard_normal((nrhs, tree.ncell * 32)).astype(_dt(prec))
+    Md = torch.from_numpy(M).to(cuda)
+    Ld = torch.zeros_like(Md)
+    plan.m2l(Md, Ld)
+    torch.cuda.synchronize()
+    Lg = Ld.cpu().numpy()
+    s = O.levsta(4)[2] * 32
+    Kd = K.astype(_dt(prec)).astype(np.float64)
+    for j in range(nrhs):
+        ref = O.m2l(tree, Kd, M[j].astype(np.float64), 32, "basic")
+        err = O.rel_l2(ref[s:], Lg[j, s:])
+        print(f"grid m2l prec={prec} scheme={scheme} form={form} rhs={j}: rel_l2={err:.3e}")
+        assert err < tol
