@@ -16,14 +16,14 @@ import numpy as np
 from ._build import lib_path
 
 __all__ = [
-    "load", "Plan", "operators", "weights", "maxlev", "m2l_drv", "BBFMMError",
+    "load", "Plan", "operators", "direct", "weights", "maxlev", "m2l_drv", "BBFMMError",
     "SCHEME_BA", "SCHEME_SI", "SCHEME_CL", "SCHEME_IJ", "FORM_PLAIN", "FORM_BF16X2", "FORM_FP32MFMA",
 ]
 
 SCHEME_BA, SCHEME_SI, SCHEME_CL, SCHEME_IJ = 0, 1, 2, 3
 FORM_PLAIN, FORM_BF16X2, FORM_FP32MFMA = 0, 1, 2
 OPT_SCHEME, OPT_FORM, OPT_TIMING = 1, 2, 3
-STAGES = ["gather", "p2m", "m2m", "m2l", "post_m2l", "l2l", "l2p", "p2p", "scatter", "total"]
+STAGES = ["gather", "p2m", "m2m", "m2l_prep", "m2l", "post_m2l", "l2l", "l2p", "p2p", "scatter", "total"]
 
 _libs: dict = {}
 
@@ -60,6 +60,7 @@ def load(prec: str = "s"):
     L.bbfmm_amd_weights.restype = None
     L.bbfmm_amd_operators.argtypes = [i, i, vp, vp, vp]
     L.bbfmm_amd_last_error_string.restype = C.c_char_p
+    L.bbfmm_amd_direct.argtypes = [i, vp, vp, i, vp, vp]
     L.bbfmm_amd_set_default_scheme.argtypes = [i]
     L.m2l_drv.argtypes = [i, vp, vp, vp, i, i, R, vp, vp, i, vp, vp, i, vp, i]
     for name in ("anotherFMMCompute",):
@@ -118,6 +119,22 @@ def operators(n: int, r: int | None = None, prec: str = "s"):
     L = load(prec)
     _check(L, L.bbfmm_amd_operators(n, r, _p(U), _p(K), _p(sig)))
     return U, K, sig
+
+
+def direct(pos, q, targets=None) -> np.ndarray:
+    """FP64 direct sum on the GPU: q_t * potential_t for the target indices."""
+    L = load("d")
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    N = pos.shape[0]
+    if targets is None:
+        nt, tp, keep = N, None, None
+    else:
+        keep = np.ascontiguousarray(targets, dtype=np.int32)
+        nt, tp = keep.size, _p(keep)
+    out = np.zeros(nt)
+    _check(L, L.bbfmm_amd_direct(N, _p(pos), _p(q), nt, tp, _p(out)))
+    return out
 
 
 def _torch():

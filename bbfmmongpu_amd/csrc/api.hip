@@ -54,7 +54,7 @@ extern "C" int bbfmm_amd_set_default_scheme(int scheme)
 static bool g_verbose = getenv("BBFMM_AMD_VERBOSE") != nullptr;
 
 // ---------------------------------------------------------------- plan
-enum { ST_GATHER, ST_P2M, ST_M2M, ST_M2L, ST_POST, ST_L2L, ST_L2P, ST_P2P, ST_SCATTER, ST_TOTAL };
+enum { ST_GATHER, ST_P2M, ST_M2M, ST_M2L_PREP, ST_M2L, ST_POST, ST_L2L, ST_L2P, ST_P2P, ST_SCATTER, ST_TOTAL };
 
 struct bbfmm_amd_plan {
   int N = 0, n = 0, n3 = 0, r = 0, nrhs = 1, maxlev = 0, ncell = 0, nleaf = 0, leaf0 = 0;
@@ -357,7 +357,7 @@ static PassArgs pass_args(const bbfmm_amd_plan *p)
   return a;
 }
 
-static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L)
+static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = nullptr)
 {
   M2LArgs m;
   m.r = p->r;
@@ -374,8 +374,12 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L)
   m.M = M;
   m.L = L;
   switch (p->scheme) {
-  case M2L_SCHEME_BA: return launch_m2l_basic(m, p->stream);
-  case M2L_SCHEME_SI: return launch_m2l_sibling(m, p->stream);
+  case M2L_SCHEME_BA:
+    if (mid) BB_HIP(hipEventRecord(mid, p->stream));
+    return launch_m2l_basic(m, p->stream);
+  case M2L_SCHEME_SI:
+    if (mid) BB_HIP(hipEventRecord(mid, p->stream));
+    return launch_m2l_sibling(m, p->stream);
   case M2L_SCHEME_CL:
   case M2L_SCHEME_IJ: {
     const int form = p->scheme == M2L_SCHEME_CL ? BBFMM_M2L_PLAIN : p->form;
@@ -385,7 +389,7 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L)
       BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, p->nrhs, form, p->h_K.data(), p->stream));
       p->grid_form = form;
     }
-    return grid_m2l_run(p->grid, M, L, p->ncell, p->stream);
+    return grid_m2l_run(p->grid, M, L, p->ncell, p->stream, mid);
   }
   default: return set_error(BBFMM_E_ARG, "unknown M2L scheme %d", p->scheme);
   }
@@ -401,8 +405,8 @@ static int solve_sorted(bbfmm_amd_plan *p)
   BB_TRY(launch_p2m(a, s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2M), s));
   BB_TRY(launch_m2m(a, p->h_levsta.data(), s, p->nrhs));
-  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2L), s));
-  BB_TRY(run_m2l(p, p->d_M, p->d_L));
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2L_PREP), s));
+  BB_TRY(run_m2l(p, p->d_M, p->d_L, tm ? stage_event(p, ST_M2L) : nullptr));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_POST), s));
   BB_TRY(launch_postm2l(a, p->h_levsta.data(), s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2L), s));
@@ -938,7 +942,7 @@ extern "C" int m2l_drv(int r, real *L, real *K, real *M, int minlev, int maxlev,
     else if (scheme == M2L_SCHEME_SI) rc = launch_m2l_sibling(m, nullptr);
     else {
       rc = grid_m2l_create(&p.grid, r, maxlev, 1, BBFMM_M2L_PLAIN, p.h_K.data(), nullptr);
-      if (rc == BBFMM_OK) rc = grid_m2l_run(p.grid, p.d_M, dL, ncell, nullptr);
+      if (rc == BBFMM_OK) rc = grid_m2l_run(p.grid, p.d_M, dL, ncell, nullptr, nullptr);
     }
     if (rc != BBFMM_OK) break;
     (void)hipEventRecord(e1, nullptr);

@@ -54,7 +54,8 @@ struct GridM2L;  // defined in m2l_grid.hip
 int grid_m2l_create(GridM2L **g, int r, int maxlev, int nrhs, int form, const real *K_colmajor_host,
                     hipStream_t s);
 void grid_m2l_destroy(GridM2L *g);
-int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s);
+// records `mid` (if not null) between the scatter into the grids and the M2L kernel
+int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, hipEvent_t mid);
 
 // ---- P2P ----
 struct P2PArgs {

@@ -457,7 +457,7 @@ void grid_m2l_destroy(GridM2L *g)
   delete g;
 }
 
-int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s)
+int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, hipEvent_t mid)
 {
   GridArgs a = g->args;
   if (ncell != a.ncell) return set_error(BBFMM_E_ARG, "dense-grid M2L: ncell %d != %d", ncell, a.ncell);
@@ -468,6 +468,7 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s)
   case BBFMM_M2L_PLAIN:
     hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_PLAIN>, dim3(sb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
+    if (mid) BB_HIP(hipEventRecord(mid, s));
     if (a.r == 32)
       hipLaunchKernelGGL(k_m2l_grid_plain<32>, dim3(g->nblocks), dim3(256), 0, s, a);
     else
@@ -478,6 +479,7 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s)
   case BBFMM_M2L_BF16X2:
     hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_BF16X2>, dim3(sb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
+    if (mid) BB_HIP(hipEventRecord(mid, s));
     if (a.wave_cols == 64)
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
     else
@@ -487,6 +489,7 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s)
   case BBFMM_M2L_FP32MFMA:
     hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_FP32MFMA>, dim3(sb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
+    if (mid) BB_HIP(hipEventRecord(mid, s));
     if (a.wave_cols == 64)
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP32MFMA, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
     else

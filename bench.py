@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X bbFMM downward pass (one JSON line on stdout).
+
+Workload (the BASELINE precision config): N = 1,048,576 particles i.i.d.
+uniform in [0,1)^3 (shifted into the root cell [-1/2,1/2)^3), charges i.i.d.
+uniform [-1,1], Laplace kernel 1/r, Chebyshev order n = 4, SVD rank r = 32,
+depth 5 by the reference's level rule.  A step is one full solve (P2M, M2M,
+M2L, post-M2L, L2L, L2P, P2P) on data already resident in HBM:
+
+  chosen form : 16 charge vectors per solve on one geometry, M2L on MFMA with
+                split-BF16 operands (hi + lo, 3 products) and fp32 accumulate;
+  plain form  : the fp32 drop-in (one vector, dense-grid M2L in fp32 FMAs).
+
+value = per-vector solves per second of the chosen form (whole job).  Errors
+are relative L2 of q*phi against the FP64 direct sum (GPU) on 4,096 sampled
+targets.  cpu_baseline times the reference's own CPU path (CPU8A build,
+oracle/_ref) on the host on the same input.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+METRIC = "FMM s/solve + M2L Gflop/s at N=1M,n=4: chosen vs plain, rel L2 err ≤1e-4"
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_FP32_TFLOPS = 157.3
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def rel_l2(ref, x):
+    ref = np.asarray(ref, np.float64)
+    x = np.asarray(x, np.float64)
+    return float(np.linalg.norm(ref - x) / np.linalg.norm(ref))
+
+
+def timed_solves(plan, q, phi, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        plan.solve(q, phi)
+    torch.cuda.synchronize()
+    plan.set(timing=steps)  # per-stage HIP events on the plan's stream, read after the region
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(steps):
+        plan.solve(q, phi)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / steps
+    stages = plan.stage_times()
+    plan.set(timing=0)
+    return ms, stages
+
+
+def cpu_baseline(pos, q, U, K, threads):
+    """Reference CPU path (oracle/_ref, CPU8A build) on the same input."""
+    import oracle as O
+    if O.ref_available("s", fast=True):
+        t0 = time.time()
+        out = O.ref_run(pos, q, 4, U, K, prec="s", dump=False, fast=True, threads=threads)
+        wall = time.time() - t0
+        info = out["info"]
+        m = re.search(r"calc_performance: kernel = ([0-9.]+)", info["stderr"])
+        p = re.search(r"num_pairwise_interactions_per_sec = ([0-9.]+)", info["stderr"])
+        return {
+            "kind": "reference",
+            "s_per_solve": info["t_solve"],
+            "m2l_gflops": float(m.group(1)) if m else None,
+            "p2p_ginteractions_per_s": float(p.group(1)) if p else None,
+            "t_upward": info["t_upward"], "t_interaction": info["t_interaction"],
+            "t_downwardx": info["t_downwardx"], "t_nearfield": info["t_nearfield"],
+            "wall_s": wall, "phi": out["phi"],
+        }
+    # restated port (oracle C build) when the reference build is absent
+    T = O.Tree(pos, 0, "s")
+    t0 = time.time()
+    res = O.fmm(T, q.astype(np.float32), 4, U, K, scheme="sibling")
+    t = time.time() - t0
+    return {"kind": "port", "s_per_solve": t, "m2l_gflops": None, "p2p_ginteractions_per_s": None,
+            "wall_s": t, "phi": res["phi"]}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--N", type=int, default=1 << 20)
+    ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--targets", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note only "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import bbfmmongpu_amd as B
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    N, nrhs = args.N, args.nrhs
+    rng = np.random.default_rng(0)
+    pos = rng.random((N, 3)) - 0.5
+    Q = rng.uniform(-1.0, 1.0, size=(nrhs, N))
+    log(f"N={N} nrhs={nrhs}: operator precompute (FP64)")
+    t0 = time.time()
+    U, K, sigma = B.operators(4, 32, prec="s")
+    t_ops = time.time() - t0
+
+    t0 = time.time()
+    plain = B.Plan(pos, 4, U, K, nrhs=1, prec="s", stream=stream.cuda_stream).set(scheme=B.SCHEME_IJ,
+                                                                                     form=B.FORM_PLAIN)
+    chosen = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", stream=stream.cuda_stream).set(scheme=B.SCHEME_IJ,
+                                                                                       form=B.FORM_BF16X2)
+    t_setup = time.time() - t0
+    log(f"maxlev={chosen.maxlev} ncell={chosen.ncell} interactions={chosen.m2l_interactions:.0f} "
+        f"pairs={chosen.p2p_pairs:.4g} setup={t_setup:.2f}s")
+
+    q1 = torch.from_numpy(Q[0].astype(np.float32)).to(dev)
+    phi1 = torch.empty_like(q1)
+    qm = torch.from_numpy(Q.astype(np.float32)).to(dev).contiguous()
+    phim = torch.empty_like(qm)
+
+    log("plain form: timing")
+    ms_plain, st_plain = timed_solves(plain, q1, phi1, args.steps, args.warmup, stream)
+    log(f"plain {ms_plain:.3f} ms/solve stages {st_plain}")
+    log("chosen form: timing")
+    ms_chosen, st_chosen = timed_solves(chosen, qm, phim, args.steps, args.warmup, stream)
+    log(f"chosen {ms_chosen:.3f} ms/step ({nrhs} vectors) stages {st_chosen}")
+
+    # accuracy against the FP64 direct sum on sampled targets
+    tg = np.sort(np.random.default_rng(1).choice(N, size=min(args.targets, N), replace=False)).astype(np.int32)
+    phi_plain = phi1.cpu().numpy()
+    phi_chosen = phim.cpu().numpy()
+    errs_chosen, errs_pot = [], []
+    direct_ref = {}
+    nerr = min(nrhs, 4)
+    for j in range(nerr):
+        d = B.direct(pos, Q[j], tg)
+        direct_ref[j] = d
+        errs_chosen.append(rel_l2(d, phi_chosen[j, tg]))
+        errs_pot.append(rel_l2(d / Q[j, tg], phi_chosen[j, tg] / Q[j, tg]))
+    err_plain = rel_l2(direct_ref[0], phi_plain[tg])
+    # the algorithm's own truncation error: the fp64 plain form
+    plan64 = B.Plan(pos, 4, U.astype(np.float64), K.astype(np.float64), nrhs=1, prec="d").set(
+        scheme=B.SCHEME_IJ, form=B.FORM_PLAIN)
+    phi64 = plan64.solve_host(Q[0])
+    err_fp64 = rel_l2(direct_ref[0], phi64[tg])
+    err_chosen_vs_fp64 = rel_l2(phi64, phi_chosen[0])
+    plan64.close()
+    log(f"errors vs FP64 direct: plain {err_plain:.3e} chosen {errs_chosen} fp64-plain {err_fp64:.3e}")
+
+    inter = chosen.m2l_interactions
+    pairs = chosen.p2p_pairs
+    r = 32
+    flop_m2l = r * (2 * r - 1) * inter  # per vector (m2l_aux.c:967-979)
+    m2l_gflops_chosen = flop_m2l * nrhs / (st_chosen["m2l"] * 1e-3) / 1e9
+    m2l_gflops_plain = flop_m2l / (st_plain["m2l"] * 1e-3) / 1e9
+    p2p_gips_chosen = pairs * nrhs / (st_chosen["p2p"] * 1e-3) / 1e9
+    p2p_gips_plain = pairs / (st_plain["p2p"] * 1e-3) / 1e9
+    s_per_solve_chosen = ms_chosen * 1e-3 / nrhs
+    s_per_solve_plain = ms_plain * 1e-3
+
+    # roofline of the dominant kernel of the chosen form
+    p2p_flop = pairs * (18 + 2 * nrhs)  # geometry (20-flop convention) shared by the vectors
+    roof_m2l = {"kernel": "k_m2l_mfma<BF16X2>", "bound": "mfma", "unit": "TFLOP/s",
+                "achieved": flop_m2l * nrhs / (st_chosen["m2l"] * 1e-3) / 1e12, "peak": PEAK_BF16_TFLOPS,
+                "launch_ms": st_chosen["m2l"], "algorithmic_flop_per_launch": flop_m2l * nrhs,
+                "issued_mfma_flop_per_launch": None, "traffic": None}
+    roof_m2l["frac"] = roof_m2l["achieved"] / roof_m2l["peak"]
+    roof_p2p = {"kernel": "k_p2p<16>", "bound": "valu", "unit": "TFLOP/s",
+                "achieved": p2p_flop / (st_chosen["p2p"] * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS,
+                "launch_ms": st_chosen["p2p"], "algorithmic_flop_per_launch": p2p_flop, "traffic": None}
+    roof_p2p["frac"] = roof_p2p["achieved"] / roof_p2p["peak"]
+    dominant = roof_m2l if st_chosen["m2l"] >= st_chosen["p2p"] else roof_p2p
+
+    result = {
+        "metric": METRIC,
+        "value": nrhs / (ms_chosen * 1e-3),
+        "unit": "solve/s (per charge vector)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_chosen,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"bbFMM Laplace 1/r, N={N} uniform, n=4, r=32, depth {chosen.maxlev}, "
+                               f"{nrhs} charge vectors per solve",
+                   "N": N, "n": 4, "r": 32, "maxlev": chosen.maxlev, "nrhs": nrhs,
+                   "m2l_scheme": "dense grid (IJ)", "m2l_form": "split-BF16 MFMA (hi+lo, 3 products), fp32 accumulate",
+                   "plain_form": "fp32 drop-in, dense-grid M2L in fp32 FMAs, one vector"},
+        "s_per_solve": s_per_solve_chosen,
+        "m2l_gflops": m2l_gflops_chosen,
+        "p2p_ginteractions_per_s": p2p_gips_chosen,
+        "speedup_per_vector_vs_plain": s_per_solve_plain / s_per_solve_chosen,
+        "rel_l2_err": {"chosen_q_phi": max(errs_chosen), "chosen_potential": max(errs_pot),
+                       "plain_q_phi": err_plain, "fp64_plain_q_phi": err_fp64,
+                       "chosen_vs_fp64_plain": err_chosen_vs_fp64, "targets": int(tg.size),
+                       "vectors_checked": nerr},
+        "plain": {"s_per_solve": s_per_solve_plain, "m2l_gflops": m2l_gflops_plain,
+                  "p2p_ginteractions_per_s": p2p_gips_plain, "stage_ms": st_plain},
+        "chosen_stage_ms": st_chosen,
+        "setup_s": {"operators": t_ops, "plans": t_setup},
+        "roofline": dominant,
+        "roofline_m2l": roof_m2l,
+        "roofline_p2p": roof_p2p,
+    }
+
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: reference CPU path, {threads} threads, full N={N} solve of one vector")
+        cb = cpu_baseline(pos, Q[0], U, K, threads)
+        phi_cpu = cb.pop("phi")
+        cb_err = rel_l2(direct_ref[0], phi_cpu[tg])
+        result["cpu_baseline"] = {
+            "value": 1.0 / cb["s_per_solve"], "unit": "solve/s (per charge vector)", "cores": threads,
+            "kind": cb["kind"],
+            "sample": f"one full solve of vector 0 at N={N} (upward + interaction + downward + near field; "
+                      f"tree build excluded), OMP_NUM_THREADS={threads}",
+            "s_per_solve": cb["s_per_solve"], "m2l_gflops": cb["m2l_gflops"],
+            "p2p_ginteractions_per_s": cb["p2p_ginteractions_per_s"], "rel_l2_err_q_phi": cb_err,
+            "detail": {k: v for k, v in cb.items() if k.startswith("t_") or k == "wall_s"},
+        }
+    plain.close()
+    chosen.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -194,9 +194,15 @@ int bbfmm_amd_fetch(bbfmm_amd_plan *plan, int which, real *dst);
 
 /* Per-stage times (ms, HIP events on the plan's stream) averaged over the
    last min(solves since timing was set, k) timed solves: 0 gather, 1 P2M,
-   2 M2M, 3 M2L, 4 post-M2L, 5 L2L, 6 L2P, 7 P2P, 8 scatter, 9 total.
+   2 M2M, 3 M2L preparation (dense-grid scatter), 4 M2L kernel, 5 post-M2L,
+   6 L2L, 7 L2P, 8 P2P, 9 scatter, 10 total.
    Synchronises the stream.  Returns the number of entries written. */
 int bbfmm_amd_stage_times(bbfmm_amd_plan *plan, double *out, int nout);
+
+/* FP64 direct sum on the GPU (DirectCalc3D, bbfmm.c:471-504 with start = 0):
+   out[t] = q_i * sum_{j: x_j != x_i} q_j / |x_i - x_j| for i = targets[t]
+   (all particles when targets is NULL, nt = N).  pos is N x 3 doubles. */
+int bbfmm_amd_direct(int N, const double *pos, const double *q, int nt, const int *targets, double *out);
 
 /* Host-side setup helpers (reference layouts) */
 int bbfmm_amd_maxlev(int N, int lmod);                                   /* bbfmm.c:151-160 */
