@@ -65,11 +65,13 @@ int main(int argc, char **argv)
   if (l < MINLEV) l = MINLEV;
   const int ncell = (POW8(l + 1) - 1) / 7;
 
-  /* Every block of the size of a per-cell int array or larger is taken from
-     fresh zero pages, so the reference's reads of uninitialised sta/end
-     entries and its accumulation into un-zeroed proxysval see zeros, as on
-     the large runs it was written for (anotherUpwardPass.c:50,388). */
-  mallopt(M_MMAP_THRESHOLD, ncell * (int)sizeof(int) - 1 > 0 ? ncell * (int)sizeof(int) - 1 : 1);
+  /* The reference reads uninitialised sta/end entries and accumulates into
+     un-zeroed proxysval (anotherUpwardPass.c:50,388), so its results depend
+     on what malloc returns.  glibc's M_PERTURB option fills every block that
+     malloc returns with the complement of the given byte: with 0xff every
+     block starts as zeros, as on the fresh pages of the runs it was written
+     for, whatever the heap held before. */
+  mallopt(M_PERTURB, 0xff);
 
   real3 *field = read_file(dir, "pos.bin", sizeof(real3) * N);
   real *q = read_file(dir, "q.bin", sizeof(real) * N);

@@ -1,0 +1,382 @@
+"""Pins the oracle (CPU restatement of the reference) before it is trusted.
+
+Every oracle function is checked by something that does not restate its own
+formula: the reference's embedded tables (known answers), closed-form counts,
+brute force on small trees, mathematical invariants of the Chebyshev
+interpolation and of the operators, an independent numpy SVD, the compiled
+reference CPU path (oracle/_ref, when built here) and golden fixtures written
+from that reference (tests/golden/).
+"""
+import hashlib
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ------------------------------------------------------------------ tables
+def test_sibling_tables_known_answers():
+    # Kindex0 / Kindex1 as tabulated in the reference
+    # (m2l_kern_sibling_blocking.cu:76-107, anotherFMMInteraction.c:7836-7869)
+    k0_ref = [57, 155, 253, 71, 169, 267, 85, 183, 281, 59, 157, 255, 73, 171, 269, 87, 185, 283,
+              61, 159, 257, 75, 173, 271, 89, 187, 285]
+    k1_ref = [57, 55, 43, 41, -41, -43, -55, -57, 56, 50, 8, 48, 6, 42, -6, -8, -42, -48, -50, -56,
+              49, 7, 1, -1, -7, -49, 0]
+    k0, k1 = O.sibling_tables()
+    assert k0.tolist() == k0_ref
+    assert k1.tolist() == k1_ref
+
+
+def test_ktable_lexicographic_and_antisymmetric():
+    T = O.Tree(np.zeros((1, 3)), maxlev=2, prec="d")
+    kt = T.Ktable
+    offs = list(itertools.product(range(-3, 4), repeat=3))
+    valid = [v for v in offs if max(abs(c) for c in v) > 1]
+    assert (kt >= 0).sum() == 316
+    for idx, v in enumerate(offs):
+        if max(abs(c) for c in v) > 1:
+            assert kt[idx] == valid.index(v)
+            neg = offs.index(tuple(-c for c in v))
+            assert kt[neg] == 315 - kt[idx]
+        else:
+            assert kt[idx] == -1
+
+
+@pytest.mark.parametrize("N,lmod,expect", [(65536, 0, 4), (1 << 20, 0, 5), (1 << 22, 0, 6), (1 << 18, 0, 4),
+                                           (4096, 0, 2), (4096, 1, 3), (100, 0, 2), (65536, -1, 3)])
+def test_level_rule(N, lmod, expect):
+    # bbfmm.c:151-160: l = lmod + floor(ln N)/ln 8 (truncated), minus 1, at least 2
+    assert O.maxlev_rule(N, lmod) == expect
+    v = int(lmod + int(np.log(N)) / np.log(8.0))
+    v = v - 1 if v > 0 else v
+    assert max(v, 2) == expect
+
+
+# ------------------------------------------------------------------ tree and lists
+def _grid_coords(T, A, level):
+    ls = O.levsta(T.maxlev)[level]
+    m = A - ls
+    x = y = z = 0
+    for b in range(level):
+        s = (m >> (3 * b)) & 7
+        x |= ((s >> 2) & 1) << b
+        y |= ((s >> 1) & 1) << b
+        z |= (s & 1) << b
+    return np.array([x, y, z])
+
+
+@pytest.fixture(scope="module")
+def tree4():
+    pos, _ = O.uniform_inputs(20000, seed=3)
+    return O.Tree(pos, 0, "d", maxlev=4)
+
+
+def test_centres_closed_form(tree4):
+    T = tree4
+    for level in range(1, T.maxlev + 1):
+        ls = O.levsta(T.maxlev)[level]
+        h = 1.0 / 2 ** level
+        for A in range(ls, ls + 8 ** level, 37):
+            g = _grid_coords(T, A, level)
+            assert np.array_equal(T.center[A], -0.5 + (g + 0.5) * h)
+
+
+def test_lists_brute_force(tree4):
+    """I(F) = same-level cells whose parent neighbours F's parent and that are not
+    neighbours of F; N(F) = cells within one cell; list order = uncle x child."""
+    T = tree4
+    for level in (2, 3, 4):
+        ls = O.levsta(T.maxlev)[level]
+        for A in range(ls, ls + 8 ** level, 53 if level == 4 else 7):
+            gf = _grid_coords(T, A, level)
+            inter = set(T.interaction[A * 192: A * 192 + T.iinter[A]].tolist())
+            neigh = set(T.neighbors[A * 64: A * 64 + T.ineigh[A]].tolist())
+            exp_i, exp_n = set(), set()
+            for C in range(ls, ls + 8 ** level):
+                gc = _grid_coords(T, C, level)
+                if np.max(np.abs(gc // 2 - gf // 2)) > 1:
+                    continue
+                (exp_n if np.max(np.abs(gc - gf)) <= 1 else exp_i).add(C)
+            assert inter == exp_i and neigh == exp_n
+
+
+def test_list_counts_closed_form(tree4):
+    # per-level sums of |I| and |N| of a complete octree
+    T = tree4
+    ls = O.levsta(T.maxlev)
+    expect_i = {2: 3096, 3: 53352, 4: 584136}
+    expect_n = {2: 1000, 3: 10648, 4: 97336}
+    for l in (2, 3, 4):
+        assert int(T.iinter[ls[l]:ls[l] + 8 ** l].sum()) == expect_i[l]
+        assert int(T.ineigh[ls[l]:ls[l] + 8 ** l].sum()) == expect_n[l]
+    assert T.iinter.max() == 189 and T.ineigh.max() == 27
+
+
+def test_list_symmetry(tree4):
+    T = tree4
+    for A in range(O.levsta(4)[3], O.levsta(4)[4]):
+        for S in T.interaction[A * 192: A * 192 + T.iinter[A]]:
+            assert A in T.interaction[S * 192: S * 192 + T.iinter[S]]
+
+
+def test_leaf_buckets_brute_force():
+    pos, _ = O.uniform_inputs(5000, seed=5)
+    pos[0] = [0.5, 0.5, 0.5]      # on the upper face: clamped into the last leaf
+    pos[1] = [-0.5, -0.5, -0.5]
+    T = O.Tree(pos, 0, "d", maxlev=3)
+    ls = O.levsta(3)[3]
+    ncpd = 8
+    ix = np.minimum(((pos + 0.5) * ncpd).astype(int), ncpd - 1)
+    for A in range(ls, ls + 512):
+        g = _grid_coords(T, A, 3)
+        members = np.where((ix == g).all(axis=1))[0]
+        got = T.list[T.sta[A]: T.end[A] + 1]
+        assert np.array_equal(got, members)  # ascending particle index
+
+
+def test_kindex_and_sourceclusters(tree4):
+    T = tree4
+    kt = T.Ktable
+    for A in range(O.levsta(4)[2], O.levsta(4)[4], 41):
+        level = 2 if A < O.levsta(4)[3] else (3 if A < O.levsta(4)[4] else 4)
+        gf = _grid_coords(T, A, level)
+        for t in range(T.iinter[A]):
+            S = T.interaction[A * 192 + t]
+            v = _grid_coords(T, S, level) - gf
+            assert T.Kindex[A * 192 + t] == kt[49 * (v[0] + 3) + 7 * (v[1] + 3) + (v[2] + 3)]
+    # source clusters: slot d of field cluster FC holds the first child of the
+    # neighbouring parent at offset (d%3-1, d/3%3-1, d/9-1), -1 if absent or FC
+    for FC in range(O.levsta(4)[1], O.levsta(4)[3], 5):
+        plev = 1 if FC < O.levsta(4)[2] else 2
+        F0 = FC - O.levsta(4)[1]
+        gp = _grid_coords(T, FC, plev)
+        for d in range(27):
+            off = np.array([d % 3 - 1, (d // 3) % 3 - 1, d // 9 - 1])
+            gs = gp + off
+            got = T.sourceclusters[F0 * 32 + d]
+            if (off == 0).all() or (gs < 0).any() or (gs >= 2 ** plev).any():
+                assert got == -1
+            else:
+                ls = O.levsta(4)[plev]
+                SC = [C for C in range(ls, ls + 8 ** plev) if (_grid_coords(T, C, plev) == gs).all()][0]
+                assert got == 8 * SC + 1
+
+
+# ------------------------------------------------------------------ weights and operators
+@pytest.mark.parametrize("n", [4, 8])
+def test_weights_closed_form(n):
+    Tkz, Kt, Kw, Cw = O.weights(n, "d")
+    x = np.cos(np.pi * (2 * np.arange(n) + 1) / (2 * n))
+    k = np.arange(n)
+    assert np.allclose(Tkz.reshape(n, n), np.cos(np.outer(np.arccos(x), k)), atol=1e-13)
+    w = 1 / np.sqrt(1 - x ** 2)
+    assert np.allclose(Kw, np.einsum("a,b,c->abc", w, w, w).ravel(), rtol=1e-13)
+
+    def S(xm, xx):  # 1/2 + sum_{k>=1} T_k(xm) T_k(xx)
+        th_m, th = np.arccos(xm), np.arccos(np.clip(xx, -1, 1))
+        return 0.5 + sum(np.cos(kk * th_m) * np.cos(kk * th) for kk in range(1, n))
+    for side, sgn in ((0, -1.0), (1, 1.0)):
+        blk = Cw[side * n * n:(side + 1) * n * n].reshape(n, n)
+        for p in range(n):
+            for c in range(n):
+                assert abs(blk[p, c] - S(x[p], 0.5 * (x[c] + sgn))) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def ops4():
+    return O.operators(4)
+
+
+def test_operators_invariants(ops4):
+    U, K, sig = ops4
+    n3, r = 64, 32
+    Um = U.reshape(r, n3).T
+    assert np.abs(Um.T @ Um - np.eye(r)).max() < 1e-12
+    K0 = O.kernel_matrix(4).reshape(316 * n3, n3).T  # n3 x 316 n3
+    s_np = np.linalg.svd(K0, compute_uv=False)
+    # singular values from the Gram matrix carry relative error ~ eps (s_0/s_i)^2:
+    # below 1e-7 for the r = 32 retained ones (s_31/s_0 ~ 5e-5)
+    assert np.allclose(sig[:r], s_np[:r], rtol=1e-7)
+    # leading singular subspace (complete clusters: first 30) agrees with numpy's
+    u_np = np.linalg.svd(K0, full_matrices=False)[0][:, :30]
+    P1, P2 = Um[:, :30] @ Um[:, :30].T, u_np @ u_np.T
+    assert np.abs(P1 - P2).max() < 1e-8  # Gram conditioning; a wrong subspace gives O(1)
+    # E_k = U^T K_k U and E_{-v} = E_v^T (symmetric kernel, k(-v) = 315 - k(v))
+    E = K.reshape(316, r, r)  # [k][col][row]
+    blocks = K0.T.reshape(316, n3, n3)  # [k][src][fld]
+    for k in (0, 17, 100, 200, 315):
+        Ek = E[k].T
+        assert np.allclose(Ek, Um.T @ blocks[k].T @ Um, atol=1e-12)
+        assert np.allclose(Ek, E[315 - k], atol=1e-12 * np.abs(Ek).max())
+
+
+def test_kernel_matrix_samples():
+    # one entry by hand: field node (0,0,0) of the unit cell at the origin,
+    # source node (1,2,3) of the cell at offset (-3,-3,-3) (operator 0)
+    n = 4
+    K0 = O.kernel_matrix(n).reshape(316, 64, 64)  # [k][src][fld]
+    x = np.cos(np.pi * (2 * np.arange(n) + 1) / (2 * n))
+    w = lambda a, b, c: 1 / np.sqrt((1 - x[a] ** 2) * (1 - x[b] ** 2) * (1 - x[c] ** 2))
+    f = 0.5 * x[[0, 0, 0]]
+    s = np.array([-3, -3, -3]) + 0.5 * x[[1, 2, 3]]
+    val = 1 / np.linalg.norm(s - f) / (w(0, 0, 0) * w(1, 2, 3))
+    assert abs(K0[0, 1 * 16 + 2 * 4 + 3, 0] - val) < 1e-15 * val * 10
+
+
+# ------------------------------------------------------------------ passes
+def test_direct_sum_brute_force():
+    rng = np.random.default_rng(2)
+    pos = rng.random((60, 3)) - 0.5
+    pos[7] = pos[3]  # coincident pair: excluded from each other's sum
+    q = rng.uniform(-1, 1, 60)
+    got = O.direct(pos, q)
+    for i in range(60):
+        s = sum(q[j] / np.linalg.norm(pos[i] - pos[j]) for j in range(60) if not np.array_equal(pos[i], pos[j]))
+        assert abs(got[i] - s) < 1e-12 * max(1, abs(s))
+    tg = np.array([5, 2, 40], np.int32)
+    assert np.array_equal(O.direct(pos, q, tg), got[tg])
+
+
+def test_nearfield_brute_force():
+    pos, q = O.uniform_inputs(3000, seed=4)
+    T = O.Tree(pos, 0, "d", maxlev=3)
+    phi = np.zeros(3000)
+    import ctypes as C
+    O.lib("d").orc_nearfield(C.c_int(3), C.c_int(3000), O._p(T.list), O._p(T.sta), O._p(T.end),
+                             O._p(T.ineigh), O._p(T.neighbors), O._p(T.pos), O._p(q), O._p(phi))
+    ls = O.levsta(3)[3]
+    leaf = np.zeros(3000, int)
+    for A in range(ls, ls + 512):
+        leaf[T.list[T.sta[A]:T.end[A] + 1]] = A
+    for i in range(0, 3000, 97):
+        A = leaf[i]
+        nb = set(T.neighbors[A * 64: A * 64 + T.ineigh[A]].tolist())
+        js = [j for j in range(3000) if leaf[j] in nb and j != i]
+        s = q[i] * sum(q[j] / np.linalg.norm(pos[i] - pos[j]) for j in js)
+        assert abs(phi[i] - s) < 1e-12 * max(1, abs(s))
+    assert abs(T.p2p_pairs() - sum((T.end[A] - T.sta[A] + 1) * sum(
+        T.end[B] - T.sta[B] + 1 for B in T.neighbors[A * 64:A * 64 + T.ineigh[A]]) for A in range(ls, ls + 512))) < 0.5
+
+
+def test_m2l_schemes_and_dense_sum(tree4, ops4):
+    U, K, _ = ops4
+    T = tree4
+    rng = np.random.default_rng(9)
+    M = rng.standard_normal(T.ncell * 32)
+    Lb = O.m2l(T, K, M, 32, "basic")
+    Ls = O.m2l(T, K, M, 32, "sibling")
+    s = O.levsta(4)[2] * 32
+    assert O.rel_l2(Lb[s:], Ls[s:]) < 1e-14
+    E = K.reshape(316, 32, 32)  # [k][col][row]
+    Mm = M.reshape(-1, 32)
+    for A in range(O.levsta(4)[2], T.ncell, 211):
+        acc = np.zeros(32)
+        for t in range(T.iinter[A]):
+            acc += E[T.Kindex[A * 192 + t]].T @ Mm[T.interaction[A * 192 + t]]
+        assert np.allclose(Lb[A * 32:(A + 1) * 32], acc, rtol=1e-12, atol=1e-12)
+
+
+def test_m2m_and_l2l_are_exact_interpolations(ops4):
+    """M2M of the children's P2M equals P2M at the parent (the kernel of the
+    interpolation has degree n-1, so it is reproduced exactly on the child's
+    nodes); this pins P2M, M2M (and Cweights) against each other."""
+    import ctypes as C
+    U, K, _ = ops4
+    pos, q = O.uniform_inputs(3000, seed=8)
+    n3, r = 64, 32
+    Tkz, Kt, Kw, Cw = O.weights(4, "d")
+    res = {}
+    for ml in (3, 2):
+        T = O.Tree(pos, 0, "d", maxlev=ml)
+        M = np.zeros(T.ncell * r)
+        O.lib("d").orc_upward(C.c_int(ml), C.c_double(1.0), C.c_int(4), C.c_int(r), O._p(T.center),
+                              O._p(T.list), O._p(T.sta), O._p(T.end), O._p(T.pos), O._p(q), O._p(U),
+                              O._p(Tkz), O._p(Kw), O._p(Cw), O._p(M))
+        res[ml] = M
+    ls2 = O.levsta(2)[2]
+    a, b = res[3][ls2 * r:(ls2 + 64) * r], res[2][ls2 * r:(ls2 + 64) * r]
+    assert O.rel_l2(b, a) < 1e-12
+
+
+def test_fmm_accuracy_against_direct(ops4):
+    U, K, _ = ops4
+    pos, q = O.uniform_inputs(4096, seed=0)
+    T = O.Tree(pos, 0, "d")
+    res = O.fmm(T, q, 4, U, K, scheme="sibling")
+    d = O.direct(pos, q) * q
+    err = O.rel_l2(d, res["phi"])
+    assert 1e-5 < err < 1e-3  # n = 4 Chebyshev / r = 32 truncation level
+
+
+# ------------------------------------------------------------------ the compiled reference
+@pytest.mark.skipif(not O.ref_available("s") or not O.ref_available("d"),
+                    reason="reference CPU path not built here (oracle/Makefile.ref)")
+@pytest.mark.parametrize("prec", ["s", "d"])
+@pytest.mark.parametrize("N,lmod", [(4096, 0), (20000, 0), (40000, 0), (4096, 1)])
+def test_oracle_matches_compiled_reference(prec, N, lmod, ops4):
+    U, K, _ = ops4
+    dt = O.dtype(prec)
+    pos, q = O.uniform_inputs(N, seed=1)
+    pos, q = pos.astype(dt), q.astype(dt)
+    ref = O.ref_run(pos, q, 4, U.astype(dt), K.astype(dt), prec=prec, lmod=lmod)
+    T = O.Tree(pos, lmod, prec)
+    assert ref["info"]["maxlev"] == T.maxlev
+    assert np.array_equal(ref["center"], T.center)
+    assert np.array_equal(ref["list"], T.list)
+    ls = O.levsta(T.maxlev)
+    assert np.array_equal(ref["sta"], T.sta[ls[-1]:]) and np.array_equal(ref["end"], T.end[ls[-1]:])
+    assert np.array_equal(ref["ineigh"], T.ineigh) and np.array_equal(ref["iinter"], T.iinter)
+    for A in range(T.ncell):
+        assert np.array_equal(ref["interaction"][A * 192:A * 192 + T.iinter[A]],
+                              T.interaction[A * 192:A * 192 + T.iinter[A]])
+        assert np.array_equal(ref["neighbors"][A * 64:A * 64 + T.ineigh[A]],
+                              T.neighbors[A * 64:A * 64 + T.ineigh[A]])
+    res = O.fmm(T, q, 4, U.astype(dt), K.astype(dt), scheme="sibling")
+    s = ls[2]
+    # the sibling-blocked oracle keeps the reference CPU7A/CPU8A operation order
+    assert np.array_equal(ref["M"][s * 32:], res["M"][s * 32:])
+    assert np.array_equal(ref["F"], res["F"])
+    assert np.array_equal(ref["phi_far"], res["phi_far"])
+    assert np.array_equal(ref["phi"], res["phi"])
+
+
+# ------------------------------------------------------------------ golden fixtures
+def _golden_cases():
+    return sorted(f for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+
+
+@pytest.mark.parametrize("fname", _golden_cases())
+def test_oracle_matches_golden(fname, ops4):
+    g = np.load(os.path.join(GOLDEN, fname))
+    prec = fname.split("_")[-1][0]
+    dt = O.dtype(prec)
+    N, lmod = int(g["N"]), int(g["lmod"])
+    pos, q = O.uniform_inputs(N, seed=int(g["seed"]))
+    pos, q = pos.astype(dt), q.astype(dt)
+    U, K, _ = ops4
+    Uc, Kc = U.astype(dt), K.astype(dt)
+
+    def digest(*arrays):
+        h = hashlib.sha256()
+        for a in arrays:
+            h.update(np.ascontiguousarray(a).tobytes())
+        return h.hexdigest()
+    assert digest(pos, q) == str(g["input_digest"])
+    assert digest(Uc, Kc) == str(g["operator_digest"]), "operator precompute changed"
+    T = O.Tree(pos, lmod, prec)
+    assert T.maxlev == int(g["maxlev"])
+    ls = O.levsta(T.maxlev)
+    assert np.array_equal(T.list, g["list"])
+    assert np.array_equal(T.sta[ls[-1]:], g["sta"]) and np.array_equal(T.end[ls[-1]:], g["end"])
+    assert np.array_equal(T.iinter, g["iinter"]) and np.array_equal(T.ineigh, g["ineigh"])
+    res = O.fmm(T, q, 4, Uc, Kc, scheme="sibling")
+    tol = 0.0
+    for k in ("phi", "phi_far"):
+        assert O.rel_l2(g[k], res[k]) <= tol or np.array_equal(g[k], res[k])
+    assert np.array_equal(g["M"], res["M"][ls[2] * 32:])
+    assert np.array_equal(g["F"], res["F"][ls[2] * 64:])
