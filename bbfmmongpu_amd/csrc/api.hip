@@ -426,6 +426,7 @@ static int solve_sorted(bbfmm_amd_plan *p)
   pp.nnbr = p->d_nnbr;
   pp.q = p->d_q;
   pp.phi = p->d_phi;
+  pp.use_mfma = p->form != BBFMM_M2L_PLAIN;
   BB_TRY(launch_p2p(pp, s));
   return BBFMM_OK;
 }

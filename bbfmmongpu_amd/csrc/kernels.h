@@ -66,6 +66,7 @@ struct P2PArgs {
   const int *nnbr;         // [nleaf]
   const real *q;           // [rhs][N]
   real *phi;               // [rhs][N] (accumulated into)
+  int use_mfma = 0;        // fp32: 16 vectors at a time on MFMA
 };
 int launch_p2p(const P2PArgs &a, hipStream_t s);
 

@@ -48,7 +48,11 @@ struct GridArgs {
   const void *Mg1;  // grids (lo) for BF16X2
   real *L;
   int wave_cols;    // columns per wave
+  const int *steps;  // [208][10]: source class, packed shift, K index per field class (-1: near)
+  int ss;            // 1: source-stationary kernel (block = columns of all 8 classes)
 };
+
+constexpr int kSteps = 208;  // 8 source classes x 26 non-zero parent shifts
 
 struct GridM2L {
   int r = 0, maxlev = 0, nrhs = 1, form = 0, ncell_cap = 0;
@@ -56,6 +60,7 @@ struct GridM2L {
   int nblocks = 0;
   size_t nslots = 0;
   int4 *d_tab = nullptr;
+  int *d_steps = nullptr;
   void *d_K = nullptr;
   void *d_Mg0 = nullptr;
   void *d_Mg1 = nullptr;
@@ -243,6 +248,98 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(GridArgs g)
   }
 }
 
+// ---------------------------------------------------------------- source-stationary MFMA kernel
+// Same products, loop order inverted: a field cell of class s at parent p and
+// a source of class s' at parent p + d interact through the offset
+// v = 2d + s' - s, so for one (s', d) the shifted source tile M_{s'}[p + d]
+// serves all 8 field classes.  A wave owns 32 columns (parent, vector) of all
+// 8 classes (8 accumulators); per step (s', d) it loads the source tile once
+// and multiplies it by the up to 8 operators K_v, which the block stages in
+// LDS (double-buffered) for all its waves.  Used for split-BF16 (fp32 build).
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
+{
+  __shared__ bf16x8 sK[2][8][4][64];  // [buffer][field class][kstep*2 + hi/lo][lane]
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  int l = g.minlev;
+  while (l < g.maxlev && (int)blockIdx.x >= g.blk0[l + 1]) l++;
+  const int grp = blockIdx.x - g.blk0[l];
+  const int P = 1 << (l - 1);
+  const int PP = P + 2;
+  const int ncols = P * P * P * g.nrhs;
+  const int half = lane >> 5;
+  const int col = (grp * WAVES + wave) * 32 + (lane & 31);
+  const bool valid = col < ncols;
+  const int cc = valid ? col : 0;
+  const int pl = cc / g.nrhs;
+  const int rh = cc - pl * g.nrhs;
+  const int pz = pl % P, py = (pl / P) % P, px = pl / (P * P);
+  const int base = ((px + 1) * PP + py + 1) * PP + pz + 1;
+  const int slot0 = g.lvl_slot0[l];
+  const int cls = PP * PP * PP;
+  const bf16x8 *Kb = (const bf16x8 *)g.Ka;
+  const __bf16 *Mh = (const __bf16 *)g.Mg0;
+  const __bf16 *Ml = (const __bf16 *)g.Mg1;
+
+  f32x16 acc[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[c][e] = 0.0f;
+
+  // stage the operators of step `st` into buffer `b`: 8 classes x 4 x 64 lanes
+  auto stage = [&](int st, int b) {
+    const int *e = g.steps + st * 10;
+    for (int o = threadIdx.x; o < 8 * 4 * 64; o += 64 * WAVES) {
+      const int c = o >> 8, q = (o >> 6) & 3, ln = o & 63;
+      const int k = e[2 + c];
+      if (k >= 0) sK[b][c][q][ln] = Kb[((size_t)k * 4 + q) * 64 + ln];
+    }
+  };
+  stage(0, 0);
+  __syncthreads();
+  for (int st = 0; st < kSteps; st++) {
+    const int b = st & 1;
+    const int *e = g.steps + st * 10;
+    const int sc = e[0];
+    const int dx = ((e[1] >> 4) & 3) - 1, dy = ((e[1] >> 2) & 3) - 1, dz = (e[1] & 3) - 1;
+    const size_t o = ((size_t)(slot0 + sc * cls + (dx * PP + dy) * PP + dz + base) * g.nrhs + rh) * 32 + half * 8;
+    bf16x8 b0h = *(const bf16x8 *)(Mh + o), b0l = *(const bf16x8 *)(Ml + o);
+    bf16x8 b1h = *(const bf16x8 *)(Mh + o + 16), b1l = *(const bf16x8 *)(Ml + o + 16);
+    if (!valid) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) b0h[q] = b0l[q] = b1h[q] = b1l[q] = (__bf16)0.0f;
+    }
+    if (st + 1 < kSteps) stage(st + 1, b ^ 1);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      if (e[2 + c] < 0) continue;  // uniform: near-field pair for this class
+      const bf16x8 a0h = sK[b][c][0][lane], a0l = sK[b][c][1][lane];
+      const bf16x8 a1h = sK[b][c][2][lane], a1l = sK[b][c][3][lane];
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0h, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0l, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, b0h, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1h, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1l, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, b1h, acc[c], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int fx = 2 * px + ((c >> 2) & 1), fy = 2 * py + ((c >> 1) & 1), fz = 2 * pz + (c & 1);
+    const int A = g.levsta[l] + morton3(fx, fy, fz, l);
+    float *Lp = (float *)g.L + ((size_t)rh * g.ncell + A) * 32 + 4 * half;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      f32x4 v = {acc[c][4 * q], acc[c][4 * q + 1], acc[c][4 * q + 2], acc[c][4 * q + 3]};
+      *reinterpret_cast<f32x4 *>(Lp + 8 * q) = v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- plain kernel
 // One thread per (column, row); 256 / R columns per block, all of one class.
 template <int R>
@@ -317,6 +414,34 @@ static void offset_table(std::vector<int4> &tab)
   }
 }
 
+// (s', d) steps of the source-stationary kernel: for every source class s'
+// and non-zero parent shift d, the operator index of each field class s
+// (offset v = 2d + s' - s), or -1 when v is a near-field offset.
+static void step_table(std::vector<int> &steps)
+{
+  int Kt[343];
+  ktable(Kt);
+  steps.assign(kSteps * 10, -1);
+  int st = 0;
+  for (int sc = 0; sc < 8; sc++)
+    for (int dxi = 0; dxi < 3; dxi++)
+      for (int dyi = 0; dyi < 3; dyi++)
+        for (int dzi = 0; dzi < 3; dzi++) {
+          if (dxi == 1 && dyi == 1 && dzi == 1) continue;
+          int *e = &steps[st * 10];
+          e[0] = sc;
+          e[1] = (dxi << 4) | (dyi << 2) | dzi;
+          for (int c = 0; c < 8; c++) {
+            const int vx = 2 * (dxi - 1) + ((sc >> 2) & 1) - ((c >> 2) & 1);
+            const int vy = 2 * (dyi - 1) + ((sc >> 1) & 1) - ((c >> 1) & 1);
+            const int vz = 2 * (dzi - 1) + (sc & 1) - (c & 1);
+            const bool near = vx >= -1 && vx <= 1 && vy >= -1 && vy <= 1 && vz >= -1 && vz <= 1;
+            e[2 + c] = near ? -1 : Kt[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
+          }
+          st++;
+        }
+}
+
 int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const real *K, hipStream_t s)
 {
   *out = nullptr;
@@ -348,8 +473,13 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   }
   g->nslots = slots;
   int tpw = 1, cpb;
+  // source-stationary blocking once the deepest level has enough columns
+  // (parents x vectors) to give every SIMD at least a wave
+  a.ss = form == BBFMM_M2L_BF16X2 && (long long)(1 << (3 * (maxlev - 1))) * nrhs >= 32 * 1024 ? 1 : 0;
   if (form == BBFMM_M2L_PLAIN) {
     cpb = 256 / r;
+  } else if (a.ss) {
+    cpb = 4 * 32;  // 4 waves x 32 columns, all 8 classes
   } else {
     // two tiles per wave once there are enough columns to fill the chip
     const long long deep = (long long)(1 << (3 * (maxlev - 1))) * nrhs;
@@ -361,13 +491,15 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   for (int l = kMinLev; l <= maxlev; l++) {
     a.blk0[l] = nb;
     const long long cols = (long long)(1 << (3 * (l - 1))) * nrhs;
-    nb += 8 * (int)((cols + cpb - 1) / cpb);
+    nb += (a.ss ? 1 : 8) * (int)((cols + cpb - 1) / cpb);
   }
   a.blk0[maxlev + 1] = nb;
   g->nblocks = nb;
 
   std::vector<int4> tab;
   offset_table(tab);
+  std::vector<int> steps;
+  step_table(steps);
   int rc = BBFMM_OK;
   auto fail = [&](hipError_t e, const char *what) {
     rc = set_error(BBFMM_E_HIP, "dense-grid M2L: %s: %s", what, hipGetErrorString(e));
@@ -375,6 +507,9 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   hipError_t e;
   if ((e = hipMalloc(&g->d_tab, sizeof(int4) * tab.size())) != hipSuccess) fail(e, "hipMalloc");
   if (rc == BBFMM_OK && (e = hipMemcpyAsync(g->d_tab, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+    fail(e, "copy");
+  if (rc == BBFMM_OK && (e = hipMalloc(&g->d_steps, sizeof(int) * steps.size())) != hipSuccess) fail(e, "hipMalloc");
+  if (rc == BBFMM_OK && (e = hipMemcpyAsync(g->d_steps, steps.data(), sizeof(int) * steps.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
     fail(e, "copy");
   // operators in the form's layout
   if (rc == BBFMM_OK) {
@@ -440,6 +575,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     return rc;
   }
   a.tab = g->d_tab;
+  a.steps = g->d_steps;
   a.Ka = g->d_K;
   a.Mg0 = g->d_Mg0;
   a.Mg1 = g->d_Mg1;
@@ -451,6 +587,7 @@ void grid_m2l_destroy(GridM2L *g)
 {
   if (!g) return;
   if (g->d_tab) (void)hipFree(g->d_tab);
+  if (g->d_steps) (void)hipFree(g->d_steps);
   if (g->d_K) (void)hipFree(g->d_K);
   if (g->d_Mg0) (void)hipFree(g->d_Mg0);
   if (g->d_Mg1) (void)hipFree(g->d_Mg1);
@@ -480,7 +617,9 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_BF16X2>, dim3(sb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (a.wave_cols == 64)
+    if (a.ss)
+      hipLaunchKernelGGL((k_m2l_mfma_ss<4>), dim3(g->nblocks), dim3(256), 0, s, a);
+    else if (a.wave_cols == 64)
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
     else
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 1>), dim3(g->nblocks), dim3(256), 0, s, a);

@@ -109,6 +109,88 @@ __global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
   }
 }
 
+#ifdef SINGLE
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Sixteen charge vectors at once on MFMA:  phi[t][v] = sum_j G[t][j] q[j][v]
+// with G[t][j] = 1/r_tj.  v_mfma_f32_16x16x4_f32 takes A = G (16 targets x 4
+// sources; lane l supplies G[l%16][l/16]) and B = q (4 sources x 16 vectors;
+// lane l supplies q[l/16][l%16]), so every lane computes exactly the one 1/r
+// value it feeds to the matrix core and the vector ALUs do only the geometry.
+// One wavefront per leaf, two 16-target tiles per pass, sources staged in LDS
+// 64 at a time with their 16 charges.
+__global__ void __launch_bounds__(64) k_p2p_mfma16(P2PArgs a, int rhs0)
+{
+  constexpr int CH = 64;
+  __shared__ float4 sp[CH];
+  __shared__ float sq[CH][16];
+  __shared__ int nsta[27], ncnt[27], nself[27];
+  const int leaf = blockIdx.x;
+  const int l = threadIdx.x;
+  const int ti = l & 15;  // A row (target in tile) and B column (vector)
+  const int kq = l >> 4;  // source within a step of four
+  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
+  const int nn = a.nnbr[leaf];
+  if (l < nn) {
+    const int B = a.nbr[(size_t)leaf * 27 + l];
+    nsta[l] = a.leafsta[B];
+    ncnt[l] = a.leafcnt[B];
+    nself[l] = (B == leaf);
+  }
+  __syncthreads();
+  const size_t N = a.N;
+  for (int tb = 0; tb < cnt; tb += 32) {
+    const int t0 = tb + ti, t1 = tb + 16 + ti;
+    float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
+    if (t0 < cnt) x0 = a.pos[sta + t0];
+    if (t1 < cnt) x1 = a.pos[sta + t1];
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < nn; b++) {
+      const int jsta = nsta[b], nj = ncnt[b];
+      const bool self = nself[b] != 0;
+      for (int jb = 0; jb < nj; jb += CH) {
+        const int nk = min(CH, nj - jb);
+        if (l < nk) {
+          sp[l] = a.pos[jsta + jb + l];
+#pragma unroll
+          for (int r = 0; r < 16; r++) sq[l][r] = a.q[(size_t)(rhs0 + r) * N + jsta + jb + l];
+        }
+        __syncthreads();
+        const int nsteps = (nk + 3) >> 2;
+        for (int st = 0; st < nsteps; st++) {
+          const int k = 4 * st + kq;
+          const bool kv = k < nk;
+          const float4 y = sp[kv ? k : 0];
+          const float dx0 = y.x - x0.x, dy0 = y.y - x0.y, dz0 = y.z - x0.z;
+          const float dx1 = y.x - x1.x, dy1 = y.y - x1.y, dz1 = y.z - x1.z;
+          const float rr0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+          const float rr1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+          float g0 = __builtin_amdgcn_rsqf(rr0), g1 = __builtin_amdgcn_rsqf(rr1);
+          if (self) {
+            g0 = rr0 != 0.f ? g0 : 0.f;
+            g1 = rr1 != 0.f ? g1 : 0.f;
+          }
+          g0 = kv ? g0 : 0.f;
+          g1 = kv ? g1 : 0.f;
+          const float bq = kv ? sq[k][ti] : 0.f;
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(g0, bq, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(g1, bq, acc1, 0, 0, 0);
+        }
+        __syncthreads();
+      }
+    }
+    // lane l holds rows (targets) 4*(l/16)+v of column (vector) l%16
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = 4 * kq + v;
+      const size_t row = (size_t)(rhs0 + ti) * N + sta;
+      if (tb + i < cnt) a.phi[row + tb + i] += a.q[row + tb + i] * acc0[v];
+      if (tb + 16 + i < cnt) a.phi[row + tb + 16 + i] += a.q[row + tb + 16 + i] * acc1[v];
+    }
+  }
+}
+#endif
+
 int launch_p2p(const P2PArgs &a, hipStream_t s)
 {
   if (a.nleaf <= 0 || a.N <= 0) return BBFMM_OK;
@@ -116,7 +198,12 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
   while (rhs < a.nrhs) {
     const int left = a.nrhs - rhs;
     if (left >= 16) {
-      hipLaunchKernelGGL(k_p2p<16>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+#ifdef SINGLE
+      if (a.use_mfma)
+        hipLaunchKernelGGL(k_p2p_mfma16, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+      else
+#endif
+        hipLaunchKernelGGL(k_p2p<16>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
       rhs += 16;
     } else if (left >= 4) {
       hipLaunchKernelGGL(k_p2p<4>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);

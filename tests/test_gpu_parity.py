@@ -102,3 +102,24 @@ def test_grid_m2l_against_oracle(prec, scheme, form, tol, nrhs, ops4, cuda):
         err = O.rel_l2(ref[s:], Lg[j, s:])
         print(f"grid m2l prec={prec} scheme={scheme} form={form} rhs={j}: rel_l2={err:.3e}")
         assert err < tol
+
+
+@pytest.mark.parametrize("N,lmod,nrhs", [(65536, 1, 16), (70000, 1, 16)])
+def test_bf16x2_many_vectors_against_oracle(N, lmod, nrhs, ops4, cuda):
+    """Split-BF16 M2L with many vectors (the source-stationary MFMA kernel at the
+    deepest levels) and the MFMA P2P, full solve against the oracle."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, Q = O.uniform_inputs(N, seed=21, nrhs=nrhs)
+    Q = Q.astype(np.float32)
+    plan = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", lmod=lmod).set(scheme=3, form=1)
+    assert plan.maxlev == 5
+    phi = plan.solve(torch.from_numpy(Q).cuda()).cpu().numpy()
+    tree = O.Tree(pos, lmod, "d")
+    Kd, Ud = K.astype(np.float32).astype(np.float64), U.astype(np.float32).astype(np.float64)
+    for j in (0, 7, nrhs - 1):
+        ref = O.fmm(tree, Q[j].astype(np.float64), 4, Ud, Kd, scheme="sibling")
+        err = O.rel_l2(ref["phi"], phi[j])
+        print(f"bf16x2 solve N={N} rhs={j}: {err:.2e}")
+        assert err < 3e-5
