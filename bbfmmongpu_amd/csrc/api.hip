@@ -67,6 +67,7 @@ struct bbfmm_amd_plan {
   // device
   real4 *d_pos = nullptr, *d_center = nullptr, *d_K4 = nullptr;
   int *d_perm = nullptr, *d_leafsta = nullptr, *d_leafcnt = nullptr, *d_leafof = nullptr;
+  int *d_iperm = nullptr;  // original index -> leaf order
   int *d_nbr = nullptr, *d_nnbr = nullptr, *d_iinter = nullptr, *d_inter = nullptr, *d_kidx = nullptr;
   int *d_Ktable = nullptr, *d_levsta = nullptr;
   real *d_Ucm = nullptr, *d_Urm = nullptr, *d_Tkz = nullptr, *d_Kw = nullptr, *d_Cw = nullptr;
@@ -74,6 +75,10 @@ struct bbfmm_amd_plan {
   real *d_q = nullptr, *d_phi = nullptr, *d_S = nullptr, *d_M = nullptr, *d_L = nullptr, *d_F = nullptr;
   real *d_qin = nullptr, *d_phiout = nullptr;  // staging for the host-array entry points
   real *d_farfield = nullptr;                  // far-field q.*phi of the last solve (vector 0)
+  real *d_qt = nullptr;                        // [Npad][16] charges, source-major (MFMA P2P)
+  real4 *d_ppos = nullptr;                     // padded particle copy for P2P
+  int *d_pad_sta = nullptr, *d_pad_cnt = nullptr, *d_pad_of = nullptr;
+  int npad = 0;
   std::vector<real> h_K;                       // column-major operators (for the grid form)
   bb::GridM2L *grid = nullptr;
   int grid_form = -1;
@@ -126,7 +131,8 @@ static void plan_free_device(bbfmm_amd_plan *p)
   void *ptrs[] = {p->d_pos, p->d_center, p->d_K4, p->d_perm, p->d_leafsta, p->d_leafcnt, p->d_leafof,
                   p->d_nbr, p->d_nnbr, p->d_iinter, p->d_inter, p->d_kidx, p->d_Ktable, p->d_levsta,
                   p->d_Ucm, p->d_Urm, p->d_Tkz, p->d_Kw, p->d_Cw, p->d_scale, p->d_q, p->d_phi,
-                  p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield};
+                  p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
+                  p->d_ppos, p->d_pad_sta, p->d_pad_cnt, p->d_pad_of, p->d_iperm};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->grid) grid_m2l_destroy(p->grid);
@@ -289,6 +295,11 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
   BB_TRY(upload(&p->d_pos, hpos.data(), (size_t)N, stream));
   BB_TRY(upload(&p->d_center, hctr.data(), (size_t)ncell, stream));
   BB_TRY(upload(&p->d_perm, p->h_perm.data(), (size_t)N, stream));
+  {
+    std::vector<int> iperm((size_t)N);
+    for (int i = 0; i < N; i++) iperm[p->h_perm[i]] = i;
+    BB_TRY(upload(&p->d_iperm, iperm.data(), (size_t)N, stream));
+  }
   BB_TRY(upload(&p->d_leafsta, lsta.data(), (size_t)nleaf, stream));
   BB_TRY(upload(&p->d_leafcnt, lcnt.data(), (size_t)nleaf, stream));
   BB_TRY(upload(&p->d_leafof, leafof.data(), (size_t)N, stream));
@@ -306,6 +317,28 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
   BB_TRY(upload(&p->d_Kw, Kw, (size_t)n3, stream));
   BB_TRY(upload(&p->d_Cw, Cw, (size_t)2 * n * n, stream));
   BB_TRY(upload(&p->d_scale, scale.data(), scale.size(), stream));
+  {
+    // padded copy for P2P: leaf ranges start at multiples of 4, dummies far away with q = 0
+    std::vector<int> psta((size_t)nleaf), pcnt((size_t)nleaf), pad_of((size_t)N);
+    int run = 0;
+    for (int k = 0; k < nleaf; k++) {
+      psta[k] = run;
+      pcnt[k] = (lcnt[k] + 3) & ~3;
+      for (int j = 0; j < lcnt[k]; j++) pad_of[lsta[k] + j] = run + j;
+      run += pcnt[k];
+    }
+    p->npad = run;
+    std::vector<real4> ppos((size_t)run);
+    for (auto &x : ppos) {
+      x.x = x.y = x.z = (real)1e6;
+      x.w = 0;
+    }
+    for (int i = 0; i < N; i++) ppos[pad_of[i]] = hpos[i];
+    BB_TRY(upload(&p->d_ppos, ppos.data(), ppos.size(), stream));
+    BB_TRY(upload(&p->d_pad_sta, psta.data(), psta.size(), stream));
+    BB_TRY(upload(&p->d_pad_cnt, pcnt.data(), pcnt.size(), stream));
+    BB_TRY(upload(&p->d_pad_of, pad_of.data(), pad_of.size(), stream));
+  }
   const size_t NR = (size_t)N * nrhs;
   BB_TRY(dalloc(&p->d_q, NR));
   BB_TRY(dalloc(&p->d_phi, NR));
@@ -427,6 +460,19 @@ static int solve_sorted(bbfmm_amd_plan *p)
   pp.q = p->d_q;
   pp.phi = p->d_phi;
   pp.use_mfma = p->form != BBFMM_M2L_PLAIN;
+  pp.ppos = p->d_ppos;
+  pp.pad_sta = p->d_pad_sta;
+  pp.pad_cnt = p->d_pad_cnt;
+  pp.pad_of = p->d_pad_of;
+#ifdef SINGLE
+  if (pp.use_mfma && p->nrhs >= 16) {
+    if (!p->d_qt) {
+      BB_TRY(dalloc(&p->d_qt, (size_t)(p->npad > 0 ? p->npad : 1) * 16));
+      BB_HIP(hipMemsetAsync(p->d_qt, 0, sizeof(real) * (size_t)(p->npad > 0 ? p->npad : 1) * 16, s));
+    }
+    pp.qt = p->d_qt;
+  }
+#endif
   BB_TRY(launch_p2p(pp, s));
   return BBFMM_OK;
 }
@@ -439,7 +485,7 @@ static int solve_dev(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev)
   BB_TRY(launch_gather(p->d_perm, q_dev, p->d_q, p->N, p->nrhs, s));
   BB_TRY(solve_sorted(p));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_SCATTER), s));
-  BB_TRY(launch_scatter(p->d_perm, p->d_phi, phi_dev, p->N, p->nrhs, s));
+  BB_TRY(launch_scatter(p->d_iperm, p->d_phi, phi_dev, p->N, p->nrhs, s));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_TOTAL), s));
   p->solves++;
   p->solved = true;
@@ -827,7 +873,7 @@ extern "C" void anotherDownwardPass(anotherTree **atree, real3 *field, real3 *so
   pp.q = p->d_q;
   pp.phi = p->d_phi;
   if (launch_p2p(pp, s) != BBFMM_OK) return;
-  if (launch_scatter(p->d_perm, p->d_phi, p->d_phiout, p->N, 1, s) != BBFMM_OK) return;
+  if (launch_scatter(p->d_iperm, p->d_phi, p->d_phiout, p->N, 1, s) != BBFMM_OK) return;
   if (hipMemcpy(phi, p->d_phiout, sizeof(real) * p->N, hipMemcpyDeviceToHost) != hipSuccess) {
     set_error(BBFMM_E_HIP, "copy of phi failed");
     return;

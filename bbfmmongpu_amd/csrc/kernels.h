@@ -26,7 +26,8 @@ struct PassArgs {
 };
 
 int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
-int launch_scatter(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
+// dst[j] = src[iperm[j]] per vector (iperm: user index -> leaf-order index)
+int launch_scatter(const int *iperm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
 int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs);
 int launch_m2m(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs);
 int launch_postm2l(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs);
@@ -67,6 +68,12 @@ struct P2PArgs {
   const real *q;           // [rhs][N]
   real *phi;               // [rhs][N] (accumulated into)
   int use_mfma = 0;        // fp32: 16 vectors at a time on MFMA
+  // padded particle copy (every leaf padded to a multiple of 4 with far-away,
+  // zero-charge dummies); kernels fall back to the unpadded ones when null
+  const real4 *ppos = nullptr;  // [Npad]
+  const int *pad_sta = nullptr, *pad_cnt = nullptr;  // [nleaf]
+  const int *pad_of = nullptr;  // [N] leaf-order index -> padded index
+  float *qt = nullptr;          // [Npad][16] charges of 16 vectors, source-major
 };
 int launch_p2p(const P2PArgs &a, hipStream_t s);
 

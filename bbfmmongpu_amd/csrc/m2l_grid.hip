@@ -288,41 +288,77 @@ __global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
 #pragma unroll
     for (int e = 0; e < 16; e++) acc[c][e] = 0.0f;
 
-  // stage the operators of step `st` into buffer `b`: 8 classes x 4 x 64 lanes
-  auto stage = [&](int st, int b) {
+  // Each thread moves 8 operator words per step (class c = i, k-part =
+  // its wave, lane = its lane); they are loaded one step ahead into
+  // registers and written to the other LDS buffer after the step's MFMAs.
+  static_assert(WAVES == 4, "one k-part per wave");
+  auto load_k = [&](int st, bf16x8 *kr) {
     const int *e = g.steps + st * 10;
-    for (int o = threadIdx.x; o < 8 * 4 * 64; o += 64 * WAVES) {
-      const int c = o >> 8, q = (o >> 6) & 3, ln = o & 63;
-      const int k = e[2 + c];
-      if (k >= 0) sK[b][c][q][ln] = Kb[((size_t)k * 4 + q) * 64 + ln];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int k = e[2 + i];
+      if (k >= 0) kr[i] = Kb[((size_t)k * 4 + wave) * 64 + lane];
     }
   };
-  stage(0, 0);
+  auto store_k = [&](int st, int b, const bf16x8 *kr) {
+    const int *e = g.steps + st * 10;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      if (e[2 + i] >= 0) sK[b][i][wave][lane] = kr[i];
+  };
+  auto src_off = [&](int st) {
+    const int *e = g.steps + st * 10;
+    const int sc = e[0];
+    const int dx = ((e[1] >> 4) & 3) - 1, dy = ((e[1] >> 2) & 3) - 1, dz = (e[1] & 3) - 1;
+    return ((size_t)(slot0 + sc * cls + (dx * PP + dy) * PP + dz + base) * g.nrhs + rh) * 32 + half * 8;
+  };
+  bf16x8 kr[8];
+  load_k(0, kr);
+  store_k(0, 0, kr);
+  bf16x8 bh0, bl0, bh1, bl1;
+  {
+    const size_t o = src_off(0);
+    bh0 = *(const bf16x8 *)(Mh + o);
+    bl0 = *(const bf16x8 *)(Ml + o);
+    bh1 = *(const bf16x8 *)(Mh + o + 16);
+    bl1 = *(const bf16x8 *)(Ml + o + 16);
+  }
   __syncthreads();
   for (int st = 0; st < kSteps; st++) {
     const int b = st & 1;
     const int *e = g.steps + st * 10;
-    const int sc = e[0];
-    const int dx = ((e[1] >> 4) & 3) - 1, dy = ((e[1] >> 2) & 3) - 1, dz = (e[1] & 3) - 1;
-    const size_t o = ((size_t)(slot0 + sc * cls + (dx * PP + dy) * PP + dz + base) * g.nrhs + rh) * 32 + half * 8;
-    bf16x8 b0h = *(const bf16x8 *)(Mh + o), b0l = *(const bf16x8 *)(Ml + o);
-    bf16x8 b1h = *(const bf16x8 *)(Mh + o + 16), b1l = *(const bf16x8 *)(Ml + o + 16);
+    const bool more = st + 1 < kSteps;
+    bf16x8 nh0, nl0, nh1, nl1;
+    if (more) {
+      load_k(st + 1, kr);
+      const size_t o = src_off(st + 1);
+      nh0 = *(const bf16x8 *)(Mh + o);
+      nl0 = *(const bf16x8 *)(Ml + o);
+      nh1 = *(const bf16x8 *)(Mh + o + 16);
+      nl1 = *(const bf16x8 *)(Ml + o + 16);
+    }
     if (!valid) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) b0h[q] = b0l[q] = b1h[q] = b1l[q] = (__bf16)0.0f;
+      for (int q = 0; q < 8; q++) bh0[q] = bl0[q] = bh1[q] = bl1[q] = (__bf16)0.0f;
     }
-    if (st + 1 < kSteps) stage(st + 1, b ^ 1);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
       if (e[2 + c] < 0) continue;  // uniform: near-field pair for this class
       const bf16x8 a0h = sK[b][c][0][lane], a0l = sK[b][c][1][lane];
       const bf16x8 a1h = sK[b][c][2][lane], a1l = sK[b][c][3][lane];
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0h, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0l, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, b0h, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1h, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1l, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, b1h, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, bh0, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, bl0, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, bh0, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, bh1, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, bl1, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, bh1, acc[c], 0, 0, 0);
+    }
+    if (more) {
+      store_k(st + 1, b ^ 1, kr);
+      bh0 = nh0;
+      bl0 = nl0;
+      bh1 = nh1;
+      bl1 = nl1;
     }
     __syncthreads();
   }

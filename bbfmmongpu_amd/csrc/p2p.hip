@@ -109,8 +109,139 @@ __global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
   }
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2):
+// give XCD x the x-th contiguous eighth of the leaves (Morton order), so that
+// neighbouring leaves, which read the same sources, share an L2.
+__device__ __forceinline__ int xcd_leaf(int b, int nleaf)
+{
+  if (nleaf & 7) return b;
+  return (b & 7) * (nleaf >> 3) + (b >> 3);
+}
+
+// One vector, no LDS: a wavefront per leaf, lane = (target t = lane % 32,
+// source half h = lane / 32); each lane sweeps every other source of the
+// neighbour leaves straight from L1/L2 (lanes of a half read the same
+// address), and the two halves are summed with a cross-lane read at the end.
+// Leaves of ~32 particles keep all 64 lanes busy.
+__global__ void __launch_bounds__(64) k_p2p_split(P2PArgs a, int rhs0)
+{
+  const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
+  const int l = threadIdx.x;
+  const int ti = l & 31, h = l >> 5;
+  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
+  const int nn = a.nnbr[leaf];
+  const size_t off = (size_t)rhs0 * a.N;
+  const real *q = a.q + off;
+  const int *nb = a.nbr + (size_t)leaf * 27;
+  for (int tb = 0; tb < cnt; tb += 32) {
+    const int i = tb + ti;
+    real4 xf;
+    if (i < cnt) {
+      xf = a.pos[sta + i];
+    } else {
+      xf.x = xf.y = xf.z = xf.w = 0;
+    }
+    real acc0 = 0, acc1 = 0;
+    for (int b = 0; b < nn; b++) {
+      const int B = nb[b];
+      const int jsta = a.leafsta[B], nj = a.leafcnt[B];
+      if (B != leaf) {
+        int k = h;
+        for (; k + 2 < nj; k += 4) {
+          const real4 y0 = a.pos[jsta + k], y1 = a.pos[jsta + k + 2];
+          const real q0 = q[jsta + k], q1 = q[jsta + k + 2];
+          const real dx0 = y0.x - xf.x, dy0 = y0.y - xf.y, dz0 = y0.z - xf.z;
+          const real dx1 = y1.x - xf.x, dy1 = y1.y - xf.y, dz1 = y1.z - xf.z;
+          acc0 += q0 * rsqrt_r(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
+          acc1 += q1 * rsqrt_r(dx1 * dx1 + dy1 * dy1 + dz1 * dz1);
+        }
+        if (k < nj) {
+          const real4 y0 = a.pos[jsta + k];
+          const real dx0 = y0.x - xf.x, dy0 = y0.y - xf.y, dz0 = y0.z - xf.z;
+          acc0 += q[jsta + k] * rsqrt_r(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
+        }
+      } else {
+        for (int k = h; k < nj; k += 2) {
+          const real4 y0 = a.pos[jsta + k];
+          const real dx0 = y0.x - xf.x, dy0 = y0.y - xf.y, dz0 = y0.z - xf.z;
+          const real rr = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+          acc0 += rr != (real)0 ? q[jsta + k] * rsqrt_r(rr) : (real)0;
+        }
+      }
+    }
+    real v = acc0 + acc1;
+    v += __shfl_down(v, 32, 64);
+    if (h == 0 && i < cnt) a.phi[off + sta + i] += q[sta + i] * v;
+  }
+}
+
 #ifdef SINGLE
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// q [16][N] (vectors rhs0..rhs0+15) -> qt [N][16]
+__global__ void k_transpose16(int N, const float *__restrict__ q, float *__restrict__ qt)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  float4 *o = reinterpret_cast<float4 *>(qt + (size_t)i * 16);
+#pragma unroll
+  for (int r = 0; r < 16; r += 4)
+    o[r / 4] = make_float4(q[(size_t)r * N + i], q[(size_t)(r + 1) * N + i], q[(size_t)(r + 2) * N + i],
+                           q[(size_t)(r + 3) * N + i]);
+}
+
+// As k_p2p_mfma16, without LDS or barriers: the B operand of a step (4
+// sources x 16 vectors) is 256 contiguous bytes of the source-major charges.
+__global__ void __launch_bounds__(64) k_p2p_mfma16_direct(P2PArgs a, int rhs0, const float *__restrict__ qt)
+{
+  const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
+  const int l = threadIdx.x;
+  const int ti = l & 15, kq = l >> 4;
+  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
+  const int nn = a.nnbr[leaf];
+  const int *nb = a.nbr + (size_t)leaf * 27;
+  const size_t N = a.N;
+  for (int tb = 0; tb < cnt; tb += 32) {
+    const int t0 = tb + ti, t1 = tb + 16 + ti;
+    float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
+    if (t0 < cnt) x0 = a.pos[sta + t0];
+    if (t1 < cnt) x1 = a.pos[sta + t1];
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < nn; b++) {
+      const int B = nb[b];
+      const int jsta = a.leafsta[B], nj = a.leafcnt[B];
+      const bool self = B == leaf;
+      for (int k0 = 0; k0 < nj; k0 += 4) {
+        const int k = k0 + kq;
+        const bool kv = k < nj;
+        const int src = jsta + (kv ? k : 0);
+        const float4 y = a.pos[src];
+        const float qv = qt[(size_t)src * 16 + ti];
+        const float dx0 = y.x - x0.x, dy0 = y.y - x0.y, dz0 = y.z - x0.z;
+        const float dx1 = y.x - x1.x, dy1 = y.y - x1.y, dz1 = y.z - x1.z;
+        const float rr0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+        const float rr1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+        float g0 = __builtin_amdgcn_rsqf(rr0), g1 = __builtin_amdgcn_rsqf(rr1);
+        if (self) {
+          g0 = rr0 != 0.f ? g0 : 0.f;
+          g1 = rr1 != 0.f ? g1 : 0.f;
+        }
+        g0 = kv ? g0 : 0.f;
+        g1 = kv ? g1 : 0.f;
+        const float bq = kv ? qv : 0.f;
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(g0, bq, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(g1, bq, acc1, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = 4 * kq + v;
+      const size_t row = (size_t)(rhs0 + ti) * N + sta;
+      if (tb + i < cnt) a.phi[row + tb + i] += a.q[row + tb + i] * acc0[v];
+      if (tb + 16 + i < cnt) a.phi[row + tb + 16 + i] += a.q[row + tb + 16 + i] * acc1[v];
+    }
+  }
+}
 
 // Sixteen charge vectors at once on MFMA:  phi[t][v] = sum_j G[t][j] q[j][v]
 // with G[t][j] = 1/r_tj.  v_mfma_f32_16x16x4_f32 takes A = G (16 targets x 4
@@ -191,6 +322,93 @@ __global__ void __launch_bounds__(64) k_p2p_mfma16(P2PArgs a, int rhs0)
 }
 #endif
 
+// ---------------------------------------------------------------- padded layout
+// The P2P kernels below read a private copy of the particles in which every
+// leaf's range starts at a multiple of 4 and is padded to a multiple of 4
+// with dummy sources far outside the root cell and of zero charge
+// (pad_sta/pad_cnt per leaf), so that their source loops need no tail masks.
+// Targets are read from the same copy; dummy targets are never written.
+
+#ifdef SINGLE
+// Sixteen vectors on MFMA over the padded layout; qt16 is [padded N][16].
+// Lane l computes G[target l%16 (two tiles)][source 4j + l/16] and feeds the
+// charges of that source for vector l%16 as the B operand.
+__global__ void __launch_bounds__(64) k_p2p_pad16(P2PArgs a, int rhs0)
+{
+  const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
+  const int l = threadIdx.x;
+  const int ti = l & 15, kq = l >> 4;
+  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
+  const int psta = a.pad_sta[leaf];
+  const int nn = a.nnbr[leaf];
+  const int *nb = a.nbr + (size_t)leaf * 27;
+  const float4 *P = reinterpret_cast<const float4 *>(a.ppos);
+  const float *Qt = a.qt;
+  const size_t N = a.N;
+  for (int tb = 0; tb < cnt; tb += 32) {
+    const float4 x0 = P[psta + min(tb + ti, cnt - 1)];
+    const float4 x1 = P[psta + min(tb + 16 + ti, cnt - 1)];
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < nn; b++) {
+      const int B = nb[b];
+      const int js = a.pad_sta[B] + kq, jn = a.pad_cnt[B];
+      if (jn == 0) continue;
+      if (B != leaf) {
+        float4 y = P[js];
+        float qv = Qt[(size_t)js * 16 + ti];
+        for (int k = 4; k <= jn; k += 4) {
+          const float4 z = y;
+          const float bq = qv;
+          if (k < jn) {
+            y = P[js + k];
+            qv = Qt[(size_t)(js + k) * 16 + ti];
+          }
+          const float dx0 = z.x - x0.x, dy0 = z.y - x0.y, dz0 = z.z - x0.z;
+          const float dx1 = z.x - x1.x, dy1 = z.y - x1.y, dz1 = z.z - x1.z;
+          const float g0 = __builtin_amdgcn_rsqf(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
+          const float g1 = __builtin_amdgcn_rsqf(dx1 * dx1 + dy1 * dy1 + dz1 * dz1);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(g0, bq, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(g1, bq, acc1, 0, 0, 0);
+        }
+      } else {
+        for (int k = 0; k < jn; k += 4) {
+          const float4 z = P[js + k];
+          const float bq = Qt[(size_t)(js + k) * 16 + ti];
+          const float dx0 = z.x - x0.x, dy0 = z.y - x0.y, dz0 = z.z - x0.z;
+          const float dx1 = z.x - x1.x, dy1 = z.y - x1.y, dz1 = z.z - x1.z;
+          const float rr0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+          const float rr1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+          const float g0 = rr0 != 0.f ? __builtin_amdgcn_rsqf(rr0) : 0.f;
+          const float g1 = rr1 != 0.f ? __builtin_amdgcn_rsqf(rr1) : 0.f;
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(g0, bq, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(g1, bq, acc1, 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = 4 * kq + v;
+      const size_t row = (size_t)(rhs0 + ti) * N + sta;
+      if (tb + i < cnt) a.phi[row + tb + i] += a.q[row + tb + i] * acc0[v];
+      if (tb + 16 + i < cnt) a.phi[row + tb + 16 + i] += a.q[row + tb + 16 + i] * acc1[v];
+    }
+  }
+}
+
+// charges of vectors rhs0..rhs0+15, leaf order [16][N] -> padded [Npad][16]
+__global__ void k_pad_charges16(int N, const int *__restrict__ pad_of, const float *__restrict__ q,
+                                float *__restrict__ qt)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  float4 *o = reinterpret_cast<float4 *>(qt + (size_t)pad_of[i] * 16);
+#pragma unroll
+  for (int r = 0; r < 16; r += 4)
+    o[r / 4] = make_float4(q[(size_t)r * N + i], q[(size_t)(r + 1) * N + i], q[(size_t)(r + 2) * N + i],
+                           q[(size_t)(r + 3) * N + i]);
+}
+#endif
+
 int launch_p2p(const P2PArgs &a, hipStream_t s)
 {
   if (a.nleaf <= 0 || a.N <= 0) return BBFMM_OK;
@@ -199,9 +417,12 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
     const int left = a.nrhs - rhs;
     if (left >= 16) {
 #ifdef SINGLE
-      if (a.use_mfma)
-        hipLaunchKernelGGL(k_p2p_mfma16, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
-      else
+      if (a.use_mfma && a.qt && a.ppos) {
+        hipLaunchKernelGGL(k_pad_charges16, dim3((a.N + 255) / 256), dim3(256), 0, s, a.N, a.pad_of,
+                           a.q + (size_t)rhs * a.N, a.qt);
+        BB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_p2p_pad16, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+      } else
 #endif
         hipLaunchKernelGGL(k_p2p<16>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
       rhs += 16;

@@ -41,12 +41,14 @@ __global__ void k_gather(int N, const int *__restrict__ perm, const real *__rest
   if (i < N) dst[off + i] = src[off + perm[i]];
 }
 
-__global__ void k_scatter(int N, const int *__restrict__ perm, const real *__restrict__ src,
+// dst (user order) from src (leaf order) through the inverse permutation:
+// reads are scattered, writes coalesced.
+__global__ void k_scatter(int N, const int *__restrict__ iperm, const real *__restrict__ src,
                           real *__restrict__ dst)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const size_t off = (size_t)blockIdx.y * N;
-  if (i < N) dst[off + perm[i]] = src[off + i];
+  if (j < N) dst[off + j] = src[off + iperm[j]];
 }
 
 // M = U^T (Kw .* S) for one cell; sw holds Kw .* S in LDS.
@@ -63,26 +65,39 @@ __device__ __forceinline__ void project_u(const real *__restrict__ sw, const rea
   }
 }
 
-// P2M at the leaf level (anotherUpwardPass.c:61-330): one block per leaf,
-// one thread per Chebyshev node (NN=8: two nodes per thread).
-template <int NN, int BLOCK>
+// Vectors per block: NR right-hand sides share the particle geometry and
+// the interpolation weights; each thread keeps NR accumulators. Rows of the
+// [item][NR] staging arrays in LDS are padded to QS(NR) entries.
+template <int NR>
+__host__ __device__ constexpr int row_stride()
+{
+  return NR == 1 ? 1 : NR + 4;
+}
+
+// P2M at the leaf level (anotherUpwardPass.c:61-330): one block per leaf and
+// group of NR vectors, one thread per Chebyshev node (n=8: two nodes per
+// thread); S = sum_j q_j Sx Sy Sz, then M = U^T (Kw .* S).
+template <int NN, int BLOCK, int NR>
 __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
 {
   constexpr int N3 = NN * NN * NN;
   constexpr int NPT = N3 / BLOCK;
   constexpr int CH = BLOCK;
+  constexpr int QS = row_stride<NR>();
   static_assert(N3 % BLOCK == 0, "nodes per thread");
-  __shared__ real sx[NN][CH], sy[NN][CH], sz[NN][CH], sq[CH];
-  __shared__ real sw[N3];
+  __shared__ real sx[NN][CH], sy[NN][CH], sz[NN][CH];
+  __shared__ __align__(16) real sq[CH * QS];
+  __shared__ __align__(16) real sw[N3 * NR];
   const int leaf = blockIdx.x;
-  const int rhs = blockIdx.y;
+  const int rhs0 = blockIdx.y * NR;
   const int A = a.leaf0 + leaf;
   const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
   const real4 c = a.center[A];
-  const real *q = a.q + (size_t)rhs * a.N;
-  real acc[NPT];
+  real acc[NPT][NR];
 #pragma unroll
-  for (int p = 0; p < NPT; p++) acc[p] = 0;
+  for (int p = 0; p < NPT; p++)
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[p][k] = 0;
   for (int base = 0; base < cnt; base += CH) {
     const int nk = min(CH, cnt - base);
     if ((int)threadIdx.x < nk) {
@@ -97,29 +112,53 @@ __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
       cheb_row<NN>(a.Tkz, a.ihalfL * (x.z - c.z), t);
 #pragma unroll
       for (int m = 0; m < NN; m++) sz[m][threadIdx.x] = t[m];
-      sq[threadIdx.x] = q[sta + base + threadIdx.x];
+    }
+    for (int o = threadIdx.x; o < NR * CH; o += BLOCK) {
+      const int k = o / CH, j = o % CH;
+      if (j < nk) sq[j * QS + k] = a.q[(size_t)(rhs0 + k) * a.N + sta + base + j];
     }
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
       const int l = threadIdx.x + p * BLOCK;
       const int l1 = l / (NN * NN), l2 = (l / NN) % NN, l3 = l % NN;
-      real s = acc[p];
-      for (int j = 0; j < nk; j++) s += sq[j] * sx[l1][j] * sy[l2][j] * sz[l3][j];
-      acc[p] = s;
+      for (int j = 0; j < nk; j++) {
+        const real w = sx[l1][j] * sy[l2][j] * sz[l3][j];
+#pragma unroll
+        for (int k = 0; k < NR; k++) acc[p][k] += sq[j * QS + k] * w;
+      }
     }
     __syncthreads();
   }
-  real *S = a.S + (size_t)rhs * a.ncell * N3 + (size_t)A * N3;
 #pragma unroll
   for (int p = 0; p < NPT; p++) {
     const int l = threadIdx.x + p * BLOCK;
-    const real v = a.prefac3 * acc[p];
-    S[l] = v;
-    sw[l] = v * a.Kw[l];
+    const real kw = a.Kw[l];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const real v = a.prefac3 * acc[p][k];
+      a.S[(size_t)(rhs0 + k) * a.ncell * N3 + (size_t)A * N3 + l] = v;
+      sw[l * NR + k] = v * kw;
+    }
   }
   __syncthreads();
-  project_u<NN, BLOCK>(sw, a.Urm, a.r, a.M + (size_t)rhs * a.ncell * a.r + (size_t)A * a.r);
+  // M[k][i] = sum_l U[l][i] sw[l][k]; a thread per (i, group of RS vectors)
+  constexpr int RS = NR >= 8 ? NR / 2 : NR, NG = NR / RS;
+  for (int o = threadIdx.x; o < a.r * NG; o += BLOCK) {
+    const int i = o % a.r, g = o / a.r;
+    real m[RS];
+#pragma unroll
+    for (int k = 0; k < RS; k++) m[k] = 0;
+#pragma unroll 4
+    for (int l = 0; l < N3; l++) {
+      const real u = a.Urm[(size_t)l * a.r + i];
+#pragma unroll
+      for (int k = 0; k < RS; k++) m[k] += u * sw[l * NR + g * RS + k];
+    }
+#pragma unroll
+    for (int k = 0; k < RS; k++)
+      a.M[(size_t)(rhs0 + g * RS + k) * a.ncell * a.r + (size_t)A * a.r + i] = m[k];
+  }
 }
 
 // M2M for one level (anotherUpwardPass.c:345-560): one block per parent,
@@ -184,27 +223,37 @@ __global__ void __launch_bounds__(BLOCK) k_m2m(PassArgs a, int Afirst)
 }
 
 // post-M2L (auxAnotherFMMInteraction.c:398-416, scale per level as
-// anotherFMMInteraction.cu:89-113): F = (1/L_l)^homogen Kw .* (U L).
-template <int NN, int BLOCK>
+// anotherFMMInteraction.cu:89-113): F = (1/L_l)^homogen Kw .* (U L), one
+// block per cell and group of NR vectors, a thread per node.
+template <int NN, int BLOCK, int NR>
 __global__ void __launch_bounds__(BLOCK) k_postm2l(PassArgs a, int Afirst)
 {
   constexpr int N3 = NN * NN * NN;
-  extern __shared__ unsigned char smem_raw[];
-  real *sl = reinterpret_cast<real *>(smem_raw);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  real *sl = reinterpret_cast<real *>(smem_raw);  // [r][NR]
   const int A = Afirst + blockIdx.x;
-  const int rhs = blockIdx.y;
+  const int rhs0 = blockIdx.y * NR;
   int level = 0;
   while (level < a.maxlev && A >= a.levsta[level + 1]) level++;
   const real scale = a.scale[level];
-  const real *Lv = a.L + (size_t)rhs * a.ncell * a.r + (size_t)A * a.r;
-  for (int i = threadIdx.x; i < a.r; i += BLOCK) sl[i] = Lv[i];
+  for (int o = threadIdx.x; o < a.r * NR; o += BLOCK) {
+    const int k = o / a.r, i = o % a.r;
+    sl[i * NR + k] = a.L[(size_t)(rhs0 + k) * a.ncell * a.r + (size_t)A * a.r + i];
+  }
   __syncthreads();
-  real *F = a.F + (size_t)rhs * a.ncell * N3 + (size_t)A * N3;
   for (int l = threadIdx.x; l < N3; l += BLOCK) {
-    real s = 0;
-#pragma unroll 8
-    for (int i = 0; i < a.r; i++) s += a.Ucm[l + (size_t)N3 * i] * sl[i];
-    F[l] = scale * a.Kw[l] * s;
+    real s[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) s[k] = 0;
+#pragma unroll 4
+    for (int i = 0; i < a.r; i++) {
+      const real u = a.Ucm[l + (size_t)N3 * i];
+#pragma unroll
+      for (int k = 0; k < NR; k++) s[k] += u * sl[i * NR + k];
+    }
+    const real f = scale * a.Kw[l];
+#pragma unroll
+    for (int k = 0; k < NR; k++) a.F[(size_t)(rhs0 + k) * a.ncell * N3 + (size_t)A * N3 + l] = f * s[k];
   }
 }
 
@@ -281,6 +330,62 @@ __global__ void __launch_bounds__(256) k_l2p(PassArgs a)
   a.phi[o] = a.q[o] * a.prefac3 * sum;
 }
 
+// L2P for NR vectors: one block per leaf, the leaf's F values in LDS
+// ([node][NR]); lanes split into NG groups of RS vectors, a lane per
+// particle within a group.
+template <int NN, int NR>
+__global__ void __launch_bounds__(64) k_l2p_leaf(PassArgs a)
+{
+  constexpr int N3 = NN * NN * NN;
+  constexpr int RS = NR >= 8 ? NR / 2 : NR, NG = NR / RS, PT = 64 / NG;
+  __shared__ __align__(16) real sf[N3 * NR];
+  const int leaf = blockIdx.x;
+  const int rhs0 = blockIdx.y * NR;
+  const int A = a.leaf0 + leaf;
+  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
+  for (int o = threadIdx.x; o < N3 * NR; o += 64) {
+    const int k = o / N3, l = o % N3;
+    sf[l * NR + k] = a.F[(size_t)(rhs0 + k) * a.ncell * N3 + (size_t)A * N3 + l];
+  }
+  __syncthreads();
+  const real4 c = a.center[A];
+  const int g = threadIdx.x / PT, jl = threadIdx.x % PT;
+  for (int j = jl; j < cnt; j += PT) {
+    const real4 x = a.pos[sta + j];
+    real sxv[NN], syv[NN], szv[NN];
+    cheb_row<NN>(a.Tkz, a.ihalfL * (x.x - c.x), sxv);
+    cheb_row<NN>(a.Tkz, a.ihalfL * (x.y - c.y), syv);
+    cheb_row<NN>(a.Tkz, a.ihalfL * (x.z - c.z), szv);
+    real sum[RS];
+#pragma unroll
+    for (int k = 0; k < RS; k++) sum[k] = 0;
+    // kept rolled: unrolled, the LDS reads are hoisted out of the particle
+    // loop and the kernel runs out of registers
+#pragma unroll 1
+    for (int l1 = 0; l1 < NN; l1++)
+#pragma unroll(NN > 4 ? 2 : NN)
+      for (int l2 = 0; l2 < NN; l2++) {
+        real inner[RS];
+#pragma unroll
+        for (int k = 0; k < RS; k++) inner[k] = 0;
+#pragma unroll
+        for (int l3 = 0; l3 < NN; l3++) {
+          const real *f = &sf[((l1 * NN + l2) * NN + l3) * NR + g * RS];
+#pragma unroll
+          for (int k = 0; k < RS; k++) inner[k] += szv[l3] * f[k];
+        }
+        const real w = sxv[l1] * syv[l2];
+#pragma unroll
+        for (int k = 0; k < RS; k++) sum[k] += w * inner[k];
+      }
+#pragma unroll
+    for (int k = 0; k < RS; k++) {
+      const size_t o = (size_t)(rhs0 + g * RS + k) * a.N + sta + j;
+      a.phi[o] = a.q[o] * a.prefac3 * sum[k];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s)
 {
@@ -290,21 +395,39 @@ int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, 
   return BBFMM_OK;
 }
 
-int launch_scatter(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s)
+int launch_scatter(const int *iperm, const real *src, real *dst, int N, int nrhs, hipStream_t s)
 {
   if (N <= 0) return BBFMM_OK;
-  hipLaunchKernelGGL(k_scatter, dim3((N + 255) / 256, nrhs), dim3(256), 0, s, N, perm, src, dst);
+  hipLaunchKernelGGL(k_scatter, dim3((N + 255) / 256, nrhs), dim3(256), 0, s, N, iperm, src, dst);
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
+}
+
+// largest group of vectors per block that divides nrhs
+static int vec_group(int nrhs)
+{
+  return nrhs % 16 == 0 ? 16 : nrhs % 4 == 0 ? 4 : 1;
+}
+
+template <int NN, int BLOCK>
+static void p2m_t(const PassArgs &a, hipStream_t s, int nrhs)
+{
+  const int nr = vec_group(nrhs);
+  if (nr == 16)
+    hipLaunchKernelGGL((k_p2m<NN, BLOCK, 16>), dim3(a.nleaf, nrhs / 16), dim3(BLOCK), 0, s, a);
+  else if (nr == 4)
+    hipLaunchKernelGGL((k_p2m<NN, BLOCK, 4>), dim3(a.nleaf, nrhs / 4), dim3(BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_p2m<NN, BLOCK, 1>), dim3(a.nleaf, nrhs), dim3(BLOCK), 0, s, a);
 }
 
 int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs)
 {
   if (a.nleaf <= 0) return BBFMM_OK;
   if (a.n == 4)
-    hipLaunchKernelGGL((k_p2m<4, 64>), dim3(a.nleaf, nrhs), dim3(64), 0, s, a);
+    p2m_t<4, 64>(a, s, nrhs);
   else if (a.n == 8)
-    hipLaunchKernelGGL((k_p2m<8, 256>), dim3(a.nleaf, nrhs), dim3(256), 0, s, a);
+    p2m_t<8, 256>(a, s, nrhs);
   else
     return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
   BB_HIP(hipGetLastError());
@@ -331,8 +454,14 @@ static int postm2l_t(const PassArgs &a, const int *h_levsta, hipStream_t s, int 
 {
   const int first = h_levsta[kMinLev], count = a.ncell - first;
   if (count <= 0) return BBFMM_OK;
-  hipLaunchKernelGGL((k_postm2l<NN, BLOCK>), dim3(count, nrhs), dim3(BLOCK), sizeof(real) * a.r, s, a,
-                     first);
+  const int nr = vec_group(nrhs);
+  const size_t sh = sizeof(real) * a.r * nr;
+  if (nr == 16)
+    hipLaunchKernelGGL((k_postm2l<NN, BLOCK, 16>), dim3(count, nrhs / 16), dim3(BLOCK), sh, s, a, first);
+  else if (nr == 4)
+    hipLaunchKernelGGL((k_postm2l<NN, BLOCK, 4>), dim3(count, nrhs / 4), dim3(BLOCK), sh, s, a, first);
+  else
+    hipLaunchKernelGGL((k_postm2l<NN, BLOCK, 1>), dim3(count, nrhs), dim3(BLOCK), sh, s, a, first);
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }
@@ -360,13 +489,25 @@ int launch_l2l(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs)
   return BBFMM_OK;
 }
 
+template <int NN>
+static void l2p_t(const PassArgs &a, hipStream_t s, int nrhs)
+{
+  const int nr = vec_group(nrhs);
+  if (nr == 16)
+    hipLaunchKernelGGL((k_l2p_leaf<NN, 16>), dim3(a.nleaf, nrhs / 16), dim3(64), 0, s, a);
+  else if (nr == 4)
+    hipLaunchKernelGGL((k_l2p_leaf<NN, 4>), dim3(a.nleaf, nrhs / 4), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_l2p<NN>), dim3((a.N + 255) / 256, nrhs), dim3(256), 0, s, a);
+}
+
 int launch_l2p(const PassArgs &a, hipStream_t s, int nrhs)
 {
   if (a.N <= 0) return BBFMM_OK;
   if (a.n == 4)
-    hipLaunchKernelGGL((k_l2p<4>), dim3((a.N + 255) / 256, nrhs), dim3(256), 0, s, a);
+    l2p_t<4>(a, s, nrhs);
   else if (a.n == 8)
-    hipLaunchKernelGGL((k_l2p<8>), dim3((a.N + 255) / 256, nrhs), dim3(256), 0, s, a);
+    l2p_t<8>(a, s, nrhs);
   else
     return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
   BB_HIP(hipGetLastError());
