@@ -77,7 +77,8 @@ struct bbfmm_amd_plan {
   real *d_farfield = nullptr;                  // far-field q.*phi of the last solve (vector 0)
   real *d_qt = nullptr;                        // [Npad][16] charges, source-major (MFMA P2P)
   real4 *d_ppos = nullptr;                     // padded particle copy for P2P
-  int *d_pad_sta = nullptr, *d_pad_cnt = nullptr, *d_pad_of = nullptr;
+  int2 *d_seg = nullptr;
+  int *d_nseg = nullptr, *d_pad_of = nullptr;
   int npad = 0;
   std::vector<real> h_K;                       // column-major operators (for the grid form)
   bb::GridM2L *grid = nullptr;
@@ -132,7 +133,7 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_nbr, p->d_nnbr, p->d_iinter, p->d_inter, p->d_kidx, p->d_Ktable, p->d_levsta,
                   p->d_Ucm, p->d_Urm, p->d_Tkz, p->d_Kw, p->d_Cw, p->d_scale, p->d_q, p->d_phi,
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
-                  p->d_ppos, p->d_pad_sta, p->d_pad_cnt, p->d_pad_of, p->d_iperm};
+                  p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->grid) grid_m2l_destroy(p->grid);
@@ -334,9 +335,20 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
       x.w = 0;
     }
     for (int i = 0; i < N; i++) ppos[pad_of[i]] = hpos[i];
+    std::vector<int2> seg((size_t)nleaf * 27, make_int2(0, 0));
+    std::vector<int> nseg((size_t)nleaf, 0);
+    for (int k = 0; k < nleaf; k++) {
+      int m = 0;
+      if (pcnt[k] > 0) seg[(size_t)k * 27 + m++] = make_int2(psta[k], pcnt[k]);
+      for (int b = 0; b < nnbr[k]; b++) {
+        const int B = nbr[(size_t)k * 27 + b];
+        if (B != k && pcnt[B] > 0) seg[(size_t)k * 27 + m++] = make_int2(psta[B], pcnt[B]);
+      }
+      nseg[k] = m;
+    }
     BB_TRY(upload(&p->d_ppos, ppos.data(), ppos.size(), stream));
-    BB_TRY(upload(&p->d_pad_sta, psta.data(), psta.size(), stream));
-    BB_TRY(upload(&p->d_pad_cnt, pcnt.data(), pcnt.size(), stream));
+    BB_TRY(upload(&p->d_seg, seg.data(), seg.size(), stream));
+    BB_TRY(upload(&p->d_nseg, nseg.data(), nseg.size(), stream));
     BB_TRY(upload(&p->d_pad_of, pad_of.data(), pad_of.size(), stream));
   }
   const size_t NR = (size_t)N * nrhs;
@@ -461,8 +473,8 @@ static int solve_sorted(bbfmm_amd_plan *p)
   pp.phi = p->d_phi;
   pp.use_mfma = p->form != BBFMM_M2L_PLAIN;
   pp.ppos = p->d_ppos;
-  pp.pad_sta = p->d_pad_sta;
-  pp.pad_cnt = p->d_pad_cnt;
+  pp.seg = p->d_seg;
+  pp.nseg = p->d_nseg;
   pp.pad_of = p->d_pad_of;
 #ifdef SINGLE
   if (pp.use_mfma && p->nrhs >= 16) {

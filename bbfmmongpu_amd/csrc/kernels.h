@@ -71,7 +71,8 @@ struct P2PArgs {
   // padded particle copy (every leaf padded to a multiple of 4 with far-away,
   // zero-charge dummies); kernels fall back to the unpadded ones when null
   const real4 *ppos = nullptr;  // [Npad]
-  const int *pad_sta = nullptr, *pad_cnt = nullptr;  // [nleaf]
+  const int2 *seg = nullptr;    // [nleaf][27] (padded start, padded count), self first, empties dropped
+  const int *nseg = nullptr;    // [nleaf]
   const int *pad_of = nullptr;  // [N] leaf-order index -> padded index
   float *qt = nullptr;          // [Npad][16] charges of 16 vectors, source-major
 };

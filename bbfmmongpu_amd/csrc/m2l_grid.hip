@@ -48,11 +48,15 @@ struct GridArgs {
   const void *Mg1;  // grids (lo) for BF16X2
   real *L;
   int wave_cols;    // columns per wave
-  const int *steps;  // [208][10]: source class, packed shift, K index per field class (-1: near)
+  const int *steps;  // [208][kStepWords] step records (step_table)
   int ss;            // 1: source-stationary kernel (block = columns of all 8 classes)
 };
 
 constexpr int kSteps = 208;  // 8 source classes x 26 non-zero parent shifts
+constexpr int kStepWords = 8;  // words per step record (see step_table)
+constexpr int kZeroOp = kNumOps;  // all-zero operator slot of the split-BF16 layout
+// s_waitcnt immediate (gfx9 encoding) for vmcnt(0), expcnt and lgkmcnt not waited on
+constexpr int kWaitVm0 = 0x0F70;
 
 struct GridM2L {
   int r = 0, maxlev = 0, nrhs = 1, form = 0, ncell_cap = 0;
@@ -262,13 +266,20 @@ __global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
   __shared__ bf16x8 sK[2][8][4][64];  // [buffer][field class][kstep*2 + hi/lo][lane]
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  // Blocks are dealt to the 8 XCDs round-robin (block b on XCD b % 8); the
+  // logical order gives each XCD a contiguous run of column groups, i.e.
+  // neighbouring parents, whose shifted source tiles then share its L2.
+  const int nb = g.blk0[g.maxlev + 1];
+  const int lb = (blockIdx.x & 7) * ((nb + 7) >> 3) + (blockIdx.x >> 3);
+  if (lb >= nb) return;
   int l = g.minlev;
-  while (l < g.maxlev && (int)blockIdx.x >= g.blk0[l + 1]) l++;
-  const int grp = blockIdx.x - g.blk0[l];
+  while (l < g.maxlev && lb >= g.blk0[l + 1]) l++;
+  const int grp = lb - g.blk0[l];
   const int P = 1 << (l - 1);
   const int PP = P + 2;
   const int ncols = P * P * P * g.nrhs;
   const int half = lane >> 5;
+  // columns past the end compute a copy of column 0 and are not stored
   const int col = (grp * WAVES + wave) * 32 + (lane & 31);
   const bool valid = col < ncols;
   const int cc = valid ? col : 0;
@@ -278,7 +289,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
   const int base = ((px + 1) * PP + py + 1) * PP + pz + 1;
   const int slot0 = g.lvl_slot0[l];
   const int cls = PP * PP * PP;
-  const bf16x8 *Kb = (const bf16x8 *)g.Ka;
+  const bf16x8 *Kl = (const bf16x8 *)g.Ka + wave * 64 + lane;
   const __bf16 *Mh = (const __bf16 *)g.Mg0;
   const __bf16 *Ml = (const __bf16 *)g.Mg1;
 
@@ -288,79 +299,103 @@ __global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
 #pragma unroll
     for (int e = 0; e < 16; e++) acc[c][e] = 0.0f;
 
-  // Each thread moves 8 operator words per step (class c = i, k-part =
-  // its wave, lane = its lane); they are loaded one step ahead into
-  // registers and written to the other LDS buffer after the step's MFMAs.
+  // Step record: e[0] = source class | packed shift << 3, e[1 + i/2] holds the
+  // operator of field classes i (low 16 bits) and i + 1 (high); near-field
+  // pairs point at the zero operator, so every step runs all 8 classes.
+  // Each wave moves its k-part of the step's 8 operators (1 KB each) from
+  // global memory straight into the other LDS buffer (global_load_lds, 16
+  // bytes per lane at lane-contiguous addresses) while the MFMAs of the
+  // current step run; the source tile of the next step is loaded into
+  // registers at the same time.  Records are read two steps ahead.
   static_assert(WAVES == 4, "one k-part per wave");
-  auto load_k = [&](int st, bf16x8 *kr) {
-    const int *e = g.steps + st * 10;
+  typedef __attribute__((address_space(3))) void lds_void;
+  auto load_k = [&](const int *e, int buf) {
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      const int k = e[2 + i];
-      if (k >= 0) kr[i] = Kb[((size_t)k * 4 + wave) * 64 + lane];
+      const int k = (e[1 + (i >> 1)] >> (16 * (i & 1))) & 0xffff;
+      __builtin_amdgcn_global_load_lds((const void *)(Kl + (size_t)k * 256), (lds_void *)&sK[buf][i][wave][0], 16,
+                                       0, 0);
     }
   };
-  auto store_k = [&](int st, int b, const bf16x8 *kr) {
-    const int *e = g.steps + st * 10;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-      if (e[2 + i] >= 0) sK[b][i][wave][lane] = kr[i];
-  };
-  auto src_off = [&](int st) {
-    const int *e = g.steps + st * 10;
-    const int sc = e[0];
-    const int dx = ((e[1] >> 4) & 3) - 1, dy = ((e[1] >> 2) & 3) - 1, dz = (e[1] & 3) - 1;
+  auto src_off = [&](const int *e) {
+    const int sc = e[0] & 7, pk = e[0] >> 3;
+    const int dx = ((pk >> 4) & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = (pk & 3) - 1;
     return ((size_t)(slot0 + sc * cls + (dx * PP + dy) * PP + dz + base) * g.nrhs + rh) * 32 + half * 8;
   };
-  bf16x8 kr[8];
-  load_k(0, kr);
-  store_k(0, 0, kr);
+  const int *S = g.steps;
+  int e1[5];
   bf16x8 bh0, bl0, bh1, bl1;
   {
-    const size_t o = src_off(0);
+    int e0[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) e0[i] = S[i];
+#pragma unroll
+    for (int i = 0; i < 5; i++) e1[i] = S[kStepWords + i];
+    load_k(e0, 0);
+    const size_t o = src_off(e0);
     bh0 = *(const bf16x8 *)(Mh + o);
     bl0 = *(const bf16x8 *)(Ml + o);
     bh1 = *(const bf16x8 *)(Mh + o + 16);
     bl1 = *(const bf16x8 *)(Ml + o + 16);
   }
+  __builtin_amdgcn_s_waitcnt(kWaitVm0);
   __syncthreads();
   for (int st = 0; st < kSteps; st++) {
     const int b = st & 1;
-    const int *e = g.steps + st * 10;
-    const bool more = st + 1 < kSteps;
-    bf16x8 nh0, nl0, nh1, nl1;
-    if (more) {
-      load_k(st + 1, kr);
-      const size_t o = src_off(st + 1);
-      nh0 = *(const bf16x8 *)(Mh + o);
-      nl0 = *(const bf16x8 *)(Ml + o);
-      nh1 = *(const bf16x8 *)(Mh + o + 16);
-      nl1 = *(const bf16x8 *)(Ml + o + 16);
-    }
-    if (!valid) {
+    // the last step reloads its own data (harmless, never used)
+    const int n2 = st + 2 < kSteps ? st + 2 : kSteps - 1;
+    // vector loads (operators arriving by DMA keep the compiler from scalar
+    // loads here); moved to scalar registers after the end-of-step wait
+    const int4 r2a = *reinterpret_cast<const int4 *>(S + n2 * kStepWords);
+    const int r2b = S[n2 * kStepWords + 4];
+    // k-step 0 of all classes, then the k-step-1 fragments; only then the
+    // loads into the other buffer (the compiler orders LDS reads after any
+    // earlier global_load_lds), then k-step 1
+    bf16x8 ah[8], al[8];
 #pragma unroll
-      for (int q = 0; q < 8; q++) bh0[q] = bl0[q] = bh1[q] = bl1[q] = (__bf16)0.0f;
+    for (int c = 0; c < 8; c++) {
+      ah[c] = sK[b][c][0][lane];
+      al[c] = sK[b][c][1][lane];
     }
 #pragma unroll
     for (int c = 0; c < 8; c++) {
-      if (e[2 + c] < 0) continue;  // uniform: near-field pair for this class
-      const bf16x8 a0h = sK[b][c][0][lane], a0l = sK[b][c][1][lane];
-      const bf16x8 a1h = sK[b][c][2][lane], a1l = sK[b][c][3][lane];
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, bh0, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, bl0, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, bh0, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, bh1, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, bl1, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, bh1, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh0, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl0, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh0, acc[c], 0, 0, 0);
     }
-    if (more) {
-      store_k(st + 1, b ^ 1, kr);
-      bh0 = nh0;
-      bl0 = nl0;
-      bh1 = nh1;
-      bl1 = nl1;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      ah[c] = sK[b][c][2][lane];
+      al[c] = sK[b][c][3][lane];
     }
+    __builtin_amdgcn_sched_barrier(0);
+    load_k(e1, b ^ 1);
+    const size_t o = src_off(e1);
+    const bf16x8 nh0 = *(const bf16x8 *)(Mh + o);
+    const bf16x8 nl0 = *(const bf16x8 *)(Ml + o);
+    const bf16x8 nh1 = *(const bf16x8 *)(Mh + o + 16);
+    const bf16x8 nl1 = *(const bf16x8 *)(Ml + o + 16);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh1, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl1, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh1, acc[c], 0, 0, 0);
+    }
+    bh0 = nh0;
+    bl0 = nl0;
+    bh1 = nh1;
+    bl1 = nl1;
+    // the next step's operators are in LDS for every wave after this (the
+    // scheduling barrier keeps the MFMAs above the wait)
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
     __syncthreads();
+    e1[0] = __builtin_amdgcn_readfirstlane(r2a.x);
+    e1[1] = __builtin_amdgcn_readfirstlane(r2a.y);
+    e1[2] = __builtin_amdgcn_readfirstlane(r2a.z);
+    e1[3] = __builtin_amdgcn_readfirstlane(r2a.w);
+    e1[4] = __builtin_amdgcn_readfirstlane(r2b);
   }
   if (!valid) return;
 #pragma unroll
@@ -452,27 +487,29 @@ static void offset_table(std::vector<int4> &tab)
 
 // (s', d) steps of the source-stationary kernel: for every source class s'
 // and non-zero parent shift d, the operator index of each field class s
-// (offset v = 2d + s' - s), or -1 when v is a near-field offset.
+// (offset v = 2d + s' - s), or the zero operator when v is a near-field
+// offset.  Record: [0] = s' | packed shift << 3, [1 + c/2] = operators of
+// classes c (low 16 bits) and c + 1 (high 16 bits), rest unused.
 static void step_table(std::vector<int> &steps)
 {
   int Kt[343];
   ktable(Kt);
-  steps.assign(kSteps * 10, -1);
+  steps.assign(kSteps * kStepWords, 0);
   int st = 0;
   for (int sc = 0; sc < 8; sc++)
     for (int dxi = 0; dxi < 3; dxi++)
       for (int dyi = 0; dyi < 3; dyi++)
         for (int dzi = 0; dzi < 3; dzi++) {
           if (dxi == 1 && dyi == 1 && dzi == 1) continue;
-          int *e = &steps[st * 10];
-          e[0] = sc;
-          e[1] = (dxi << 4) | (dyi << 2) | dzi;
+          int *e = &steps[st * kStepWords];
+          e[0] = sc | (((dxi << 4) | (dyi << 2) | dzi) << 3);
           for (int c = 0; c < 8; c++) {
             const int vx = 2 * (dxi - 1) + ((sc >> 2) & 1) - ((c >> 2) & 1);
             const int vy = 2 * (dyi - 1) + ((sc >> 1) & 1) - ((c >> 1) & 1);
             const int vz = 2 * (dzi - 1) + (sc & 1) - (c & 1);
             const bool near = vx >= -1 && vx <= 1 && vy >= -1 && vy <= 1 && vz >= -1 && vz <= 1;
-            e[2 + c] = near ? -1 : Kt[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
+            const int k = near ? kZeroOp : Kt[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
+            e[1 + (c >> 1)] |= k << (16 * (c & 1));
           }
           st++;
         }
@@ -567,7 +604,8 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
 #ifdef SINGLE
     else if (form == BBFMM_M2L_BF16X2) {
       // [k][kstep][hi/lo][lane][8]: lane l holds row l%32, k = 16*kstep + 8*(l/32) + e
-      std::vector<__bf16> Kb((size_t)kNumOps * 2 * 2 * 64 * 8);
+      // one extra all-zero operator (kZeroOp) after the 316
+      std::vector<__bf16> Kb((size_t)(kNumOps + 1) * 2 * 2 * 64 * 8, (__bf16)0.0f);
       for (int k = 0; k < kNumOps; k++)
         for (int ks = 0; ks < 2; ks++)
           for (int lane = 0; lane < 64; lane++)
@@ -653,8 +691,8 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_BF16X2>, dim3(sb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (a.ss)
-      hipLaunchKernelGGL((k_m2l_mfma_ss<4>), dim3(g->nblocks), dim3(256), 0, s, a);
+    if (a.ss)  // rounded up to whole rounds of the 8 XCDs; the extra blocks exit
+      hipLaunchKernelGGL((k_m2l_mfma_ss<4>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else if (a.wave_cols == 64)
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
     else

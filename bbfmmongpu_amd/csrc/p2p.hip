@@ -323,74 +323,130 @@ __global__ void __launch_bounds__(64) k_p2p_mfma16(P2PArgs a, int rhs0)
 #endif
 
 // ---------------------------------------------------------------- padded layout
-// The P2P kernels below read a private copy of the particles in which every
+// The kernel below reads a private copy of the particles in which every
 // leaf's range starts at a multiple of 4 and is padded to a multiple of 4
-// with dummy sources far outside the root cell and of zero charge
-// (pad_sta/pad_cnt per leaf), so that their source loops need no tail masks.
-// Targets are read from the same copy; dummy targets are never written.
+// with dummy sources far outside the root cell and of zero charge, so that
+// its source loops need no tail masks.  seg[leaf][0..nseg) lists the
+// (padded start, padded count) of the leaf's non-empty neighbours, the leaf
+// itself first.  Targets are read from the same copy; dummy targets are
+// never written.
 
 #ifdef SINGLE
-// Sixteen vectors on MFMA over the padded layout; qt16 is [padded N][16].
-// Lane l computes G[target l%16 (two tiles)][source 4j + l/16] and feeds the
-// charges of that source for vector l%16 as the B operand.
-__global__ void __launch_bounds__(64) k_p2p_pad16(P2PArgs a, int rhs0)
+template <int NT>
+__device__ __forceinline__ void p2p16_step(const float4 &z, float bq, const float4 *x, f32x4 *acc)
+{
+  float g[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const float dx = z.x - x[t].x, dy = z.y - x[t].y, dz = z.z - x[t].z;
+    g[t] = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(g[t], bq, acc[t], 0, 0, 0);
+}
+
+// One pass over all sources for NT tiles of 16 targets starting at target
+// tb of the leaf; results added into phi (q * potential).
+template <int NT>
+__device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, int nseg, int T, int sta,
+                                           int cnt, int psta, int pself, int tb)
+{
+  const int l = threadIdx.x;
+  const int ti = l & 15, kq = l >> 4;
+  const float4 *P = reinterpret_cast<const float4 *>(a.ppos);
+  const float *Qt = a.qt;
+  const int qoff = kq * 16 + ti;
+  float4 x[NT];
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    x[t] = P[psta + min(tb + 16 * t + ti, cnt - 1)];
+    acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  for (int k = 0; k < pself; k += 4) {
+    const float4 z = P[psta + k + kq];
+    const float bq = Qt[(size_t)(psta + k) * 16 + qoff];
+    float g[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const float dx = z.x - x[t].x, dy = z.y - x[t].y, dz = z.z - x[t].z;
+      const float rr = dx * dx + dy * dy + dz * dz;
+      g[t] = rr != 0.f ? __builtin_amdgcn_rsqf(rr) : 0.f;
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(g[t], bq, acc[t], 0, 0, 0);
+  }
+  if (T > 0) {
+    int b = 1;
+    int pos = __builtin_amdgcn_readlane(sg.x, 1);
+    int end = pos + __builtin_amdgcn_readlane(sg.y, 1);
+    // next group; stays on the last group at the end (loaded, never summed)
+    auto advance = [&]() {
+      if (pos + 4 < end) {
+        pos += 4;
+      } else if (b + 1 < nseg) {
+        b++;
+        pos = __builtin_amdgcn_readlane(sg.x, b);
+        end = pos + __builtin_amdgcn_readlane(sg.y, b);
+      }
+    };
+    float4 yA = (P + pos)[kq];
+    float qA = (Qt + (size_t)pos * 16)[qoff];
+    advance();
+    for (int t = 0; t < T; t += 2) {
+      const float4 yB = (P + pos)[kq];
+      const float qB = (Qt + (size_t)pos * 16)[qoff];
+      advance();
+      p2p16_step<NT>(yA, qA, x, acc);
+      if (t + 1 >= T) break;
+      yA = (P + pos)[kq];
+      qA = (Qt + (size_t)pos * 16)[qoff];
+      advance();
+      p2p16_step<NT>(yB, qB, x, acc);
+    }
+  }
+  const size_t row = (size_t)(rhs0 + ti) * a.N + sta;
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = tb + 16 * t + 4 * kq + v;
+      if (i < cnt) a.phi[row + i] += a.q[row + i] * acc[t][v];
+    }
+}
+
+// Sixteen vectors on MFMA; qt is [padded N][16].  Lane l computes
+// G[target l%16 of each 16-target tile][source 4j + l/16] and feeds the
+// charges of that source for vector l%16 as the B operand.  The self leaf is
+// summed first with the r != 0 exclusion; the other neighbours form one
+// flattened sequence of 4-source groups, loaded one group ahead into two
+// alternating register sets.  Up to four target tiles share a pass.
+__global__ void __launch_bounds__(64) k_p2p_seg16(P2PArgs a, int rhs0)
 {
   const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
   const int l = threadIdx.x;
-  const int ti = l & 15, kq = l >> 4;
   const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
-  const int psta = a.pad_sta[leaf];
-  const int nn = a.nnbr[leaf];
-  const int *nb = a.nbr + (size_t)leaf * 27;
-  const float4 *P = reinterpret_cast<const float4 *>(a.ppos);
-  const float *Qt = a.qt;
-  const size_t N = a.N;
-  for (int tb = 0; tb < cnt; tb += 32) {
-    const float4 x0 = P[psta + min(tb + ti, cnt - 1)];
-    const float4 x1 = P[psta + min(tb + 16 + ti, cnt - 1)];
-    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    for (int b = 0; b < nn; b++) {
-      const int B = nb[b];
-      const int js = a.pad_sta[B] + kq, jn = a.pad_cnt[B];
-      if (jn == 0) continue;
-      if (B != leaf) {
-        float4 y = P[js];
-        float qv = Qt[(size_t)js * 16 + ti];
-        for (int k = 4; k <= jn; k += 4) {
-          const float4 z = y;
-          const float bq = qv;
-          if (k < jn) {
-            y = P[js + k];
-            qv = Qt[(size_t)(js + k) * 16 + ti];
-          }
-          const float dx0 = z.x - x0.x, dy0 = z.y - x0.y, dz0 = z.z - x0.z;
-          const float dx1 = z.x - x1.x, dy1 = z.y - x1.y, dz1 = z.z - x1.z;
-          const float g0 = __builtin_amdgcn_rsqf(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
-          const float g1 = __builtin_amdgcn_rsqf(dx1 * dx1 + dy1 * dy1 + dz1 * dz1);
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(g0, bq, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(g1, bq, acc1, 0, 0, 0);
-        }
-      } else {
-        for (int k = 0; k < jn; k += 4) {
-          const float4 z = P[js + k];
-          const float bq = Qt[(size_t)(js + k) * 16 + ti];
-          const float dx0 = z.x - x0.x, dy0 = z.y - x0.y, dz0 = z.z - x0.z;
-          const float dx1 = z.x - x1.x, dy1 = z.y - x1.y, dz1 = z.z - x1.z;
-          const float rr0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
-          const float rr1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-          const float g0 = rr0 != 0.f ? __builtin_amdgcn_rsqf(rr0) : 0.f;
-          const float g1 = rr1 != 0.f ? __builtin_amdgcn_rsqf(rr1) : 0.f;
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(g0, bq, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(g1, bq, acc1, 0, 0, 0);
-        }
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int i = 4 * kq + v;
-      const size_t row = (size_t)(rhs0 + ti) * N + sta;
-      if (tb + i < cnt) a.phi[row + tb + i] += a.q[row + tb + i] * acc0[v];
-      if (tb + 16 + i < cnt) a.phi[row + tb + 16 + i] += a.q[row + tb + 16 + i] * acc1[v];
+  if (cnt == 0) return;
+  const int nseg = a.nseg[leaf];
+  const int2 sg = l < nseg ? a.seg[(size_t)leaf * 27 + l] : make_int2(0, 0);
+  const int psta = __builtin_amdgcn_readfirstlane(sg.x);
+  const int pself = __builtin_amdgcn_readfirstlane(sg.y);
+  int T = 0;
+  for (int b = 1; b < nseg; b++) T += __builtin_amdgcn_readlane(sg.y, b) >> 2;
+  for (int tb = 0; tb < cnt;) {
+    const int tiles = (cnt - tb + 15) >> 4;
+    if (tiles >= 4) {
+      p2p16_pass<4>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      tb += 64;
+    } else if (tiles == 3) {
+      p2p16_pass<3>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      tb += 48;
+    } else if (tiles == 2) {
+      p2p16_pass<2>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      tb += 32;
+    } else {
+      p2p16_pass<1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      tb += 16;
     }
   }
 }
@@ -417,11 +473,11 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
     const int left = a.nrhs - rhs;
     if (left >= 16) {
 #ifdef SINGLE
-      if (a.use_mfma && a.qt && a.ppos) {
+      if (a.use_mfma && a.qt && a.ppos && a.seg) {
         hipLaunchKernelGGL(k_pad_charges16, dim3((a.N + 255) / 256), dim3(256), 0, s, a.N, a.pad_of,
                            a.q + (size_t)rhs * a.N, a.qt);
         BB_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_p2p_pad16, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+        hipLaunchKernelGGL(k_p2p_seg16, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
       } else
 #endif
         hipLaunchKernelGGL(k_p2p<16>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
