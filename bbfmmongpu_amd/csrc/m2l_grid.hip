@@ -411,6 +411,137 @@ __global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
   }
 }
 
+// ---------------------------------------------------------------- r = 256 FP32 MFMA kernel
+// 256 threads = 4 waves, all of one class s at one level; each wave owns 32
+// columns and all 256 rows (8 row tiles of v_mfma_f32_32x32x2f32).  For each
+// of the class's 189 offsets v, K_v streams through LDS in 8 chunks of 32
+// k-values (32 KB, layout [row tile][lane][16]: lane l holds row l%32 of the
+// tile and k = 16*(l/32) + 0..15 of the chunk), double-buffered and loaded
+// by LDS-DMA one chunk ahead; the lane's B values are the 16 consecutive
+// coefficients k = 16*(l/32) + 0..15 of its source column.
+constexpr int kR256Chunks = 8;  // 256 / 32
+
+__global__ void __launch_bounds__(256) k_m2l_fp32_r256(GridArgs g)
+{
+  __shared__ f32x4 sK[2][8][64][4];  // [buffer][row tile][lane][16 floats]
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  // XCD-aware order: each XCD takes a contiguous run of (column group, class)
+  const int nb = g.blk0[g.maxlev + 1];
+  const int lb = (blockIdx.x & 7) * ((nb + 7) >> 3) + (blockIdx.x >> 3);
+  if (lb >= nb) return;
+  int l, s, grp;
+  decode_block(g, lb, l, s, grp);
+  const int P = 1 << (l - 1);
+  const int PP = P + 2;
+  const int ncols = P * P * P * g.nrhs;
+  const int half = lane >> 5;
+  // columns past the end compute a copy of column 0 and are not stored
+  const int col = (grp * 4 + wave) * 32 + (lane & 31);
+  const bool valid = col < ncols;
+  const int cc = valid ? col : 0;
+  const int pl = cc / g.nrhs;
+  const int rh = cc - pl * g.nrhs;
+  const int pz = pl % P, py = (pl / P) % P, px = pl / (P * P);
+  const int base = ((px + 1) * PP + py + 1) * PP + pz + 1;
+  const int slot0 = g.lvl_slot0[l];
+  const int cls = PP * PP * PP;
+  const float *Mf = (const float *)g.Mg0;
+  const int4 *tab = g.tab + s * kOffsets;
+  // chunk (v, kc) of the operator stream: 32 KB at Ka + (k * 8 + kc) * 32 KB
+  const f32x4 *Kq = (const f32x4 *)g.Ka;
+  typedef __attribute__((address_space(3))) void lds_void;
+  auto load_chunk = [&](int k, int kc, int buf) {
+    const f32x4 *src = Kq + ((size_t)k * kR256Chunks + kc) * 2048 + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      __builtin_amdgcn_global_load_lds((const void *)(src + i * 256),
+                                       (lds_void *)((f32x4 *)&sK[buf][0][0][0] + i * 256 + wave * 64), 16, 0, 0);
+  };
+  auto src_col = [&](int4 e) {
+    const int dx = ((e.z >> 4) & 3) - 1, dy = ((e.z >> 2) & 3) - 1, dz = (e.z & 3) - 1;
+    const int off = slot0 + e.y * cls + (dx * PP + dy) * PP + dz;
+    return ((size_t)(off + base) * g.nrhs + rh) * 256 + half * 16;
+  };
+  auto load_b = [&](size_t o, int kc, f32x4 *bq) {
+    const f32x4 *bp = (const f32x4 *)(Mf + o + kc * 32);
+#pragma unroll
+    for (int q = 0; q < 4; q++) bq[q] = bp[q];
+  };
+
+  f32x16 acc[8];
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[t][e] = 0.0f;
+
+  int4 e = tab[0];
+  size_t o = src_col(e);
+  f32x4 bq[4];
+  load_chunk(e.x, 0, 0);
+  load_b(o, 0, bq);
+  __builtin_amdgcn_s_waitcnt(kWaitVm0);
+  __syncthreads();
+  const int nsteps = kOffsets * kR256Chunks;
+  for (int st = 0; st < nsteps; st++) {
+    const int b = st & 1;
+    const int kc = st & (kR256Chunks - 1);
+    // next chunk: same offset, next k-chunk, or the next offset
+    const int sn = st + 1 < nsteps ? st + 1 : st;
+    const int kcn = sn & (kR256Chunks - 1);
+    const int4 en = tab[sn / kR256Chunks];
+    const size_t on = kcn == 0 ? src_col(en) : o;
+    // row tiles 0-3, then the fragments of tiles 4-7, then the loads of the
+    // next chunk (LDS reads must precede the DMA into the other buffer)
+    f32x4 a[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) a[t][q] = sK[b][t][lane][q];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t][q][w], bq[q][w], acc[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) a[t][q] = sK[b][4 + t][lane][q];
+    __builtin_amdgcn_sched_barrier(0);
+    f32x4 bn[4];
+    load_chunk(en.x, kcn, b ^ 1);
+    load_b(on, kcn, bn);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+          acc[4 + t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t][q][w], bq[q][w], acc[4 + t], 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) bq[q] = bn[q];
+    o = on;
+    (void)kc;
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    __syncthreads();
+  }
+  if (!valid) return;
+  const int fx = 2 * px + ((s >> 2) & 1), fy = 2 * py + ((s >> 1) & 1), fz = 2 * pz + (s & 1);
+  const int A = g.levsta[l] + morton3(fx, fy, fz, l);
+  float *Lp = (float *)g.L + ((size_t)rh * g.ncell + A) * 256 + 4 * half;
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      f32x4 v = {acc[t][4 * q], acc[t][4 * q + 1], acc[t][4 * q + 2], acc[t][4 * q + 3]};
+      *reinterpret_cast<f32x4 *>(Lp + 32 * t + 8 * q) = v;
+    }
+}
+
 // ---------------------------------------------------------------- plain kernel
 // One thread per (column, row); 256 / R columns per block, all of one class.
 template <int R>
@@ -521,8 +652,8 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   if (maxlev > kMaxLev) return set_error(BBFMM_E_ARG, "dense-grid M2L: maxlev %d > %d", maxlev, kMaxLev);
   if (r != 32 && r != 256) return set_error(BBFMM_E_ARG, "dense-grid M2L: r=%d not supported", r);
 #ifdef SINGLE
-  if (form != BBFMM_M2L_PLAIN && r != 32)
-    return set_error(BBFMM_E_ARG, "MFMA M2L forms are built for r = 32 (got r=%d)", r);
+  if (form == BBFMM_M2L_BF16X2 && r != 32)
+    return set_error(BBFMM_E_ARG, "the split-BF16 M2L form is built for r = 32 (got r=%d)", r);
 #else
   if (form != BBFMM_M2L_PLAIN) return set_error(BBFMM_E_ARG, "MFMA M2L forms need the fp32 build");
 #endif
@@ -556,7 +687,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   } else {
     // two tiles per wave once there are enough columns to fill the chip
     const long long deep = (long long)(1 << (3 * (maxlev - 1))) * nrhs;
-    tpw = deep >= 8 * 1024 ? 2 : 1;
+    tpw = r == 32 && deep >= 8 * 1024 ? 2 : 1;  // r = 256: one 32-column tile per wave
     cpb = 4 * tpw * 32;
   }
   a.wave_cols = tpw * 32;
@@ -625,6 +756,27 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
       if (rc == BBFMM_OK && (e = hipMalloc(&g->d_Mg1, nb2)) != hipSuccess) fail(e, "hipMalloc");
       if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_Mg0, 0, nb2, s)) != hipSuccess) fail(e, "memset");
       if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_Mg1, 0, nb2, s)) != hipSuccess) fail(e, "memset");
+    } else if (r == 256) {
+      // FP32, r = 256: [k][k-chunk][row tile][lane][16]: lane l holds row
+      // 32*tile + l%32 and columns 32*chunk + 16*(l/32) + 0..15
+      std::vector<float> Kf((size_t)kNumOps * 256 * 256);
+      for (int k = 0; k < kNumOps; k++)
+        for (int kc = 0; kc < 8; kc++)
+          for (int t = 0; t < 8; t++)
+            for (int lane = 0; lane < 64; lane++)
+              for (int ks = 0; ks < 16; ks++) {
+                const int row = 32 * t + (lane & 31), col = 32 * kc + 16 * (lane >> 5) + ks;
+                Kf[((((size_t)k * 8 + kc) * 8 + t) * 64 + lane) * 16 + ks] =
+                    K[(size_t)k * 256 * 256 + (size_t)col * 256 + row];
+              }
+      if ((e = hipMalloc(&g->d_K, sizeof(float) * Kf.size())) != hipSuccess) fail(e, "hipMalloc");
+      else if ((e = hipMemcpyAsync(g->d_K, Kf.data(), sizeof(float) * Kf.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+        fail(e, "copy");
+      // the host copy must outlive the asynchronous copy
+      if (rc == BBFMM_OK && (e = hipStreamSynchronize(s)) != hipSuccess) fail(e, "sync");
+      const size_t nb2 = sizeof(float) * slots * nrhs * r;
+      if (rc == BBFMM_OK && (e = hipMalloc(&g->d_Mg0, nb2)) != hipSuccess) fail(e, "hipMalloc");
+      if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_Mg0, 0, nb2, s)) != hipSuccess) fail(e, "memset");
     } else {
       // FP32: [k][lane][16]: lane l holds row l%32, k = 16*(l/32) + ks
       std::vector<float> Kf((size_t)kNumOps * 64 * 16);
@@ -703,7 +855,9 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_FP32MFMA>, dim3(sb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (a.wave_cols == 64)
+    if (a.r == 256)  // rounded up to whole rounds of the 8 XCDs; the extra blocks exit
+      hipLaunchKernelGGL(k_m2l_fp32_r256, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+    else if (a.wave_cols == 64)
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP32MFMA, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
     else
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP32MFMA, 1>), dim3(g->nblocks), dim3(256), 0, s, a);
