@@ -16,7 +16,7 @@ import numpy as np
 from ._build import lib_path
 
 __all__ = [
-    "load", "Plan", "operators", "direct", "weights", "maxlev", "m2l_drv", "BBFMMError",
+    "load", "Plan", "operators", "write_operators", "read_operators", "direct", "weights", "maxlev", "m2l_drv", "BBFMMError",
     "SCHEME_BA", "SCHEME_SI", "SCHEME_CL", "SCHEME_IJ", "FORM_PLAIN", "FORM_BF16X2", "FORM_FP32MFMA",
 ]
 
@@ -59,6 +59,10 @@ def load(prec: str = "s"):
     L.bbfmm_amd_weights.argtypes = [i, vp, vp, vp, vp]
     L.bbfmm_amd_weights.restype = None
     L.bbfmm_amd_operators.argtypes = [i, i, vp, vp, vp]
+    L.bbfmm_amd_operators_device.argtypes = [i, i, vp, vp, vp]
+    cp = C.c_char_p
+    L.bbfmm_amd_write_operators.argtypes = [cp, cp, i, i, vp, vp]
+    L.bbfmm_amd_read_operators.argtypes = [cp, cp, i, vp, vp, vp]
     L.bbfmm_amd_last_error_string.restype = C.c_char_p
     L.bbfmm_amd_direct.argtypes = [i, vp, vp, i, vp, vp]
     L.bbfmm_amd_set_default_scheme.argtypes = [i]
@@ -107,9 +111,11 @@ def weights(n: int, prec: str = "s"):
     return Tkz, Kt, Kw, Cw
 
 
-def operators(n: int, r: int | None = None, prec: str = "s"):
+def operators(n: int, r: int | None = None, prec: str = "s", device: bool = False):
     """FP64 operator precompute, cast to the build's real: (U, K, sigma)
-    with U n^3 x r and K 316 blocks of r x r, column-major (reference layout)."""
+    with U n^3 x r and K 316 blocks of r x r, column-major (reference layout).
+    device=True samples the kernel and forms the Gram matrix and the 316
+    blocks on the GPU (the eigenproblem stays on the host)."""
     n3 = n ** 3
     r = r or n3 // 2
     dt = dtype(prec)
@@ -117,8 +123,31 @@ def operators(n: int, r: int | None = None, prec: str = "s"):
     K = np.zeros(316 * r * r, dt)
     sig = np.zeros(n3)
     L = load(prec)
-    _check(L, L.bbfmm_amd_operators(n, r, _p(U), _p(K), _p(sig)))
+    fn = L.bbfmm_amd_operators_device if device else L.bbfmm_amd_operators
+    _check(L, fn(n, r, _p(U), _p(K), _p(sig)))
     return U, K, sig
+
+
+def write_operators(Kpath: str, Upath: str, n: int, U, K, prec: str = "s"):
+    """Operator files in the reference's binary layout (bbfmm.c:1355-1403)."""
+    dt = dtype(prec)
+    U = np.ascontiguousarray(U, dt)
+    K = np.ascontiguousarray(K, dt)
+    r = U.size // n ** 3
+    L = load(prec)
+    _check(L, L.bbfmm_amd_write_operators(os.fsencode(Kpath), os.fsencode(Upath), n, r, _p(U), _p(K)))
+
+
+def read_operators(Kpath: str, Upath: str, n: int, prec: str = "s"):
+    """Operator files as FMMReadMatrices reads them (bbfmm.c:216-254): (U, K, r)."""
+    L = load(prec)
+    r = C.c_int(0)
+    _check(L, L.bbfmm_amd_read_operators(os.fsencode(Kpath), os.fsencode(Upath), n, C.byref(r), None, None))
+    dt = dtype(prec)
+    U = np.zeros(r.value * n ** 3, dt)
+    K = np.zeros(316 * r.value ** 2, dt)
+    _check(L, L.bbfmm_amd_read_operators(os.fsencode(Kpath), os.fsencode(Upath), n, C.byref(r), _p(U), _p(K)))
+    return U, K, r.value
 
 
 def direct(pos, q, targets=None) -> np.ndarray:

@@ -59,7 +59,8 @@ def build(precs=("s", "d"), jobs: int | None = None) -> list[str]:
     for prec in precs:
         lib = lib_path(prec)
         tmp = lib + ".tmp"
-        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *sorted(objs[prec]), "-o", tmp]
+        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *sorted(objs[prec]), "-o", tmp,
+               "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"link failed for {lib}:\n{res.stderr[-4000:]}")

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -673,6 +674,72 @@ extern "C" int bbfmm_amd_operators(int n, int r, real *U, real *K, double *sigma
   for (size_t i = 0; i < Kd.size(); i++) K[i] = (real)Kd[i];
   if (sigma)
     for (int i = 0; i < n3; i++) sigma[i] = sg[i];
+  return BBFMM_OK;
+}
+
+extern "C" int bbfmm_amd_operators_device(int n, int r, real *U, real *K, double *sigma)
+{
+  clear_error();
+  if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
+  const int n3 = n * n * n;
+  if (r < 1 || r > n3) return set_error(BBFMM_E_ARG, "rank r=%d out of range", r);
+  BB_TRY(check_device());
+  std::vector<double> Ud((size_t)n3 * r), Kd((size_t)kNumOps * r * r), sg((size_t)n3);
+  BB_TRY(operators_fp64_device(n, r, Ud.data(), Kd.data(), sg.data()));
+  for (size_t i = 0; i < Ud.size(); i++) U[i] = (real)Ud[i];
+  for (size_t i = 0; i < Kd.size(); i++) K[i] = (real)Kd[i];
+  if (sigma)
+    for (int i = 0; i < n3; i++) sigma[i] = sg[i];
+  return BBFMM_OK;
+}
+
+// Operator files in the reference's binary layout (bbfmm.c:216-254 reads,
+// bbfmm.c:1355-1403 writes): U file = int32 r, then n^3 x r reals
+// column-major; K file = 316 blocks of r x r reals, column-major.
+extern "C" int bbfmm_amd_write_operators(const char *Kpath, const char *Upath, int n, int r, const real *U,
+                                         const real *K)
+{
+  clear_error();
+  if (!Kpath || !Upath || !U || !K || n < 1 || r < 1) return set_error(BBFMM_E_ARG, "bad operator-file arguments");
+  const size_t usize = (size_t)r * n * n * n, ksize = (size_t)kNumOps * r * r;
+  FILE *f = fopen(Upath, "wb");
+  if (!f) return set_error(BBFMM_E_ARG, "cannot open %s for writing", Upath);
+  const bool okU = fwrite(&r, sizeof(int), 1, f) == 1 && fwrite(U, sizeof(real), usize, f) == usize;
+  const bool closeU = fclose(f) == 0;
+  if (!okU || !closeU) return set_error(BBFMM_E_ARG, "short write to %s", Upath);
+  f = fopen(Kpath, "wb");
+  if (!f) return set_error(BBFMM_E_ARG, "cannot open %s for writing", Kpath);
+  const bool okK = fwrite(K, sizeof(real), ksize, f) == ksize;
+  const bool closeK = fclose(f) == 0;
+  if (!okK || !closeK) return set_error(BBFMM_E_ARG, "short write to %s", Kpath);
+  return BBFMM_OK;
+}
+
+extern "C" int bbfmm_amd_read_operators(const char *Kpath, const char *Upath, int n, int *r, real *U, real *K)
+{
+  clear_error();
+  if (!Kpath || !Upath || !r || n < 1) return set_error(BBFMM_E_ARG, "bad operator-file arguments");
+  FILE *f = fopen(Upath, "rb");
+  if (!f) return set_error(BBFMM_E_ARG, "cannot open %s", Upath);
+  int cutoff = 0;
+  if (fread(&cutoff, sizeof(int), 1, f) != 1 || cutoff < 1 || cutoff > n * n * n) {
+    fclose(f);
+    return set_error(BBFMM_E_ARG, "%s: bad rank header", Upath);
+  }
+  *r = cutoff;
+  if (!U || !K) {  // size query
+    fclose(f);
+    return BBFMM_OK;
+  }
+  const size_t usize = (size_t)cutoff * n * n * n, ksize = (size_t)kNumOps * cutoff * cutoff;
+  const bool okU = fread(U, sizeof(real), usize, f) == usize;
+  fclose(f);
+  if (!okU) return set_error(BBFMM_E_ARG, "%s: short file", Upath);
+  f = fopen(Kpath, "rb");
+  if (!f) return set_error(BBFMM_E_ARG, "cannot open %s", Kpath);
+  const bool okK = fread(K, sizeof(real), ksize, f) == ksize;
+  fclose(f);
+  if (!okK) return set_error(BBFMM_E_ARG, "%s: short file", Kpath);
   return BBFMM_OK;
 }
 

@@ -3,6 +3,8 @@
 
 #include "common.h"
 
+#include <vector>
+
 namespace bb {
 
 // Everything the pass kernels read, by value (device pointers).
@@ -80,5 +82,12 @@ int launch_p2p(const P2PArgs &a, hipStream_t s);
 
 // ---- FP64 operator precompute ----
 int operators_fp64(int n, int r, double *U, double *K, double *sigma);
+// same results with the kernel sampling, the Gram matrix and the 316
+// products E_k = U^T K0_k U on the GPU (the eigenproblem stays on the host)
+int operators_fp64_device(int n, int r, double *U, double *K, double *sigma);
+// eigen-decomposition of a symmetric m x m matrix (column-major, destroyed):
+// eigenvalues w descending, eigenvectors as the columns of V, each signed so
+// that its largest-magnitude entry is positive
+void jacobi_eig(int m, std::vector<double> &A, std::vector<double> &w, std::vector<double> &V);
 
 } // namespace bb

@@ -43,7 +43,7 @@ static void sample_K0(int n, std::vector<double> &K0)
 }
 
 // symmetric eigenproblem, cyclic Jacobi with Rutishauser's threshold strategy
-static void jacobi_eig(int m, std::vector<double> &A, std::vector<double> &w, std::vector<double> &V)
+void jacobi_eig(int m, std::vector<double> &A, std::vector<double> &w, std::vector<double> &V)
 {
   V.assign((size_t)m * m, 0.0);
   for (int i = 0; i < m; i++) V[(size_t)i * m + i] = 1;

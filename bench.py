@@ -90,6 +90,71 @@ def cpu_baseline(pos, q, U, K, threads):
             "wall_s": t, "phi": res["phi"]}
 
 
+def run_c3(args):
+    """N = 4,194,304 uniform, n = 8, r = 256, depth 6, 16 charge vectors per
+    solve, M2L on FP32 MFMA (dense grid), MFMA P2P; operators precomputed in
+    FP64 on the GPU.  Errors against the FP64 direct sum on sampled targets."""
+    import torch
+    import bbfmmongpu_amd as B
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    N = 1 << 22 if args.N == 1 << 20 else args.N
+    nrhs = args.nrhs
+    steps = min(args.steps, 3)
+    warmup = min(args.warmup, 1)
+    rng = np.random.default_rng(0)
+    pos = rng.random((N, 3)) - 0.5
+    Q = rng.uniform(-1.0, 1.0, size=(nrhs, N))
+    log(f"c3: N={N} nrhs={nrhs}: operator precompute n=8 r=256 (FP64, GPU)")
+    t0 = time.time()
+    U, K, sigma = B.operators(8, 256, prec="s", device=True)
+    t_ops = time.time() - t0
+    t0 = time.time()
+    plan = B.Plan(pos, 8, U, K, nrhs=nrhs, prec="s", stream=stream.cuda_stream).set(
+        scheme=B.SCHEME_IJ, form=B.FORM_FP32MFMA)
+    t_setup = time.time() - t0
+    log(f"c3: maxlev={plan.maxlev} ncell={plan.ncell} interactions={plan.m2l_interactions:.0f} "
+        f"pairs={plan.p2p_pairs:.4g} ops={t_ops:.1f}s setup={t_setup:.1f}s")
+    qm = torch.from_numpy(Q.astype(np.float32)).to(dev).contiguous()
+    phim = torch.empty_like(qm)
+    ms, st = timed_solves(plan, qm, phim, steps, warmup, stream)
+    log(f"c3: {ms:.1f} ms/step ({nrhs} vectors) stages {st}")
+    tg = np.sort(np.random.default_rng(1).choice(N, size=min(args.targets, N), replace=False)).astype(np.int32)
+    phi = phim.cpu().numpy()
+    errs = []
+    for j in range(min(nrhs, 2)):
+        d = B.direct(pos, Q[j], tg)
+        errs.append(rel_l2(d, phi[j, tg]))
+    log(f"c3: errors vs FP64 direct {errs}")
+    r = 256
+    inter = plan.m2l_interactions
+    flop_m2l = r * (2 * r - 1) * inter * nrhs
+    roof = {"kernel": "k_m2l_fp32_r256", "bound": "mfma", "unit": "TFLOP/s",
+            "achieved": flop_m2l / (st["m2l"] * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS,
+            "launch_ms": st["m2l"], "algorithmic_flop_per_launch": flop_m2l, "traffic": None}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    result = {
+        "metric": "FMM s/solve + M2L Gflop/s at N=4M, n=8, 16 charge vectors, rel L2 err vs FP64 direct",
+        "value": nrhs / (ms * 1e-3), "unit": "solve/s (per charge vector)", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"bbFMM Laplace 1/r, N={N} uniform, n=8, r=256, depth {plan.maxlev}, "
+                               f"{nrhs} charge vectors per solve",
+                   "N": N, "n": 8, "r": r, "maxlev": plan.maxlev, "nrhs": nrhs,
+                   "m2l_scheme": "dense grid (IJ)", "m2l_form": "FP32 MFMA"},
+        "s_per_solve": ms * 1e-3 / nrhs,
+        "m2l_gflops": flop_m2l / (st["m2l"] * 1e-3) / 1e9,
+        "p2p_ginteractions_per_s": plan.p2p_pairs * nrhs / (st["p2p"] * 1e-3) / 1e9,
+        "rel_l2_err": {"q_phi": errs, "targets": int(tg.size)},
+        "stage_ms": st, "setup_s": {"operators": t_ops, "plan": t_setup},
+        "roofline": roof,
+    }
+    plan.close()
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,11 +167,17 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for HBM byte counters)")
+    ap.add_argument("--config", choices=["c2", "c3"], default="c2",
+                    help="c2: N=1M n=4 chosen vs plain (default); c3: N=4M n=8 r=256 depth 6, "
+                         "16 vectors, FP32 MFMA M2L")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note only "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c3":
+        run_c3(args)
+        return
 
     import torch
     import bbfmmongpu_amd as B

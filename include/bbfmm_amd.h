@@ -211,6 +211,15 @@ void bbfmm_amd_weights(int n, real *Tkz, int *Ktable, real *Kweights,
 /* Operator precompute in FP64 (bbfmm.c:1209-1421): U (n^3 x r) and K
    (316 r x r), column-major, cast to real.  sigma (n^3 values) optional. */
 int bbfmm_amd_operators(int n, int r, real *U, real *K, double *sigma);
+/* The same precompute with the kernel sampling, K0 K0^T and the 316 products
+   U^T K0_k U in FP64 on the GPU (the eigenproblem on the host). */
+int bbfmm_amd_operators_device(int n, int r, real *U, real *K, double *sigma);
+/* Operator files in the reference's binary layout (FMMReadMatrices,
+   bbfmm.c:216-254; written as ComputeKernelSVD writes them, bbfmm.c:1355-1403):
+   U file = int32 r followed by n^3 x r reals, K file = 316 r x r reals, both
+   column-major.  read: with U or K NULL only *r is returned (size query). */
+int bbfmm_amd_write_operators(const char *Kpath, const char *Upath, int n, int r, const real *U, const real *K);
+int bbfmm_amd_read_operators(const char *Kpath, const char *Upath, int n, int *r, real *U, real *K);
 
 #ifdef __cplusplus
 }

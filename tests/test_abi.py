@@ -155,3 +155,33 @@ def test_no_gpu_is_reported_not_faked():
     h = C.c_void_p()
     rc = lib.bbfmm_amd_plan_create(C.byref(h), 1000, O._p(pos), 1.0, 4, 0, 32, O._p(K), O._p(U), 1, None)
     assert rc == 4, lib.bbfmm_amd_last_error_string()
+
+
+@pytest.mark.parametrize("prec", ["s", "d"])
+def test_operator_files_match_reference_layout(prec, tmp_path):
+    """The library writes the reference's K/U file layout byte for byte (the
+    oracle's writer is the one the compiled reference reads) and reads it back."""
+    U, K, _ = B.operators(4, 32, prec=prec)
+    B.write_operators(str(tmp_path / "K.bin"), str(tmp_path / "U.bin"), 4, U, K, prec=prec)
+    ref_dir = tmp_path / "oracle"
+    ref_dir.mkdir()
+    O.write_operator_files(str(ref_dir), prec, U, K, 4)
+    for name in ("K.bin", "U.bin"):
+        assert (tmp_path / name).read_bytes() == (ref_dir / name).read_bytes(), name
+    U2, K2, r = B.read_operators(str(tmp_path / "K.bin"), str(tmp_path / "U.bin"), 4, prec=prec)
+    assert r == 32 and np.array_equal(U2, U) and np.array_equal(K2, K)
+    # truncated file: an error code, not a crash
+    (tmp_path / "K.bin").write_bytes((tmp_path / "K.bin").read_bytes()[:100])
+    with pytest.raises(B.BBFMMError):
+        B.read_operators(str(tmp_path / "K.bin"), str(tmp_path / "U.bin"), 4, prec=prec)
+
+
+def test_device_operators_need_a_gpu():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is visible")
+    except ImportError:
+        pass
+    with pytest.raises(B.BBFMMError, match="error 4"):
+        B.operators(4, 32, prec="d", device=True)
