@@ -16,13 +16,14 @@ import numpy as np
 from ._build import lib_path
 
 __all__ = [
-    "load", "Plan", "operators", "write_operators", "read_operators", "direct", "weights", "maxlev", "m2l_drv", "BBFMMError",
+    "load", "Plan", "operators", "write_operators", "read_operators", "direct", "direct_field",
+    "FMMPotential", "potential", "weights", "maxlev", "m2l_drv", "BBFMMError",
     "SCHEME_BA", "SCHEME_SI", "SCHEME_CL", "SCHEME_IJ", "FORM_PLAIN", "FORM_BF16X2", "FORM_FP32MFMA",
 ]
 
 SCHEME_BA, SCHEME_SI, SCHEME_CL, SCHEME_IJ = 0, 1, 2, 3
 FORM_PLAIN, FORM_BF16X2, FORM_FP32MFMA = 0, 1, 2
-OPT_SCHEME, OPT_FORM, OPT_TIMING = 1, 2, 3
+OPT_SCHEME, OPT_FORM, OPT_TIMING, OPT_OUTPUT = 1, 2, 3, 4
 STAGES = ["gather", "p2m", "m2m", "m2l_prep", "m2l", "post_m2l", "l2l", "l2p", "p2p", "scatter", "total"]
 
 _libs: dict = {}
@@ -65,6 +66,9 @@ def load(prec: str = "s"):
     L.bbfmm_amd_read_operators.argtypes = [cp, cp, i, vp, vp, vp]
     L.bbfmm_amd_last_error_string.restype = C.c_char_p
     L.bbfmm_amd_direct.argtypes = [i, vp, vp, i, vp, vp]
+    L.bbfmm_amd_direct_field.argtypes = [i, vp, i, vp, i, vp, vp, vp]
+    L.bbfmm_amd_gradient.argtypes = [vp, vp, vp, vp, vp]
+    L.bbfmm_amd_plan_update_positions.argtypes = [vp, vp]
     L.bbfmm_amd_set_default_scheme.argtypes = [i]
     L.m2l_drv.argtypes = [i, vp, vp, vp, i, i, R, vp, vp, i, vp, vp, i, vp, i]
     for name in ("anotherFMMCompute",):
@@ -166,6 +170,25 @@ def direct(pos, q, targets=None) -> np.ndarray:
     return out
 
 
+def direct_field(pos, q, targets=None):
+    """FP64 direct sum on the GPU for nvec charge vectors q (nvec x N):
+    (potentials nvec x nt, gradients nvec x nt x 3) at the target indices."""
+    L = load("d")
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(np.atleast_2d(q), dtype=np.float64)
+    N = pos.shape[0]
+    nvec = q.shape[0]
+    if targets is None:
+        nt, tp = N, None
+    else:
+        keep = np.ascontiguousarray(targets, dtype=np.int32)
+        nt, tp = keep.size, _p(keep)
+    pot = np.zeros((nvec, nt))
+    grad = np.zeros((nvec, nt, 3))
+    _check(L, L.bbfmm_amd_direct_field(N, _p(pos), nvec, _p(q), nt, tp, _p(pot), _p(grad)))
+    return pot, grad
+
+
 def _torch():
     import torch
     return torch
@@ -197,14 +220,48 @@ class Plan:
         self.maxlev, self.ncell = int(info[0]), int(info[1])
         self.m2l_interactions, self.p2p_pairs = float(dinfo[0]), float(dinfo[1])
 
-    def set(self, scheme: int | None = None, form: int | None = None, timing: int | None = None):
+    def set(self, scheme: int | None = None, form: int | None = None, timing: int | None = None,
+            output: str | None = None):
+        """output: "q_phi" (q .* potential, the reference's output) or "potential"."""
         if scheme is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_SCHEME, scheme))
         if form is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_FORM, form))
         if timing is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_TIMING, int(timing)))
+        if output is not None:
+            if output not in ("q_phi", "potential"):
+                raise BBFMMError(f"unknown output {output!r}")
+            _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_OUTPUT, int(output == "potential")))
         return self
+
+    def update_positions(self, pos):
+        """Move the particles within their leaves (same tree and lists)."""
+        dt = dtype(self.prec)
+        p = np.ascontiguousarray(pos, dtype=dt).reshape(-1, 3)
+        if p.shape[0] != self.N:
+            raise BBFMMError("pos has the wrong size")
+        _check(self.lib, self.lib.bbfmm_amd_plan_update_positions(self.h, _p(p)))
+        self.pos = p
+        return self
+
+    def gradient(self, q, g, dq=None, dx=None):
+        """Adjoint of the potentials for a plan of nrhs = 2m vectors: q, g
+        (m x N torch tensors on the GPU; g = dLoss/dpotential) -> (dLoss/dq
+        (m x N), dLoss/dpositions (N x 3))."""
+        torch = _torch()
+        want = torch.float32 if self.prec == "s" else torch.float64
+        m = self.nrhs // 2
+        for name, t in (("q", q), ("g", g)):
+            if t.dtype != want or not t.is_contiguous() or t.numel() != m * self.N:
+                raise BBFMMError(f"{name} must be a contiguous {want} tensor of {m} x {self.N}")
+        if dq is None:
+            dq = torch.empty_like(q)
+        if dx is None:
+            dx = torch.empty((self.N, 3), dtype=want, device=q.device)
+        _check(self.lib, self.lib.bbfmm_amd_gradient(self.h, C.c_void_p(q.data_ptr()), C.c_void_p(g.data_ptr()),
+                                                     C.c_void_p(dq.data_ptr()), C.c_void_p(dx.data_ptr())))
+        return dq, dx
 
     def solve(self, q, phi=None):
         """q: torch tensor on the GPU (nrhs x N or N), returns phi = q .* potential."""
@@ -267,3 +324,69 @@ def m2l_drv(r, Lv, K, M, minlev, maxlev, L0, center, ineigh, neighbors, iinter, 
     return L.m2l_drv(r, _p(Lv), _p(K), _p(M), minlev, maxlev, L0, _p(center), _p(ineigh),
                      pitch_neighbors, _p(neighbors), _p(iinter), pitch_interaction,
                      _p(interaction), scheme)
+
+
+# ---------------------------------------------------------------- autograd
+def _make_autograd():
+    torch = _torch()
+
+    class FMMPotential(torch.autograd.Function):
+        """phi = potential(pos, q): the FMM potentials of m charge vectors
+        (q: m x N, pos: N x 3, both on the GPU) with dLoss/dq and
+        dLoss/dpos from the adjoint solve (bbfmm_amd_gradient).  The
+        backward recomputes the expansions (a solve of the 2m vectors
+        [q; dLoss/dphi]); nothing but the inputs is kept from the forward.
+        cfg = (n, U, K, prec, lmod, scheme, form)."""
+
+        @staticmethod
+        def forward(ctx, pos, q, cfg):
+            n, U, K, prec, lmod, scheme, form = cfg
+            m = q.shape[0]
+            plan = Plan(pos.detach().cpu().numpy(), n, U, K, nrhs=m, prec=prec, lmod=lmod,
+                        stream=torch.cuda.current_stream().cuda_stream)
+            try:
+                plan.set(scheme=scheme, form=form, output="potential")
+                phi = plan.solve(q.detach().contiguous())
+            finally:
+                plan.close()
+            ctx.save_for_backward(pos, q)
+            ctx.cfg = cfg
+            return phi
+
+        @staticmethod
+        def backward(ctx, gphi):
+            pos, q = ctx.saved_tensors
+            n, U, K, prec, lmod, scheme, form = ctx.cfg
+            m = q.shape[0]
+            plan = Plan(pos.detach().cpu().numpy(), n, U, K, nrhs=2 * m, prec=prec, lmod=lmod,
+                        stream=torch.cuda.current_stream().cuda_stream)
+            try:
+                plan.set(scheme=scheme, form=form)
+                dq, dx = plan.gradient(q.detach().contiguous(), gphi.detach().contiguous())
+                torch.cuda.current_stream().synchronize()
+            finally:
+                plan.close()
+            return dx.to(pos.dtype), dq, None
+
+    return FMMPotential
+
+
+class _Lazy:
+    cls = None
+
+
+def __getattr__(name):
+    if name == "FMMPotential":
+        if _Lazy.cls is None:
+            _Lazy.cls = _make_autograd()
+        return _Lazy.cls
+    raise AttributeError(name)
+
+
+def potential(pos, q, n: int, U, K, prec: str = "s", lmod: int = 0, scheme: int = 3,
+              form: int | None = None):
+    """Differentiable FMM potentials (torch): pos N x 3, q m x N on the GPU,
+    returns m x N potentials; gradients flow to pos and q."""
+    if form is None:
+        form = FORM_BF16X2 if (prec == "s" and n == 4 and q.shape[0] >= 16) else FORM_PLAIN
+    return __getattr__("FMMPotential").apply(pos, q, (n, U, K, prec, lmod, scheme, form))

@@ -81,6 +81,11 @@ struct bbfmm_amd_plan {
   int2 *d_seg = nullptr;
   int *d_nseg = nullptr, *d_pad_of = nullptr;
   int npad = 0;
+  int potential = 0;           // output: 0 = q * potential (reference), 1 = potential
+  // host copies for moving particles inside their leaves (update_positions)
+  std::vector<int> h_leafof, h_pad_of, h_leafxyz;  // [N], [N], [nleaf*3]
+  real h_inv_leaf = 0;
+  real *d_grad = nullptr;      // [3][N] position-gradient scratch (allocated by the first gradient call)
   std::vector<real> h_K;                       // column-major operators (for the grid form)
   bb::GridM2L *grid = nullptr;
   int grid_form = -1;
@@ -117,7 +122,7 @@ static int upload(T **p, const T *h, size_t count, hipStream_t s)
   return BBFMM_OK;
 }
 
-static int check_device()
+int check_device()
 {
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
@@ -134,7 +139,7 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_nbr, p->d_nnbr, p->d_iinter, p->d_inter, p->d_kidx, p->d_Ktable, p->d_levsta,
                   p->d_Ucm, p->d_Urm, p->d_Tkz, p->d_Kw, p->d_Cw, p->d_scale, p->d_q, p->d_phi,
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
-                  p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm};
+                  p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->grid) grid_m2l_destroy(p->grid);
@@ -320,7 +325,17 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
   BB_TRY(upload(&p->d_Cw, Cw, (size_t)2 * n * n, stream));
   BB_TRY(upload(&p->d_scale, scale.data(), scale.size(), stream));
   {
-    // padded copy for P2P: leaf ranges start at multiples of 4, dummies far away with q = 0
+    // leaf grid coordinates (the bucketing of build_tree) for update_positions
+  p->h_leafof = leafof;
+  p->h_inv_leaf = 1 / t->celeng[t->maxlev];
+  p->h_leafxyz.resize((size_t)nleaf * 3);
+  for (int k = 0; k < nleaf; k++) {
+    const real3 cc = c->center[p->leaf0 + k];
+    p->h_leafxyz[3 * k] = (int)((cc.x + p->L / 2) * p->h_inv_leaf);
+    p->h_leafxyz[3 * k + 1] = (int)((cc.y + p->L / 2) * p->h_inv_leaf);
+    p->h_leafxyz[3 * k + 2] = (int)((cc.z + p->L / 2) * p->h_inv_leaf);
+  }
+  // padded copy for P2P: leaf ranges start at multiples of 4, dummies far away with q = 0
     std::vector<int> psta((size_t)nleaf), pcnt((size_t)nleaf), pad_of((size_t)N);
     int run = 0;
     for (int k = 0; k < nleaf; k++) {
@@ -330,6 +345,7 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
       run += pcnt[k];
     }
     p->npad = run;
+    p->h_pad_of = pad_of;
     std::vector<real4> ppos((size_t)run);
     for (auto &x : ppos) {
       x.x = x.y = x.z = (real)1e6;
@@ -400,6 +416,7 @@ static PassArgs pass_args(const bbfmm_amd_plan *p)
   a.M = p->d_M;
   a.L = p->d_L;
   a.F = p->d_F;
+  a.potential = p->potential;
   return a;
 }
 
@@ -473,6 +490,7 @@ static int solve_sorted(bbfmm_amd_plan *p)
   pp.q = p->d_q;
   pp.phi = p->d_phi;
   pp.use_mfma = p->form != BBFMM_M2L_PLAIN;
+  pp.potential = p->potential;
   pp.ppos = p->d_ppos;
   pp.seg = p->d_seg;
   pp.nseg = p->d_nseg;
@@ -575,6 +593,10 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     if (value < 0 || value > 4096) return set_error(BBFMM_E_ARG, "timing sets %d out of range", value);
     p->timing = value;
     return set_timing(p, value);
+  case BBFMM_OPT_OUTPUT:
+    if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "output %d: 0 (q .* potential) or 1 (potential)", value);
+    p->potential = value;
+    return BBFMM_OK;
   default: return set_error(BBFMM_E_ARG, "unknown option %d", option);
   }
 }
@@ -615,6 +637,86 @@ extern "C" int bbfmm_amd_solve_host(bbfmm_amd_plan *p, const real *q, real *phi)
   BB_TRY(solve_dev(p, p->d_qin, p->d_phiout));
   BB_HIP(hipMemcpyAsync(phi, p->d_phiout, sizeof(real) * NR, hipMemcpyDeviceToHost, p->stream));
   BB_HIP(hipStreamSynchronize(p->stream));
+  return BBFMM_OK;
+}
+
+extern "C" int bbfmm_amd_plan_update_positions(bbfmm_amd_plan *p, const real3 *pos)
+{
+  clear_error();
+  if (!p || !pos) return set_error(BBFMM_E_ARG, "null argument");
+  const int N = p->N;
+  const real half = p->L / 2;
+  const int ncpd = 1 << p->maxlev;
+  std::vector<real4> hpos((size_t)N), ppos((size_t)(p->npad > 0 ? p->npad : 1));
+  for (auto &x : ppos) {
+    x.x = x.y = x.z = (real)1e6;
+    x.w = 0;
+  }
+  for (int i = 0; i < N; i++) {
+    const real3 x = pos[p->h_perm[i]];
+    int ix = (int)((x.x + half) * p->h_inv_leaf), iy = (int)((x.y + half) * p->h_inv_leaf),
+        iz = (int)((x.z + half) * p->h_inv_leaf);
+    if (ix < 0 || iy < 0 || iz < 0 || !(x.x <= half) || !(x.y <= half) || !(x.z <= half))
+      return set_error(BBFMM_E_RANGE, "a particle lies outside the root cell [-L/2, L/2]^3");
+    ix = ix < ncpd - 1 ? ix : ncpd - 1;
+    iy = iy < ncpd - 1 ? iy : ncpd - 1;
+    iz = iz < ncpd - 1 ? iz : ncpd - 1;
+    const int *c = &p->h_leafxyz[(size_t)3 * p->h_leafof[i]];
+    if (ix != c[0] || iy != c[1] || iz != c[2])
+      return set_error(BBFMM_E_RANGE, "particle %d moved to another leaf: build a new plan", p->h_perm[i]);
+    hpos[i] = real4{x.x, x.y, x.z, 0};
+    ppos[p->h_pad_of[i]] = hpos[i];
+  }
+  BB_HIP(hipMemcpyAsync(p->d_pos, hpos.data(), sizeof(real4) * (size_t)N, hipMemcpyHostToDevice, p->stream));
+  BB_HIP(hipMemcpyAsync(p->d_ppos, ppos.data(), sizeof(real4) * ppos.size(), hipMemcpyHostToDevice, p->stream));
+  BB_HIP(hipStreamSynchronize(p->stream));
+  return BBFMM_OK;
+}
+
+extern "C" int bbfmm_amd_gradient(bbfmm_amd_plan *p, const real *q_dev, const real *g_dev, real *dq_dev,
+                                  real *dx_dev)
+{
+  clear_error();
+  if (!p || !q_dev || !g_dev || !dq_dev || !dx_dev) return set_error(BBFMM_E_ARG, "null argument");
+  if (p->nrhs % 2) return set_error(BBFMM_E_ARG, "gradient: the plan needs an even vector count (2m), got %d", p->nrhs);
+  const int m = p->nrhs / 2;
+  const int N = p->N;
+  hipStream_t s = p->stream;
+  if (!p->d_grad) BB_TRY(dalloc(&p->d_grad, (size_t)3 * N));
+  // [q; g] in leaf order, one solve of the 2m vectors in potential mode
+  BB_TRY(launch_gather(p->d_perm, q_dev, p->d_q, N, m, s));
+  BB_TRY(launch_gather(p->d_perm, g_dev, p->d_q + (size_t)m * N, N, m, s));
+  const int saved_potential = p->potential, saved_sets = p->nsets;
+  p->potential = 1;
+  p->nsets = 0;  // not a timed solve
+  const int rc = solve_sorted(p);
+  p->potential = saved_potential;
+  p->nsets = saved_sets;
+  BB_TRY(rc);
+  // dLoss/dq = A g: the potentials of the g vectors
+  BB_TRY(launch_scatter(p->d_iperm, p->d_phi + (size_t)m * N, dq_dev, N, m, s));
+  GradArgs ga;
+  ga.N = N;
+  ga.n = p->n;
+  ga.nleaf = p->nleaf;
+  ga.leaf0 = p->leaf0;
+  ga.ncell = p->ncell;
+  ga.m = m;
+  const PassArgs pa = pass_args(p);
+  ga.ihalfL = pa.ihalfL;
+  ga.prefac3 = pa.prefac3;
+  ga.pos = p->d_pos;
+  ga.center = p->d_center;
+  ga.leafsta = p->d_leafsta;
+  ga.leafcnt = p->d_leafcnt;
+  ga.nbr = p->d_nbr;
+  ga.nnbr = p->d_nnbr;
+  ga.Tkz = p->d_Tkz;
+  ga.F = p->d_F;
+  ga.qg = p->d_q;
+  ga.dx = p->d_grad;
+  BB_TRY(launch_gradient(ga, p->d_iperm, dx_dev, s));
+  p->solved = true;
   return BBFMM_OK;
 }
 

@@ -20,6 +20,9 @@ constexpr int kPitchNeighbors = 64;     // FLOORUP(27, ALIGN_SIZE=64)
 constexpr int kPitchInteraction = 192;  // FLOORUP(189, 64)
 constexpr int kNumOps = 316;            // offsets in [-3,3]^3 outside [-1,1]^3
 
+// api.hip: BBFMM_E_NODEVICE unless a GPU is visible
+int check_device();
+
 // host_setup.cpp
 int maxlev_rule(int N, int lmod);
 void level_tables(int maxlev, real L, int *levsta, int *levend, real *celeng);

@@ -25,6 +25,7 @@ struct PassArgs {
   const real *q;        // [rhs][N] leaf order
   real *phi;            // [rhs][N] leaf order
   real *S, *M, *L, *F;
+  int potential = 0;    // 1: L2P writes the potential, 0: q * potential (the reference's output)
 };
 
 int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
@@ -70,6 +71,7 @@ struct P2PArgs {
   const real *q;           // [rhs][N]
   real *phi;               // [rhs][N] (accumulated into)
   int use_mfma = 0;        // fp32: 16 vectors at a time on MFMA
+  int potential = 0;       // 1: add the potential, 0: q * potential (the reference's output)
   // padded particle copy (every leaf padded to a multiple of 4 with far-away,
   // zero-charge dummies); kernels fall back to the unpadded ones when null
   const real4 *ppos = nullptr;  // [Npad]
@@ -79,6 +81,20 @@ struct P2PArgs {
   float *qt = nullptr;          // [Npad][16] charges of 16 vectors, source-major
 };
 int launch_p2p(const P2PArgs &a, hipStream_t s);
+
+// ---- adjoint (position gradient) ----
+struct GradArgs {
+  int N, n, nleaf, leaf0, ncell, m;  // m vectors: charges q in rows 0..m-1 of qg, dLoss/dphi in m..2m-1
+  real ihalfL, prefac3;
+  const real4 *pos, *center;         // leaf order
+  const int *leafsta, *leafcnt, *nbr, *nnbr;
+  const real *Tkz;
+  const real *F;                     // [2m][ncell][n^3] local values after L2L
+  const real *qg;                    // [2m][N] leaf order
+  real *dx;                          // [3][N] leaf order (scratch)
+};
+// dx_out [N][3] (original order) = position gradient; iperm: original -> leaf order
+int launch_gradient(const GradArgs &a, const int *iperm, real *dx_out, hipStream_t s);
 
 // ---- FP64 operator precompute ----
 int operators_fp64(int n, int r, double *U, double *K, double *sigma);

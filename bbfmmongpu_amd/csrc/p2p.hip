@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         const size_t o = (size_t)(rhs0 + r) * N + sta + i;
-        a.phi[o] += a.q[o] * (acc[r] + acc2[r]);
+        a.phi[o] += (a.potential ? (real)1 : a.q[o]) * (acc[r] + acc2[r]);
       }
     }
   }
@@ -118,208 +118,9 @@ __device__ __forceinline__ int xcd_leaf(int b, int nleaf)
   return (b & 7) * (nleaf >> 3) + (b >> 3);
 }
 
-// One vector, no LDS: a wavefront per leaf, lane = (target t = lane % 32,
-// source half h = lane / 32); each lane sweeps every other source of the
-// neighbour leaves straight from L1/L2 (lanes of a half read the same
-// address), and the two halves are summed with a cross-lane read at the end.
-// Leaves of ~32 particles keep all 64 lanes busy.
-__global__ void __launch_bounds__(64) k_p2p_split(P2PArgs a, int rhs0)
-{
-  const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
-  const int l = threadIdx.x;
-  const int ti = l & 31, h = l >> 5;
-  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
-  const int nn = a.nnbr[leaf];
-  const size_t off = (size_t)rhs0 * a.N;
-  const real *q = a.q + off;
-  const int *nb = a.nbr + (size_t)leaf * 27;
-  for (int tb = 0; tb < cnt; tb += 32) {
-    const int i = tb + ti;
-    real4 xf;
-    if (i < cnt) {
-      xf = a.pos[sta + i];
-    } else {
-      xf.x = xf.y = xf.z = xf.w = 0;
-    }
-    real acc0 = 0, acc1 = 0;
-    for (int b = 0; b < nn; b++) {
-      const int B = nb[b];
-      const int jsta = a.leafsta[B], nj = a.leafcnt[B];
-      if (B != leaf) {
-        int k = h;
-        for (; k + 2 < nj; k += 4) {
-          const real4 y0 = a.pos[jsta + k], y1 = a.pos[jsta + k + 2];
-          const real q0 = q[jsta + k], q1 = q[jsta + k + 2];
-          const real dx0 = y0.x - xf.x, dy0 = y0.y - xf.y, dz0 = y0.z - xf.z;
-          const real dx1 = y1.x - xf.x, dy1 = y1.y - xf.y, dz1 = y1.z - xf.z;
-          acc0 += q0 * rsqrt_r(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
-          acc1 += q1 * rsqrt_r(dx1 * dx1 + dy1 * dy1 + dz1 * dz1);
-        }
-        if (k < nj) {
-          const real4 y0 = a.pos[jsta + k];
-          const real dx0 = y0.x - xf.x, dy0 = y0.y - xf.y, dz0 = y0.z - xf.z;
-          acc0 += q[jsta + k] * rsqrt_r(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
-        }
-      } else {
-        for (int k = h; k < nj; k += 2) {
-          const real4 y0 = a.pos[jsta + k];
-          const real dx0 = y0.x - xf.x, dy0 = y0.y - xf.y, dz0 = y0.z - xf.z;
-          const real rr = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
-          acc0 += rr != (real)0 ? q[jsta + k] * rsqrt_r(rr) : (real)0;
-        }
-      }
-    }
-    real v = acc0 + acc1;
-    v += __shfl_down(v, 32, 64);
-    if (h == 0 && i < cnt) a.phi[off + sta + i] += q[sta + i] * v;
-  }
-}
-
 #ifdef SINGLE
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// q [16][N] (vectors rhs0..rhs0+15) -> qt [N][16]
-__global__ void k_transpose16(int N, const float *__restrict__ q, float *__restrict__ qt)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  float4 *o = reinterpret_cast<float4 *>(qt + (size_t)i * 16);
-#pragma unroll
-  for (int r = 0; r < 16; r += 4)
-    o[r / 4] = make_float4(q[(size_t)r * N + i], q[(size_t)(r + 1) * N + i], q[(size_t)(r + 2) * N + i],
-                           q[(size_t)(r + 3) * N + i]);
-}
-
-// As k_p2p_mfma16, without LDS or barriers: the B operand of a step (4
-// sources x 16 vectors) is 256 contiguous bytes of the source-major charges.
-__global__ void __launch_bounds__(64) k_p2p_mfma16_direct(P2PArgs a, int rhs0, const float *__restrict__ qt)
-{
-  const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
-  const int l = threadIdx.x;
-  const int ti = l & 15, kq = l >> 4;
-  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
-  const int nn = a.nnbr[leaf];
-  const int *nb = a.nbr + (size_t)leaf * 27;
-  const size_t N = a.N;
-  for (int tb = 0; tb < cnt; tb += 32) {
-    const int t0 = tb + ti, t1 = tb + 16 + ti;
-    float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
-    if (t0 < cnt) x0 = a.pos[sta + t0];
-    if (t1 < cnt) x1 = a.pos[sta + t1];
-    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    for (int b = 0; b < nn; b++) {
-      const int B = nb[b];
-      const int jsta = a.leafsta[B], nj = a.leafcnt[B];
-      const bool self = B == leaf;
-      for (int k0 = 0; k0 < nj; k0 += 4) {
-        const int k = k0 + kq;
-        const bool kv = k < nj;
-        const int src = jsta + (kv ? k : 0);
-        const float4 y = a.pos[src];
-        const float qv = qt[(size_t)src * 16 + ti];
-        const float dx0 = y.x - x0.x, dy0 = y.y - x0.y, dz0 = y.z - x0.z;
-        const float dx1 = y.x - x1.x, dy1 = y.y - x1.y, dz1 = y.z - x1.z;
-        const float rr0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
-        const float rr1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-        float g0 = __builtin_amdgcn_rsqf(rr0), g1 = __builtin_amdgcn_rsqf(rr1);
-        if (self) {
-          g0 = rr0 != 0.f ? g0 : 0.f;
-          g1 = rr1 != 0.f ? g1 : 0.f;
-        }
-        g0 = kv ? g0 : 0.f;
-        g1 = kv ? g1 : 0.f;
-        const float bq = kv ? qv : 0.f;
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(g0, bq, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(g1, bq, acc1, 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int i = 4 * kq + v;
-      const size_t row = (size_t)(rhs0 + ti) * N + sta;
-      if (tb + i < cnt) a.phi[row + tb + i] += a.q[row + tb + i] * acc0[v];
-      if (tb + 16 + i < cnt) a.phi[row + tb + 16 + i] += a.q[row + tb + 16 + i] * acc1[v];
-    }
-  }
-}
-
-// Sixteen charge vectors at once on MFMA:  phi[t][v] = sum_j G[t][j] q[j][v]
-// with G[t][j] = 1/r_tj.  v_mfma_f32_16x16x4_f32 takes A = G (16 targets x 4
-// sources; lane l supplies G[l%16][l/16]) and B = q (4 sources x 16 vectors;
-// lane l supplies q[l/16][l%16]), so every lane computes exactly the one 1/r
-// value it feeds to the matrix core and the vector ALUs do only the geometry.
-// One wavefront per leaf, two 16-target tiles per pass, sources staged in LDS
-// 64 at a time with their 16 charges.
-__global__ void __launch_bounds__(64) k_p2p_mfma16(P2PArgs a, int rhs0)
-{
-  constexpr int CH = 64;
-  __shared__ float4 sp[CH];
-  __shared__ float sq[CH][16];
-  __shared__ int nsta[27], ncnt[27], nself[27];
-  const int leaf = blockIdx.x;
-  const int l = threadIdx.x;
-  const int ti = l & 15;  // A row (target in tile) and B column (vector)
-  const int kq = l >> 4;  // source within a step of four
-  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
-  const int nn = a.nnbr[leaf];
-  if (l < nn) {
-    const int B = a.nbr[(size_t)leaf * 27 + l];
-    nsta[l] = a.leafsta[B];
-    ncnt[l] = a.leafcnt[B];
-    nself[l] = (B == leaf);
-  }
-  __syncthreads();
-  const size_t N = a.N;
-  for (int tb = 0; tb < cnt; tb += 32) {
-    const int t0 = tb + ti, t1 = tb + 16 + ti;
-    float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
-    if (t0 < cnt) x0 = a.pos[sta + t0];
-    if (t1 < cnt) x1 = a.pos[sta + t1];
-    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    for (int b = 0; b < nn; b++) {
-      const int jsta = nsta[b], nj = ncnt[b];
-      const bool self = nself[b] != 0;
-      for (int jb = 0; jb < nj; jb += CH) {
-        const int nk = min(CH, nj - jb);
-        if (l < nk) {
-          sp[l] = a.pos[jsta + jb + l];
-#pragma unroll
-          for (int r = 0; r < 16; r++) sq[l][r] = a.q[(size_t)(rhs0 + r) * N + jsta + jb + l];
-        }
-        __syncthreads();
-        const int nsteps = (nk + 3) >> 2;
-        for (int st = 0; st < nsteps; st++) {
-          const int k = 4 * st + kq;
-          const bool kv = k < nk;
-          const float4 y = sp[kv ? k : 0];
-          const float dx0 = y.x - x0.x, dy0 = y.y - x0.y, dz0 = y.z - x0.z;
-          const float dx1 = y.x - x1.x, dy1 = y.y - x1.y, dz1 = y.z - x1.z;
-          const float rr0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
-          const float rr1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-          float g0 = __builtin_amdgcn_rsqf(rr0), g1 = __builtin_amdgcn_rsqf(rr1);
-          if (self) {
-            g0 = rr0 != 0.f ? g0 : 0.f;
-            g1 = rr1 != 0.f ? g1 : 0.f;
-          }
-          g0 = kv ? g0 : 0.f;
-          g1 = kv ? g1 : 0.f;
-          const float bq = kv ? sq[k][ti] : 0.f;
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(g0, bq, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(g1, bq, acc1, 0, 0, 0);
-        }
-        __syncthreads();
-      }
-    }
-    // lane l holds rows (targets) 4*(l/16)+v of column (vector) l%16
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int i = 4 * kq + v;
-      const size_t row = (size_t)(rhs0 + ti) * N + sta;
-      if (tb + i < cnt) a.phi[row + tb + i] += a.q[row + tb + i] * acc0[v];
-      if (tb + 16 + i < cnt) a.phi[row + tb + 16 + i] += a.q[row + tb + 16 + i] * acc1[v];
-    }
-  }
-}
 #endif
 
 // ---------------------------------------------------------------- padded layout
@@ -346,7 +147,7 @@ __device__ __forceinline__ void p2p16_step(const float4 &z, float bq, const floa
 }
 
 // One pass over all sources for NT tiles of 16 targets starting at target
-// tb of the leaf; results added into phi (q * potential).
+// tb of the leaf; results added into phi (q * potential, or the potential).
 template <int NT>
 __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, int nseg, int T, int sta,
                                            int cnt, int psta, int pself, int tb)
@@ -411,7 +212,7 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int i = tb + 16 * t + 4 * kq + v;
-      if (i < cnt) a.phi[row + i] += a.q[row + i] * acc[t][v];
+      if (i < cnt) a.phi[row + i] += (a.potential ? 1.0f : a.q[row + i]) * acc[t][v];
     }
 }
 

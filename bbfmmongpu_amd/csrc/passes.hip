@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(256) k_l2p(PassArgs a)
       sum += sxv[l1] * syv[l2] * inner;
     }
   const size_t o = (size_t)rhs * a.N + i;
-  a.phi[o] = a.q[o] * a.prefac3 * sum;
+  a.phi[o] = (a.potential ? (real)1 : a.q[o]) * a.prefac3 * sum;
 }
 
 // L2P for NR vectors: one block per leaf, the leaf's F values in LDS
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(64) k_l2p_leaf(PassArgs a)
 #pragma unroll
     for (int k = 0; k < RS; k++) {
       const size_t o = (size_t)(rhs0 + g * RS + k) * a.N + sta + j;
-      a.phi[o] = a.q[o] * a.prefac3 * sum[k];
+      a.phi[o] = (a.potential ? (real)1 : a.q[o]) * a.prefac3 * sum[k];
     }
   }
 }

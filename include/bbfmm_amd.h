@@ -157,6 +157,8 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
 #define BBFMM_OPT_FORM 2      /* BBFMM_M2L_* */
 #define BBFMM_OPT_TIMING 3    /* k > 0: record per-stage HIP events on every solve into a
                                  ring of k event sets; 0: off (resets the averages) */
+#define BBFMM_OPT_OUTPUT 4    /* 0: q .* potential (the reference's output, default);
+                                 1: the potential */
 
 /* Create a plan for N particles at host positions pos (N real3, inside the
    root cell [-L/2,L/2]^3), Chebyshev order n, level modifier lmod (the -d
@@ -183,6 +185,24 @@ int bbfmm_amd_solve(bbfmm_amd_plan *plan, const real *q_dev, real *phi_dev);
 /* Same from host arrays (PCIe copies included; synchronous). */
 int bbfmm_amd_solve_host(bbfmm_amd_plan *plan, const real *q, real *phi);
 
+/* Move the particles to new host positions pos (N real3, original order)
+   that stay in their leaves: the tree, lists and operators are kept.
+   Returns BBFMM_E_RANGE, leaving the plan unchanged, if a particle would
+   change leaf (build a new plan then).  No counterpart in the reference
+   (which rebuilds the tree for every call). */
+int bbfmm_amd_plan_update_positions(bbfmm_amd_plan *plan, const real3 *pos);
+
+/* Adjoint of the solve (no counterpart in the reference): for a plan made
+   with nrhs = 2m vectors, charges q_dev (m x N) and g_dev = dLoss/dphi
+   (m x N, phi the potentials), writes dq_dev = dLoss/dq (m x N, the
+   potentials of the charges g, the FMM operator being symmetric) and
+   dx_dev = dLoss/dpositions (N x 3, original order).  The expansions are
+   recomputed by one solve of the 2m vectors [q; g]; the far part of dx
+   differentiates the L2P interpolant (Chebyshev derivative) and the near
+   part the 1/r kernel over the P2P neighbourhoods.  At most 16 vectors. */
+int bbfmm_amd_gradient(bbfmm_amd_plan *plan, const real *q_dev, const real *g_dev, real *dq_dev,
+                       real *dx_dev);
+
 /* M2L alone on the plan's tree: M_dev and L_dev are nrhs x ncell x r
    device arrays in global cell order; L is written for levels 2..maxlev. */
 int bbfmm_amd_m2l(bbfmm_amd_plan *plan, const real *M_dev, real *L_dev);
@@ -203,6 +223,13 @@ int bbfmm_amd_stage_times(bbfmm_amd_plan *plan, double *out, int nout);
    out[t] = q_i * sum_{j: x_j != x_i} q_j / |x_i - x_j| for i = targets[t]
    (all particles when targets is NULL, nt = N).  pos is N x 3 doubles. */
 int bbfmm_amd_direct(int N, const double *pos, const double *q, int nt, const int *targets, double *out);
+/* FP64 direct-sum potentials and their gradients for nvec charge vectors
+   (q: nvec x N): for target i = targets[t], pot[v*nt + t] =
+   sum_{j: x_j != x_i} q_vj / r_ij and grad[(v*nt + t)*3 + c] = the c-th
+   component of its gradient with respect to x_i.  The analytic reference of
+   the adjoint; no counterpart in the reference. */
+int bbfmm_amd_direct_field(int N, const double *pos, int nvec, const double *q, int nt, const int *targets,
+                           double *pot, double *grad);
 
 /* Host-side setup helpers (reference layouts) */
 int bbfmm_amd_maxlev(int N, int lmod);                                   /* bbfmm.c:151-160 */
