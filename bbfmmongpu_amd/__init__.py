@@ -69,6 +69,7 @@ def load(prec: str = "s"):
     L.bbfmm_amd_direct_field.argtypes = [i, vp, i, vp, i, vp, vp, vp]
     L.bbfmm_amd_gradient.argtypes = [vp, vp, vp, vp, vp]
     L.bbfmm_amd_plan_update_positions.argtypes = [vp, vp]
+    L.bbfmm_amd_batch_create.argtypes = [C.POINTER(vp), i, vp, vp, vp, i, R, i, i, vp, vp, i, vp, vp]
     L.bbfmm_amd_set_default_scheme.argtypes = [i]
     L.m2l_drv.argtypes = [i, vp, vp, vp, i, i, R, vp, vp, i, vp, vp, i, vp, i]
     for name in ("anotherFMMCompute",):
@@ -195,10 +196,12 @@ def _torch():
 
 
 class Plan:
-    """Persistent device plan (bbfmm_amd_plan_create)."""
+    """Persistent device plan (bbfmm_amd_plan_create), or with
+    Plan.batch(...) one plan over many independent problems
+    (bbfmm_amd_batch_create)."""
 
     def __init__(self, pos, n: int, U, K, nrhs: int = 1, lmod: int = 0, L: float = 1.0,
-                 prec: str = "s", stream=None):
+                 prec: str = "s", stream=None, _batch=None):
         self.prec = prec
         self.lib = load(prec)
         dt = dtype(prec)
@@ -210,15 +213,41 @@ class Plan:
         self._K = np.ascontiguousarray(K, dtype=dt)
         self.r = self._U.size // n ** 3
         h = C.c_void_p()
-        _check(self.lib, self.lib.bbfmm_amd_plan_create(
-            C.byref(h), self.N, _p(self.pos), L, n, lmod, self.r, _p(self._K), _p(self._U), nrhs,
-            C.c_void_p(stream)))
+        if _batch is None:
+            _check(self.lib, self.lib.bbfmm_amd_plan_create(
+                C.byref(h), self.N, _p(self.pos), L, n, lmod, self.r, _p(self._K), _p(self._U), nrhs,
+                C.c_void_p(stream)))
+            self.sizes, self.depths = [self.N], None
+        else:
+            sizes, depths, leafmax = _batch
+            self.sizes = [int(x) for x in sizes]
+            if sum(self.sizes) != self.N:
+                raise BBFMMError("instance sizes do not add up to the points given")
+            ni = len(self.sizes)
+            Ns = np.asarray(self.sizes, np.int32)
+            dreq = None if depths is None else np.ascontiguousarray(depths, np.int32)
+            dout = np.zeros(ni, np.int32)
+            _check(self.lib, self.lib.bbfmm_amd_batch_create(
+                C.byref(h), ni, _p(Ns), _p(self.pos), None if dreq is None else _p(dreq), leafmax, L, n,
+                self.r, _p(self._K), _p(self._U), nrhs, C.c_void_p(stream), _p(dout)))
+            self.depths = dout.tolist()
         self.h = h
         info = np.zeros(6, np.int32)
         dinfo = np.zeros(2)
         self.lib.bbfmm_amd_plan_info(self.h, _p(info), _p(dinfo))
         self.maxlev, self.ncell = int(info[0]), int(info[1])
         self.m2l_interactions, self.p2p_pairs = float(dinfo[0]), float(dinfo[1])
+
+    @classmethod
+    def batch(cls, pos_list, n: int, U, K, nrhs: int = 1, depths=None, leafmax: int = 32, L: float = 1.0,
+              prec: str = "s", stream=None):
+        """One plan over independent problems pos_list[i] (N_i x 3 each),
+        each on a sparse octree of uniform depth (depths[i], or the smallest
+        depth whose leaves hold at most leafmax points).  Charges and
+        results are nrhs x sum(N_i), the instances concatenated."""
+        sizes = [np.asarray(p).reshape(-1, 3).shape[0] for p in pos_list]
+        pos = np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1, 3) for p in pos_list])
+        return cls(pos, n, U, K, nrhs=nrhs, L=L, prec=prec, stream=stream, _batch=(sizes, depths, leafmax))
 
     def set(self, scheme: int | None = None, form: int | None = None, timing: int | None = None,
             output: str | None = None):

@@ -4,6 +4,7 @@
 // the event timers) and the tree builder.
 #include "common.h"
 #include "kernels.h"
+#include "forest.h"
 
 #include <cmath>
 #include <cstdarg>
@@ -59,6 +60,9 @@ enum { ST_GATHER, ST_P2M, ST_M2M, ST_M2L_PREP, ST_M2L, ST_POST, ST_L2L, ST_L2P, 
 
 struct bbfmm_amd_plan {
   int N = 0, n = 0, n3 = 0, r = 0, nrhs = 1, maxlev = 0, ncell = 0, nleaf = 0, leaf0 = 0;
+  bool complete = true;  // one complete octree (reference layout); false: a batch of sparse instances
+  int ninst = 1;
+  std::vector<int> h_ploff;  // [maxlev+1] offsets of each level's parents in d_plist
   real L = 1, homogen = 1;
   hipStream_t stream = nullptr;
   int scheme = M2L_SCHEME_BA, form = BBFMM_M2L_PLAIN, timing = 0;
@@ -69,6 +73,8 @@ struct bbfmm_amd_plan {
   real4 *d_pos = nullptr, *d_center = nullptr, *d_K4 = nullptr;
   int *d_perm = nullptr, *d_leafsta = nullptr, *d_leafcnt = nullptr, *d_leafof = nullptr;
   int *d_iperm = nullptr;  // original index -> leaf order
+  int *d_celllev = nullptr, *d_child = nullptr, *d_plist = nullptr, *d_leafcell = nullptr;
+  real *d_leafihalf = nullptr;
   int *d_nbr = nullptr, *d_nnbr = nullptr, *d_iinter = nullptr, *d_inter = nullptr, *d_kidx = nullptr;
   int *d_Ktable = nullptr, *d_levsta = nullptr;
   real *d_Ucm = nullptr, *d_Urm = nullptr, *d_Tkz = nullptr, *d_Kw = nullptr, *d_Cw = nullptr;
@@ -83,8 +89,8 @@ struct bbfmm_amd_plan {
   int npad = 0;
   int potential = 0;           // output: 0 = q * potential (reference), 1 = potential
   // host copies for moving particles inside their leaves (update_positions)
-  std::vector<int> h_leafof, h_pad_of, h_leafxyz;  // [N], [N], [nleaf*3]
-  real h_inv_leaf = 0;
+  std::vector<int> h_leafof, h_pad_of, h_leafxyz, h_leafdepth;  // [N], [N], [nleaf*3], [nleaf]
+  std::vector<real> h_leafinv;                                   // [nleaf] 1 / leaf length
   real *d_grad = nullptr;      // [3][N] position-gradient scratch (allocated by the first gradient call)
   std::vector<real> h_K;                       // column-major operators (for the grid form)
   bb::GridM2L *grid = nullptr;
@@ -139,7 +145,8 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_nbr, p->d_nnbr, p->d_iinter, p->d_inter, p->d_kidx, p->d_Ktable, p->d_levsta,
                   p->d_Ucm, p->d_Urm, p->d_Tkz, p->d_Kw, p->d_Cw, p->d_scale, p->d_q, p->d_phi,
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
-                  p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad};
+                  p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad,
+                  p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->grid) grid_m2l_destroy(p->grid);
@@ -188,97 +195,49 @@ static inline hipEvent_t stage_event(bbfmm_amd_plan *p, int st)
   return p->ev[(size_t)(p->solves % p->nsets) * (ST_TOTAL + 1) + st];
 }
 
-// Upload everything a solve needs, from a tree in the reference's layout.
-// Tkz/Kw/Cw are the caller's Chebyshev weights; K and U are in the
-// reference's column-major file layouts.
-static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, int n, int r,
-                     const real *K, const real *U, const real *Tkz, const real *Kw, const real *Cw,
-                     real homogen, int nrhs, hipStream_t stream)
+// Upload everything a solve needs from a forest (one complete tree in the
+// reference's layout, or a batch of sparse instances).  Tkz/Kw/Cw are the
+// caller's Chebyshev weights; K and U are in the reference's column-major
+// file layouts.
+static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 *pos, int n, int r,
+                            const real *K, const real *U, const real *Tkz, const real *Kw, const real *Cw,
+                            real homogen, int nrhs, hipStream_t stream)
 {
   BB_TRY(check_device());
   if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
   if (r != 32 && r != 256) return set_error(BBFMM_E_ARG, "SVD rank r=%d not supported (r = 32 or 256)", r);
   if (r > n * n * n) return set_error(BBFMM_E_ARG, "rank r=%d exceeds n^3=%d", r, n * n * n);
   if (nrhs < 1) return set_error(BBFMM_E_ARG, "nrhs must be >= 1");
-  if (t->maxlev < kMinLev) return set_error(BBFMM_E_ARG, "maxlev=%d < %d", t->maxlev, kMinLev);
-  const cell *c = t->c;
-  p->N = t->Nf;
+  if (f.maxdepth < kMinLev) return set_error(BBFMM_E_ARG, "maxlev=%d < %d", f.maxdepth, kMinLev);
+  p->complete = f.complete;
+  p->ninst = (int)f.inst_N.size();
+  p->N = f.N;
   p->n = n;
   p->n3 = n * n * n;
   p->r = r;
   p->nrhs = nrhs;
-  p->maxlev = t->maxlev;
-  p->ncell = t->ncell;
-  p->leaf0 = t->levsta[t->maxlev];
-  p->nleaf = t->levend[t->maxlev] - p->leaf0 + 1;
-  p->L = t->celeng[0];
+  p->maxlev = f.maxdepth;
+  p->ncell = f.ncell;
+  p->nleaf = f.nleaf;
+  p->leaf0 = f.complete ? f.levsta[f.maxdepth] : 0;
+  p->L = f.celeng[0];
   p->homogen = homogen;
   p->stream = stream;
-  p->h_levsta.assign(t->levsta, t->levsta + t->maxlev + 1);
-  p->h_levsta.push_back(t->levend[t->maxlev] + 1);
+  if (f.complete) p->h_levsta = f.levsta;
+  p->h_ploff = f.ploff;
   const int N = p->N, ncell = p->ncell, nleaf = p->nleaf, n3 = p->n3;
 
   // particles in leaf order
-  p->h_perm.assign(t->fieldlist, t->fieldlist + N);
+  p->h_perm = f.perm;
   std::vector<real4> hpos((size_t)N);
-  std::vector<int> leafof((size_t)N), lsta((size_t)nleaf), lcnt((size_t)nleaf);
-  for (int k = 0; k < nleaf; k++) {
-    const int A = p->leaf0 + k;
-    lsta[k] = c->fieldsta[A];
-    lcnt[k] = c->fieldend[A] - c->fieldsta[A] + 1;
-    for (int i = c->fieldsta[A]; i <= c->fieldend[A]; i++) leafof[i] = k;
-  }
   for (int i = 0; i < N; i++) {
     const real3 x = pos[p->h_perm[i]];
-    hpos[i].x = x.x;
-    hpos[i].y = x.y;
-    hpos[i].z = x.z;
-    hpos[i].w = 0;
+    hpos[i] = real4{x.x, x.y, x.z, 0};
   }
-  std::vector<real4> hctr((size_t)ncell);
-  for (int A = 0; A < ncell; A++) {
-    hctr[A].x = c->center[A].x;
-    hctr[A].y = c->center[A].y;
-    hctr[A].z = c->center[A].z;
-    hctr[A].w = 0;
-  }
-  // neighbour leaves (local indices), in the reference's list order
-  std::vector<int> nbr((size_t)nleaf * 27, -1), nnbr((size_t)nleaf);
-  double pairs = 0;
-  for (int k = 0; k < nleaf; k++) {
-    const int A = p->leaf0 + k;
-    const int ne = c->ineigh[A];
-    if (ne > 27) return set_error(BBFMM_E_ARG, "leaf %d has %d neighbours (> 27)", A, ne);
-    nnbr[k] = ne;
-    for (int b = 0; b < ne; b++) {
-      const int B = c->neighbors[(size_t)c->pitch_neighbors * A + b] - p->leaf0;
-      nbr[(size_t)k * 27 + b] = B;
-      pairs += (double)lcnt[k] * (double)lcnt[B];
-    }
-  }
-  p->p2p_pairs = pairs;
-  // interaction lists with pitch 192 and their operator indices
-  std::vector<int> inter((size_t)ncell * kPitchInteraction, 0), kidx((size_t)ncell * kPitchInteraction, 0);
-  std::vector<int> iinter(c->iinter, c->iinter + ncell);
+  p->p2p_pairs = f.pairs;
+  p->m2l_interactions = f.ninter;
   int Kt[343];
   ktable(Kt);
-  double ninter = 0;
-  for (int A = 0; A < ncell; A++) {
-    if (iinter[A] > kPitchInteraction) return set_error(BBFMM_E_ARG, "cell %d has %d interactions", A, iinter[A]);
-    for (int i = 0; i < iinter[A]; i++)
-      inter[(size_t)kPitchInteraction * A + i] = c->interaction[(size_t)c->pitch_interaction * A + i];
-    if (A >= p->h_levsta[kMinLev]) ninter += iinter[A];
-  }
-  p->m2l_interactions = ninter;
-  {
-    // operator index per interaction (m2l_aux.c:1135-1191)
-    anotherTree tv = *t;
-    cell cv = *c;
-    cv.interaction = inter.data();
-    cv.pitch_interaction = kPitchInteraction;
-    tv.c = &cv;
-    kindex(&tv, Kt, kidx.data());
-  }
   // operators: K4 (four consecutive columns of a row), U both ways
   std::vector<real4> K4((size_t)kNumOps * r * r / 4);
   for (int k = 0; k < kNumOps; k++)
@@ -297,26 +256,31 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
   for (int l = 0; l < n3; l++)
     for (int i = 0; i < r; i++) Urm[(size_t)l * r + i] = U[l + (size_t)n3 * i];
   std::vector<real> scale((size_t)p->maxlev + 1);
-  for (int l = 0; l <= p->maxlev; l++) scale[l] = std::pow((real)1 / t->celeng[l], homogen);
+  for (int l = 0; l <= p->maxlev; l++) scale[l] = std::pow((real)1 / f.celeng[l], homogen);
 
   BB_TRY(upload(&p->d_pos, hpos.data(), (size_t)N, stream));
-  BB_TRY(upload(&p->d_center, hctr.data(), (size_t)ncell, stream));
+  BB_TRY(upload(&p->d_center, f.center.data(), (size_t)ncell, stream));
+  BB_TRY(upload(&p->d_celllev, f.celllev.data(), (size_t)ncell, stream));
+  BB_TRY(upload(&p->d_child, f.child.data(), f.child.size(), stream));
+  BB_TRY(upload(&p->d_plist, f.plist.data(), f.plist.size(), stream));
+  BB_TRY(upload(&p->d_leafcell, f.leafcell.data(), (size_t)nleaf, stream));
+  BB_TRY(upload(&p->d_leafihalf, f.leafihalf.data(), (size_t)nleaf, stream));
   BB_TRY(upload(&p->d_perm, p->h_perm.data(), (size_t)N, stream));
   {
     std::vector<int> iperm((size_t)N);
     for (int i = 0; i < N; i++) iperm[p->h_perm[i]] = i;
     BB_TRY(upload(&p->d_iperm, iperm.data(), (size_t)N, stream));
   }
-  BB_TRY(upload(&p->d_leafsta, lsta.data(), (size_t)nleaf, stream));
-  BB_TRY(upload(&p->d_leafcnt, lcnt.data(), (size_t)nleaf, stream));
-  BB_TRY(upload(&p->d_leafof, leafof.data(), (size_t)N, stream));
-  BB_TRY(upload(&p->d_nbr, nbr.data(), nbr.size(), stream));
-  BB_TRY(upload(&p->d_nnbr, nnbr.data(), nnbr.size(), stream));
-  BB_TRY(upload(&p->d_iinter, iinter.data(), iinter.size(), stream));
-  BB_TRY(upload(&p->d_inter, inter.data(), inter.size(), stream));
-  BB_TRY(upload(&p->d_kidx, kidx.data(), kidx.size(), stream));
+  BB_TRY(upload(&p->d_leafsta, f.leafsta.data(), (size_t)nleaf, stream));
+  BB_TRY(upload(&p->d_leafcnt, f.leafcnt.data(), (size_t)nleaf, stream));
+  BB_TRY(upload(&p->d_leafof, f.leafof.data(), (size_t)N, stream));
+  BB_TRY(upload(&p->d_nbr, f.nbr.data(), f.nbr.size(), stream));
+  BB_TRY(upload(&p->d_nnbr, f.nnbr.data(), f.nnbr.size(), stream));
+  BB_TRY(upload(&p->d_iinter, f.iinter.data(), f.iinter.size(), stream));
+  BB_TRY(upload(&p->d_inter, f.inter.data(), f.inter.size(), stream));
+  BB_TRY(upload(&p->d_kidx, f.kidx.data(), f.kidx.size(), stream));
   BB_TRY(upload(&p->d_Ktable, Kt, (size_t)343, stream));
-  BB_TRY(upload(&p->d_levsta, p->h_levsta.data(), p->h_levsta.size(), stream));
+  if (f.complete) BB_TRY(upload(&p->d_levsta, p->h_levsta.data(), p->h_levsta.size(), stream));
   BB_TRY(upload(&p->d_K4, K4.data(), K4.size(), stream));
   BB_TRY(upload(&p->d_Ucm, U, (size_t)n3 * r, stream));
   BB_TRY(upload(&p->d_Urm, Urm.data(), Urm.size(), stream));
@@ -326,16 +290,24 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
   BB_TRY(upload(&p->d_scale, scale.data(), scale.size(), stream));
   {
     // leaf grid coordinates (the bucketing of build_tree) for update_positions
-  p->h_leafof = leafof;
-  p->h_inv_leaf = 1 / t->celeng[t->maxlev];
-  p->h_leafxyz.resize((size_t)nleaf * 3);
-  for (int k = 0; k < nleaf; k++) {
-    const real3 cc = c->center[p->leaf0 + k];
-    p->h_leafxyz[3 * k] = (int)((cc.x + p->L / 2) * p->h_inv_leaf);
-    p->h_leafxyz[3 * k + 1] = (int)((cc.y + p->L / 2) * p->h_inv_leaf);
-    p->h_leafxyz[3 * k + 2] = (int)((cc.z + p->L / 2) * p->h_inv_leaf);
+    p->h_leafof = f.leafof;
+    p->h_leafinv.resize(nleaf);
+    p->h_leafxyz.resize((size_t)nleaf * 3);
+    for (int k = 0; k < nleaf; k++) {
+      const int A = f.leafcell[k];
+      const real inv = 1 / f.celeng[f.celllev[A]];
+      p->h_leafinv[k] = inv;
+      const real4 cc = f.center[A];
+      p->h_leafxyz[3 * k] = (int)((cc.x + p->L / 2) * inv);
+      p->h_leafxyz[3 * k + 1] = (int)((cc.y + p->L / 2) * inv);
+      p->h_leafxyz[3 * k + 2] = (int)((cc.z + p->L / 2) * inv);
+    }
+    p->h_leafdepth.resize(nleaf);
+    for (int k = 0; k < nleaf; k++) p->h_leafdepth[k] = f.celllev[f.leafcell[k]];
   }
-  // padded copy for P2P: leaf ranges start at multiples of 4, dummies far away with q = 0
+  {
+    // padded copy for P2P: leaf ranges start at multiples of 4, dummies far away with q = 0
+    const std::vector<int> &lsta = f.leafsta, &lcnt = f.leafcnt, &nbr = f.nbr, &nnbr = f.nnbr;
     std::vector<int> psta((size_t)nleaf), pcnt((size_t)nleaf), pad_of((size_t)N);
     int run = 0;
     for (int k = 0; k < nleaf; k++) {
@@ -384,6 +356,16 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
   return BBFMM_OK;
 }
 
+static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, int n, int r, const real *K,
+                     const real *U, const real *Tkz, const real *Kw, const real *Cw, real homogen, int nrhs,
+                     hipStream_t stream)
+{
+  if (t->maxlev < kMinLev) return set_error(BBFMM_E_ARG, "maxlev=%d < %d", t->maxlev, kMinLev);
+  HostForest f;
+  BB_TRY(forest_from_tree(f, t));
+  return plan_init_forest(p, f, pos, n, r, K, U, Tkz, Kw, Cw, homogen, nrhs, stream);
+}
+
 static PassArgs pass_args(const bbfmm_amd_plan *p)
 {
   PassArgs a;
@@ -393,9 +375,6 @@ static PassArgs pass_args(const bbfmm_amd_plan *p)
   a.ncell = p->ncell;
   a.maxlev = p->maxlev;
   a.nleaf = p->nleaf;
-  a.leaf0 = p->leaf0;
-  const real leaflen = p->L / (real)(1 << p->maxlev);
-  a.ihalfL = (real)1 / (leaflen / (real)2);
   const real pf = (real)2 / (real)p->n;
   a.prefac3 = pf * pf * pf;
   a.pos = p->d_pos;
@@ -403,7 +382,11 @@ static PassArgs pass_args(const bbfmm_amd_plan *p)
   a.leafsta = p->d_leafsta;
   a.leafcnt = p->d_leafcnt;
   a.leafof = p->d_leafof;
-  a.levsta = p->d_levsta;
+  a.leafcell = p->d_leafcell;
+  a.leafihalf = p->d_leafihalf;
+  a.celllev = p->d_celllev;
+  a.child = p->d_child;
+  a.plist = p->d_plist;
   a.scale = p->d_scale;
   a.Tkz = p->d_Tkz;
   a.Kw = p->d_Kw;
@@ -428,7 +411,7 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
   m.nrhs = p->nrhs;
   m.minlev = kMinLev;
   m.maxlev = p->maxlev;
-  m.h_levsta = p->h_levsta.data();
+  m.h_levsta = p->complete ? p->h_levsta.data() : nullptr;
   m.iinter = p->d_iinter;
   m.inter = p->d_inter;
   m.kidx = p->d_kidx;
@@ -436,6 +419,8 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
   m.Ktable = p->d_Ktable;
   m.M = M;
   m.L = L;
+  if (!p->complete && p->scheme != M2L_SCHEME_BA)
+    return set_error(BBFMM_E_ARG, "a batch of sparse instances runs the basic M2L scheme only");
   switch (p->scheme) {
   case M2L_SCHEME_BA:
     if (mid) BB_HIP(hipEventRecord(mid, p->stream));
@@ -467,13 +452,13 @@ static int solve_sorted(bbfmm_amd_plan *p)
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2M), s));
   BB_TRY(launch_p2m(a, s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2M), s));
-  BB_TRY(launch_m2m(a, p->h_levsta.data(), s, p->nrhs));
+  BB_TRY(launch_m2m(a, p->h_ploff.data(), s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2L_PREP), s));
   BB_TRY(run_m2l(p, p->d_M, p->d_L, tm ? stage_event(p, ST_M2L) : nullptr));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_POST), s));
-  BB_TRY(launch_postm2l(a, p->h_levsta.data(), s, p->nrhs));
+  BB_TRY(launch_postm2l(a, s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2L), s));
-  BB_TRY(launch_l2l(a, p->h_levsta.data(), s, p->nrhs));
+  BB_TRY(launch_l2l(a, p->h_ploff.data(), s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2P), s));
   BB_TRY(launch_l2p(a, s, p->nrhs));
   BB_HIP(hipMemcpyAsync(p->d_farfield, p->d_phi, sizeof(real) * p->N, hipMemcpyDeviceToDevice, s));
@@ -564,6 +549,35 @@ extern "C" int bbfmm_amd_plan_create(bbfmm_amd_plan **plan, int N, const real3 *
   return BBFMM_OK;
 }
 
+extern "C" int bbfmm_amd_batch_create(bbfmm_amd_plan **plan, int ninst, const int *N, const real3 *pos,
+                                     const int *depth, int leafmax, real L, int n, int r, const real *K,
+                                     const real *U, int nrhs, void *stream, int *depth_out)
+{
+  clear_error();
+  if (!plan || !N || !pos || !K || !U || ninst < 1) return set_error(BBFMM_E_ARG, "null argument");
+  *plan = nullptr;
+  if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
+  HostForest f;
+  std::vector<int> dep;
+  BB_TRY(forest_from_points(f, ninst, N, pos, depth, leafmax, L, &dep));
+  std::vector<real> Tkz((size_t)n * n), Kw((size_t)n * n * n), Cw((size_t)2 * n * n);
+  int Kt[343];
+  weights(n, Tkz.data(), Kt, Kw.data(), Cw.data());
+  bbfmm_amd_plan *p = new bbfmm_amd_plan();
+  const int rc = plan_init_forest(p, f, pos, n, r, K, U, Tkz.data(), Kw.data(), Cw.data(), (real)1, nrhs,
+                                  (hipStream_t)stream);
+  if (rc != BBFMM_OK) {
+    plan_free_device(p);
+    delete p;
+    return rc;
+  }
+  p->scheme = M2L_SCHEME_BA;
+  if (depth_out)
+    for (int i = 0; i < ninst; i++) depth_out[i] = dep[i];
+  *plan = p;
+  return BBFMM_OK;
+}
+
 extern "C" int bbfmm_amd_plan_destroy(bbfmm_amd_plan *p)
 {
   if (!p) return BBFMM_OK;
@@ -580,6 +594,8 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
   switch (option) {
   case BBFMM_OPT_SCHEME:
     if (value < M2L_SCHEME_BA || value > M2L_SCHEME_IJ) return set_error(BBFMM_E_ARG, "unknown M2L scheme %d", value);
+    if (!p->complete && value != M2L_SCHEME_BA)
+      return set_error(BBFMM_E_ARG, "a batch of sparse instances runs the basic M2L scheme only");
     p->scheme = value;
     return BBFMM_OK;
   case BBFMM_OPT_FORM:
@@ -646,7 +662,6 @@ extern "C" int bbfmm_amd_plan_update_positions(bbfmm_amd_plan *p, const real3 *p
   if (!p || !pos) return set_error(BBFMM_E_ARG, "null argument");
   const int N = p->N;
   const real half = p->L / 2;
-  const int ncpd = 1 << p->maxlev;
   std::vector<real4> hpos((size_t)N), ppos((size_t)(p->npad > 0 ? p->npad : 1));
   for (auto &x : ppos) {
     x.x = x.y = x.z = (real)1e6;
@@ -654,14 +669,16 @@ extern "C" int bbfmm_amd_plan_update_positions(bbfmm_amd_plan *p, const real3 *p
   }
   for (int i = 0; i < N; i++) {
     const real3 x = pos[p->h_perm[i]];
-    int ix = (int)((x.x + half) * p->h_inv_leaf), iy = (int)((x.y + half) * p->h_inv_leaf),
-        iz = (int)((x.z + half) * p->h_inv_leaf);
+    const int k = p->h_leafof[i];
+    const real inv = p->h_leafinv[k];
+    const int ncpd = 1 << p->h_leafdepth[k];
+    int ix = (int)((x.x + half) * inv), iy = (int)((x.y + half) * inv), iz = (int)((x.z + half) * inv);
     if (ix < 0 || iy < 0 || iz < 0 || !(x.x <= half) || !(x.y <= half) || !(x.z <= half))
       return set_error(BBFMM_E_RANGE, "a particle lies outside the root cell [-L/2, L/2]^3");
     ix = ix < ncpd - 1 ? ix : ncpd - 1;
     iy = iy < ncpd - 1 ? iy : ncpd - 1;
     iz = iz < ncpd - 1 ? iz : ncpd - 1;
-    const int *c = &p->h_leafxyz[(size_t)3 * p->h_leafof[i]];
+    const int *c = &p->h_leafxyz[(size_t)3 * k];
     if (ix != c[0] || iy != c[1] || iz != c[2])
       return set_error(BBFMM_E_RANGE, "particle %d moved to another leaf: build a new plan", p->h_perm[i]);
     hpos[i] = real4{x.x, x.y, x.z, 0};
@@ -699,12 +716,12 @@ extern "C" int bbfmm_amd_gradient(bbfmm_amd_plan *p, const real *q_dev, const re
   ga.N = N;
   ga.n = p->n;
   ga.nleaf = p->nleaf;
-  ga.leaf0 = p->leaf0;
   ga.ncell = p->ncell;
   ga.m = m;
   const PassArgs pa = pass_args(p);
-  ga.ihalfL = pa.ihalfL;
   ga.prefac3 = pa.prefac3;
+  ga.leafcell = p->d_leafcell;
+  ga.leafihalf = p->d_leafihalf;
   ga.pos = p->d_pos;
   ga.center = p->d_center;
   ga.leafsta = p->d_leafsta;
@@ -739,7 +756,7 @@ extern "C" int bbfmm_amd_fetch(bbfmm_amd_plan *p, int which, real *dst)
   case 2: {
     // F after post-M2L: recompute it from L (the solve's F also holds L2L)
     const PassArgs a = pass_args(p);
-    BB_TRY(launch_postm2l(a, p->h_levsta.data(), p->stream, 1));
+    BB_TRY(launch_postm2l(a, p->stream, 1));
     BB_HIP(hipMemcpyAsync(dst, p->d_F, sizeof(real) * (size_t)p->ncell * p->n3, hipMemcpyDeviceToHost, p->stream));
     BB_HIP(hipStreamSynchronize(p->stream));
     break;
@@ -985,7 +1002,7 @@ extern "C" void anotherFMMInteraction(anotherTree **atree, real *E, int *Ktable,
   if (run_m2l(p, p->d_M, p->d_L) != BBFMM_OK) return;
   (void)hipEventRecord(e1, p->stream);
   const PassArgs a = pass_args(p);
-  if (launch_postm2l(a, p->h_levsta.data(), p->stream, 1) != BBFMM_OK) return;
+  if (launch_postm2l(a, p->stream, 1) != BBFMM_OK) return;
   float ms = 0;
   (void)hipEventSynchronize(e1);
   (void)hipEventElapsedTime(&ms, e0, e1);
@@ -1040,7 +1057,7 @@ extern "C" void anotherDownwardPass(anotherTree **atree, real3 *field, real3 *so
   }
   if (launch_gather(p->d_perm, p->d_qin, p->d_q, p->N, 1, s) != BBFMM_OK) return;
   const PassArgs a = pass_args(p);
-  if (launch_l2l(a, p->h_levsta.data(), s, 1) != BBFMM_OK) return;
+  if (launch_l2l(a, p->h_ploff.data(), s, 1) != BBFMM_OK) return;
   if (launch_l2p(a, s, 1) != BBFMM_OK) return;
   P2PArgs pp;
   pp.N = p->N;

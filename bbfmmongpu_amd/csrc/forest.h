@@ -1,0 +1,44 @@
+// forest.h -- host description of the cells, leaves, particles and lists a
+// plan uploads: one complete octree in the reference's numbering, or a
+// batch of independent instances each on its own sparse octree of uniform
+// depth (only the cells that hold particles), concatenated into one global
+// numbering of cells, leaves and particles.
+#pragma once
+
+#include "common.h"
+
+#include <vector>
+
+namespace bb {
+
+struct HostForest {
+  int N = 0, ncell = 0, nleaf = 0, maxdepth = 0;
+  bool complete = false;            // one complete octree (cells 0..ncell-1 as cell.h:27-28)
+  std::vector<int> levsta;          // complete: first cell of each level, [maxdepth + 2]
+  std::vector<real> celeng;         // cell length per level from the root, [maxdepth + 1]
+  std::vector<real4> center;        // [ncell]
+  std::vector<int> celllev;         // [ncell]
+  std::vector<int> child;           // [ncell * 8], -1 when absent
+  std::vector<int> plist, ploff;    // cells with children, grouped by level; ploff [maxdepth + 1]
+  std::vector<int> leafcell, leafsta, leafcnt;  // [nleaf]
+  std::vector<real> leafihalf;      // [nleaf] 1 / (half the leaf's length)
+  std::vector<int> perm;            // [N] leaf order -> input index
+  std::vector<int> leafof;          // [N] leaf of each leaf-ordered particle
+  std::vector<int> nbr, nnbr;       // [nleaf * 27] neighbour leaves (self included), [nleaf]
+  std::vector<int> iinter, inter, kidx;  // [ncell], [ncell * kPitchInteraction] x 2
+  double pairs = 0, ninter = 0;     // P2P pairs and M2L interactions (levels >= 2), per vector
+  std::vector<int> inst_N, inst_depth, inst_p0, inst_leaf0;  // batch instances
+};
+
+// The reference's complete tree (anotherTree after build_tree).
+int forest_from_tree(HostForest &f, const anotherTree *t);
+
+// ninst instances: N[i] points each, concatenated in pos; depth[i] the
+// uniform depth of instance i, or (depth == NULL or depth[i] <= 0) the
+// smallest depth >= 2 whose leaves hold at most leafmax points (at most
+// kMaxForestDepth).  Every instance sits in the root cell [-L/2, L/2]^3.
+constexpr int kMaxForestDepth = 10;
+int forest_from_points(HostForest &f, int ninst, const int *N, const real3 *pos, const int *depth, int leafmax,
+                       real L, std::vector<int> *depth_out = nullptr);
+
+}  // namespace bb

@@ -64,16 +64,17 @@ __global__ void __launch_bounds__(64) k_l2p_grad(GradArgs a)
 {
   constexpr int N3 = NN * NN * NN;
   const int leaf = blockIdx.x;
-  const int A = a.leaf0 + leaf;
+  const int A = a.leafcell[leaf];
+  const real ih = a.leafihalf[leaf];
   const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
   const real4 c = a.center[A];
   for (int j = threadIdx.x; j < cnt; j += 64) {
     const int i = sta + j;
     const real4 x = a.pos[i];
     real sx[NN], dsx[NN], sy[NN], dsy[NN], sz[NN], dsz[NN];
-    cheb_val_der<NN>(a.Tkz, a.ihalfL * (x.x - c.x), sx, dsx);
-    cheb_val_der<NN>(a.Tkz, a.ihalfL * (x.y - c.y), sy, dsy);
-    cheb_val_der<NN>(a.Tkz, a.ihalfL * (x.z - c.z), sz, dsz);
+    cheb_val_der<NN>(a.Tkz, ih * (x.x - c.x), sx, dsx);
+    cheb_val_der<NN>(a.Tkz, ih * (x.y - c.y), sy, dsy);
+    cheb_val_der<NN>(a.Tkz, ih * (x.z - c.z), sz, dsz);
     real qk[MMAX], gk[MMAX];
 #pragma unroll
     for (int v = 0; v < MMAX; v++) {
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(64) k_l2p_grad(GradArgs a)
         gy += sx[l1] * dsy[l2] * in_s;
         gz += sx[l1] * sy[l2] * in_d;
       }
-    const real sc = a.prefac3 * a.ihalfL;
+    const real sc = a.prefac3 * ih;
     a.dx[i] = sc * gx;
     a.dx[(size_t)a.N + i] = sc * gy;
     a.dx[(size_t)2 * a.N + i] = sc * gz;

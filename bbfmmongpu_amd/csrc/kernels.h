@@ -9,15 +9,18 @@ namespace bb {
 
 // Everything the pass kernels read, by value (device pointers).
 struct PassArgs {
-  int N, n, r, ncell, maxlev, nleaf, leaf0;
-  real ihalfL;          // 1 / (half leaf length)
+  int N, n, r, ncell, maxlev, nleaf;
   real prefac3;         // (2/n)^3
   const real4 *pos;     // [N] leaf order
   const real4 *center;  // [ncell]
   const int *leafsta;   // [nleaf]
   const int *leafcnt;   // [nleaf]
   const int *leafof;    // [N] leaf (local index) of each leaf-ordered particle
-  const int *levsta;    // [maxlev+2]
+  const int *leafcell;  // [nleaf] cell index of each leaf
+  const real *leafihalf;  // [nleaf] 1 / (half the leaf's length)
+  const int *celllev;   // [ncell] level of each cell
+  const int *child;     // [ncell * 8] children (-1: none)
+  const int *plist;     // cells with children, grouped by level
   const real *scale;    // [maxlev+1] (1/L_l)^homogen
   const real *Tkz, *Kw, *Cw;
   const real *Ucm;      // U column-major (n^3 x r)
@@ -32,15 +35,16 @@ int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, 
 // dst[j] = src[iperm[j]] per vector (iperm: user index -> leaf-order index)
 int launch_scatter(const int *iperm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
 int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs);
-int launch_m2m(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs);
-int launch_postm2l(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs);
-int launch_l2l(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs);
+// h_ploff[l]: offset of level l's parents in a.plist (levels 0..maxlev)
+int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs);
+int launch_postm2l(const PassArgs &a, hipStream_t s, int nrhs);
+int launch_l2l(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs);
 int launch_l2p(const PassArgs &a, hipStream_t s, int nrhs);
 
 // ---- M2L ----
 struct M2LArgs {
   int r, ncell, nrhs, minlev, maxlev;
-  const int *h_levsta;     // host [maxlev+2]
+  const int *h_levsta;     // host [maxlev+2]; NULL for a batch of sparse instances (basic scheme only)
   // basic / sibling schemes
   const int *iinter;       // [ncell]
   const int *inter;        // [ncell*192]
@@ -84,8 +88,10 @@ int launch_p2p(const P2PArgs &a, hipStream_t s);
 
 // ---- adjoint (position gradient) ----
 struct GradArgs {
-  int N, n, nleaf, leaf0, ncell, m;  // m vectors: charges q in rows 0..m-1 of qg, dLoss/dphi in m..2m-1
-  real ihalfL, prefac3;
+  int N, n, nleaf, ncell, m;  // m vectors: charges q in rows 0..m-1 of qg, dLoss/dphi in m..2m-1
+  real prefac3;
+  const int *leafcell;        // [nleaf]
+  const real *leafihalf;      // [nleaf]
   const real4 *pos, *center;         // leaf order
   const int *leafsta, *leafcnt, *nbr, *nnbr;
   const real *Tkz;

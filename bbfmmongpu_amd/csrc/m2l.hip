@@ -147,6 +147,14 @@ template <int R>
 static int basic_t(const M2LArgs &a, hipStream_t s)
 {
   constexpr int CPB = 256 / R;
+  if (!a.h_levsta) {
+    // no level table (a batch of sparse instances): one launch over every
+    // cell; cells without a list (levels 0-1) write L = 0
+    hipLaunchKernelGGL((k_m2l_basic<R>), dim3((a.ncell + CPB - 1) / CPB, a.nrhs), dim3(256), 0, s, 0, a.ncell,
+                       a.ncell, a.iinter, a.inter, a.kidx, a.K4, a.M, a.L);
+    BB_HIP(hipGetLastError());
+    return BBFMM_OK;
+  }
   for (int level = a.minlev; level <= a.maxlev; level++) {
     const int F0 = a.h_levsta[level], nF = a.h_levsta[level + 1] - F0;
     hipLaunchKernelGGL((k_m2l_basic<R>), dim3((nF + CPB - 1) / CPB, a.nrhs), dim3(256), 0, s, F0, nF,

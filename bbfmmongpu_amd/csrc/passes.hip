@@ -90,7 +90,8 @@ __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
   __shared__ __align__(16) real sw[N3 * NR];
   const int leaf = blockIdx.x;
   const int rhs0 = blockIdx.y * NR;
-  const int A = a.leaf0 + leaf;
+  const int A = a.leafcell[leaf];
+  const real ih = a.leafihalf[leaf];
   const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
   const real4 c = a.center[A];
   real acc[NPT][NR];
@@ -103,13 +104,13 @@ __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
     if ((int)threadIdx.x < nk) {
       const real4 x = a.pos[sta + base + threadIdx.x];
       real t[NN];
-      cheb_row<NN>(a.Tkz, a.ihalfL * (x.x - c.x), t);
+      cheb_row<NN>(a.Tkz, ih * (x.x - c.x), t);
 #pragma unroll
       for (int m = 0; m < NN; m++) sx[m][threadIdx.x] = t[m];
-      cheb_row<NN>(a.Tkz, a.ihalfL * (x.y - c.y), t);
+      cheb_row<NN>(a.Tkz, ih * (x.y - c.y), t);
 #pragma unroll
       for (int m = 0; m < NN; m++) sy[m][threadIdx.x] = t[m];
-      cheb_row<NN>(a.Tkz, a.ihalfL * (x.z - c.z), t);
+      cheb_row<NN>(a.Tkz, ih * (x.z - c.z), t);
 #pragma unroll
       for (int m = 0; m < NN; m++) sz[m][threadIdx.x] = t[m];
     }
@@ -164,7 +165,7 @@ __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
 // M2M for one level (anotherUpwardPass.c:345-560): one block per parent,
 // separable interpolation of the eight children's node values (z, y, x).
 template <int NN, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_m2m(PassArgs a, int Afirst)
+__global__ void __launch_bounds__(BLOCK) k_m2m(PassArgs a, int off)
 {
   constexpr int N3 = NN * NN * NN, N2 = NN * NN;
   __shared__ real sc[8 * N3];
@@ -172,10 +173,13 @@ __global__ void __launch_bounds__(BLOCK) k_m2m(PassArgs a, int Afirst)
   __shared__ real syy[2 * N3];
   __shared__ real sw[N3];
   __shared__ real cw[2 * N2];
-  const int A = Afirst + blockIdx.x;
+  const int A = a.plist[off + blockIdx.x];
   const int rhs = blockIdx.y;
   real *S = a.S + (size_t)rhs * a.ncell * N3;
-  for (int o = threadIdx.x; o < 8 * N3; o += BLOCK) sc[o] = S[(size_t)(8 * A + 1) * N3 + o];
+  for (int o = threadIdx.x; o < 8 * N3; o += BLOCK) {
+    const int ch = a.child[(size_t)8 * A + o / N3];
+    sc[o] = ch >= 0 ? S[(size_t)ch * N3 + o % N3] : (real)0;
+  }
   for (int o = threadIdx.x; o < 2 * N2; o += BLOCK) cw[o] = a.Cw[o];
   __syncthreads();
   for (int o = threadIdx.x; o < 4 * N3; o += BLOCK) { // z
@@ -233,8 +237,8 @@ __global__ void __launch_bounds__(BLOCK) k_postm2l(PassArgs a, int Afirst)
   real *sl = reinterpret_cast<real *>(smem_raw);  // [r][NR]
   const int A = Afirst + blockIdx.x;
   const int rhs0 = blockIdx.y * NR;
-  int level = 0;
-  while (level < a.maxlev && A >= a.levsta[level + 1]) level++;
+  const int level = a.celllev[A];
+  if (level < kMinLev) return;  // whole block, before any barrier
   const real scale = a.scale[level];
   for (int o = threadIdx.x; o < a.r * NR; o += BLOCK) {
     const int k = o / a.r, i = o % a.r;
@@ -260,14 +264,14 @@ __global__ void __launch_bounds__(BLOCK) k_postm2l(PassArgs a, int Afirst)
 // L2L for one level (anotherDownwardPassX.c:40-211): one block per parent,
 // children F += (2/n)^3 * interpolation of the parent's F (x, y, z).
 template <int NN, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_l2l(PassArgs a, int Afirst)
+__global__ void __launch_bounds__(BLOCK) k_l2l(PassArgs a, int off)
 {
   constexpr int N3 = NN * NN * NN, N2 = NN * NN;
   __shared__ real fa[N3];
   __shared__ real fx[2 * N3];
   __shared__ real fy[4 * N3];
   __shared__ real cw[2 * N2];
-  const int A = Afirst + blockIdx.x;
+  const int A = a.plist[off + blockIdx.x];
   const int rhs = blockIdx.y;
   real *F = a.F + (size_t)rhs * a.ncell * N3;
   for (int o = threadIdx.x; o < N3; o += BLOCK) fa[o] = F[(size_t)A * N3 + o];
@@ -295,7 +299,8 @@ __global__ void __launch_bounds__(BLOCK) k_l2l(PassArgs a, int Afirst)
     real v = 0;
 #pragma unroll
     for (int k = 0; k < NN; k++) v += fy[(j >> 1) * N3 + l1 * NN + k] * cw[(j & 1) * N2 + k * NN + cz];
-    F[(size_t)(8 * A + 1 + j) * N3 + rem] += a.prefac3 * v;
+    const int ch = a.child[(size_t)8 * A + j];
+    if (ch >= 0) F[(size_t)ch * N3 + rem] += a.prefac3 * v;
   }
 }
 
@@ -308,13 +313,15 @@ __global__ void __launch_bounds__(256) k_l2p(PassArgs a)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int rhs = blockIdx.y;
   if (i >= a.N) return;
-  const int A = a.leaf0 + a.leafof[i];
+  const int lf = a.leafof[i];
+  const int A = a.leafcell[lf];
+  const real ih = a.leafihalf[lf];
   const real4 x = a.pos[i];
   const real4 c = a.center[A];
   real sxv[NN], syv[NN], szv[NN];
-  cheb_row<NN>(a.Tkz, a.ihalfL * (x.x - c.x), sxv);
-  cheb_row<NN>(a.Tkz, a.ihalfL * (x.y - c.y), syv);
-  cheb_row<NN>(a.Tkz, a.ihalfL * (x.z - c.z), szv);
+  cheb_row<NN>(a.Tkz, ih * (x.x - c.x), sxv);
+  cheb_row<NN>(a.Tkz, ih * (x.y - c.y), syv);
+  cheb_row<NN>(a.Tkz, ih * (x.z - c.z), szv);
   const real *FA = a.F + (size_t)rhs * a.ncell * N3 + (size_t)A * N3;
   real sum = 0;
 #pragma unroll
@@ -341,7 +348,8 @@ __global__ void __launch_bounds__(64) k_l2p_leaf(PassArgs a)
   __shared__ __align__(16) real sf[N3 * NR];
   const int leaf = blockIdx.x;
   const int rhs0 = blockIdx.y * NR;
-  const int A = a.leaf0 + leaf;
+  const int A = a.leafcell[leaf];
+  const real ih = a.leafihalf[leaf];
   const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
   for (int o = threadIdx.x; o < N3 * NR; o += 64) {
     const int k = o / N3, l = o % N3;
@@ -353,9 +361,9 @@ __global__ void __launch_bounds__(64) k_l2p_leaf(PassArgs a)
   for (int j = jl; j < cnt; j += PT) {
     const real4 x = a.pos[sta + j];
     real sxv[NN], syv[NN], szv[NN];
-    cheb_row<NN>(a.Tkz, a.ihalfL * (x.x - c.x), sxv);
-    cheb_row<NN>(a.Tkz, a.ihalfL * (x.y - c.y), syv);
-    cheb_row<NN>(a.Tkz, a.ihalfL * (x.z - c.z), szv);
+    cheb_row<NN>(a.Tkz, ih * (x.x - c.x), sxv);
+    cheb_row<NN>(a.Tkz, ih * (x.y - c.y), syv);
+    cheb_row<NN>(a.Tkz, ih * (x.z - c.z), szv);
     real sum[RS];
 #pragma unroll
     for (int k = 0; k < RS; k++) sum[k] = 0;
@@ -434,25 +442,27 @@ int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs)
   return BBFMM_OK;
 }
 
-int launch_m2m(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs)
+int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs)
 {
   if (a.n != 4 && a.n != 8)
     return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
   for (int level = a.maxlev - 1; level >= kMinLev; level--) {
-    const int first = h_levsta[level], count = h_levsta[level + 1] - first;
+    const int off = h_ploff[level], count = h_ploff[level + 1] - off;
+    if (count <= 0) continue;
     if (a.n == 4)
-      hipLaunchKernelGGL((k_m2m<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, first);
+      hipLaunchKernelGGL((k_m2m<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, off);
     else
-      hipLaunchKernelGGL((k_m2m<8, 256>), dim3(count, nrhs), dim3(256), 0, s, a, first);
+      hipLaunchKernelGGL((k_m2m<8, 256>), dim3(count, nrhs), dim3(256), 0, s, a, off);
     BB_HIP(hipGetLastError());
   }
   return BBFMM_OK;
 }
 
+// every cell; blocks of cells above level 2 return at once
 template <int NN, int BLOCK>
-static int postm2l_t(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs)
+static int postm2l_t(const PassArgs &a, hipStream_t s, int nrhs)
 {
-  const int first = h_levsta[kMinLev], count = a.ncell - first;
+  const int first = 0, count = a.ncell;
   if (count <= 0) return BBFMM_OK;
   const int nr = vec_group(nrhs);
   const size_t sh = sizeof(real) * a.r * nr;
@@ -466,24 +476,25 @@ static int postm2l_t(const PassArgs &a, const int *h_levsta, hipStream_t s, int 
   return BBFMM_OK;
 }
 
-int launch_postm2l(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs)
+int launch_postm2l(const PassArgs &a, hipStream_t s, int nrhs)
 {
-  if (a.n == 4) return postm2l_t<4, 64>(a, h_levsta, s, nrhs);
-  if (a.n == 8) return postm2l_t<8, 256>(a, h_levsta, s, nrhs);
+  if (a.n == 4) return postm2l_t<4, 64>(a, s, nrhs);
+  if (a.n == 8) return postm2l_t<8, 256>(a, s, nrhs);
   return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
 }
 
-int launch_l2l(const PassArgs &a, const int *h_levsta, hipStream_t s, int nrhs)
+int launch_l2l(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs)
 {
-  // L2L only (levels minlev..maxlev-1)
+  // L2L only (parents at levels minlev..maxlev-1)
   if (a.n != 4 && a.n != 8)
     return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
   for (int level = kMinLev; level < a.maxlev; level++) {
-    const int first = h_levsta[level], count = h_levsta[level + 1] - first;
+    const int off = h_ploff[level], count = h_ploff[level + 1] - off;
+    if (count <= 0) continue;
     if (a.n == 4)
-      hipLaunchKernelGGL((k_l2l<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, first);
+      hipLaunchKernelGGL((k_l2l<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, off);
     else
-      hipLaunchKernelGGL((k_l2l<8, 256>), dim3(count, nrhs), dim3(256), 0, s, a, first);
+      hipLaunchKernelGGL((k_l2l<8, 256>), dim3(count, nrhs), dim3(256), 0, s, a, off);
     BB_HIP(hipGetLastError());
   }
   return BBFMM_OK;

@@ -170,6 +170,21 @@ int bbfmm_amd_plan_create(bbfmm_amd_plan **plan, int N, const real3 *pos, real L
                           int n, int lmod, int r, const real *K, const real *U,
                           int nrhs, void *stream);
 int bbfmm_amd_plan_destroy(bbfmm_amd_plan *plan);
+
+/* A plan over ninst independent problems solved together (no counterpart in
+   the reference, which solves one problem per call): instance i has N[i]
+   points, concatenated in pos (sum N real3, each inside [-L/2,L/2]^3), and
+   its own octree of uniform depth depth[i] (depth NULL or depth[i] <= 0: the
+   smallest depth >= 2 whose leaves hold at most leafmax points, at most 10),
+   storing only the cells that hold points.  Each instance's field equals the
+   reference's complete-octree field at the same depth (empty cells add
+   exact zeros).  Every stage is one launch over all instances (M2M and L2L
+   one per level); q and phi are nrhs x sum(N), instances concatenated.
+   M2L: the basic scheme.  depth_out (optional, ninst ints) receives the
+   depths used.  Destroy with bbfmm_amd_plan_destroy. */
+int bbfmm_amd_batch_create(bbfmm_amd_plan **plan, int ninst, const int *N, const real3 *pos,
+                           const int *depth, int leafmax, real L, int n, int r, const real *K,
+                           const real *U, int nrhs, void *stream, int *depth_out);
 int bbfmm_amd_plan_set(bbfmm_amd_plan *plan, int option, int value);
 
 /* Tree facts: out[0]=maxlev, out[1]=ncell, out[2]=N, out[3]=r, out[4]=n,
