@@ -23,7 +23,7 @@ __all__ = [
 
 SCHEME_BA, SCHEME_SI, SCHEME_CL, SCHEME_IJ = 0, 1, 2, 3
 FORM_PLAIN, FORM_BF16X2, FORM_FP32MFMA = 0, 1, 2
-OPT_SCHEME, OPT_FORM, OPT_TIMING, OPT_OUTPUT = 1, 2, 3, 4
+OPT_SCHEME, OPT_FORM, OPT_TIMING, OPT_OUTPUT, OPT_GRAPH = 1, 2, 3, 4, 5
 STAGES = ["gather", "p2m", "m2m", "m2l_prep", "m2l", "post_m2l", "l2l", "l2p", "p2p", "scatter", "total"]
 
 _libs: dict = {}
@@ -250,8 +250,9 @@ class Plan:
         return cls(pos, n, U, K, nrhs=nrhs, L=L, prec=prec, stream=stream, _batch=(sizes, depths, leafmax))
 
     def set(self, scheme: int | None = None, form: int | None = None, timing: int | None = None,
-            output: str | None = None):
-        """output: "q_phi" (q .* potential, the reference's output) or "potential"."""
+            output: str | None = None, graph: bool | None = None):
+        """output: "q_phi" (q .* potential, the reference's output) or "potential";
+        graph: capture the whole solve into one hipGraph and replay it."""
         if scheme is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_SCHEME, scheme))
         if form is not None:
@@ -262,6 +263,8 @@ class Plan:
             if output not in ("q_phi", "potential"):
                 raise BBFMMError(f"unknown output {output!r}")
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_OUTPUT, int(output == "potential")))
+        if graph is not None:
+            _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_GRAPH, int(bool(graph))))
         return self
 
     def update_positions(self, pos):

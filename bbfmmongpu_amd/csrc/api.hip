@@ -102,6 +102,14 @@ struct bbfmm_amd_plan {
   long long solves = 0, timed_from = 0;
   double stage_ms[ST_TOTAL + 1] = {};
   bool solved = false;
+  // whole-solve hipGraph (BBFMM_OPT_GRAPH): captured on a private stream for
+  // one (q, phi) pointer pair, replayed on the plan's stream
+  int use_graph = 0;
+  hipGraphExec_t gexec = nullptr;
+  const real *gq = nullptr;
+  real *gphi = nullptr;
+  hipStream_t gstream = nullptr;
+  bool graph_timed = false;  // the timed solves were graph launches (total time only)
 };
 
 namespace bb {
@@ -149,6 +157,10 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  p->gexec = nullptr;
+  if (p->gstream) (void)hipStreamDestroy(p->gstream);
+  p->gstream = nullptr;
   if (p->grid) grid_m2l_destroy(p->grid);
   p->grid = nullptr;
   for (auto &e : p->ev)
@@ -166,6 +178,7 @@ static int set_timing(bbfmm_amd_plan *p, int nsets)
   for (auto &e : p->ev) BB_HIP(hipEventCreate(&e));
   p->nsets = nsets;
   p->timed_from = p->solves;
+  p->graph_timed = p->use_graph != 0;
   return BBFMM_OK;
 }
 
@@ -178,6 +191,12 @@ static int read_timing(bbfmm_amd_plan *p)
   BB_HIP(hipStreamSynchronize(p->stream));
   for (int k = 0; k < sets; k++) {
     hipEvent_t *e = &p->ev[(size_t)k * (ST_TOTAL + 1)];
+    if (p->graph_timed) {  // a graph launch: the total only
+      float tot = 0;
+      BB_HIP(hipEventElapsedTime(&tot, e[0], e[ST_TOTAL]));
+      p->stage_ms[ST_TOTAL] += tot / sets;
+      continue;
+    }
     for (int st = 0; st < ST_TOTAL; st++) {
       float ms = 0;
       BB_HIP(hipEventElapsedTime(&ms, e[st], e[st + 1]));
@@ -403,6 +422,36 @@ static PassArgs pass_args(const bbfmm_amd_plan *p)
   return a;
 }
 
+// the dense grids of the CL / IJ schemes (created on first use: allocates,
+// uploads the operator layout and synchronises the stream)
+static int ensure_grid(bbfmm_amd_plan *p)
+{
+  if (p->scheme != M2L_SCHEME_CL && p->scheme != M2L_SCHEME_IJ) return BBFMM_OK;
+  const int form = p->scheme == M2L_SCHEME_CL ? BBFMM_M2L_PLAIN : p->form;
+  if (!p->grid || p->grid_form != form) {
+    if (p->grid) grid_m2l_destroy(p->grid);
+    p->grid = nullptr;
+    BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, p->nrhs, form, p->h_K.data(), p->stream));
+    p->grid_form = form;
+  }
+  return BBFMM_OK;
+}
+
+// everything a solve allocates on first use, so that a solve itself (and a
+// captured graph of it) allocates and clears nothing
+static int prepare_solve(bbfmm_amd_plan *p)
+{
+  BB_TRY(ensure_grid(p));
+#ifdef SINGLE
+  if (p->form != BBFMM_M2L_PLAIN && p->nrhs >= 16 && !p->d_qt) {
+    const size_t cnt = (size_t)(p->npad > 0 ? p->npad : 1) * 16;
+    BB_TRY(dalloc(&p->d_qt, cnt));
+    BB_HIP(hipMemsetAsync(p->d_qt, 0, sizeof(real) * cnt, p->stream));
+  }
+#endif
+  return BBFMM_OK;
+}
+
 static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = nullptr)
 {
   M2LArgs m;
@@ -429,26 +478,18 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
     if (mid) BB_HIP(hipEventRecord(mid, p->stream));
     return launch_m2l_sibling(m, p->stream);
   case M2L_SCHEME_CL:
-  case M2L_SCHEME_IJ: {
-    const int form = p->scheme == M2L_SCHEME_CL ? BBFMM_M2L_PLAIN : p->form;
-    if (!p->grid || p->grid_form != form) {
-      if (p->grid) grid_m2l_destroy(p->grid);
-      p->grid = nullptr;
-      BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, p->nrhs, form, p->h_K.data(), p->stream));
-      p->grid_form = form;
-    }
+  case M2L_SCHEME_IJ:
+    BB_TRY(ensure_grid(p));
     return grid_m2l_run(p->grid, M, L, p->ncell, p->stream, mid);
-  }
   default: return set_error(BBFMM_E_ARG, "unknown M2L scheme %d", p->scheme);
   }
 }
 
 // The solve pipeline on device arrays in leaf order (d_q -> d_phi).
-static int solve_sorted(bbfmm_amd_plan *p)
+static int solve_sorted(bbfmm_amd_plan *p, bool tm)
 {
   hipStream_t s = p->stream;
   const PassArgs a = pass_args(p);
-  const bool tm = p->nsets > 0;
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2M), s));
   BB_TRY(launch_p2m(a, s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2M), s));
@@ -482,10 +523,7 @@ static int solve_sorted(bbfmm_amd_plan *p)
   pp.pad_of = p->d_pad_of;
 #ifdef SINGLE
   if (pp.use_mfma && p->nrhs >= 16) {
-    if (!p->d_qt) {
-      BB_TRY(dalloc(&p->d_qt, (size_t)(p->npad > 0 ? p->npad : 1) * 16));
-      BB_HIP(hipMemsetAsync(p->d_qt, 0, sizeof(real) * (size_t)(p->npad > 0 ? p->npad : 1) * 16, s));
-    }
+    if (!p->d_qt) return set_error(BBFMM_E_ARG, "internal: solve not prepared");
     pp.qt = p->d_qt;
   }
 #endif
@@ -493,18 +531,80 @@ static int solve_sorted(bbfmm_amd_plan *p)
   return BBFMM_OK;
 }
 
+static void drop_graph(bbfmm_amd_plan *p)
+{
+  if (p->gexec) {
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipGraphExecDestroy(p->gexec);
+  }
+  p->gexec = nullptr;
+  p->gq = nullptr;
+  p->gphi = nullptr;
+}
+
+static int solve_eager(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev, bool tm);
+
+// Capture gather, solve and scatter for (q_dev, phi_dev) on a private
+// stream into one graph; replay it on the plan's stream.
+static int solve_graph(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev)
+{
+  if (!p->gexec || p->gq != q_dev || p->gphi != phi_dev) {
+    drop_graph(p);
+    BB_TRY(prepare_solve(p));
+    BB_HIP(hipStreamSynchronize(p->stream));
+    if (!p->gstream) BB_HIP(hipStreamCreateWithFlags(&p->gstream, hipStreamNonBlocking));
+    hipStream_t user = p->stream;
+    p->stream = p->gstream;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(p->gstream, hipStreamCaptureModeThreadLocal);
+    int rc = e == hipSuccess ? solve_eager(p, q_dev, phi_dev, false)
+                             : set_error(BBFMM_E_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    if (e == hipSuccess) {
+      const hipError_t e2 = hipStreamEndCapture(p->gstream, &g);
+      if (rc == BBFMM_OK && e2 != hipSuccess)
+        rc = set_error(BBFMM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+    }
+    p->stream = user;
+    if (rc == BBFMM_OK) {
+      const hipError_t e3 = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+      if (e3 != hipSuccess) {
+        p->gexec = nullptr;
+        rc = set_error(BBFMM_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e3));
+      }
+    }
+    if (g) (void)hipGraphDestroy(g);
+    BB_TRY(rc);
+    p->gq = q_dev;
+    p->gphi = phi_dev;
+  }
+  const bool tm = p->nsets > 0;
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_GATHER), p->stream));
+  BB_HIP(hipGraphLaunch(p->gexec, p->stream));
+  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_TOTAL), p->stream));
+  p->solves++;
+  p->solved = true;
+  return BBFMM_OK;
+}
+
 static int solve_dev(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev)
 {
+  if (p->use_graph) return solve_graph(p, q_dev, phi_dev);
+  BB_TRY(prepare_solve(p));
+  BB_TRY(solve_eager(p, q_dev, phi_dev, p->nsets > 0));
+  p->solves++;
+  p->solved = true;
+  return BBFMM_OK;
+}
+
+static int solve_eager(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev, bool tm)
+{
   hipStream_t s = p->stream;
-  const bool tm = p->nsets > 0;
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_GATHER), s));
   BB_TRY(launch_gather(p->d_perm, q_dev, p->d_q, p->N, p->nrhs, s));
-  BB_TRY(solve_sorted(p));
+  BB_TRY(solve_sorted(p, tm));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_SCATTER), s));
   BB_TRY(launch_scatter(p->d_iperm, p->d_phi, phi_dev, p->N, p->nrhs, s));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_TOTAL), s));
-  p->solves++;
-  p->solved = true;
   return BBFMM_OK;
 }
 
@@ -597,6 +697,7 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     if (!p->complete && value != M2L_SCHEME_BA)
       return set_error(BBFMM_E_ARG, "a batch of sparse instances runs the basic M2L scheme only");
     p->scheme = value;
+    drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_FORM:
     if (value < BBFMM_M2L_PLAIN || value > BBFMM_M2L_FP32MFMA) return set_error(BBFMM_E_ARG, "unknown M2L form %d", value);
@@ -604,6 +705,7 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     if (value != BBFMM_M2L_PLAIN) return set_error(BBFMM_E_ARG, "MFMA forms are fp32-build only");
 #endif
     p->form = value;
+    drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_TIMING:
     if (value < 0 || value > 4096) return set_error(BBFMM_E_ARG, "timing sets %d out of range", value);
@@ -612,6 +714,13 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
   case BBFMM_OPT_OUTPUT:
     if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "output %d: 0 (q .* potential) or 1 (potential)", value);
     p->potential = value;
+    drop_graph(p);
+    return BBFMM_OK;
+  case BBFMM_OPT_GRAPH:
+    if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "graph %d: 0 (eager) or 1 (captured)", value);
+    drop_graph(p);
+    p->use_graph = value;
+    if (p->nsets > 0) return set_timing(p, p->nsets);
     return BBFMM_OK;
   default: return set_error(BBFMM_E_ARG, "unknown option %d", option);
   }
@@ -703,12 +812,11 @@ extern "C" int bbfmm_amd_gradient(bbfmm_amd_plan *p, const real *q_dev, const re
   // [q; g] in leaf order, one solve of the 2m vectors in potential mode
   BB_TRY(launch_gather(p->d_perm, q_dev, p->d_q, N, m, s));
   BB_TRY(launch_gather(p->d_perm, g_dev, p->d_q + (size_t)m * N, N, m, s));
-  const int saved_potential = p->potential, saved_sets = p->nsets;
+  BB_TRY(prepare_solve(p));
+  const int saved_potential = p->potential;
   p->potential = 1;
-  p->nsets = 0;  // not a timed solve
-  const int rc = solve_sorted(p);
+  const int rc = solve_sorted(p, false);  // not a timed solve
   p->potential = saved_potential;
-  p->nsets = saved_sets;
   BB_TRY(rc);
   // dLoss/dq = A g: the potentials of the g vectors
   BB_TRY(launch_scatter(p->d_iperm, p->d_phi + (size_t)m * N, dq_dev, N, m, s));

@@ -159,6 +159,9 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                  ring of k event sets; 0: off (resets the averages) */
 #define BBFMM_OPT_OUTPUT 4    /* 0: q .* potential (the reference's output, default);
                                  1: the potential */
+#define BBFMM_OPT_GRAPH 5     /* 1: capture the whole solve (gather .. scatter) into one
+                                 hipGraph per (q, phi) pointer pair on first use and
+                                 replay it; stage times then report the total only */
 
 /* Create a plan for N particles at host positions pos (N real3, inside the
    root cell [-L/2,L/2]^3), Chebyshev order n, level modifier lmod (the -d

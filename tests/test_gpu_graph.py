@@ -1,0 +1,51 @@
+"""Whole-solve hipGraph (BBFMM_OPT_GRAPH): every replay, with eager work
+and synchronisations in between and the charges changed in place between
+replays, matches the eager solve of the same charges (bit for bit: the
+graph runs the same kernels on the same data)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _replays_match(plan, Q, cuda):
+    import torch
+    q = torch.from_numpy(Q).to(cuda).contiguous()
+    eager = []
+    for scale in (1.0, 2.0, -0.5):
+        eager.append(plan.solve((q * scale).contiguous()).cpu().numpy())
+    plan.set(graph=True)
+    qg = q.clone()
+    phi = torch.empty_like(qg)
+    for rep in range(2):
+        for k, scale in enumerate((1.0, 2.0, -0.5)):
+            qg.copy_(q * scale)              # eager work on the graph's input
+            plan.solve(qg, phi)               # graph replay
+            torch.cuda.synchronize()
+            got = phi.cpu().numpy()
+            err = O.rel_l2(eager[k], got)
+            print(f"replay {rep}/{k}: max abs diff {np.abs(got - eager[k]).max():.3e} rel {err:.2e}", flush=True)
+            assert np.array_equal(got, eager[k])
+    plan.set(graph=False)
+
+
+def test_graph_single_plan_chosen_form(ops4, cuda):
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, Q = O.uniform_inputs(65536, seed=8, nrhs=16)
+    plan = B.Plan(pos, 4, U, K, nrhs=16, prec="s", lmod=1).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2)
+    _replays_match(plan, Q.astype(np.float32), cuda)
+    plan.close()
+
+
+def test_graph_batch_plan(ops4, cuda):
+    import bbfmmongpu_amd as B
+    from bbfmmongpu_amd import workloads as W
+    U, K, _ = ops4
+    pos_list, q_list = W.uneven_instances(count=6, nmin=4096, nmax=16384, seed=5)
+    plan = B.Plan.batch(pos_list, 4, U, K, nrhs=1, prec="s")
+    Q = np.concatenate(q_list).astype(np.float32)[None, :]
+    _replays_match(plan, Q, cuda)
+    plan.close()
