@@ -23,7 +23,7 @@ __all__ = [
 
 SCHEME_BA, SCHEME_SI, SCHEME_CL, SCHEME_IJ = 0, 1, 2, 3
 FORM_PLAIN, FORM_BF16X2, FORM_FP32MFMA = 0, 1, 2
-OPT_SCHEME, OPT_FORM, OPT_TIMING, OPT_OUTPUT, OPT_GRAPH = 1, 2, 3, 4, 5
+OPT_SCHEME, OPT_FORM, OPT_TIMING, OPT_OUTPUT, OPT_GRAPH, OPT_M2L_VARIANT, OPT_LEVEL_SWITCH = 1, 2, 3, 4, 5, 6, 7
 STAGES = ["gather", "p2m", "m2m", "m2l_prep", "m2l", "post_m2l", "l2l", "l2p", "p2p", "scatter", "total"]
 
 _libs: dict = {}
@@ -69,6 +69,7 @@ def load(prec: str = "s"):
     L.bbfmm_amd_direct_field.argtypes = [i, vp, i, vp, i, vp, vp, vp]
     L.bbfmm_amd_gradient.argtypes = [vp, vp, vp, vp, vp]
     L.bbfmm_amd_plan_update_positions.argtypes = [vp, vp]
+    L.bbfmm_amd_plan_tune.argtypes = [vp, i, vp, i, vp]
     L.bbfmm_amd_batch_create.argtypes = [C.POINTER(vp), i, vp, vp, vp, i, R, i, i, vp, vp, i, vp, vp]
     L.bbfmm_amd_set_default_scheme.argtypes = [i]
     L.m2l_drv.argtypes = [i, vp, vp, vp, i, i, R, vp, vp, i, vp, vp, i, vp, i]
@@ -250,7 +251,8 @@ class Plan:
         return cls(pos, n, U, K, nrhs=nrhs, L=L, prec=prec, stream=stream, _batch=(sizes, depths, leafmax))
 
     def set(self, scheme: int | None = None, form: int | None = None, timing: int | None = None,
-            output: str | None = None, graph: bool | None = None):
+            output: str | None = None, graph: bool | None = None, variant: int | None = None,
+            level_switch: int | None = None):
         """output: "q_phi" (q .* potential, the reference's output) or "potential";
         graph: capture the whole solve into one hipGraph and replay it."""
         if scheme is not None:
@@ -263,9 +265,22 @@ class Plan:
             if output not in ("q_phi", "potential"):
                 raise BBFMMError(f"unknown output {output!r}")
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_OUTPUT, int(output == "potential")))
+        if variant is not None:
+            _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_M2L_VARIANT, int(variant)))
+        if level_switch is not None:
+            _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_LEVEL_SWITCH, int(level_switch)))
         if graph is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_GRAPH, int(bool(graph))))
         return self
+
+    def tune(self, reps: int = 3):
+        """M2L autotuner: times every valid (scheme, variant, level switch) on
+        this plan and keeps the fastest; returns the table of
+        (scheme, variant, level_switch, ms)."""
+        table = np.zeros((64, 4))
+        rows = C.c_int(0)
+        _check(self.lib, self.lib.bbfmm_amd_plan_tune(self.h, reps, _p(table), 64, C.byref(rows)))
+        return [tuple(r) for r in table[:min(rows.value, 64)].tolist()]
 
     def update_positions(self, pos):
         """Move the particles within their leaves (same tree and lists)."""

@@ -105,6 +105,11 @@ struct bbfmm_amd_plan {
   // whole-solve hipGraph (BBFMM_OPT_GRAPH): captured on a private stream for
   // one (q, phi) pointer pair, replayed on the plan's stream
   int use_graph = 0;
+  // M2L configuration picked by hand or by the autotuner: tile variant of the
+  // scheme, and for the dense-grid schemes the first level they handle (the
+  // basic scheme takes the levels below)
+  int m2l_variant = 0, level_switch = bb::kMinLev;
+  int grid_minlev = bb::kMinLev, grid_variant = 0;
   hipGraphExec_t gexec = nullptr;
   const real *gq = nullptr;
   real *gphi = nullptr;
@@ -428,11 +433,15 @@ static int ensure_grid(bbfmm_amd_plan *p)
 {
   if (p->scheme != M2L_SCHEME_CL && p->scheme != M2L_SCHEME_IJ) return BBFMM_OK;
   const int form = p->scheme == M2L_SCHEME_CL ? BBFMM_M2L_PLAIN : p->form;
-  if (!p->grid || p->grid_form != form) {
+  const int minlev = p->level_switch > kMinLev ? p->level_switch : kMinLev;
+  const int variant = p->scheme == M2L_SCHEME_IJ ? p->m2l_variant : 0;
+  if (!p->grid || p->grid_form != form || p->grid_minlev != minlev || p->grid_variant != variant) {
     if (p->grid) grid_m2l_destroy(p->grid);
     p->grid = nullptr;
-    BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, p->nrhs, form, p->h_K.data(), p->stream));
+    BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, p->nrhs, form, p->h_K.data(), p->stream, minlev, variant));
     p->grid_form = form;
+    p->grid_minlev = minlev;
+    p->grid_variant = variant;
   }
   return BBFMM_OK;
 }
@@ -468,6 +477,7 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
   m.Ktable = p->d_Ktable;
   m.M = M;
   m.L = L;
+  m.variant = p->scheme == M2L_SCHEME_BA ? p->m2l_variant : 0;
   if (!p->complete && p->scheme != M2L_SCHEME_BA)
     return set_error(BBFMM_E_ARG, "a batch of sparse instances runs the basic M2L scheme only");
   switch (p->scheme) {
@@ -480,6 +490,14 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
   case M2L_SCHEME_CL:
   case M2L_SCHEME_IJ:
     BB_TRY(ensure_grid(p));
+    if (p->level_switch > kMinLev) {
+      // levels below the switch on the basic scheme (m2l_drv.cu:39-60 splits
+      // the levels the same way)
+      M2LArgs b = m;
+      b.maxlev = p->level_switch - 1;
+      b.variant = 0;
+      BB_TRY(launch_m2l_basic(b, p->stream));
+    }
     return grid_m2l_run(p->grid, M, L, p->ncell, p->stream, mid);
   default: return set_error(BBFMM_E_ARG, "unknown M2L scheme %d", p->scheme);
   }
@@ -716,6 +734,17 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     p->potential = value;
     drop_graph(p);
     return BBFMM_OK;
+  case BBFMM_OPT_M2L_VARIANT:
+    if (value < 0 || value > 3) return set_error(BBFMM_E_ARG, "M2L variant %d out of range", value);
+    p->m2l_variant = value;
+    drop_graph(p);
+    return BBFMM_OK;
+  case BBFMM_OPT_LEVEL_SWITCH:
+    if (value < kMinLev || value > p->maxlev)
+      return set_error(BBFMM_E_ARG, "level switch %d outside [%d, %d]", value, kMinLev, p->maxlev);
+    p->level_switch = value;
+    drop_graph(p);
+    return BBFMM_OK;
   case BBFMM_OPT_GRAPH:
     if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "graph %d: 0 (eager) or 1 (captured)", value);
     drop_graph(p);
@@ -724,6 +753,88 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     return BBFMM_OK;
   default: return set_error(BBFMM_E_ARG, "unknown option %d", option);
   }
+}
+
+// M2L autotuner: times every configuration valid for the plan's tree and
+// arithmetic form on the plan's own multipoles and keeps the fastest.
+extern "C" int bbfmm_amd_plan_tune(bbfmm_amd_plan *p, int reps, double *table, int maxrows, int *nrows)
+{
+  clear_error();
+  if (!p) return set_error(BBFMM_E_ARG, "null plan");
+  if (reps < 1) reps = 3;
+  struct Cand {
+    int scheme, variant, sw;
+  };
+  std::vector<Cand> cands;
+  if (!p->complete) {
+    for (int v = 0; v < 3; v++) cands.push_back({M2L_SCHEME_BA, v, kMinLev});
+  } else {
+    std::vector<int> sws{kMinLev};
+    if (p->maxlev > kMinLev) sws.push_back(kMinLev + 1);
+    if (p->form == BBFMM_M2L_PLAIN) {
+      for (int v = 0; v < 3; v++) cands.push_back({M2L_SCHEME_BA, v, kMinLev});
+      cands.push_back({M2L_SCHEME_SI, 0, kMinLev});
+      for (int sw : sws) cands.push_back({M2L_SCHEME_IJ, 0, sw});
+    } else if (p->form == BBFMM_M2L_BF16X2) {
+      for (int sw : sws)
+        for (int v = 1; v <= 3; v++) cands.push_back({M2L_SCHEME_IJ, v, sw});
+    } else if (p->r == 32) {
+      for (int sw : sws)
+        for (int v = 2; v <= 3; v++) cands.push_back({M2L_SCHEME_IJ, v, sw});
+    } else {
+      for (int sw : sws) cands.push_back({M2L_SCHEME_IJ, 0, sw});
+    }
+  }
+  const int s0 = p->scheme, v0 = p->m2l_variant, w0 = p->level_switch;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  BB_HIP(hipEventCreate(&e0));
+  BB_HIP(hipEventCreate(&e1));
+  int best = -1;
+  double best_ms = 0;
+  int rc = BBFMM_OK, rows = 0;
+  for (size_t c = 0; c < cands.size() && rc == BBFMM_OK; c++) {
+    p->scheme = cands[c].scheme;
+    p->m2l_variant = cands[c].variant;
+    p->level_switch = cands[c].sw;
+    rc = run_m2l(p, p->d_M, p->d_L);  // warm-up (creates the grids)
+    if (rc != BBFMM_OK) break;
+    if (hipEventRecord(e0, p->stream) != hipSuccess) rc = set_error(BBFMM_E_HIP, "event record");
+    for (int k = 0; k < reps && rc == BBFMM_OK; k++) rc = run_m2l(p, p->d_M, p->d_L);
+    if (rc != BBFMM_OK) break;
+    float ms = 0;
+    if (hipEventRecord(e1, p->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = set_error(BBFMM_E_HIP, "tuner timing failed");
+      break;
+    }
+    const double per = ms / reps;
+    if (table && rows < maxrows) {
+      table[4 * rows] = cands[c].scheme;
+      table[4 * rows + 1] = cands[c].variant;
+      table[4 * rows + 2] = cands[c].sw;
+      table[4 * rows + 3] = per;
+    }
+    rows++;
+    if (best < 0 || per < best_ms) {
+      best = (int)c;
+      best_ms = per;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (nrows) *nrows = rows;
+  if (rc != BBFMM_OK || best < 0) {
+    p->scheme = s0;
+    p->m2l_variant = v0;
+    p->level_switch = w0;
+    return rc != BBFMM_OK ? rc : set_error(BBFMM_E_ARG, "no M2L configuration to tune");
+  }
+  p->scheme = cands[best].scheme;
+  p->m2l_variant = cands[best].variant;
+  p->level_switch = cands[best].sw;
+  drop_graph(p);
+  BB_TRY(ensure_grid(p));
+  return BBFMM_OK;
 }
 
 extern "C" int bbfmm_amd_plan_info(const bbfmm_amd_plan *p, int *out, double *dout)
@@ -1294,7 +1405,7 @@ extern "C" int m2l_drv(int r, real *L, real *K, real *M, int minlev, int maxlev,
     if (scheme == M2L_SCHEME_BA) rc = launch_m2l_basic(m, nullptr);
     else if (scheme == M2L_SCHEME_SI) rc = launch_m2l_sibling(m, nullptr);
     else {
-      rc = grid_m2l_create(&p.grid, r, maxlev, 1, BBFMM_M2L_PLAIN, p.h_K.data(), nullptr);
+      rc = grid_m2l_create(&p.grid, r, maxlev, 1, BBFMM_M2L_PLAIN, p.h_K.data(), nullptr, minlev, 0);
       if (rc == BBFMM_OK) rc = grid_m2l_run(p.grid, p.d_M, dL, ncell, nullptr, nullptr);
     }
     if (rc != BBFMM_OK) break;

@@ -53,14 +53,18 @@ struct M2LArgs {
   const int *Ktable;       // [343] (device)
   const real *M;           // [rhs][ncell][r]
   real *L;                 // [rhs][ncell][r]
+  int variant = 0;         // basic scheme: 0 / 1 / 2 = 256 / 128 / 512 threads per block
 };
 int launch_m2l_basic(const M2LArgs &a, hipStream_t s);
 int launch_m2l_sibling(const M2LArgs &a, hipStream_t s);
 
 // ---- dense-grid M2L (ghost-padded per-sibling-class grids) ----
 struct GridM2L;  // defined in m2l_grid.hip
+// minlev: first level handled by the grids (levels below are left to the
+// caller); variant: 0 automatic, 1 source-stationary (split-BF16), 2 / 3 one
+// / two 32-column tiles per wave (r = 32 MFMA forms)
 int grid_m2l_create(GridM2L **g, int r, int maxlev, int nrhs, int form, const real *K_colmajor_host,
-                    hipStream_t s);
+                    hipStream_t s, int minlev, int variant);
 void grid_m2l_destroy(GridM2L *g);
 // records `mid` (if not null) between the scatter into the grids and the M2L kernel
 int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, hipEvent_t mid);

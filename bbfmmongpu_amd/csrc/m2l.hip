@@ -13,14 +13,14 @@
 
 namespace bb {
 
-template <int R>
-__global__ void __launch_bounds__(256) k_m2l_basic(int F0, int nF, int ncell, const int *__restrict__ iinter,
+template <int R, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_m2l_basic(int F0, int nF, int ncell, const int *__restrict__ iinter,
                                                    const int *__restrict__ inter,
                                                    const int *__restrict__ kidx,
                                                    const real4 *__restrict__ K4,
                                                    const real *__restrict__ M, real *__restrict__ L)
 {
-  constexpr int CPB = 256 / R;  // field cells per block
+  constexpr int CPB = BLOCK / R;  // field cells per block
   const int cl = threadIdx.x / R;
   const int i = threadIdx.x % R;
   const int F = F0 + blockIdx.x * CPB + cl;
@@ -143,31 +143,42 @@ __global__ void __launch_bounds__(R < 64 ? 64 : R) k_m2l_sibling(int level, int 
   }
 }
 
-template <int R>
+template <int R, int BLOCK>
 static int basic_t(const M2LArgs &a, hipStream_t s)
 {
-  constexpr int CPB = 256 / R;
+  constexpr int CPB = BLOCK / R;
   if (!a.h_levsta) {
     // no level table (a batch of sparse instances): one launch over every
     // cell; cells without a list (levels 0-1) write L = 0
-    hipLaunchKernelGGL((k_m2l_basic<R>), dim3((a.ncell + CPB - 1) / CPB, a.nrhs), dim3(256), 0, s, 0, a.ncell,
-                       a.ncell, a.iinter, a.inter, a.kidx, a.K4, a.M, a.L);
+    hipLaunchKernelGGL((k_m2l_basic<R, BLOCK>), dim3((a.ncell + CPB - 1) / CPB, a.nrhs), dim3(BLOCK), 0, s, 0,
+                       a.ncell, a.ncell, a.iinter, a.inter, a.kidx, a.K4, a.M, a.L);
     BB_HIP(hipGetLastError());
     return BBFMM_OK;
   }
   for (int level = a.minlev; level <= a.maxlev; level++) {
     const int F0 = a.h_levsta[level], nF = a.h_levsta[level + 1] - F0;
-    hipLaunchKernelGGL((k_m2l_basic<R>), dim3((nF + CPB - 1) / CPB, a.nrhs), dim3(256), 0, s, F0, nF,
+    hipLaunchKernelGGL((k_m2l_basic<R, BLOCK>), dim3((nF + CPB - 1) / CPB, a.nrhs), dim3(BLOCK), 0, s, F0, nF,
                        a.ncell, a.iinter, a.inter, a.kidx, a.K4, a.M, a.L);
     BB_HIP(hipGetLastError());
   }
   return BBFMM_OK;
 }
 
+template <int R>
+static int basic_r(const M2LArgs &a, hipStream_t s)
+{
+  switch (a.variant) {
+  case 0: return basic_t<R, 256>(a, s);
+  case 1: return basic_t<R, (R > 128 ? R : 128)>(a, s);
+  case 2: return basic_t<R, 512>(a, s);
+  default: return set_error(BBFMM_E_ARG, "M2L basic: variant %d", a.variant);
+  }
+}
+
 int launch_m2l_basic(const M2LArgs &a, hipStream_t s)
 {
-  if (a.r == 32) return basic_t<32>(a, s);
-  if (a.r == 256) return basic_t<256>(a, s);
+  if (a.r == 32) return basic_r<32>(a, s);
+  if (a.r == 256) return basic_r<256>(a, s);
   return set_error(BBFMM_E_ARG, "M2L: r=%d not supported (r = 32 or 256)", a.r);
 }
 

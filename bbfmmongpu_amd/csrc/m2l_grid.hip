@@ -59,7 +59,7 @@ constexpr int kZeroOp = kNumOps;  // all-zero operator slot of the split-BF16 la
 constexpr int kWaitVm0 = 0x0F70;
 
 struct GridM2L {
-  int r = 0, maxlev = 0, nrhs = 1, form = 0, ncell_cap = 0;
+  int r = 0, maxlev = 0, nrhs = 1, form = 0, ncell_cap = 0, minlev = kMinLev, variant = 0;
   GridArgs args{};
   int nblocks = 0;
   size_t nslots = 0;
@@ -646,10 +646,18 @@ static void step_table(std::vector<int> &steps)
         }
 }
 
-int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const real *K, hipStream_t s)
+int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const real *K, hipStream_t s,
+                    int minlev, int variant)
 {
   *out = nullptr;
   if (maxlev > kMaxLev) return set_error(BBFMM_E_ARG, "dense-grid M2L: maxlev %d > %d", maxlev, kMaxLev);
+  if (minlev < kMinLev || minlev > maxlev)
+    return set_error(BBFMM_E_ARG, "dense-grid M2L: first level %d outside [%d, %d]", minlev, kMinLev, maxlev);
+  if (variant < 0 || variant > 3) return set_error(BBFMM_E_ARG, "dense-grid M2L: variant %d", variant);
+  if (variant == 1 && form != BBFMM_M2L_BF16X2)
+    return set_error(BBFMM_E_ARG, "dense-grid M2L: the source-stationary variant is split-BF16 only");
+  if (variant >= 2 && (form == BBFMM_M2L_PLAIN || r != 32))
+    return set_error(BBFMM_E_ARG, "dense-grid M2L: tile variants are for the r = 32 MFMA forms");
   if (r != 32 && r != 256) return set_error(BBFMM_E_ARG, "dense-grid M2L: r=%d not supported", r);
 #ifdef SINGLE
   if (form == BBFMM_M2L_BF16X2 && r != 32)
@@ -665,12 +673,14 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   GridArgs &a = g->args;
   a.r = r;
   a.nrhs = nrhs;
-  a.minlev = kMinLev;
+  a.minlev = minlev;
   a.maxlev = maxlev;
+  g->minlev = minlev;
+  g->variant = variant;
   for (int l = 0; l <= maxlev + 1; l++) a.levsta[l] = ((1 << (3 * l)) - 1) / 7;
   a.ncell = a.levsta[maxlev + 1];
   size_t slots = 0;
-  for (int l = kMinLev; l <= maxlev; l++) {
+  for (int l = minlev; l <= maxlev; l++) {
     a.lvl_slot0[l] = (int)slots;
     const size_t PP = ((size_t)1 << (l - 1)) + 2;
     slots += 8 * PP * PP * PP;
@@ -679,20 +689,23 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   int tpw = 1, cpb;
   // source-stationary blocking once the deepest level has enough columns
   // (parents x vectors) to give every SIMD at least a wave
-  a.ss = form == BBFMM_M2L_BF16X2 && (long long)(1 << (3 * (maxlev - 1))) * nrhs >= 32 * 1024 ? 1 : 0;
+  // variant 0 (automatic): source-stationary blocking once the deepest level
+  // has enough columns (parents x vectors) to give every SIMD at least a wave;
+  // 1: source-stationary; 2 / 3: one / two 32-column tiles per wave
+  const long long deep = (long long)(1 << (3 * (maxlev - 1))) * nrhs;
+  a.ss = form == BBFMM_M2L_BF16X2 && (variant == 1 || (variant == 0 && deep >= 32 * 1024)) ? 1 : 0;
   if (form == BBFMM_M2L_PLAIN) {
     cpb = 256 / r;
   } else if (a.ss) {
     cpb = 4 * 32;  // 4 waves x 32 columns, all 8 classes
   } else {
     // two tiles per wave once there are enough columns to fill the chip
-    const long long deep = (long long)(1 << (3 * (maxlev - 1))) * nrhs;
-    tpw = r == 32 && deep >= 8 * 1024 ? 2 : 1;  // r = 256: one 32-column tile per wave
+    tpw = variant == 3 ? 2 : variant == 2 ? 1 : (r == 32 && deep >= 8 * 1024 ? 2 : 1);  // r = 256: one tile
     cpb = 4 * tpw * 32;
   }
   a.wave_cols = tpw * 32;
   int nb = 0;
-  for (int l = kMinLev; l <= maxlev; l++) {
+  for (int l = minlev; l <= maxlev; l++) {
     a.blk0[l] = nb;
     const long long cols = (long long)(1 << (3 * (l - 1))) * nrhs;
     nb += (a.ss ? 1 : 8) * (int)((cols + cpb - 1) / cpb);

@@ -162,6 +162,12 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
 #define BBFMM_OPT_GRAPH 5     /* 1: capture the whole solve (gather .. scatter) into one
                                  hipGraph per (q, phi) pointer pair on first use and
                                  replay it; stage times then report the total only */
+#define BBFMM_OPT_M2L_VARIANT 6  /* tile shape of the M2L scheme: basic 0/1/2 = 256/128/512
+                                    threads per block; IJ MFMA forms 0 automatic,
+                                    1 source-stationary (split-BF16), 2/3 one/two 32-column
+                                    tiles per wave */
+#define BBFMM_OPT_LEVEL_SWITCH 7 /* CL/IJ schemes: first level on the dense grids, the
+                                    levels below on the basic scheme (default 2) */
 
 /* Create a plan for N particles at host positions pos (N real3, inside the
    root cell [-L/2,L/2]^3), Chebyshev order n, level modifier lmod (the -d
@@ -189,6 +195,14 @@ int bbfmm_amd_batch_create(bbfmm_amd_plan **plan, int ninst, const int *N, const
                            const int *depth, int leafmax, real L, int n, int r, const real *K,
                            const real *U, int nrhs, void *stream, int *depth_out);
 int bbfmm_amd_plan_set(bbfmm_amd_plan *plan, int option, int value);
+
+/* M2L autotuner (no counterpart in the reference, which fixes the scheme at
+   compile time): runs every M2L configuration valid for the plan's tree and
+   arithmetic form (scheme basic/sibling/cluster/ij, tile variant, level
+   switch) reps times on the plan's current multipoles, keeps the fastest
+   and writes up to maxrows rows (scheme, variant, level switch, ms per M2L)
+   to table.  The arithmetic form is never changed. */
+int bbfmm_amd_plan_tune(bbfmm_amd_plan *plan, int reps, double *table, int maxrows, int *nrows);
 
 /* Tree facts: out[0]=maxlev, out[1]=ncell, out[2]=N, out[3]=r, out[4]=n,
    out[5]=nrhs.  dout[0]=M2L interactions (levels 2..maxlev, per vector),

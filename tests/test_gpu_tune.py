@@ -1,0 +1,84 @@
+"""M2L configurations and the autotuner (bbfmm_amd_plan_tune): every
+scheme / tile variant / level switch the tuner can pick computes the same
+L as the oracle (relative L2 1e-5, split-BF16 3e-5), and a tuned plan
+solves as before."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# (form, scheme, variant, level switch, tolerance)
+CONFIGS = [
+    (0, 0, 0, 2, 1e-5), (0, 0, 1, 2, 1e-5), (0, 0, 2, 2, 1e-5), (0, 1, 0, 2, 1e-5),
+    (0, 3, 0, 2, 1e-5), (0, 3, 0, 3, 1e-5), (0, 2, 0, 3, 1e-5),
+    (1, 3, 1, 2, 3e-5), (1, 3, 2, 2, 3e-5), (1, 3, 3, 2, 3e-5), (1, 3, 1, 3, 3e-5), (1, 3, 3, 4, 3e-5),
+    (2, 3, 2, 2, 1e-5), (2, 3, 3, 3, 1e-5),
+]
+
+
+@pytest.mark.parametrize("form,scheme,variant,switch,tol", CONFIGS)
+def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, ops4, cuda):
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    nrhs = 16 if form == 1 else 2
+    pos, _q = O.uniform_inputs(65536, seed=0)
+    plan = B.Plan(pos, 4, U, K, prec="s", nrhs=nrhs).set(scheme=scheme, form=form, variant=variant,
+                                                         level_switch=switch)
+    tree = O.Tree(pos, 0, "d")
+    rng = np.random.default_rng(13)
+    M = rng.standard_normal((nrhs, tree.ncell * 32)).astype(np.float32)
+    Md = torch.from_numpy(M).to(cuda)
+    Ld = torch.zeros_like(Md)
+    plan.m2l(Md, Ld)
+    torch.cuda.synchronize()
+    Lg = Ld.cpu().numpy()
+    s = O.levsta(4)[2] * 32
+    Kd = K.astype(np.float32).astype(np.float64)
+    for j in (0, nrhs - 1):
+        ref = O.m2l(tree, Kd, M[j].astype(np.float64), 32, "basic")
+        err = O.rel_l2(ref[s:], Lg[j, s:])
+        print(f"form={form} scheme={scheme} variant={variant} switch={switch} rhs={j}: {err:.2e}", flush=True)
+        assert err < tol
+    plan.close()
+
+
+@pytest.mark.parametrize("form,nrhs", [(0, 1), (1, 16)])
+def test_tuned_plan_solves_like_oracle(form, nrhs, ops4, cuda):
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, Q = O.uniform_inputs(65536, seed=2, nrhs=nrhs)
+    Q = np.atleast_2d(Q).astype(np.float32)
+    plan = B.Plan(pos, 4, U, K, prec="s", nrhs=nrhs).set(scheme=B.SCHEME_IJ, form=form)
+    plan.solve(torch.from_numpy(Q).to(cuda))
+    table = plan.tune(reps=2)
+    print("tuning table:", [(int(a), int(b), int(c), round(d, 4)) for a, b, c, d in table], flush=True)
+    assert len(table) >= 2
+    best = min(table, key=lambda t: t[3])
+    assert all(t[3] > 0 for t in table)
+    phi = plan.solve(torch.from_numpy(Q).to(cuda)).cpu().numpy().reshape(nrhs, -1)
+    tree = O.Tree(pos, 0, "d")
+    ref = O.fmm(tree, Q[0].astype(np.float64), 4, U.astype(np.float32).astype(np.float64),
+                K.astype(np.float32).astype(np.float64), scheme="sibling")
+    err = O.rel_l2(ref["phi"], phi[0])
+    print(f"tuned (best {best}) solve rel_l2 {err:.2e}", flush=True)
+    assert err < (3e-5 if form == 1 else 1e-5)
+    plan.close()
+
+
+def test_tuned_batch_plan(ops4, cuda):
+    import bbfmmongpu_amd as B
+    from bbfmmongpu_amd import workloads as W
+    U, K, _ = ops4
+    pos_list, q_list = W.uneven_instances(count=6, nmin=4096, nmax=16384, seed=7)
+    plan = B.Plan.batch(pos_list, 4, U, K, nrhs=1, prec="s")
+    q = np.concatenate(q_list).astype(np.float32)
+    phi0 = plan.solve_host(q)
+    table = plan.tune(reps=2)
+    assert sorted(int(t[1]) for t in table) == [0, 1, 2] and all(int(t[0]) == 0 for t in table)
+    phi1 = plan.solve_host(q)
+    assert O.rel_l2(phi0, phi1) < 1e-6
+    plan.close()
