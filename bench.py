@@ -155,6 +155,72 @@ def run_c3(args):
     print(json.dumps(result), flush=True)
 
 
+def run_c4(args):
+    """64 independent instances, N log-uniform in [4096, 262144]; half Plummer
+    (a = 0.1, r <= 1, mapped into the unit cube), half on a sphere of radius
+    0.4; charges uniform [0, 1]; each instance on its own sparse octree with
+    leaves of at most 32 points; n = 4, r = 32, fp32.  One plan, M2L
+    configuration by the autotuner, the whole solve replayed as one hipGraph.
+    Errors against the FP64 direct sum on sampled targets of every instance."""
+    import torch
+    import bbfmmongpu_amd as B
+    from bbfmmongpu_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    pos_list, q_list = W.uneven_instances(count=64, nmin=4096, nmax=262144, seed=0)
+    Ntot = sum(p.shape[0] for p in pos_list)
+    U, K, _ = B.operators(4, 32, prec="s")
+    t0 = time.time()
+    plan = B.Plan.batch(pos_list, 4, U, K, nrhs=1, prec="s", stream=stream.cuda_stream)
+    t_setup = time.time() - t0
+    depths = plan.depths
+    log(f"c4: {len(pos_list)} instances, N total {Ntot}, depths {min(depths)}..{max(depths)}, "
+        f"interactions {plan.m2l_interactions:.4g} pairs {plan.p2p_pairs:.4g} setup {t_setup:.1f}s")
+    q = torch.from_numpy(np.concatenate(q_list).astype(np.float32)).to(dev)[None, :].contiguous()
+    phi = torch.empty_like(q)
+    plan.solve(q, phi)
+    table = plan.tune(reps=3)
+    best = min(table, key=lambda t: t[3])
+    log(f"c4: tuner {[(int(a), int(b), int(c), round(d, 3)) for a, b, c, d in table]} -> {best}")
+    ms_eager, st = timed_solves(plan, q, phi, args.steps, args.warmup, stream)
+    plan.set(graph=True)
+    ms_graph, _ = timed_solves(plan, q, phi, args.steps, args.warmup, stream)
+    log(f"c4: eager {ms_eager:.3f} ms/solve, graph {ms_graph:.3f} ms/solve, stages {st}")
+    out = phi.cpu().numpy()[0]
+    errs = []
+    o = 0
+    for i, (p, qi) in enumerate(zip(pos_list, q_list)):
+        n = p.shape[0]
+        tg = np.sort(np.random.default_rng(100 + i).choice(n, min(n, 256), replace=False)).astype(np.int32)
+        d = B.direct(p, qi, tg)
+        errs.append(rel_l2(d, out[o + tg]))
+        o += n
+    log(f"c4: errors vs FP64 direct: max {max(errs):.3e} median {float(np.median(errs)):.3e}")
+    flop_m2l = 32 * 63 * plan.m2l_interactions
+    result = {
+        "metric": "FMM s/solve for 64 uneven instances in one hipGraph (n=4, r=32), rel L2 err vs FP64 direct",
+        "value": len(pos_list) / (ms_graph * 1e-3), "unit": "instance solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_graph, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"64 instances, N log-uniform [4096, 262144] (total {Ntot}), half Plummer "
+                               f"a=0.1, half sphere surface r=0.4, leaves <= 32 points, n=4, r=32",
+                   "instances": len(pos_list), "N_total": Ntot, "depth_min": min(depths),
+                   "depth_max": max(depths), "m2l_config": {"scheme": int(best[0]), "variant": int(best[1]),
+                                                            "level_switch": int(best[2])}},
+        "particles_per_s": Ntot / (ms_graph * 1e-3),
+        "ms_eager": ms_eager, "ms_graph": ms_graph,
+        "m2l_gflops": flop_m2l / (st["m2l"] * 1e-3) / 1e9,
+        "p2p_ginteractions_per_s": plan.p2p_pairs / (st["p2p"] * 1e-3) / 1e9,
+        "stage_ms": st, "tuning_table": table,
+        "rel_l2_err": {"max": max(errs), "median": float(np.median(errs)), "targets_per_instance": 256},
+        "setup_s": t_setup,
+    }
+    plan.close()
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,9 +233,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for HBM byte counters)")
-    ap.add_argument("--config", choices=["c2", "c3"], default="c2",
+    ap.add_argument("--config", choices=["c2", "c3", "c4"], default="c2",
                     help="c2: N=1M n=4 chosen vs plain (default); c3: N=4M n=8 r=256 depth 6, "
-                         "16 vectors, FP32 MFMA M2L")
+                         "16 vectors, FP32 MFMA M2L; c4: 64 uneven instances in one plan and one hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note only "
@@ -177,6 +243,9 @@ def main():
         sys.exit(2)
     if args.config == "c3":
         run_c3(args)
+        return
+    if args.config == "c4":
+        run_c4(args)
         return
 
     import torch
