@@ -95,6 +95,12 @@ struct bbfmm_amd_plan {
   std::vector<real> h_K;                       // column-major operators (for the grid form)
   bb::GridM2L *grid = nullptr;
   int grid_form = -1;
+  // list MFMA M2L of sparse forests: class lists and source table (device),
+  // and the per-form operator layout (created on first use)
+  int h_cls_off[9] = {};
+  int *d_cls_cells = nullptr, *d_vsrc = nullptr;
+  bb::ListM2L *list = nullptr;
+  int list_form = -1;
   // per-stage timing: a ring of `nsets` event sets, one set per solve,
   // read (and averaged) only when stage times are requested
   int nsets = 0;
@@ -159,7 +165,8 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_Ucm, p->d_Urm, p->d_Tkz, p->d_Kw, p->d_Cw, p->d_scale, p->d_q, p->d_phi,
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
                   p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad,
-                  p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf};
+                  p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf, p->d_cls_cells,
+                  p->d_vsrc};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
@@ -168,6 +175,8 @@ static void plan_free_device(bbfmm_amd_plan *p)
   p->gstream = nullptr;
   if (p->grid) grid_m2l_destroy(p->grid);
   p->grid = nullptr;
+  if (p->list) list_m2l_destroy(p->list);
+  p->list = nullptr;
   for (auto &e : p->ev)
     if (e) (void)hipEventDestroy(e);
   p->ev.clear();
@@ -305,6 +314,11 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
   BB_TRY(upload(&p->d_kidx, f.kidx.data(), f.kidx.size(), stream));
   BB_TRY(upload(&p->d_Ktable, Kt, (size_t)343, stream));
   if (f.complete) BB_TRY(upload(&p->d_levsta, p->h_levsta.data(), p->h_levsta.size(), stream));
+  if (!f.complete) {
+    for (int c = 0; c < 9; c++) p->h_cls_off[c] = f.cls_off[c];
+    BB_TRY(upload(&p->d_cls_cells, f.cls_cells.data(), f.cls_cells.size(), stream));
+    BB_TRY(upload(&p->d_vsrc, f.vsrc.data(), f.vsrc.size(), stream));
+  }
   BB_TRY(upload(&p->d_K4, K4.data(), K4.size(), stream));
   BB_TRY(upload(&p->d_Ucm, U, (size_t)n3 * r, stream));
   BB_TRY(upload(&p->d_Urm, Urm.data(), Urm.size(), stream));
@@ -432,6 +446,18 @@ static PassArgs pass_args(const bbfmm_amd_plan *p)
 static int ensure_grid(bbfmm_amd_plan *p)
 {
   if (p->scheme != M2L_SCHEME_CL && p->scheme != M2L_SCHEME_IJ) return BBFMM_OK;
+  if (!p->complete) {
+    // sparse forest: the list MFMA form of the IJ blocking
+    if (p->scheme != M2L_SCHEME_IJ) return set_error(BBFMM_E_ARG, "scheme CL needs a complete tree");
+    if (!p->list || p->list_form != p->form) {
+      if (p->list) list_m2l_destroy(p->list);
+      p->list = nullptr;
+      BB_TRY(list_m2l_create(&p->list, p->r, p->nrhs, p->form, p->h_K.data(), p->ncell, p->h_cls_off,
+                             p->d_cls_cells, p->d_vsrc, p->stream));
+      p->list_form = p->form;
+    }
+    return BBFMM_OK;
+  }
   const int form = p->scheme == M2L_SCHEME_CL ? BBFMM_M2L_PLAIN : p->form;
   const int minlev = p->level_switch > kMinLev ? p->level_switch : kMinLev;
   const int variant = p->scheme == M2L_SCHEME_IJ ? p->m2l_variant : 0;
@@ -478,8 +504,12 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
   m.M = M;
   m.L = L;
   m.variant = p->scheme == M2L_SCHEME_BA ? p->m2l_variant : 0;
+  if (!p->complete && p->scheme == M2L_SCHEME_IJ) {
+    BB_TRY(ensure_grid(p));
+    return list_m2l_run(p->list, M, L, p->stream, mid);
+  }
   if (!p->complete && p->scheme != M2L_SCHEME_BA)
-    return set_error(BBFMM_E_ARG, "a batch of sparse instances runs the basic M2L scheme only");
+    return set_error(BBFMM_E_ARG, "a batch of sparse instances runs the basic or the IJ (list MFMA) scheme");
   switch (p->scheme) {
   case M2L_SCHEME_BA:
     if (mid) BB_HIP(hipEventRecord(mid, p->stream));
@@ -712,8 +742,8 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
   switch (option) {
   case BBFMM_OPT_SCHEME:
     if (value < M2L_SCHEME_BA || value > M2L_SCHEME_IJ) return set_error(BBFMM_E_ARG, "unknown M2L scheme %d", value);
-    if (!p->complete && value != M2L_SCHEME_BA)
-      return set_error(BBFMM_E_ARG, "a batch of sparse instances runs the basic M2L scheme only");
+    if (!p->complete && value != M2L_SCHEME_BA && value != M2L_SCHEME_IJ)
+      return set_error(BBFMM_E_ARG, "a batch of sparse instances runs the basic or the IJ (list MFMA) scheme");
     p->scheme = value;
     drop_graph(p);
     return BBFMM_OK;
@@ -768,6 +798,9 @@ extern "C" int bbfmm_amd_plan_tune(bbfmm_amd_plan *p, int reps, double *table, i
   std::vector<Cand> cands;
   if (!p->complete) {
     for (int v = 0; v < 3; v++) cands.push_back({M2L_SCHEME_BA, v, kMinLev});
+#ifdef SINGLE
+    if (p->form != BBFMM_M2L_PLAIN && p->r == 32) cands.push_back({M2L_SCHEME_IJ, 0, kMinLev});
+#endif
   } else {
     std::vector<int> sws{kMinLev};
     if (p->maxlev > kMinLev) sws.push_back(kMinLev + 1);

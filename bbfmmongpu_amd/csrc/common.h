@@ -19,6 +19,7 @@ constexpr int kMinLev = 2;              // another.h:24-26
 constexpr int kPitchNeighbors = 64;     // FLOORUP(27, ALIGN_SIZE=64)
 constexpr int kPitchInteraction = 192;  // FLOORUP(189, 64)
 constexpr int kNumOps = 316;            // offsets in [-3,3]^3 outside [-1,1]^3
+constexpr int kM2LOffsetsPerClass = 189;  // M2L offsets of one sibling class
 
 // api.hip: BBFMM_E_NODEVICE unless a GPU is visible
 int check_device();

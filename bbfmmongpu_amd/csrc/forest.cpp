@@ -10,6 +10,7 @@
 // each instance's field equals the reference's complete-octree result at
 // the same depth up to summation order.
 #include "forest.h"
+#include "kernels.h"
 
 #include <algorithm>
 #include <cstdint>
@@ -205,6 +206,7 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
   }
   f.ploff.assign(dmax + 1, 0);
   std::vector<std::vector<int>> plev(dmax);  // parents per level across instances
+  std::vector<int> cellclass;                // sibling class of each cell
   p0 = 0;
   for (int i = 0; i < ninst; i++) {
     const int d = dep[i], n = Ns[i];
@@ -301,6 +303,10 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
     f.iinter.resize(run, 0);
     f.inter.resize((size_t)run * kPitchInteraction, 0);
     f.kidx.resize((size_t)run * kPitchInteraction, 0);
+    f.vsrc.resize((size_t)run * kM2LOffsetsPerClass, -1);
+    cellclass.resize(run, 0);
+    for (int l = 1; l <= d; l++)
+      for (size_t j = 0; j < lev[l].key.size(); j++) cellclass[lev[l].first + j] = (int)(lev[l].key[j] & 7);
     for (int l = kMinLev; l <= d; l++) {
       const int np = 1 << (l - 1);
       for (size_t j = 0; j < lev[l].key.size(); j++) {
@@ -325,6 +331,8 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
                 f.inter[(size_t)kPitchInteraction * F + m] = C;
                 f.kidx[(size_t)kPitchInteraction * F + m] = Ktab[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
                 m++;
+                const int slot = class_slot(cellclass[F], vx, vy, vz);
+                if (slot >= 0) f.vsrc[(size_t)kM2LOffsetsPerClass * F + slot] = C;
               }
             }
         f.iinter[F] = m;
@@ -338,6 +346,19 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
     f.plist.insert(f.plist.end(), plev[l].begin(), plev[l].end());
   }
   f.ploff[dmax] = (int)f.plist.size();
+  // class lists for the list MFMA M2L; empty sources point at the zero cell
+  for (auto &x : f.vsrc)
+    if (x < 0) x = f.ncell;
+  f.cls_off.assign(9, 0);
+  for (int A = 0; A < f.ncell; A++)
+    if (f.celllev[A] >= kMinLev) f.cls_off[cellclass[A] + 1]++;
+  for (int c = 0; c < 8; c++) f.cls_off[c + 1] += f.cls_off[c];
+  f.cls_cells.assign(f.cls_off[8], 0);
+  {
+    std::vector<int> fill(f.cls_off.begin(), f.cls_off.end() - 1);
+    for (int A = 0; A < f.ncell; A++)
+      if (f.celllev[A] >= kMinLev) f.cls_cells[fill[cellclass[A]]++] = A;
+  }
   if (depth_out) *depth_out = dep;
   return BBFMM_OK;
 }

@@ -28,6 +28,9 @@ struct HostForest {
   std::vector<int> iinter, inter, kidx;  // [ncell], [ncell * kPitchInteraction] x 2
   double pairs = 0, ninter = 0;     // P2P pairs and M2L interactions (levels >= 2), per vector
   std::vector<int> inst_N, inst_depth, inst_p0, inst_leaf0;  // batch instances
+  // sparse forests: field cells of levels >= 2 grouped by sibling class, and
+  // the source cell of each of the class's 189 M2L offsets (ncell: empty)
+  std::vector<int> cls_off, cls_cells, vsrc;  // [9], [cells of levels >= 2], [ncell * 189]
 };
 
 // The reference's complete tree (anotherTree after build_tree).

@@ -69,6 +69,18 @@ void grid_m2l_destroy(GridM2L *g);
 // records `mid` (if not null) between the scatter into the grids and the M2L kernel
 int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, hipEvent_t mid);
 
+// ---- list MFMA M2L (trees storing only occupied cells) ----
+struct ListM2L;  // defined in m2l_grid.hip
+// cls_off[9] (host) / d_cls_cells (device): the field cells of levels >= 2
+// grouped by sibling class; d_vsrc [ncell * 189] (device): source cell of
+// each class offset (class_slot order), ncell for an empty source
+int list_m2l_create(ListM2L **g, int r, int nrhs, int form, const real *K_colmajor_host, int ncell,
+                    const int *cls_off, const int *d_cls_cells, const int *d_vsrc, hipStream_t s);
+void list_m2l_destroy(ListM2L *g);
+int list_m2l_run(ListM2L *g, const real *M, real *L, hipStream_t s, hipEvent_t mid);
+// slot of offset (vx, vy, vz) among the 189 M2L offsets of sibling class s, or -1
+int class_slot(int s, int vx, int vy, int vz);
+
 // ---- P2P ----
 struct P2PArgs {
   int N, nleaf, nrhs;

@@ -881,4 +881,277 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
   }
 }
 
+// ---------------------------------------------------------------- list MFMA (sparse trees)
+// For trees that store only occupied cells (batches of instances) the dense
+// grids do not exist; the same (class, offset) products are formed on
+// gathered columns.  The field cells of levels >= 2 are grouped by sibling
+// class s; for class s and its 189 offsets v (the dense-grid table order)
+// vsrc[F * 189 + j] names the source cell F + v, or the zero cell ncell when
+// that cell holds no points.  A block (4 waves x 32 columns, columns = (field
+// cell, vector) pairs of one class) multiplies K_v by the gathered source
+// columns for every v: the operator is shared by the whole block, as in the
+// dense-grid kernel.  M is copied per solve to [rhs][ncell + 1][r] (bf16 hi
+// and lo for the split form) with a zero last cell.
+
+struct ListArgs {
+  int ncell, nrhs;
+  int cls_off[9];   // class c cells: cls_cells[cls_off[c] .. cls_off[c + 1])
+  int blk0[9];      // first block of each class
+  const int *cls_cells;
+  const int *vsrc;  // [ncell * 189]
+  const int4 *tab;  // [8][189] (operator, ...)
+  const void *Ka;
+  const void *M0, *M1;
+  real *L;
+};
+
+struct ListM2L {
+  int form = 0, nrhs = 1, ncell = 0, nblocks = 0;
+  ListArgs args{};
+  int4 *d_tab = nullptr;
+  void *d_K = nullptr, *d_M0 = nullptr, *d_M1 = nullptr;
+};
+
+template <int FORM>
+__global__ void __launch_bounds__(256) k_m2l_list_prep(ListArgs g, const real *__restrict__ M)
+{
+  const long long total = (long long)g.nrhs * g.ncell * 4;  // 8 coefficients per thread
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int ch = (int)(t & 3);
+  const long long cr = t >> 2;
+  const int A = (int)(cr % g.ncell), rh = (int)(cr / g.ncell);
+  const real *src = M + ((size_t)rh * g.ncell + A) * 32 + ch * 8;
+  const size_t dst = ((size_t)rh * (g.ncell + 1) + A) * 32 + ch * 8;
+  if (FORM == BBFMM_M2L_BF16X2) {
+    bf16x8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const float v = (float)src[e];
+      const __bf16 h = (__bf16)v;
+      hi[e] = h;
+      lo[e] = (__bf16)(v - (float)h);
+    }
+    *reinterpret_cast<bf16x8 *>((__bf16 *)g.M0 + dst) = hi;
+    *reinterpret_cast<bf16x8 *>((__bf16 *)g.M1 + dst) = lo;
+  } else {
+    float *d = (float *)g.M0 + dst;
+#pragma unroll
+    for (int e = 0; e < 8; e++) d[e] = (float)src[e];
+  }
+}
+
+template <int FORM>
+__global__ void __launch_bounds__(256) k_m2l_list_mfma(ListArgs g)
+{
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  int s = 0;
+  while (s < 7 && (int)blockIdx.x >= g.blk0[s + 1]) s++;
+  const int grp = blockIdx.x - g.blk0[s];
+  const int ncols = (g.cls_off[s + 1] - g.cls_off[s]) * g.nrhs;
+  const int half = lane >> 5;
+  // columns past the end compute a copy of the class's first column and are not stored
+  const int col = (grp * 4 + wave) * 32 + (lane & 31);
+  const bool valid = col < ncols;
+  const int cc = valid ? col : 0;
+  const int F = g.cls_cells[g.cls_off[s] + cc / g.nrhs];
+  const int rh = cc % g.nrhs;
+  const int *vs = g.vsrc + (size_t)F * kOffsets;
+  const int4 *tab = g.tab + s * kOffsets;
+  const size_t rbase = (size_t)rh * (g.ncell + 1);
+  f32x16 acc;
+#pragma unroll
+  for (int e = 0; e < 16; e++) acc[e] = 0.0f;
+  int Sn = vs[0];
+  for (int v = 0; v < kOffsets; v++) {
+    const int S = Sn;
+    if (v + 1 < kOffsets) Sn = vs[v + 1];
+    const int k = tab[v].x;
+    if (FORM == BBFMM_M2L_BF16X2) {
+      const bf16x8 *Ka = (const bf16x8 *)g.Ka + (size_t)k * 4 * 64 + lane;
+      const bf16x8 a0h = Ka[0 * 64], a0l = Ka[1 * 64], a1h = Ka[2 * 64], a1l = Ka[3 * 64];
+      const size_t o = (rbase + S) * 32 + half * 8;
+      const bf16x8 b0h = *(const bf16x8 *)((const __bf16 *)g.M0 + o);
+      const bf16x8 b0l = *(const bf16x8 *)((const __bf16 *)g.M1 + o);
+      const bf16x8 b1h = *(const bf16x8 *)((const __bf16 *)g.M0 + o + 16);
+      const bf16x8 b1l = *(const bf16x8 *)((const __bf16 *)g.M1 + o + 16);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0l, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, b0h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1l, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, b1h, acc, 0, 0, 0);
+    } else {
+      const f32x4 *Ka = (const f32x4 *)g.Ka + ((size_t)k * 64 + lane) * 4;
+      const f32x4 *bp = (const f32x4 *)((const float *)g.M0 + (rbase + S) * 32 + half * 16);
+      f32x4 a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        a[q] = Ka[q];
+        b[q] = bp[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int w = 0; w < 4; w++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q][w], b[q][w], acc, 0, 0, 0);
+    }
+  }
+  if (!valid) return;
+  float *Lp = (float *)g.L + ((size_t)rh * g.ncell + F) * 32 + 4 * half;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    f32x4 o = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+    *reinterpret_cast<f32x4 *>(Lp + 8 * q) = o;
+  }
+}
+
+// slot j of offset v in class s's order (offset_table), or -1
+static void slot_table(std::vector<int> &slot)
+{
+  slot.assign(8 * 343, -1);
+  for (int s = 0; s < 8; s++) {
+    const int sb[3] = {(s >> 2) & 1, (s >> 1) & 1, s & 1};
+    int nv = 0;
+    for (int vx = -2 - sb[0]; vx <= 3 - sb[0]; vx++)
+      for (int vy = -2 - sb[1]; vy <= 3 - sb[1]; vy++)
+        for (int vz = -2 - sb[2]; vz <= 3 - sb[2]; vz++) {
+          if (vx >= -1 && vx <= 1 && vy >= -1 && vy <= 1 && vz >= -1 && vz <= 1) continue;
+          slot[s * 343 + 49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)] = nv++;
+        }
+  }
+}
+
+int class_slot(int s, int vx, int vy, int vz)
+{
+  static std::vector<int> slot;
+  if (slot.empty()) slot_table(slot);
+  if (vx < -3 || vx > 3 || vy < -3 || vy > 3 || vz < -3 || vz > 3) return -1;
+  return slot[s * 343 + 49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
+}
+
+int list_m2l_create(ListM2L **out, int r, int nrhs, int form, const real *K, int ncell, const int *cls_off,
+                    const int *d_cls_cells, const int *d_vsrc, hipStream_t s)
+{
+  *out = nullptr;
+#ifndef SINGLE
+  (void)r; (void)nrhs; (void)form; (void)K; (void)ncell; (void)cls_off; (void)d_cls_cells; (void)d_vsrc; (void)s;
+  return set_error(BBFMM_E_ARG, "the list MFMA M2L needs the fp32 build");
+#else
+  if (r != 32) return set_error(BBFMM_E_ARG, "the list MFMA M2L is built for r = 32 (got r=%d)", r);
+  if (form != BBFMM_M2L_BF16X2 && form != BBFMM_M2L_FP32MFMA)
+    return set_error(BBFMM_E_ARG, "the list MFMA M2L runs the split-BF16 or FP32 MFMA form");
+  ListM2L *g = new ListM2L();
+  g->form = form;
+  g->nrhs = nrhs;
+  g->ncell = ncell;
+  ListArgs &a = g->args;
+  a.ncell = ncell;
+  a.nrhs = nrhs;
+  int nb = 0;
+  for (int c = 0; c <= 8; c++) a.cls_off[c] = cls_off[c];
+  for (int c = 0; c < 8; c++) {
+    a.blk0[c] = nb;
+    const long long cols = (long long)(cls_off[c + 1] - cls_off[c]) * nrhs;
+    nb += (int)((cols + 127) / 128);
+  }
+  a.blk0[8] = nb;
+  g->nblocks = nb;
+  std::vector<int4> tab;
+  offset_table(tab);
+  int rc = BBFMM_OK;
+  auto fail = [&](hipError_t e, const char *what) {
+    rc = set_error(BBFMM_E_HIP, "list M2L: %s: %s", what, hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&g->d_tab, sizeof(int4) * tab.size())) != hipSuccess) fail(e, "hipMalloc");
+  if (rc == BBFMM_OK && (e = hipMemcpy(g->d_tab, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    fail(e, "copy");
+  if (rc == BBFMM_OK) {
+    if (form == BBFMM_M2L_BF16X2) {
+      std::vector<__bf16> Kb((size_t)kNumOps * 2 * 2 * 64 * 8);
+      for (int k = 0; k < kNumOps; k++)
+        for (int ks = 0; ks < 2; ks++)
+          for (int lane = 0; lane < 64; lane++)
+            for (int q = 0; q < 8; q++) {
+              const int row = lane & 31, col = 16 * ks + 8 * (lane >> 5) + q;
+              const float v = K[(size_t)k * 32 * 32 + (size_t)col * 32 + row];
+              const __bf16 h = (__bf16)v;
+              Kb[((((size_t)k * 2 + ks) * 2 + 0) * 64 + lane) * 8 + q] = h;
+              Kb[((((size_t)k * 2 + ks) * 2 + 1) * 64 + lane) * 8 + q] = (__bf16)(v - (float)h);
+            }
+      if ((e = hipMalloc(&g->d_K, sizeof(__bf16) * Kb.size())) != hipSuccess) fail(e, "hipMalloc");
+      else if ((e = hipMemcpy(g->d_K, Kb.data(), sizeof(__bf16) * Kb.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        fail(e, "copy");
+      const size_t nb2 = sizeof(__bf16) * (size_t)nrhs * (ncell + 1) * 32;
+      if (rc == BBFMM_OK && (e = hipMalloc(&g->d_M0, nb2)) != hipSuccess) fail(e, "hipMalloc");
+      if (rc == BBFMM_OK && (e = hipMalloc(&g->d_M1, nb2)) != hipSuccess) fail(e, "hipMalloc");
+      if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_M0, 0, nb2, s)) != hipSuccess) fail(e, "memset");
+      if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_M1, 0, nb2, s)) != hipSuccess) fail(e, "memset");
+    } else {
+      std::vector<float> Kf((size_t)kNumOps * 64 * 16);
+      for (int k = 0; k < kNumOps; k++)
+        for (int lane = 0; lane < 64; lane++)
+          for (int ks = 0; ks < 16; ks++) {
+            const int row = lane & 31, col = 16 * (lane >> 5) + ks;
+            Kf[((size_t)k * 64 + lane) * 16 + ks] = K[(size_t)k * 32 * 32 + (size_t)col * 32 + row];
+          }
+      if ((e = hipMalloc(&g->d_K, sizeof(float) * Kf.size())) != hipSuccess) fail(e, "hipMalloc");
+      else if ((e = hipMemcpy(g->d_K, Kf.data(), sizeof(float) * Kf.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        fail(e, "copy");
+      const size_t nb2 = sizeof(float) * (size_t)nrhs * (ncell + 1) * 32;
+      if (rc == BBFMM_OK && (e = hipMalloc(&g->d_M0, nb2)) != hipSuccess) fail(e, "hipMalloc");
+      if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_M0, 0, nb2, s)) != hipSuccess) fail(e, "memset");
+    }
+  }
+  if (rc == BBFMM_OK && (e = hipStreamSynchronize(s)) != hipSuccess) fail(e, "sync");
+  if (rc != BBFMM_OK) {
+    list_m2l_destroy(g);
+    return rc;
+  }
+  a.cls_cells = d_cls_cells;
+  a.vsrc = d_vsrc;
+  a.tab = g->d_tab;
+  a.Ka = g->d_K;
+  a.M0 = g->d_M0;
+  a.M1 = g->d_M1;
+  *out = g;
+  return BBFMM_OK;
+#endif
+}
+
+void list_m2l_destroy(ListM2L *g)
+{
+  if (!g) return;
+  for (void *p : {(void *)g->d_tab, g->d_K, g->d_M0, g->d_M1})
+    if (p) (void)hipFree(p);
+  delete g;
+}
+
+int list_m2l_run(ListM2L *g, const real *M, real *L, hipStream_t s, hipEvent_t mid)
+{
+  ListArgs a = g->args;
+  a.L = L;
+  const long long prep = (long long)a.nrhs * a.ncell * 4;
+  const int pb = (int)((prep + 255) / 256);
+#ifdef SINGLE
+  if (g->form == BBFMM_M2L_BF16X2) {
+    hipLaunchKernelGGL(k_m2l_list_prep<BBFMM_M2L_BF16X2>, dim3(pb), dim3(256), 0, s, a, M);
+    BB_HIP(hipGetLastError());
+    if (mid) BB_HIP(hipEventRecord(mid, s));
+    if (g->nblocks) hipLaunchKernelGGL(k_m2l_list_mfma<BBFMM_M2L_BF16X2>, dim3(g->nblocks), dim3(256), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(k_m2l_list_prep<BBFMM_M2L_FP32MFMA>, dim3(pb), dim3(256), 0, s, a, M);
+    BB_HIP(hipGetLastError());
+    if (mid) BB_HIP(hipEventRecord(mid, s));
+    if (g->nblocks) hipLaunchKernelGGL(k_m2l_list_mfma<BBFMM_M2L_FP32MFMA>, dim3(g->nblocks), dim3(256), 0, s, a);
+  }
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
+#else
+  (void)pb; (void)M; (void)s; (void)mid;
+  return set_error(BBFMM_E_ARG, "the list MFMA M2L needs the fp32 build");
+#endif
+}
+
 } // namespace bb

@@ -32,6 +32,22 @@ PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_c2.txt")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            f = dict(kv.split("=", 1) for kv in line.split() if "=" in kv)
+            if f.get("kernel") == kernel and "traffic_bytes" in f:
+                return float(f["traffic_bytes"])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -173,7 +189,10 @@ def run_c4(args):
     Ntot = sum(p.shape[0] for p in pos_list)
     U, K, _ = B.operators(4, 32, prec="s")
     t0 = time.time()
-    plan = B.Plan.batch(pos_list, 4, U, K, nrhs=1, prec="s", stream=stream.cuda_stream)
+    # fp32 arithmetic throughout; the FP32 MFMA form lets the tuner choose
+    # between the basic kernel and the class-grouped list MFMA kernel
+    plan = B.Plan.batch(pos_list, 4, U, K, nrhs=1, prec="s", stream=stream.cuda_stream).set(
+        form=B.FORM_FP32MFMA)
     t_setup = time.time() - t0
     depths = plan.depths
     log(f"c4: {len(pos_list)} instances, N total {Ntot}, depths {min(depths)}..{max(depths)}, "
@@ -207,8 +226,8 @@ def run_c4(args):
         "config": {"workload": f"64 instances, N log-uniform [4096, 262144] (total {Ntot}), half Plummer "
                                f"a=0.1, half sphere surface r=0.4, leaves <= 32 points, n=4, r=32",
                    "instances": len(pos_list), "N_total": Ntot, "depth_min": min(depths),
-                   "depth_max": max(depths), "m2l_config": {"scheme": int(best[0]), "variant": int(best[1]),
-                                                            "level_switch": int(best[2])}},
+                   "depth_max": max(depths), "m2l_form": "FP32 MFMA allowed (fp32 arithmetic)",
+                   "m2l_config": {"scheme": int(best[0]), "variant": int(best[1]), "level_switch": int(best[2])}},
         "particles_per_s": Ntot / (ms_graph * 1e-3),
         "ms_eager": ms_eager, "ms_graph": ms_graph,
         "m2l_gflops": flop_m2l / (st["m2l"] * 1e-3) / 1e9,
@@ -325,14 +344,21 @@ def main():
 
     # roofline of the dominant kernel of the chosen form
     p2p_flop = pairs * (18 + 2 * nrhs)  # geometry (20-flop convention) shared by the vectors
-    roof_m2l = {"kernel": "k_m2l_mfma<BF16X2>", "bound": "mfma", "unit": "TFLOP/s",
+    # issued MFMA flops of the source-stationary kernel: 208 steps x 8 classes
+    # x 6 products of 32x32x16 per 32-column tile (zero operator included)
+    blocks = sum(((1 << (3 * (lev - 1))) * nrhs + 127) // 128 for lev in range(2, chosen.maxlev + 1))
+    issued = 208 * 8 * 6 * (2 * 32 * 32 * 16) * blocks * 4
+    roof_m2l = {"kernel": "k_m2l_mfma_ss<4>", "bound": "mfma", "unit": "TFLOP/s",
                 "achieved": flop_m2l * nrhs / (st_chosen["m2l"] * 1e-3) / 1e12, "peak": PEAK_BF16_TFLOPS,
                 "launch_ms": st_chosen["m2l"], "algorithmic_flop_per_launch": flop_m2l * nrhs,
-                "issued_mfma_flop_per_launch": None, "traffic": None}
+                "issued_mfma_flop_per_launch": issued,
+                "issued_mfma_frac": issued / (st_chosen["m2l"] * 1e-3) / (PEAK_BF16_TFLOPS * 1e12),
+                "traffic": measured_traffic("k_m2l_mfma_ss<4>")}
     roof_m2l["frac"] = roof_m2l["achieved"] / roof_m2l["peak"]
-    roof_p2p = {"kernel": "k_p2p<16>", "bound": "valu", "unit": "TFLOP/s",
+    roof_p2p = {"kernel": "k_p2p_seg16", "bound": "valu", "unit": "TFLOP/s",
                 "achieved": p2p_flop / (st_chosen["p2p"] * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS,
-                "launch_ms": st_chosen["p2p"], "algorithmic_flop_per_launch": p2p_flop, "traffic": None}
+                "launch_ms": st_chosen["p2p"], "algorithmic_flop_per_launch": p2p_flop,
+                "traffic": measured_traffic("k_p2p_seg16")}
     roof_p2p["frac"] = roof_p2p["achieved"] / roof_p2p["peak"]
     dominant = roof_m2l if st_chosen["m2l"] >= st_chosen["p2p"] else roof_p2p
 

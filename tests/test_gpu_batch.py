@@ -66,3 +66,31 @@ def test_batch_auto_depth_against_direct(ops4, cuda):
         o += n
     assert max(plan.depths) >= 5  # the Plummer cores need deep leaves
     plan.close()
+
+
+@pytest.mark.parametrize("form,nrhs,tol", [(2, 1, 1e-5), (2, 3, 1e-5), (1, 16, 3e-5)])
+def test_batch_list_mfma_against_oracle(form, nrhs, tol, ops4, cuda):
+    """The IJ blocking on sparse trees (class-grouped gathered columns on
+    MFMA): FP32 MFMA and split BF16, per instance against the oracle."""
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    inst = _instances()
+    pos_list = [p for p, _ in inst]
+    depths = [d for _, d in inst]
+    Ntot = sum(p.shape[0] for p in pos_list)
+    Q = np.random.default_rng(16).uniform(-1.0, 1.0, size=(nrhs, Ntot)).astype(np.float32)
+    plan = B.Plan.batch(pos_list, 4, U, K, nrhs=nrhs, depths=depths, prec="s").set(form=form, scheme=B.SCHEME_IJ)
+    phi = plan.solve_host(Q).reshape(nrhs, Ntot)
+    Ud, Kd = U.astype(np.float32).astype(np.float64), K.astype(np.float32).astype(np.float64)
+    o = 0
+    for (p, d) in inst:
+        n = p.shape[0]
+        # identical inputs: the plan computes with fp32-rounded positions
+        tree = O.Tree(p.astype(np.float32).astype(np.float64), 0, "d", maxlev=d)
+        for j in sorted({0, nrhs - 1}):
+            ref = O.fmm(tree, Q[j, o:o + n].astype(np.float64), 4, Ud, Kd, scheme="basic")
+            err = O.rel_l2(ref["phi"], phi[j, o:o + n])
+            print(f"list-MFMA form={form} instance N={n} depth={d} rhs={j}: {err:.2e}", flush=True)
+            assert err < tol
+        o += n
+    plan.close()
