@@ -83,6 +83,8 @@ struct bbfmm_amd_plan {
   real *d_qin = nullptr, *d_phiout = nullptr;  // staging for the host-array entry points
   real *d_farfield = nullptr;                  // far-field q.*phi of the last solve (vector 0)
   real *d_qt = nullptr;                        // [Npad][16] charges, source-major (MFMA P2P)
+  real *d_phit = nullptr;                      // [Npad][16] potentials of the particle-row form
+  int *d_ipad = nullptr;                       // [N] user index -> padded leaf-order index
   real4 *d_ppos = nullptr;                     // padded particle copy for P2P
   int2 *d_seg = nullptr;
   int *d_nseg = nullptr, *d_pad_of = nullptr;
@@ -164,7 +166,7 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_nbr, p->d_nnbr, p->d_iinter, p->d_inter, p->d_kidx, p->d_Ktable, p->d_levsta,
                   p->d_Ucm, p->d_Urm, p->d_Tkz, p->d_Kw, p->d_Cw, p->d_scale, p->d_q, p->d_phi,
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
-                  p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad,
+                  p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad, p->d_phit, p->d_ipad,
                   p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf, p->d_cls_cells,
                   p->d_vsrc};
   for (void *q : ptrs)
@@ -377,6 +379,9 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
     BB_TRY(upload(&p->d_seg, seg.data(), seg.size(), stream));
     BB_TRY(upload(&p->d_nseg, nseg.data(), nseg.size(), stream));
     BB_TRY(upload(&p->d_pad_of, pad_of.data(), pad_of.size(), stream));
+    std::vector<int> ipad((size_t)N);
+    for (int i = 0; i < N; i++) ipad[p->h_perm[i]] = pad_of[i];
+    BB_TRY(upload(&p->d_ipad, ipad.data(), ipad.size(), stream));
   }
   const size_t NR = (size_t)N * nrhs;
   BB_TRY(dalloc(&p->d_q, NR));
@@ -404,7 +409,20 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
   return plan_init_forest(p, f, pos, n, r, K, U, Tkz, Kw, Cw, homogen, nrhs, stream);
 }
 
-static PassArgs pass_args(const bbfmm_amd_plan *p)
+// Particle-row form of a solve: 16 vectors in an MFMA form (fp32 build); the
+// charges and potentials move as rows of 16 in the padded leaf order that the
+// MFMA P2P reads.
+static bool rows16(const bbfmm_amd_plan *p)
+{
+#ifdef SINGLE
+  return p->form != BBFMM_M2L_PLAIN && p->nrhs == 16 && p->npad > 0;
+#else
+  (void)p;
+  return false;
+#endif
+}
+
+static PassArgs pass_args(const bbfmm_amd_plan *p, bool rows = false)
 {
   PassArgs a;
   a.N = p->N;
@@ -438,6 +456,13 @@ static PassArgs pass_args(const bbfmm_amd_plan *p)
   a.L = p->d_L;
   a.F = p->d_F;
   a.potential = p->potential;
+  if (rows) {
+    a.rows = 1;
+    a.qt = p->d_qt;
+    a.phit = p->d_phit;
+    a.pad_of = p->d_pad_of;
+    a.farfield = p->d_farfield;
+  }
   return a;
 }
 
@@ -483,6 +508,7 @@ static int prepare_solve(bbfmm_amd_plan *p)
     BB_TRY(dalloc(&p->d_qt, cnt));
     BB_HIP(hipMemsetAsync(p->d_qt, 0, sizeof(real) * cnt, p->stream));
   }
+  if (rows16(p) && !p->d_phit) BB_TRY(dalloc(&p->d_phit, (size_t)p->npad * 16));
 #endif
   return BBFMM_OK;
 }
@@ -534,10 +560,12 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
 }
 
 // The solve pipeline on device arrays in leaf order (d_q -> d_phi).
-static int solve_sorted(bbfmm_amd_plan *p, bool tm)
+// rows: the charges are in d_qt as particle rows (the particle-row form);
+// otherwise in d_q [nrhs][N] leaf order, the potentials going to d_phi
+static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
 {
   hipStream_t s = p->stream;
-  const PassArgs a = pass_args(p);
+  const PassArgs a = pass_args(p, rows);
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2M), s));
   BB_TRY(launch_p2m(a, s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2M), s));
@@ -550,7 +578,8 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm)
   BB_TRY(launch_l2l(a, p->h_ploff.data(), s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2P), s));
   BB_TRY(launch_l2p(a, s, p->nrhs));
-  BB_HIP(hipMemcpyAsync(p->d_farfield, p->d_phi, sizeof(real) * p->N, hipMemcpyDeviceToDevice, s));
+  if (!rows)  // the particle-row L2P writes the far field itself
+    BB_HIP(hipMemcpyAsync(p->d_farfield, p->d_phi, sizeof(real) * p->N, hipMemcpyDeviceToDevice, s));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2P), s));
   P2PArgs pp;
   pp.N = p->N;
@@ -573,6 +602,7 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm)
   if (pp.use_mfma && p->nrhs >= 16) {
     if (!p->d_qt) return set_error(BBFMM_E_ARG, "internal: solve not prepared");
     pp.qt = p->d_qt;
+    if (rows) pp.phit = p->d_phit;
   }
 #endif
   BB_TRY(launch_p2p(pp, s));
@@ -647,11 +677,18 @@ static int solve_dev(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev)
 static int solve_eager(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev, bool tm)
 {
   hipStream_t s = p->stream;
+  const bool rows = rows16(p);
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_GATHER), s));
-  BB_TRY(launch_gather(p->d_perm, q_dev, p->d_q, p->N, p->nrhs, s));
-  BB_TRY(solve_sorted(p, tm));
+  if (rows)
+    BB_TRY(launch_gather_rows16(p->d_ipad, q_dev, p->d_qt, p->N, s));
+  else
+    BB_TRY(launch_gather(p->d_perm, q_dev, p->d_q, p->N, p->nrhs, s));
+  BB_TRY(solve_sorted(p, tm, rows));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_SCATTER), s));
-  BB_TRY(launch_scatter(p->d_iperm, p->d_phi, phi_dev, p->N, p->nrhs, s));
+  if (rows)
+    BB_TRY(launch_scatter_rows16(p->d_ipad, p->d_phit, phi_dev, p->N, s));
+  else
+    BB_TRY(launch_scatter(p->d_iperm, p->d_phi, phi_dev, p->N, p->nrhs, s));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_TOTAL), s));
   return BBFMM_OK;
 }
@@ -765,7 +802,7 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_M2L_VARIANT:
-    if (value < 0 || value > 3) return set_error(BBFMM_E_ARG, "M2L variant %d out of range", value);
+    if (value < 0 || value > 5) return set_error(BBFMM_E_ARG, "M2L variant %d out of range", value);
     p->m2l_variant = value;
     drop_graph(p);
     return BBFMM_OK;
@@ -810,7 +847,7 @@ extern "C" int bbfmm_amd_plan_tune(bbfmm_amd_plan *p, int reps, double *table, i
       for (int sw : sws) cands.push_back({M2L_SCHEME_IJ, 0, sw});
     } else if (p->form == BBFMM_M2L_BF16X2) {
       for (int sw : sws)
-        for (int v = 1; v <= 3; v++) cands.push_back({M2L_SCHEME_IJ, v, sw});
+        for (int v = 1; v <= 5; v++) cands.push_back({M2L_SCHEME_IJ, v, sw});
     } else if (p->r == 32) {
       for (int sw : sws)
         for (int v = 2; v <= 3; v++) cands.push_back({M2L_SCHEME_IJ, v, sw});
@@ -959,7 +996,7 @@ extern "C" int bbfmm_amd_gradient(bbfmm_amd_plan *p, const real *q_dev, const re
   BB_TRY(prepare_solve(p));
   const int saved_potential = p->potential;
   p->potential = 1;
-  const int rc = solve_sorted(p, false);  // not a timed solve
+  const int rc = solve_sorted(p, false, false);  // not a timed solve
   p->potential = saved_potential;
   BB_TRY(rc);
   // dLoss/dq = A g: the potentials of the g vectors

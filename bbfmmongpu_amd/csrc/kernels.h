@@ -29,11 +29,24 @@ struct PassArgs {
   real *phi;            // [rhs][N] leaf order
   real *S, *M, *L, *F;
   int potential = 0;    // 1: L2P writes the potential, 0: q * potential (the reference's output)
+  // particle-row form of 16 vectors (rows = 1): charges and potentials as
+  // rows of 16 in the padded leaf order of the MFMA P2P; P2M reads qt, L2P
+  // writes the far-field potential into phit and vector 0's far field (with
+  // the output's q factor) into farfield
+  int rows = 0;
+  const real *qt = nullptr;   // [Npad][16]
+  real *phit = nullptr;       // [Npad][16]
+  const int *pad_of = nullptr;  // [N] leaf-order index -> padded index
+  real *farfield = nullptr;   // [N] leaf order
 };
 
 int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
 // dst[j] = src[iperm[j]] per vector (iperm: user index -> leaf-order index)
 int launch_scatter(const int *iperm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
+// particle-row form of 16 vectors: qt[ipad[j]][v] = q[v][j] and
+// phi[v][j] = phit[ipad[j]][v] (ipad: user index -> padded leaf-order index)
+int launch_gather_rows16(const int *ipad, const real *q, real *qt, int N, hipStream_t s);
+int launch_scatter_rows16(const int *ipad, const real *phit, real *phi, int N, hipStream_t s);
 int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs);
 // h_ploff[l]: offset of level l's parents in a.plist (levels 0..maxlev)
 int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs);
@@ -99,6 +112,9 @@ struct P2PArgs {
   const int *nseg = nullptr;    // [nleaf]
   const int *pad_of = nullptr;  // [N] leaf-order index -> padded index
   float *qt = nullptr;          // [Npad][16] charges of 16 vectors, source-major
+  // particle-row form: the far-field potentials are in phit [Npad][16]; P2P
+  // adds the near field and applies the output's q factor there (phi unused)
+  float *phit = nullptr;
 };
 int launch_p2p(const P2PArgs &a, hipStream_t s);
 

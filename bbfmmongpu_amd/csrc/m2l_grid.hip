@@ -48,8 +48,9 @@ struct GridArgs {
   const void *Mg1;  // grids (lo) for BF16X2
   real *L;
   int wave_cols;    // columns per wave
-  const int *steps;  // [208][kStepWords] step records (step_table)
-  int ss;            // 1: source-stationary kernel (block = columns of all 8 classes)
+  const int *steps;  // [class group][208][kStepWords] step records (step_table)
+  int nsteps[8];     // steps of each class group
+  int ss;            // source-stationary kernel: field classes per wave (8, 4 or 2); 0: not used
 };
 
 constexpr int kSteps = 208;  // 8 source classes x 26 non-zero parent shifts
@@ -256,25 +257,33 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(GridArgs g)
 // Same products, loop order inverted: a field cell of class s at parent p and
 // a source of class s' at parent p + d interact through the offset
 // v = 2d + s' - s, so for one (s', d) the shifted source tile M_{s'}[p + d]
-// serves all 8 field classes.  A wave owns 32 columns (parent, vector) of all
-// 8 classes (8 accumulators); per step (s', d) it loads the source tile once
-// and multiplies it by the up to 8 operators K_v, which the block stages in
-// LDS (double-buffered) for all its waves.  Used for split-BF16 (fp32 build).
-template <int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
+// serves every field class.  A wave owns 32 columns (parent, vector) of CLS
+// field classes (CLS accumulators); per step (s', d) it loads the source tile
+// once and multiplies it by the CLS operators K_v, which the block stages in
+// LDS (double-buffered) for all its waves.  CLS = 8 takes all classes of a
+// column group in one block; CLS = 4 or 2 splits a column group over 2 or 4
+// blocks (class groups), which evens out the work per CU when the column
+// groups do not fill whole rounds of the chip, and drops the steps in which
+// every class of the group has a near-field offset.  Split-BF16 (fp32 build).
+template <int CLS>
+__global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
 {
-  __shared__ bf16x8 sK[2][8][4][64];  // [buffer][field class][kstep*2 + hi/lo][lane]
+  constexpr int WAVES = 4;  // one k-part (k-step x hi/lo) of the operators per wave
+  constexpr int NCG = 8 / CLS;
+  __shared__ bf16x8 sK[2][CLS][4][64];  // [buffer][class of the group][kstep*2 + hi/lo][lane]
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   // Blocks are dealt to the 8 XCDs round-robin (block b on XCD b % 8); the
   // logical order gives each XCD a contiguous run of column groups, i.e.
-  // neighbouring parents, whose shifted source tiles then share its L2.
+  // neighbouring parents, whose shifted source tiles then share its L2; the
+  // class groups of one column group are adjacent.
   const int nb = g.blk0[g.maxlev + 1];
   const int lb = (blockIdx.x & 7) * ((nb + 7) >> 3) + (blockIdx.x >> 3);
   if (lb >= nb) return;
   int l = g.minlev;
   while (l < g.maxlev && lb >= g.blk0[l + 1]) l++;
-  const int grp = lb - g.blk0[l];
+  const int rel = lb - g.blk0[l];
+  const int grp = rel / NCG, cg = rel - grp * NCG;
   const int P = 1 << (l - 1);
   const int PP = P + 2;
   const int ncols = P * P * P * g.nrhs;
@@ -293,25 +302,24 @@ __global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
   const __bf16 *Mh = (const __bf16 *)g.Mg0;
   const __bf16 *Ml = (const __bf16 *)g.Mg1;
 
-  f32x16 acc[8];
+  f32x16 acc[CLS];
 #pragma unroll
-  for (int c = 0; c < 8; c++)
+  for (int c = 0; c < CLS; c++)
 #pragma unroll
     for (int e = 0; e < 16; e++) acc[c][e] = 0.0f;
 
-  // Step record: e[0] = source class | packed shift << 3, e[1 + i/2] holds the
-  // operator of field classes i (low 16 bits) and i + 1 (high); near-field
-  // pairs point at the zero operator, so every step runs all 8 classes.
-  // Each wave moves its k-part of the step's 8 operators (1 KB each) from
-  // global memory straight into the other LDS buffer (global_load_lds, 16
-  // bytes per lane at lane-contiguous addresses) while the MFMAs of the
-  // current step run; the source tile of the next step is loaded into
-  // registers at the same time.  Records are read two steps ahead.
-  static_assert(WAVES == 4, "one k-part per wave");
+  // Step record of the class group: e[0] = source class | packed shift << 3,
+  // e[1 + i/2] holds the operator of the group's classes i (low 16 bits) and
+  // i + 1 (high); near-field pairs point at the zero operator, so every step
+  // runs all CLS classes.  Each wave moves its k-part of the step's operators
+  // (1 KB each) from global memory straight into the other LDS buffer
+  // (global_load_lds, 16 bytes per lane at lane-contiguous addresses) while
+  // the MFMAs of the current step run; the source tile of the next step is
+  // loaded into registers at the same time.  Records are read two steps ahead.
   typedef __attribute__((address_space(3))) void lds_void;
   auto load_k = [&](const int *e, int buf) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < CLS; i++) {
       const int k = (e[1 + (i >> 1)] >> (16 * (i & 1))) & 0xffff;
       __builtin_amdgcn_global_load_lds((const void *)(Kl + (size_t)k * 256), (lds_void *)&sK[buf][i][wave][0], 16,
                                        0, 0);
@@ -322,15 +330,17 @@ __global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
     const int dx = ((pk >> 4) & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = (pk & 3) - 1;
     return ((size_t)(slot0 + sc * cls + (dx * PP + dy) * PP + dz + base) * g.nrhs + rh) * 32 + half * 8;
   };
-  const int *S = g.steps;
+  const int nst = g.nsteps[cg];
+  const int *S = g.steps + (size_t)cg * kSteps * kStepWords;
   int e1[5];
   bf16x8 bh0, bl0, bh1, bl1;
   {
     int e0[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) e0[i] = S[i];
+    const int s1 = nst > 1 ? 1 : 0;
 #pragma unroll
-    for (int i = 0; i < 5; i++) e1[i] = S[kStepWords + i];
+    for (int i = 0; i < 5; i++) e1[i] = S[s1 * kStepWords + i];
     load_k(e0, 0);
     const size_t o = src_off(e0);
     bh0 = *(const bf16x8 *)(Mh + o);
@@ -340,31 +350,31 @@ __global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
   }
   __builtin_amdgcn_s_waitcnt(kWaitVm0);
   __syncthreads();
-  for (int st = 0; st < kSteps; st++) {
+  for (int st = 0; st < nst; st++) {
     const int b = st & 1;
-    // the last step reloads its own data (harmless, never used)
-    const int n2 = st + 2 < kSteps ? st + 2 : kSteps - 1;
+    // the last steps reload their own data (harmless, never used)
+    const int n2 = st + 2 < nst ? st + 2 : nst - 1;
     // vector loads (operators arriving by DMA keep the compiler from scalar
     // loads here); moved to scalar registers after the end-of-step wait
     const int4 r2a = *reinterpret_cast<const int4 *>(S + n2 * kStepWords);
-    const int r2b = S[n2 * kStepWords + 4];
+    const int r2b = CLS > 6 ? S[n2 * kStepWords + 4] : 0;
     // k-step 0 of all classes, then the k-step-1 fragments; only then the
     // loads into the other buffer (the compiler orders LDS reads after any
     // earlier global_load_lds), then k-step 1
-    bf16x8 ah[8], al[8];
+    bf16x8 ah[CLS], al[CLS];
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
+    for (int c = 0; c < CLS; c++) {
       ah[c] = sK[b][c][0][lane];
       al[c] = sK[b][c][1][lane];
     }
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
+    for (int c = 0; c < CLS; c++) {
       acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh0, acc[c], 0, 0, 0);
       acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl0, acc[c], 0, 0, 0);
       acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh0, acc[c], 0, 0, 0);
     }
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
+    for (int c = 0; c < CLS; c++) {
       ah[c] = sK[b][c][2][lane];
       al[c] = sK[b][c][3][lane];
     }
@@ -377,7 +387,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
     const bf16x8 nl1 = *(const bf16x8 *)(Ml + o + 16);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
+    for (int c = 0; c < CLS; c++) {
       acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh1, acc[c], 0, 0, 0);
       acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl1, acc[c], 0, 0, 0);
       acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh1, acc[c], 0, 0, 0);
@@ -399,13 +409,14 @@ __global__ void __launch_bounds__(64 * WAVES) k_m2l_mfma_ss(GridArgs g)
   }
   if (!valid) return;
 #pragma unroll
-  for (int c = 0; c < 8; c++) {
+  for (int i = 0; i < CLS; i++) {
+    const int c = cg * CLS + i;
     const int fx = 2 * px + ((c >> 2) & 1), fy = 2 * py + ((c >> 1) & 1), fz = 2 * pz + (c & 1);
     const int A = g.levsta[l] + morton3(fx, fy, fz, l);
     float *Lp = (float *)g.L + ((size_t)rh * g.ncell + A) * 32 + 4 * half;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      f32x4 v = {acc[c][4 * q], acc[c][4 * q + 1], acc[c][4 * q + 2], acc[c][4 * q + 3]};
+      f32x4 v = {acc[i][4 * q], acc[i][4 * q + 1], acc[i][4 * q + 2], acc[i][4 * q + 3]};
       *reinterpret_cast<f32x4 *>(Lp + 8 * q) = v;
     }
   }
@@ -616,34 +627,47 @@ static void offset_table(std::vector<int4> &tab)
   }
 }
 
-// (s', d) steps of the source-stationary kernel: for every source class s'
-// and non-zero parent shift d, the operator index of each field class s
-// (offset v = 2d + s' - s), or the zero operator when v is a near-field
-// offset.  Record: [0] = s' | packed shift << 3, [1 + c/2] = operators of
-// classes c (low 16 bits) and c + 1 (high 16 bits), rest unused.
-static void step_table(std::vector<int> &steps)
+// (s', d) steps of the source-stationary kernel for field classes split into
+// groups of cls: for every source class s' and non-zero parent shift d, the
+// operator index of each field class s of the group (offset v = 2d + s' - s),
+// or the zero operator when v is a near-field offset; steps in which every
+// class of the group is near are left out.  Record: [0] = s' | packed shift
+// << 3, [1 + i/2] = operators of the group's classes i (low 16 bits) and
+// i + 1 (high 16 bits), rest unused.  Layout [group][kSteps][kStepWords].
+static void step_table(std::vector<int> &steps, int cls, int *nsteps)
 {
   int Kt[343];
   ktable(Kt);
-  steps.assign(kSteps * kStepWords, 0);
-  int st = 0;
-  for (int sc = 0; sc < 8; sc++)
-    for (int dxi = 0; dxi < 3; dxi++)
-      for (int dyi = 0; dyi < 3; dyi++)
-        for (int dzi = 0; dzi < 3; dzi++) {
-          if (dxi == 1 && dyi == 1 && dzi == 1) continue;
-          int *e = &steps[st * kStepWords];
-          e[0] = sc | (((dxi << 4) | (dyi << 2) | dzi) << 3);
-          for (int c = 0; c < 8; c++) {
-            const int vx = 2 * (dxi - 1) + ((sc >> 2) & 1) - ((c >> 2) & 1);
-            const int vy = 2 * (dyi - 1) + ((sc >> 1) & 1) - ((c >> 1) & 1);
-            const int vz = 2 * (dzi - 1) + (sc & 1) - (c & 1);
-            const bool near = vx >= -1 && vx <= 1 && vy >= -1 && vy <= 1 && vz >= -1 && vz <= 1;
-            const int k = near ? kZeroOp : Kt[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
-            e[1 + (c >> 1)] |= k << (16 * (c & 1));
+  const int ncg = 8 / cls;
+  steps.assign((size_t)ncg * kSteps * kStepWords, 0);
+  for (int cg = 0; cg < ncg; cg++) {
+    int st = 0;
+    for (int sc = 0; sc < 8; sc++)
+      for (int dxi = 0; dxi < 3; dxi++)
+        for (int dyi = 0; dyi < 3; dyi++)
+          for (int dzi = 0; dzi < 3; dzi++) {
+            if (dxi == 1 && dyi == 1 && dzi == 1) continue;
+            int *e = &steps[((size_t)cg * kSteps + st) * kStepWords];
+            e[0] = sc | (((dxi << 4) | (dyi << 2) | dzi) << 3);
+            bool far_any = false;
+            for (int i = 0; i < cls; i++) {
+              const int c = cg * cls + i;
+              const int vx = 2 * (dxi - 1) + ((sc >> 2) & 1) - ((c >> 2) & 1);
+              const int vy = 2 * (dyi - 1) + ((sc >> 1) & 1) - ((c >> 1) & 1);
+              const int vz = 2 * (dzi - 1) + (sc & 1) - (c & 1);
+              const bool near = vx >= -1 && vx <= 1 && vy >= -1 && vy <= 1 && vz >= -1 && vz <= 1;
+              far_any |= !near;
+              const int k = near ? kZeroOp : Kt[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
+              e[1 + (i >> 1)] |= k << (16 * (i & 1));
+            }
+            if (far_any) {
+              st++;
+            } else {
+              for (int w = 0; w < kStepWords; w++) e[w] = 0;
+            }
           }
-          st++;
-        }
+    nsteps[cg] = st;
+  }
 }
 
 int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const real *K, hipStream_t s,
@@ -653,8 +677,8 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   if (maxlev > kMaxLev) return set_error(BBFMM_E_ARG, "dense-grid M2L: maxlev %d > %d", maxlev, kMaxLev);
   if (minlev < kMinLev || minlev > maxlev)
     return set_error(BBFMM_E_ARG, "dense-grid M2L: first level %d outside [%d, %d]", minlev, kMinLev, maxlev);
-  if (variant < 0 || variant > 3) return set_error(BBFMM_E_ARG, "dense-grid M2L: variant %d", variant);
-  if (variant == 1 && form != BBFMM_M2L_BF16X2)
+  if (variant < 0 || variant > 5) return set_error(BBFMM_E_ARG, "dense-grid M2L: variant %d", variant);
+  if ((variant == 1 || variant >= 4) && form != BBFMM_M2L_BF16X2)
     return set_error(BBFMM_E_ARG, "dense-grid M2L: the source-stationary variant is split-BF16 only");
   if (variant >= 2 && (form == BBFMM_M2L_PLAIN || r != 32))
     return set_error(BBFMM_E_ARG, "dense-grid M2L: tile variants are for the r = 32 MFMA forms");
@@ -687,17 +711,21 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   }
   g->nslots = slots;
   int tpw = 1, cpb;
-  // source-stationary blocking once the deepest level has enough columns
-  // (parents x vectors) to give every SIMD at least a wave
   // variant 0 (automatic): source-stationary blocking once the deepest level
   // has enough columns (parents x vectors) to give every SIMD at least a wave;
-  // 1: source-stationary; 2 / 3: one / two 32-column tiles per wave
+  // 1 / 4 / 5: source-stationary with 8 / 4 / 2 field classes per wave;
+  // 2 / 3: one / two 32-column tiles per wave
   const long long deep = (long long)(1 << (3 * (maxlev - 1))) * nrhs;
-  a.ss = form == BBFMM_M2L_BF16X2 && (variant == 1 || (variant == 0 && deep >= 32 * 1024)) ? 1 : 0;
+  a.ss = 0;
+  if (form == BBFMM_M2L_BF16X2) {
+    if (variant == 1) a.ss = 8;
+    if (variant == 4 || (variant == 0 && deep >= 32 * 1024)) a.ss = 4;
+    if (variant == 5) a.ss = 2;
+  }
   if (form == BBFMM_M2L_PLAIN) {
     cpb = 256 / r;
   } else if (a.ss) {
-    cpb = 4 * 32;  // 4 waves x 32 columns, all 8 classes
+    cpb = 4 * 32;  // 4 waves x 32 columns, one class group per block
   } else {
     // two tiles per wave once there are enough columns to fill the chip
     tpw = variant == 3 ? 2 : variant == 2 ? 1 : (r == 32 && deep >= 8 * 1024 ? 2 : 1);  // r = 256: one tile
@@ -708,7 +736,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   for (int l = minlev; l <= maxlev; l++) {
     a.blk0[l] = nb;
     const long long cols = (long long)(1 << (3 * (l - 1))) * nrhs;
-    nb += (a.ss ? 1 : 8) * (int)((cols + cpb - 1) / cpb);
+    nb += (a.ss ? 8 / a.ss : 8) * (int)((cols + cpb - 1) / cpb);
   }
   a.blk0[maxlev + 1] = nb;
   g->nblocks = nb;
@@ -716,7 +744,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   std::vector<int4> tab;
   offset_table(tab);
   std::vector<int> steps;
-  step_table(steps);
+  step_table(steps, a.ss ? a.ss : 8, a.nsteps);
   int rc = BBFMM_OK;
   auto fail = [&](hipError_t e, const char *what) {
     rc = set_error(BBFMM_E_HIP, "dense-grid M2L: %s: %s", what, hipGetErrorString(e));
@@ -856,8 +884,12 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_BF16X2>, dim3(sb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (a.ss)  // rounded up to whole rounds of the 8 XCDs; the extra blocks exit
+    if (a.ss == 8)  // rounded up to whole rounds of the 8 XCDs; the extra blocks exit
+      hipLaunchKernelGGL((k_m2l_mfma_ss<8>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+    else if (a.ss == 4)
       hipLaunchKernelGGL((k_m2l_mfma_ss<4>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+    else if (a.ss == 2)
+      hipLaunchKernelGGL((k_m2l_mfma_ss<2>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else if (a.wave_cols == 64)
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
     else

@@ -206,6 +206,18 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
       p2p16_step<NT>(yB, qB, x, acc);
     }
   }
+  if (a.phit) {
+    // particle rows: far field + near field, then the output's q factor
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = tb + 16 * t + 4 * kq + v;
+        const size_t o = (size_t)(psta + i) * 16 + ti;
+        if (i < cnt) a.phit[o] = (a.potential ? 1.0f : Qt[o]) * (a.phit[o] + acc[t][v]);
+      }
+    return;
+  }
   const size_t row = (size_t)(rhs0 + ti) * a.N + sta;
 #pragma unroll
   for (int t = 0; t < NT; t++)
@@ -275,9 +287,11 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
     if (left >= 16) {
 #ifdef SINGLE
       if (a.use_mfma && a.qt && a.ppos && a.seg) {
-        hipLaunchKernelGGL(k_pad_charges16, dim3((a.N + 255) / 256), dim3(256), 0, s, a.N, a.pad_of,
-                           a.q + (size_t)rhs * a.N, a.qt);
-        BB_HIP(hipGetLastError());
+        if (!a.phit) {  // particle rows arrive filled
+          hipLaunchKernelGGL(k_pad_charges16, dim3((a.N + 255) / 256), dim3(256), 0, s, a.N, a.pad_of,
+                             a.q + (size_t)rhs * a.N, a.qt);
+          BB_HIP(hipGetLastError());
+        }
         hipLaunchKernelGGL(k_p2p_seg16, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
       } else
 #endif

@@ -51,6 +51,37 @@ __global__ void k_scatter(int N, const int *__restrict__ iperm, const real *__re
   if (j < N) dst[off + j] = src[off + iperm[j]];
 }
 
+// Particle-row form of 16 vectors.  A thread per user-order particle j reads
+// its 16 charges (one coalesced load per vector) and writes them as one
+// 64-byte row at its padded leaf-order slot; the scatter reads the row of
+// potentials back and writes 16 coalesced values.  Random accesses are whole
+// rows, not single words.
+__global__ void k_gather_rows16(int N, const int *__restrict__ ipad, const real *__restrict__ q,
+                                real *__restrict__ qt)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  real v[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) v[k] = q[(size_t)k * N + j];
+  real *row = qt + (size_t)ipad[j] * 16;
+#pragma unroll
+  for (int k = 0; k < 16; k++) row[k] = v[k];
+}
+
+__global__ void k_scatter_rows16(int N, const int *__restrict__ ipad, const real *__restrict__ phit,
+                                 real *__restrict__ phi)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  const real *row = phit + (size_t)ipad[j] * 16;
+  real v[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) v[k] = row[k];
+#pragma unroll
+  for (int k = 0; k < 16; k++) phi[(size_t)k * N + j] = v[k];
+}
+
 // M = U^T (Kw .* S) for one cell; sw holds Kw .* S in LDS.
 template <int NN, int BLOCK>
 __device__ __forceinline__ void project_u(const real *__restrict__ sw, const real *__restrict__ Urm,
@@ -77,9 +108,11 @@ __host__ __device__ constexpr int row_stride()
 // P2M at the leaf level (anotherUpwardPass.c:61-330): one block per leaf and
 // group of NR vectors, one thread per Chebyshev node (n=8: two nodes per
 // thread); S = sum_j q_j Sx Sy Sz, then M = U^T (Kw .* S).
-template <int NN, int BLOCK, int NR>
+// ROWS: the 16 charges of a particle are one row of a.qt (padded leaf order).
+template <int NN, int BLOCK, int NR, bool ROWS = false>
 __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
 {
+  static_assert(!ROWS || NR == 16, "particle rows hold 16 vectors");
   constexpr int N3 = NN * NN * NN;
   constexpr int NPT = N3 / BLOCK;
   constexpr int CH = BLOCK;
@@ -114,9 +147,17 @@ __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
 #pragma unroll
       for (int m = 0; m < NN; m++) sz[m][threadIdx.x] = t[m];
     }
-    for (int o = threadIdx.x; o < NR * CH; o += BLOCK) {
-      const int k = o / CH, j = o % CH;
-      if (j < nk) sq[j * QS + k] = a.q[(size_t)(rhs0 + k) * a.N + sta + base + j];
+    if (ROWS) {
+      const real *qr = a.qt + (size_t)a.pad_of[sta] * 16;  // the leaf's rows are contiguous
+      for (int o = threadIdx.x; o < 16 * CH; o += BLOCK) {
+        const int j = o >> 4, k = o & 15;
+        if (j < nk) sq[j * QS + k] = qr[(size_t)(base + j) * 16 + k];
+      }
+    } else {
+      for (int o = threadIdx.x; o < NR * CH; o += BLOCK) {
+        const int k = o / CH, j = o % CH;
+        if (j < nk) sq[j * QS + k] = a.q[(size_t)(rhs0 + k) * a.N + sta + base + j];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -340,9 +381,13 @@ __global__ void __launch_bounds__(256) k_l2p(PassArgs a)
 // L2P for NR vectors: one block per leaf, the leaf's F values in LDS
 // ([node][NR]); lanes split into NG groups of RS vectors, a lane per
 // particle within a group.
-template <int NN, int NR>
+// ROWS: the potentials go to rows of a.phit (padded leaf order) without the
+// output's q factor, which P2P applies; vector 0's far field with that factor
+// goes to a.farfield.
+template <int NN, int NR, bool ROWS = false>
 __global__ void __launch_bounds__(64) k_l2p_leaf(PassArgs a)
 {
+  static_assert(!ROWS || NR == 16, "particle rows hold 16 vectors");
   constexpr int N3 = NN * NN * NN;
   constexpr int RS = NR >= 8 ? NR / 2 : NR, NG = NR / RS, PT = 64 / NG;
   __shared__ __align__(16) real sf[N3 * NR];
@@ -386,10 +431,17 @@ __global__ void __launch_bounds__(64) k_l2p_leaf(PassArgs a)
 #pragma unroll
         for (int k = 0; k < RS; k++) sum[k] += w * inner[k];
       }
+    if (ROWS) {
+      const size_t row = (size_t)a.pad_of[sta + j] * 16 + g * RS;
 #pragma unroll
-    for (int k = 0; k < RS; k++) {
-      const size_t o = (size_t)(rhs0 + g * RS + k) * a.N + sta + j;
-      a.phi[o] = (a.potential ? (real)1 : a.q[o]) * a.prefac3 * sum[k];
+      for (int k = 0; k < RS; k++) a.phit[row + k] = a.prefac3 * sum[k];
+      if (g == 0) a.farfield[sta + j] = (a.potential ? (real)1 : a.qt[row]) * a.prefac3 * sum[0];
+    } else {
+#pragma unroll
+      for (int k = 0; k < RS; k++) {
+        const size_t o = (size_t)(rhs0 + g * RS + k) * a.N + sta + j;
+        a.phi[o] = (a.potential ? (real)1 : a.q[o]) * a.prefac3 * sum[k];
+      }
     }
   }
 }
@@ -411,6 +463,22 @@ int launch_scatter(const int *iperm, const real *src, real *dst, int N, int nrhs
   return BBFMM_OK;
 }
 
+int launch_gather_rows16(const int *ipad, const real *q, real *qt, int N, hipStream_t s)
+{
+  if (N <= 0) return BBFMM_OK;
+  hipLaunchKernelGGL(k_gather_rows16, dim3((N + 255) / 256), dim3(256), 0, s, N, ipad, q, qt);
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
+}
+
+int launch_scatter_rows16(const int *ipad, const real *phit, real *phi, int N, hipStream_t s)
+{
+  if (N <= 0) return BBFMM_OK;
+  hipLaunchKernelGGL(k_scatter_rows16, dim3((N + 255) / 256), dim3(256), 0, s, N, ipad, phit, phi);
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
+}
+
 // largest group of vectors per block that divides nrhs
 static int vec_group(int nrhs)
 {
@@ -421,7 +489,9 @@ template <int NN, int BLOCK>
 static void p2m_t(const PassArgs &a, hipStream_t s, int nrhs)
 {
   const int nr = vec_group(nrhs);
-  if (nr == 16)
+  if (a.rows)
+    hipLaunchKernelGGL((k_p2m<NN, BLOCK, 16, true>), dim3(a.nleaf, 1), dim3(BLOCK), 0, s, a);
+  else if (nr == 16)
     hipLaunchKernelGGL((k_p2m<NN, BLOCK, 16>), dim3(a.nleaf, nrhs / 16), dim3(BLOCK), 0, s, a);
   else if (nr == 4)
     hipLaunchKernelGGL((k_p2m<NN, BLOCK, 4>), dim3(a.nleaf, nrhs / 4), dim3(BLOCK), 0, s, a);
@@ -504,7 +574,9 @@ template <int NN>
 static void l2p_t(const PassArgs &a, hipStream_t s, int nrhs)
 {
   const int nr = vec_group(nrhs);
-  if (nr == 16)
+  if (a.rows)
+    hipLaunchKernelGGL((k_l2p_leaf<NN, 16, true>), dim3(a.nleaf, 1), dim3(64), 0, s, a);
+  else if (nr == 16)
     hipLaunchKernelGGL((k_l2p_leaf<NN, 16>), dim3(a.nleaf, nrhs / 16), dim3(64), 0, s, a);
   else if (nr == 4)
     hipLaunchKernelGGL((k_l2p_leaf<NN, 4>), dim3(a.nleaf, nrhs / 4), dim3(64), 0, s, a);

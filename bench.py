@@ -19,6 +19,7 @@ oracle/_ref) on the host on the same input.
 from __future__ import annotations
 
 import argparse
+import itertools
 import json
 import os
 import re
@@ -33,6 +34,26 @@ PEAK_FP32_TFLOPS = 157.3
 
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_c2.txt")
+
+
+def ss_group_steps(cls: int) -> list[int]:
+    """Steps (source class s', parent shift d != 0) of each group of cls field
+    classes in the source-stationary M2L: those in which some class of the
+    group has a far offset v = 2d + s' - s (|v|_inf >= 2)."""
+    out = []
+    for g in range(8 // cls):
+        n = 0
+        for sc in range(8):
+            for d in itertools.product((-1, 0, 1), repeat=3):
+                if d == (0, 0, 0):
+                    continue
+                far = False
+                for c in range(g * cls, (g + 1) * cls):
+                    v = [2 * d[a] + ((sc >> (2 - a)) & 1) - ((c >> (2 - a)) & 1) for a in range(3)]
+                    far |= max(abs(x) for x in v) >= 2
+                n += far
+        out.append(n)
+    return out
 
 
 def measured_traffic(kernel: str):
@@ -344,10 +365,11 @@ def main():
 
     # roofline of the dominant kernel of the chosen form
     p2p_flop = pairs * (18 + 2 * nrhs)  # geometry (20-flop convention) shared by the vectors
-    # issued MFMA flops of the source-stationary kernel: 208 steps x 8 classes
-    # x 6 products of 32x32x16 per 32-column tile (zero operator included)
-    blocks = sum(((1 << (3 * (lev - 1))) * nrhs + 127) // 128 for lev in range(2, chosen.maxlev + 1))
-    issued = 208 * 8 * 6 * (2 * 32 * 32 * 16) * blocks * 4
+    # issued MFMA flops of the source-stationary kernel (4 field classes per
+    # wave): per 32-column tile and class group, the group's steps x 4 classes
+    # x 6 products of 32x32x16 (zero operator included)
+    groups = sum(((1 << (3 * (lev - 1))) * nrhs + 127) // 128 for lev in range(2, chosen.maxlev + 1))
+    issued = sum(ss_group_steps(4)) * 4 * 6 * (2 * 32 * 32 * 16) * groups * 4
     roof_m2l = {"kernel": "k_m2l_mfma_ss<4>", "bound": "mfma", "unit": "TFLOP/s",
                 "achieved": flop_m2l * nrhs / (st_chosen["m2l"] * 1e-3) / 1e12, "peak": PEAK_BF16_TFLOPS,
                 "launch_ms": st_chosen["m2l"], "algorithmic_flop_per_launch": flop_m2l * nrhs,

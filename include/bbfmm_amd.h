@@ -164,8 +164,8 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                  replay it; stage times then report the total only */
 #define BBFMM_OPT_M2L_VARIANT 6  /* tile shape of the M2L scheme: basic 0/1/2 = 256/128/512
                                     threads per block; IJ MFMA forms 0 automatic,
-                                    1 source-stationary (split-BF16), 2/3 one/two 32-column
-                                    tiles per wave */
+                                    1/4/5 source-stationary (split-BF16) with 8/4/2 field
+                                    classes per wave, 2/3 one/two 32-column tiles per wave */
 #define BBFMM_OPT_LEVEL_SWITCH 7 /* CL/IJ schemes: first level on the dense grids, the
                                     levels below on the basic scheme (default 2) */
 

@@ -14,16 +14,21 @@ CONFIGS = [
     (0, 0, 0, 2, 1e-5), (0, 0, 1, 2, 1e-5), (0, 0, 2, 2, 1e-5), (0, 1, 0, 2, 1e-5),
     (0, 3, 0, 2, 1e-5), (0, 3, 0, 3, 1e-5), (0, 2, 0, 3, 1e-5),
     (1, 3, 1, 2, 3e-5), (1, 3, 2, 2, 3e-5), (1, 3, 3, 2, 3e-5), (1, 3, 1, 3, 3e-5), (1, 3, 3, 4, 3e-5),
+    (1, 3, 4, 2, 3e-5), (1, 3, 5, 2, 3e-5), (1, 3, 4, 3, 3e-5), (1, 3, 5, 4, 3e-5),
     (2, 3, 2, 2, 1e-5), (2, 3, 3, 3, 1e-5),
 ]
 
 
-@pytest.mark.parametrize("form,scheme,variant,switch,tol", CONFIGS)
-def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, ops4, cuda):
+# source-stationary class groups with a vector count that leaves partial column tiles
+RAGGED = [(1, 3, 1, 2, 3e-5, 3), (1, 3, 4, 2, 3e-5, 3), (1, 3, 5, 2, 3e-5, 5)]
+
+
+@pytest.mark.parametrize("form,scheme,variant,switch,tol,nrhs",
+                         [c + (16 if c[0] == 1 else 2,) for c in CONFIGS] + RAGGED)
+def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, nrhs, ops4, cuda):
     import torch
     import bbfmmongpu_amd as B
     U, K, _ = ops4
-    nrhs = 16 if form == 1 else 2
     pos, _q = O.uniform_inputs(65536, seed=0)
     plan = B.Plan(pos, 4, U, K, prec="s", nrhs=nrhs).set(scheme=scheme, form=form, variant=variant,
                                                          level_switch=switch)
@@ -40,7 +45,8 @@ def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, op
     for j in (0, nrhs - 1):
         ref = O.m2l(tree, Kd, M[j].astype(np.float64), 32, "basic")
         err = O.rel_l2(ref[s:], Lg[j, s:])
-        print(f"form={form} scheme={scheme} variant={variant} switch={switch} rhs={j}: {err:.2e}", flush=True)
+        print(f"form={form} scheme={scheme} variant={variant} switch={switch} nrhs={nrhs} rhs={j}: {err:.2e}",
+              flush=True)
         assert err < tol
     plan.close()
 
