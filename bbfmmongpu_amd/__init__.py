@@ -18,11 +18,11 @@ from ._build import lib_path
 __all__ = [
     "load", "Plan", "operators", "write_operators", "read_operators", "direct", "direct_field",
     "FMMPotential", "potential", "weights", "maxlev", "m2l_drv", "BBFMMError",
-    "SCHEME_BA", "SCHEME_SI", "SCHEME_CL", "SCHEME_IJ", "FORM_PLAIN", "FORM_BF16X2", "FORM_FP32MFMA",
+    "SCHEME_BA", "SCHEME_SI", "SCHEME_CL", "SCHEME_IJ", "FORM_PLAIN", "FORM_BF16X2", "FORM_FP32MFMA", "FORM_FP16X3",
 ]
 
 SCHEME_BA, SCHEME_SI, SCHEME_CL, SCHEME_IJ = 0, 1, 2, 3
-FORM_PLAIN, FORM_BF16X2, FORM_FP32MFMA = 0, 1, 2
+FORM_PLAIN, FORM_BF16X2, FORM_FP32MFMA, FORM_FP16X3 = 0, 1, 2, 3
 OPT_SCHEME, OPT_FORM, OPT_TIMING, OPT_OUTPUT, OPT_GRAPH, OPT_M2L_VARIANT, OPT_LEVEL_SWITCH = 1, 2, 3, 4, 5, 6, 7
 STAGES = ["gather", "p2m", "m2m", "m2l_prep", "m2l", "post_m2l", "l2l", "l2p", "p2p", "scatter", "total"]
 

@@ -785,7 +785,7 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_FORM:
-    if (value < BBFMM_M2L_PLAIN || value > BBFMM_M2L_FP32MFMA) return set_error(BBFMM_E_ARG, "unknown M2L form %d", value);
+    if (value < BBFMM_M2L_PLAIN || value > BBFMM_M2L_FP16X3) return set_error(BBFMM_E_ARG, "unknown M2L form %d", value);
 #ifndef SINGLE
     if (value != BBFMM_M2L_PLAIN) return set_error(BBFMM_E_ARG, "MFMA forms are fp32-build only");
 #endif
@@ -836,7 +836,8 @@ extern "C" int bbfmm_amd_plan_tune(bbfmm_amd_plan *p, int reps, double *table, i
   if (!p->complete) {
     for (int v = 0; v < 3; v++) cands.push_back({M2L_SCHEME_BA, v, kMinLev});
 #ifdef SINGLE
-    if (p->form != BBFMM_M2L_PLAIN && p->r == 32) cands.push_back({M2L_SCHEME_IJ, 0, kMinLev});
+    if ((p->form == BBFMM_M2L_BF16X2 || p->form == BBFMM_M2L_FP32MFMA) && p->r == 32)
+      cands.push_back({M2L_SCHEME_IJ, 0, kMinLev});
 #endif
   } else {
     std::vector<int> sws{kMinLev};

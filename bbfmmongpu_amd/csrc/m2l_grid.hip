@@ -21,16 +21,27 @@
 //   PLAIN   real-precision FMAs on the vector ALUs (drop-in parity);
 //   BF16X2  K and M split into bf16 hi + lo, three MFMA products
 //           (hi*hi + hi*lo + lo*hi) accumulated in fp32 (fp32 build);
-//   FP32    v_mfma_f32_32x32x2_f32 (fp32 build).
+//   FP32    v_mfma_f32_32x32x2_f32 (fp32 build);
+//   FP16X3  K and M each split into three fp16 terms t1 + t2 + t3 (about 33
+//           bits), six MFMA products (t1t1, t1t2, t2t1, t1t3, t2t2, t3t1)
+//           accumulated in fp32 (fp32 build).  fp16 has a narrow exponent
+//           range, so K is scaled by a power of two chosen from its largest
+//           entry and the M of every level by one chosen from that level's
+//           largest entry (measured on the device each solve); all sources
+//           of a field cell lie on its level, so the product is unscaled
+//           exactly in the epilogue.
 // MFMA forms use r = 32 (one 32 x 32 output tile per 32 columns).
 #include "common.h"
 #include "kernels.h"
 
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 namespace bb {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -45,7 +56,10 @@ struct GridArgs {
   const int4 *tab;  // [8][189]: (k, source class, packed shift, 0)
   const void *Ka;   // operators in the form's layout
   const void *Mg0;  // grids (hi / real)
-  const void *Mg1;  // grids (lo) for BF16X2
+  const void *Mg1;  // grids (lo) for BF16X2, (t2) for FP16X3
+  const void *Mg2;  // grids (t3) for FP16X3
+  unsigned *lvl_max;  // FP16X3: [kMaxLev + 1] bits of each level's largest |M|
+  float kscale_inv;   // FP16X3: 1 / the power of two K is scaled by
   real *L;
   int wave_cols;    // columns per wave
   const int *steps;  // [class group][208][kStepWords] step records (step_table)
@@ -69,6 +83,8 @@ struct GridM2L {
   void *d_K = nullptr;
   void *d_Mg0 = nullptr;
   void *d_Mg1 = nullptr;
+  void *d_Mg2 = nullptr;
+  unsigned *d_lvl_max = nullptr;
 };
 
 __device__ __forceinline__ int morton3(int x, int y, int z, int bits)
@@ -95,6 +111,57 @@ __device__ __forceinline__ int grid_slot(const GridArgs &g, int l, int c, int px
 {
   const int PP = (1 << (l - 1)) + 2;
   return g.lvl_slot0[l] + ((c * PP + px + 1) * PP + py + 1) * PP + pz + 1;
+}
+
+// ---------------------------------------------------------------- FP16X3 scaling
+// The power of two that maps a largest magnitude m (bits of a non-negative
+// float) into [2^14, 2^15): 2^(15 - e) with m = f 2^e, f in [0.5, 1).
+__device__ __forceinline__ int fp16_scale_exp(unsigned mbits)
+{
+  int e = 0;
+  (void)frexpf(__uint_as_float(mbits), &e);
+  return 15 - e;
+}
+
+__device__ __forceinline__ void split3_f16(float v, _Float16 &t1, _Float16 &t2, _Float16 &t3)
+{
+  t1 = (_Float16)v;
+  const float r1 = v - (float)t1;
+  t2 = (_Float16)r1;
+  t3 = (_Float16)(r1 - (float)t2);
+}
+
+__global__ void k_zero_u32(unsigned *p, int n)
+{
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// Largest |M| of every level minlev..maxlev: blockIdx.y = level - minlev; a
+// block reduces a grid-strided slice of the level's [rhs][cells][r] values
+// and posts one atomicMax on the bits (order-preserving for non-negative
+// floats).
+__global__ void __launch_bounds__(256) k_level_absmax(GridArgs g, const real *__restrict__ M)
+{
+  __shared__ unsigned red[4];
+  const int l = g.minlev + blockIdx.y;
+  const int c0 = g.levsta[l], nc = g.levsta[l + 1] - c0;
+  const long long per = (long long)nc * g.r / 4;  // float4 per vector
+  const long long total = per * g.nrhs;
+  float m = 0.0f;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+    const int rhs = (int)(t / per);
+    const long long k = t - (long long)rhs * per;
+    const float4 v = *reinterpret_cast<const float4 *>(M + ((size_t)rhs * g.ncell + c0) * g.r + 4 * k);
+    m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = __float_as_uint(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned b = red[0];
+    for (int w = 1; w < 4; w++) b = max(b, red[w]);
+    atomicMax(g.lvl_max + l, b);
+  }
 }
 
 // ---------------------------------------------------------------- scatter
@@ -130,6 +197,20 @@ __global__ void k_grid_scatter(GridArgs g, const real *__restrict__ M)
     }
     *reinterpret_cast<bf16x8 *>((__bf16 *)g.Mg0 + dst) = hi;
     *reinterpret_cast<bf16x8 *>((__bf16 *)g.Mg1 + dst) = lo;
+  } else if (FORM == BBFMM_M2L_FP16X3) {
+    const float sc = ldexpf(1.0f, fp16_scale_exp(g.lvl_max[l]));
+    f16x8 t1, t2, t3;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      _Float16 a1, a2, a3;
+      split3_f16((float)src[e] * sc, a1, a2, a3);
+      t1[e] = a1;
+      t2[e] = a2;
+      t3[e] = a3;
+    }
+    *reinterpret_cast<f16x8 *>((_Float16 *)g.Mg0 + dst) = t1;
+    *reinterpret_cast<f16x8 *>((_Float16 *)g.Mg1 + dst) = t2;
+    *reinterpret_cast<f16x8 *>((_Float16 *)g.Mg2 + dst) = t3;
   } else {
     real *d = (real *)g.Mg0 + dst;
 #pragma unroll
@@ -211,6 +292,36 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(GridArgs g)
         acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1l, acc[t], 0, 0, 0);
         acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, b1h, acc[t], 0, 0, 0);
       }
+    } else if (FORM == BBFMM_M2L_FP16X3) {
+      // A: [k][kstep][term][lane] f16x8, as the BF16X2 layout with three terms
+      const f16x8 *Ka = (const f16x8 *)g.Ka + (size_t)k * 6 * 64 + lane;
+      f16x8 a[2][3];
+#pragma unroll
+      for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) a[ks][q] = Ka[(ks * 3 + q) * 64];
+      const _Float16 *Mt[3] = {(const _Float16 *)g.Mg0, (const _Float16 *)g.Mg1, (const _Float16 *)g.Mg2};
+#pragma unroll
+      for (int t = 0; t < TPW; t++) {
+        const size_t o = ((size_t)(off + base[t]) * g.nrhs + rhs[t]) * 32 + half * 8;
+        f16x8 b[2][3];
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            b[ks][q] = *(const f16x8 *)(Mt[q] + o + 16 * ks);
+            if (!valid[t]) b[ks][q] = f16x8{0, 0, 0, 0, 0, 0, 0, 0};
+          }
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[ks][0], b[ks][0], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[ks][0], b[ks][1], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[ks][1], b[ks][0], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[ks][0], b[ks][2], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[ks][1], b[ks][1], acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[ks][2], b[ks][0], acc[t], 0, 0, 0);
+        }
+      }
     } else {
       // FP32: lane holds k = 16*half + ks for ks = 0..15; A: [k][lane][16]
       const f32x4 *Ka = (const f32x4 *)g.Ka + ((size_t)k * 64 + lane) * 4;
@@ -237,9 +348,14 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(GridArgs g)
     }
   }
   // epilogue: lane holds rows 8*(e/4) + 4*half + e%4 of its column
+  float unscale = 1.0f;
+  if (FORM == BBFMM_M2L_FP16X3) unscale = ldexpf(g.kscale_inv, -fp16_scale_exp(g.lvl_max[l]));
 #pragma unroll
   for (int t = 0; t < TPW; t++) {
     if (!valid[t]) continue;
+    if (FORM == BBFMM_M2L_FP16X3)
+#pragma unroll
+      for (int e = 0; e < 16; e++) acc[t][e] *= unscale;
     const int pl = pcol[t];
     const int pz = pl % P, py = (pl / P) % P, px = pl / (P * P);
     const int fx = 2 * px + ((s >> 2) & 1), fy = 2 * py + ((s >> 1) & 1), fz = 2 * pz + (s & 1);
@@ -686,6 +802,8 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
 #ifdef SINGLE
   if (form == BBFMM_M2L_BF16X2 && r != 32)
     return set_error(BBFMM_E_ARG, "the split-BF16 M2L form is built for r = 32 (got r=%d)", r);
+  if (form == BBFMM_M2L_FP16X3 && r != 32)
+    return set_error(BBFMM_E_ARG, "the split-FP16 M2L form is built for r = 32 (got r=%d)", r);
 #else
   if (form != BBFMM_M2L_PLAIN) return set_error(BBFMM_E_ARG, "MFMA M2L forms need the fp32 build");
 #endif
@@ -797,6 +915,44 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
       if (rc == BBFMM_OK && (e = hipMalloc(&g->d_Mg1, nb2)) != hipSuccess) fail(e, "hipMalloc");
       if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_Mg0, 0, nb2, s)) != hipSuccess) fail(e, "memset");
       if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_Mg1, 0, nb2, s)) != hipSuccess) fail(e, "memset");
+    } else if (form == BBFMM_M2L_FP16X3) {
+      // [k][kstep][term][lane][8]: lane l holds row l%32, k = 16*kstep + 8*(l/32) + e,
+      // scaled by 2^(15 - e) for the largest |K| = f 2^e
+      float kmax = 0.0f;
+      for (size_t i = 0; i < (size_t)kNumOps * 32 * 32; i++) kmax = std::max(kmax, std::fabs((float)K[i]));
+      int ke = 0;
+      (void)std::frexp(kmax, &ke);
+      const float ksc = std::ldexp(1.0f, 15 - ke);
+      a.kscale_inv = std::ldexp(1.0f, ke - 15);
+      std::vector<_Float16> Kh((size_t)kNumOps * 2 * 3 * 64 * 8);
+      for (int k = 0; k < kNumOps; k++)
+        for (int ks = 0; ks < 2; ks++)
+          for (int lane = 0; lane < 64; lane++)
+            for (int q = 0; q < 8; q++) {
+              const int row = lane & 31, col = 16 * ks + 8 * (lane >> 5) + q;
+              const float v = (float)K[(size_t)k * 32 * 32 + (size_t)col * 32 + row] * ksc;
+              const _Float16 t1 = (_Float16)v;
+              const float r1 = v - (float)t1;
+              const _Float16 t2 = (_Float16)r1;
+              const _Float16 t3 = (_Float16)(r1 - (float)t2);
+              const size_t b = (((size_t)k * 2 + ks) * 3) * 64 * 8 + (size_t)lane * 8 + q;
+              Kh[b] = t1;
+              Kh[b + 64 * 8] = t2;
+              Kh[b + 2 * 64 * 8] = t3;
+            }
+      if ((e = hipMalloc(&g->d_K, sizeof(_Float16) * Kh.size())) != hipSuccess) fail(e, "hipMalloc");
+      else if ((e = hipMemcpyAsync(g->d_K, Kh.data(), sizeof(_Float16) * Kh.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+        fail(e, "copy");
+      // the host copy must outlive the asynchronous copy
+      if (rc == BBFMM_OK && (e = hipStreamSynchronize(s)) != hipSuccess) fail(e, "sync");
+      const size_t nb2 = sizeof(_Float16) * slots * nrhs * r;
+      void **grids[3] = {&g->d_Mg0, &g->d_Mg1, &g->d_Mg2};
+      for (void **gp : grids) {
+        if (rc == BBFMM_OK && (e = hipMalloc(gp, nb2)) != hipSuccess) fail(e, "hipMalloc");
+        if (rc == BBFMM_OK && (e = hipMemsetAsync(*gp, 0, nb2, s)) != hipSuccess) fail(e, "memset");
+      }
+      if (rc == BBFMM_OK && (e = hipMalloc(&g->d_lvl_max, sizeof(unsigned) * (kMaxLev + 1))) != hipSuccess)
+        fail(e, "hipMalloc");
     } else if (r == 256) {
       // FP32, r = 256: [k][k-chunk][row tile][lane][16]: lane l holds row
       // 32*tile + l%32 and columns 32*chunk + 16*(l/32) + 0..15
@@ -846,6 +1002,8 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   a.Ka = g->d_K;
   a.Mg0 = g->d_Mg0;
   a.Mg1 = g->d_Mg1;
+  a.Mg2 = g->d_Mg2;
+  a.lvl_max = g->d_lvl_max;
   *out = g;
   return BBFMM_OK;
 }
@@ -858,6 +1016,8 @@ void grid_m2l_destroy(GridM2L *g)
   if (g->d_K) (void)hipFree(g->d_K);
   if (g->d_Mg0) (void)hipFree(g->d_Mg0);
   if (g->d_Mg1) (void)hipFree(g->d_Mg1);
+  if (g->d_Mg2) (void)hipFree(g->d_Mg2);
+  if (g->d_lvl_max) (void)hipFree(g->d_lvl_max);
   delete g;
 }
 
@@ -894,6 +1054,19 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
     else
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 1>), dim3(g->nblocks), dim3(256), 0, s, a);
+    BB_HIP(hipGetLastError());
+    return BBFMM_OK;
+  case BBFMM_M2L_FP16X3:
+    // per-level scales first (a fill kernel, not a memset: the solve may be captured)
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, a.lvl_max, kMaxLev + 1);
+    hipLaunchKernelGGL(k_level_absmax, dim3(256, a.maxlev - a.minlev + 1), dim3(256), 0, s, a, M);
+    hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_FP16X3>, dim3(sb), dim3(256), 0, s, a, M);
+    BB_HIP(hipGetLastError());
+    if (mid) BB_HIP(hipEventRecord(mid, s));
+    if (a.wave_cols == 64)
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP16X3, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP16X3, 1>), dim3(g->nblocks), dim3(256), 0, s, a);
     BB_HIP(hipGetLastError());
     return BBFMM_OK;
   case BBFMM_M2L_FP32MFMA:

@@ -352,6 +352,21 @@ def main():
     plan64.close()
     log(f"errors vs FP64 direct: plain {err_plain:.3e} chosen {errs_chosen} fp64-plain {err_fp64:.3e}")
 
+    # the other arithmetic forms of the 16-vector solve on the same inputs:
+    # FP32 MFMA and the FP16 3-term split (both on the dense grids)
+    variants = {}
+    for name, form in (("fp32_mfma", B.FORM_FP32MFMA), ("fp16x3", B.FORM_FP16X3)):
+        vp = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", stream=stream.cuda_stream).set(scheme=B.SCHEME_IJ,
+                                                                                        form=form)
+        phiv = torch.empty_like(qm)
+        ms_v, st_v = timed_solves(vp, qm, phiv, max(args.steps // 2, 1), 1, stream)
+        pv = phiv.cpu().numpy()
+        variants[name] = {"ms_per_step": ms_v, "m2l_ms": st_v["m2l"],
+                          "rel_l2_err_q_phi": [rel_l2(direct_ref[j], pv[j, tg]) for j in range(nerr)],
+                          "vs_fp64_plain": rel_l2(phi64, pv[0])}
+        vp.close()
+        log(f"form {name}: {ms_v:.3f} ms/step, M2L {st_v['m2l']:.3f} ms, errors {variants[name]['rel_l2_err_q_phi']}")
+
     inter = chosen.m2l_interactions
     pairs = chosen.p2p_pairs
     r = 32
@@ -413,6 +428,7 @@ def main():
         "plain": {"s_per_solve": s_per_solve_plain, "m2l_gflops": m2l_gflops_plain,
                   "p2p_ginteractions_per_s": p2p_gips_plain, "stage_ms": st_plain},
         "chosen_stage_ms": st_chosen,
+        "other_forms": variants,
         "setup_s": {"operators": t_ops, "plans": t_setup},
         "roofline": dominant,
         "roofline_m2l": roof_m2l,

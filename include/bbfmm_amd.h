@@ -151,6 +151,9 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
 #define BBFMM_M2L_PLAIN 0     /* real-precision FMAs (drop-in parity) */
 #define BBFMM_M2L_BF16X2 1    /* split BF16 hi/lo operands, 3 MFMA products, FP32 accumulate */
 #define BBFMM_M2L_FP32MFMA 2  /* FP32 MFMA */
+#define BBFMM_M2L_FP16X3 3    /* split FP16 operands in three terms (K and each level's M
+                                 scaled by powers of two), 6 MFMA products, FP32 accumulate;
+                                 dense grids (complete trees), r = 32 */
 
 /* Plan options (bbfmm_amd_plan_set) */
 #define BBFMM_OPT_SCHEME 1    /* M2L_SCHEME_* */

@@ -16,6 +16,7 @@ CONFIGS = [
     (1, 3, 1, 2, 3e-5), (1, 3, 2, 2, 3e-5), (1, 3, 3, 2, 3e-5), (1, 3, 1, 3, 3e-5), (1, 3, 3, 4, 3e-5),
     (1, 3, 4, 2, 3e-5), (1, 3, 5, 2, 3e-5), (1, 3, 4, 3, 3e-5), (1, 3, 5, 4, 3e-5),
     (2, 3, 2, 2, 1e-5), (2, 3, 3, 3, 1e-5),
+    (3, 3, 2, 2, 1e-6), (3, 3, 3, 3, 1e-6), (3, 3, 0, 2, 1e-6),
 ]
 
 
@@ -24,7 +25,7 @@ RAGGED = [(1, 3, 1, 2, 3e-5, 3), (1, 3, 4, 2, 3e-5, 3), (1, 3, 5, 2, 3e-5, 5)]
 
 
 @pytest.mark.parametrize("form,scheme,variant,switch,tol,nrhs",
-                         [c + (16 if c[0] == 1 else 2,) for c in CONFIGS] + RAGGED)
+                         [c + (16 if c[0] in (1, 3) else 2,) for c in CONFIGS] + RAGGED)
 def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, nrhs, ops4, cuda):
     import torch
     import bbfmmongpu_amd as B
@@ -51,7 +52,36 @@ def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, nr
     plan.close()
 
 
-@pytest.mark.parametrize("form,nrhs", [(0, 1), (1, 16)])
+def test_fp16x3_scales_each_level(ops4, cuda):
+    """The split-FP16 form scales K and every level's M by powers of two:
+    multipoles whose size differs by 1e10 between levels (beyond fp16's
+    range) keep fp32-level accuracy on every level."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, _q = O.uniform_inputs(65536, seed=0)
+    plan = B.Plan(pos, 4, U, K, prec="s", nrhs=16).set(scheme=B.SCHEME_IJ, form=B.FORM_FP16X3)
+    tree = O.Tree(pos, 0, "d")
+    ls = O.levsta(5)  # first cell of levels 0..5
+    rng = np.random.default_rng(17)
+    M = rng.standard_normal((16, tree.ncell * 32))
+    for lev, f in ((2, 1e5), (3, 1.0), (4, 1e-5)):
+        M[:, ls[lev] * 32:ls[lev + 1] * 32] *= f
+    M = M.astype(np.float32)
+    Ld = torch.zeros((16, tree.ncell * 32), device=cuda)
+    plan.m2l(torch.from_numpy(M).to(cuda), Ld)
+    Lg = Ld.cpu().numpy()
+    Kd = K.astype(np.float32).astype(np.float64)
+    ref = O.m2l(tree, Kd, M[5].astype(np.float64), 32, "basic")
+    for lev in (2, 3, 4):
+        sl = slice(ls[lev] * 32, ls[lev + 1] * 32)
+        err = O.rel_l2(ref[sl], Lg[5, sl])
+        print(f"level {lev}: rel_l2 {err:.2e}", flush=True)
+        assert err < 1e-6
+    plan.close()
+
+
+@pytest.mark.parametrize("form,nrhs", [(0, 1), (1, 16), (3, 16)])
 def test_tuned_plan_solves_like_oracle(form, nrhs, ops4, cuda):
     import torch
     import bbfmmongpu_amd as B
@@ -71,7 +101,7 @@ def test_tuned_plan_solves_like_oracle(form, nrhs, ops4, cuda):
                 K.astype(np.float32).astype(np.float64), scheme="sibling")
     err = O.rel_l2(ref["phi"], phi[0])
     print(f"tuned (best {best}) solve rel_l2 {err:.2e}", flush=True)
-    assert err < (3e-5 if form == 1 else 1e-5)
+    assert err < (3e-5 if form == 1 else 1e-5)  # form 3 (FP16 3-term): fp32 level
     plan.close()
 
 
