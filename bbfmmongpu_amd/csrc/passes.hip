@@ -203,6 +203,318 @@ __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
   }
 }
 
+#ifdef SINGLE
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// v[k] for a lane-dependent k in 0..3 without a dynamically indexed array
+__device__ __forceinline__ float pick4(const float *v, int k)
+{
+  const float a = (k & 1) ? v[1] : v[0];
+  const float b = (k & 1) ? v[3] : v[2];
+  return (k & 2) ? b : a;
+}
+
+// P2M of the particle-row form, n = 4, on MFMA (v_mfma_f32_16x16x4_f32):
+// one wave per leaf, four leaves per block.  For 4 particles at a time,
+// lane l supplies the A value W[node 16t + l%16][particle l/16] (the
+// Chebyshev weight Sx Sy Sz of that node, row tile t = x index) and the B
+// value q[particle l/16][vector l%16] from the charge rows, so four MFMAs
+// accumulate S (64 nodes x 16 vectors).  M = U^T (Kw .* S) follows on the
+// same unit: S goes to LDS, A = U^T (lane: coefficient l%16 of a row tile,
+// node l/16 of a k-step), B = Kw .* S.  Dummy rows of the padded leaf have
+// zero charge and are masked anyway.
+__global__ void __launch_bounds__(256) k_p2m_rows_mfma4(PassArgs a)
+{
+  constexpr int N3 = 64, LP = 17;  // padded LDS row of 16 vectors
+  __shared__ float sS[4][N3 * LP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int leaf = blockIdx.x * 4 + wave;
+  const bool active = leaf < a.nleaf;
+  const int l16 = lane & 15, kq = lane >> 4;
+  int A = 0, cnt = 0, sta = 0, psta = 0;
+  if (active) {
+    A = a.leafcell[leaf];
+    cnt = a.leafcnt[leaf];
+    sta = a.leafsta[leaf];
+    psta = cnt > 0 ? a.pad_of[sta] : 0;
+  }
+  f32x4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  if (cnt > 0) {
+    const real ih = a.leafihalf[leaf];
+    const real4 c = a.center[A];
+    for (int k0 = 0; k0 < cnt; k0 += 4) {
+      const int p = k0 + kq;
+      const bool ok = p < cnt;
+      const real4 x = ok ? a.pos[sta + p] : c;
+      float sx[4], sy[4], sz[4];
+      cheb_row<4>(a.Tkz, ih * (x.x - c.x), sx);
+      cheb_row<4>(a.Tkz, ih * (x.y - c.y), sy);
+      cheb_row<4>(a.Tkz, ih * (x.z - c.z), sz);
+      const float syz = ok ? pick4(sy, l16 >> 2) * pick4(sz, l16 & 3) : 0.f;
+      const float bq = a.qt[(size_t)(psta + p) * 16 + l16];  // rows up to the padded count exist
+#pragma unroll
+      for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(sx[t] * syz, bq, acc[t], 0, 0, 0);
+    }
+  }
+  // acc[t][v] = S[node 16t + 4 kq + v][vector l16]
+  float *sw = sS[wave];
+  if (active) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      f32x4 v = acc[t] * a.prefac3;
+      *reinterpret_cast<f32x4 *>(a.S + ((size_t)l16 * a.ncell + A) * N3 + 16 * t + 4 * kq) = v;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int node = 16 * t + 4 * kq + e;
+        sw[node * LP + l16] = v[e] * a.Kw[node];
+      }
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  // M[coef 16t + i][v] = sum_node U[node][coef] sw[node][v]
+  f32x4 m[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll 4
+  for (int kk = 0; kk < 16; kk++) {
+    const int node = 4 * kk + kq;
+    const float b = sw[node * LP + l16];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+      m[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.Urm[(size_t)node * 32 + 16 * t + l16], b, m[t], 0, 0, 0);
+  }
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+    *reinterpret_cast<f32x4 *>(a.M + ((size_t)l16 * a.ncell + A) * 32 + 16 * t + 4 * kq) = m[t];
+}
+
+// L2P of the particle-row form, n = 4, on MFMA: one wave per leaf, four
+// leaves per block.  The leaf's F (64 nodes x 16 vectors) is staged in LDS;
+// per tile of 16 particles and k-step of 4 nodes, lane l supplies
+// A = W[particle l%16][node 4kk + l/16] and B = F[node 4kk + l/16][vector
+// l%16].  The far-field potentials go to the particle rows (no q factor)
+// and vector 0's far field, with the output's factor, to farfield.
+__global__ void __launch_bounds__(256) k_l2p_rows_mfma4(PassArgs a)
+{
+  constexpr int N3 = 64, LP = 17;
+  __shared__ float sF[4][N3 * LP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int leaf = blockIdx.x * 4 + wave;
+  if (leaf >= a.nleaf) return;  // wave-private LDS: no block barrier below
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int cnt = a.leafcnt[leaf];
+  if (cnt == 0) return;
+  const int A = a.leafcell[leaf], sta = a.leafsta[leaf];
+  const int psta = a.pad_of[sta];
+  float *sf = sF[wave];
+  // 16 vectors x 64 nodes: lane reads 16 consecutive nodes of vector v
+  for (int o = lane; o < 16 * 16; o += 64) {
+    const int v = o >> 4, q4 = o & 15;
+    const f32x4 f = *reinterpret_cast<const f32x4 *>(a.F + ((size_t)v * a.ncell + A) * N3 + 4 * q4);
+#pragma unroll
+    for (int e = 0; e < 4; e++) sf[(4 * q4 + e) * LP + v] = f[e];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const real ih = a.leafihalf[leaf];
+  const real4 c = a.center[A];
+  for (int tb = 0; tb < cnt; tb += 16) {
+    const int p = tb + l16;
+    const bool ok = p < cnt;
+    const real4 x = ok ? a.pos[sta + p] : c;
+    float sx[4], sy[4], sz[4];
+    cheb_row<4>(a.Tkz, ih * (x.x - c.x), sx);
+    cheb_row<4>(a.Tkz, ih * (x.y - c.y), sy);
+    cheb_row<4>(a.Tkz, ih * (x.z - c.z), sz);
+    const float zq = pick4(sz, kq);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      const float w = sx[kk >> 2] * sy[kk & 3] * zq;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(w, sf[(4 * kk + kq) * LP + l16], acc, 0, 0, 0);
+    }
+    // acc[v] = phi[particle tb + 4 kq + v][vector l16]
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int i = tb + 4 * kq + e;
+      if (i < cnt) {
+        const size_t o = (size_t)(psta + i) * 16 + l16;
+        const float ph = a.prefac3 * acc[e];
+        a.phit[o] = ph;
+        if (l16 == 0) a.farfield[sta + i] = (a.potential ? 1.0f : a.qt[o]) * ph;
+      }
+    }
+  }
+}
+#endif
+
+// ---------------------------------------------------------------- 16 vectors per block, n = 4
+// The per-cell passes below give each of the 16 vectors a wave of its own
+// inside one block per cell: the operator tables (U, Cw) are staged in LDS
+// once per block and every wave works in its private LDS region, so a cell
+// costs one round trip to memory for all vectors instead of one per vector.
+
+// LDS writes of a wave visible to its other lanes
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// M2M (as k_m2m) for one level, 16 vectors: block = parent, wave = vector.
+__global__ void __launch_bounds__(1024) k_m2m_v16(PassArgs a, int off)
+{
+  constexpr int NN = 4, N3 = 64, N2 = 16, R = 32;
+  __shared__ real sU[N3 * R];  // [node][coefficient]
+  __shared__ real cw[2 * N2];
+  __shared__ real wsc[16][8 * N3];
+  __shared__ real wzz[16][4 * N3];
+  __shared__ real wyy[16][2 * N3];
+  __shared__ real wsw[16][N3];
+  const int A = a.plist[off + blockIdx.x];
+  const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+  const real *S = a.S + (size_t)v * a.ncell * N3;
+  real *sc = wsc[v], *szz = wzz[v], *syy = wyy[v], *sw = wsw[v];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int ch = a.child[(size_t)8 * A + j];
+    sc[j * N3 + lane] = ch >= 0 ? S[(size_t)ch * N3 + lane] : (real)0;
+  }
+  for (int o = threadIdx.x; o < N3 * R; o += 1024) sU[o] = a.Urm[o];
+  if (threadIdx.x < 2 * N2) cw[threadIdx.x] = a.Cw[threadIdx.x];
+  __syncthreads();
+#pragma unroll
+  for (int o = lane; o < 4 * N3; o += 64) {  // z
+    const int pair = o / N3, rem = o % N3, l1 = rem / NN, pz = rem % NN;
+    real t = 0;
+#pragma unroll
+    for (int zb = 0; zb < 2; zb++)
+#pragma unroll
+      for (int k = 0; k < NN; k++) t += sc[(2 * pair + zb) * N3 + l1 * NN + k] * cw[zb * N2 + pz * NN + k];
+    szz[o] = t;
+  }
+  wave_sync();
+#pragma unroll
+  for (int o = lane; o < 2 * N3; o += 64) {  // y
+    const int xb = o / N3, rem = o % N3, cx = rem / N2, py = (rem / NN) % NN, pz = rem % NN;
+    real t = 0;
+#pragma unroll
+    for (int yb = 0; yb < 2; yb++)
+#pragma unroll
+      for (int k = 0; k < NN; k++) t += szz[(2 * xb + yb) * N3 + cx * N2 + k * NN + pz] * cw[yb * N2 + py * NN + k];
+    syy[o] = t;
+  }
+  wave_sync();
+  {  // x
+    const int px = lane / N2, rem = lane % N2;
+    real t = 0;
+#pragma unroll
+    for (int xb = 0; xb < 2; xb++)
+#pragma unroll
+      for (int k = 0; k < NN; k++) t += syy[xb * N3 + k * N2 + rem] * cw[xb * N2 + px * NN + k];
+    t *= a.prefac3;
+    a.S[((size_t)v * a.ncell + A) * N3 + lane] = t;
+    sw[lane] = t * a.Kw[lane];
+  }
+  wave_sync();
+  // M = U^T sw: lanes i and i + 32 take half of the nodes each
+  const int i = lane & 31, h = lane >> 5;
+  real m = 0;
+#pragma unroll 8
+  for (int l = 32 * h; l < 32 * h + 32; l++) m += sU[l * R + i] * sw[l];
+  m += __shfl_down(m, 32);
+  if (h == 0) a.M[((size_t)v * a.ncell + A) * R + i] = m;
+}
+
+// post-M2L (as k_postm2l), 16 vectors: four cells per block, a wave per
+// cell, a lane per node with 16 accumulators; U staged once per block.
+__global__ void __launch_bounds__(256) k_postm2l_v16(PassArgs a)
+{
+  constexpr int N3 = 64, R = 32;
+  __shared__ real sU[N3 * R];   // column-major as Ucm: [coefficient][node]
+  __shared__ __align__(32) real wl[4][R * 16];  // [coefficient][vector]
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int A = blockIdx.x * 4 + w;
+  for (int o = threadIdx.x; o < N3 * R; o += 256) sU[o] = a.Ucm[o];
+  const bool act = A < a.ncell && a.celllev[A] >= kMinLev;
+  real *sl = wl[w];
+  if (act) {
+#pragma unroll
+    for (int o = lane; o < R * 16; o += 64) {
+      const int v = o / R, i = o % R;
+      sl[i * 16 + v] = a.L[((size_t)v * a.ncell + A) * R + i];
+    }
+  }
+  __syncthreads();
+  if (!act) return;
+  real acc[16];
+#pragma unroll
+  for (int v = 0; v < 16; v++) acc[v] = 0;
+#pragma unroll 4
+  for (int i = 0; i < R; i++) {
+    const real u = sU[i * N3 + lane];
+    const real4 *l4 = reinterpret_cast<const real4 *>(sl + i * 16);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const real4 lv = l4[q];
+      acc[4 * q] += u * lv.x;
+      acc[4 * q + 1] += u * lv.y;
+      acc[4 * q + 2] += u * lv.z;
+      acc[4 * q + 3] += u * lv.w;
+    }
+  }
+  const real f = a.scale[a.celllev[A]] * a.Kw[lane];
+#pragma unroll
+  for (int v = 0; v < 16; v++) a.F[((size_t)v * a.ncell + A) * N3 + lane] = f * acc[v];
+}
+
+// L2L (as k_l2l) for one level, 16 vectors: block = parent, wave = vector.
+__global__ void __launch_bounds__(1024) k_l2l_v16(PassArgs a, int off)
+{
+  constexpr int NN = 4, N3 = 64, N2 = 16;
+  __shared__ real cw[2 * N2];
+  __shared__ real wfa[16][N3];
+  __shared__ real wfx[16][2 * N3];
+  __shared__ real wfy[16][4 * N3];
+  const int A = a.plist[off + blockIdx.x];
+  const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+  real *F = a.F + (size_t)v * a.ncell * N3;
+  real *fa = wfa[v], *fx = wfx[v], *fy = wfy[v];
+  fa[lane] = F[(size_t)A * N3 + lane];
+  if (threadIdx.x < 2 * N2) cw[threadIdx.x] = a.Cw[threadIdx.x];
+  __syncthreads();
+#pragma unroll
+  for (int o = lane; o < 2 * N3; o += 64) {  // x
+    const int xb = o / N3, rem = o % N3, cx = rem / N2, yz = rem % N2;
+    real t = 0;
+#pragma unroll
+    for (int k = 0; k < NN; k++) t += fa[k * N2 + yz] * cw[xb * N2 + k * NN + cx];
+    fx[o] = t;
+  }
+  wave_sync();
+#pragma unroll
+  for (int o = lane; o < 4 * N3; o += 64) {  // y
+    const int jj = o / N3, rem = o % N3, cx = rem / N2, cy = (rem / NN) % NN, pz = rem % NN;
+    real t = 0;
+#pragma unroll
+    for (int k = 0; k < NN; k++) t += fx[(jj >> 1) * N3 + cx * N2 + k * NN + pz] * cw[(jj & 1) * N2 + k * NN + cy];
+    fy[o] = t;
+  }
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < 8; j++) {  // z, added into the children
+    const int l1 = lane / NN, cz = lane % NN;
+    real t = 0;
+#pragma unroll
+    for (int k = 0; k < NN; k++) t += fy[(j >> 1) * N3 + l1 * NN + k] * cw[(j & 1) * N2 + k * NN + cz];
+    const int ch = a.child[(size_t)8 * A + j];
+    if (ch >= 0) F[(size_t)ch * N3 + lane] += a.prefac3 * t;
+  }
+}
+
 // M2M for one level (anotherUpwardPass.c:345-560): one block per parent,
 // separable interpolation of the eight children's node values (z, y, x).
 template <int NN, int BLOCK>
@@ -489,6 +801,12 @@ template <int NN, int BLOCK>
 static void p2m_t(const PassArgs &a, hipStream_t s, int nrhs)
 {
   const int nr = vec_group(nrhs);
+#ifdef SINGLE
+  if (a.rows && NN == 4) {
+    hipLaunchKernelGGL(k_p2m_rows_mfma4, dim3((a.nleaf + 3) / 4), dim3(256), 0, s, a);
+    return;
+  }
+#endif
   if (a.rows)
     hipLaunchKernelGGL((k_p2m<NN, BLOCK, 16, true>), dim3(a.nleaf, 1), dim3(BLOCK), 0, s, a);
   else if (nr == 16)
@@ -519,7 +837,9 @@ int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs)
   for (int level = a.maxlev - 1; level >= kMinLev; level--) {
     const int off = h_ploff[level], count = h_ploff[level + 1] - off;
     if (count <= 0) continue;
-    if (a.n == 4)
+    if (a.n == 4 && nrhs == 16)
+      hipLaunchKernelGGL(k_m2m_v16, dim3(count), dim3(1024), 0, s, a, off);
+    else if (a.n == 4)
       hipLaunchKernelGGL((k_m2m<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, off);
     else
       hipLaunchKernelGGL((k_m2m<8, 256>), dim3(count, nrhs), dim3(256), 0, s, a, off);
@@ -548,6 +868,12 @@ static int postm2l_t(const PassArgs &a, hipStream_t s, int nrhs)
 
 int launch_postm2l(const PassArgs &a, hipStream_t s, int nrhs)
 {
+  if (a.n == 4 && nrhs == 16 && a.r == 32) {
+    if (a.ncell <= 0) return BBFMM_OK;
+    hipLaunchKernelGGL(k_postm2l_v16, dim3((a.ncell + 3) / 4), dim3(256), 0, s, a);
+    BB_HIP(hipGetLastError());
+    return BBFMM_OK;
+  }
   if (a.n == 4) return postm2l_t<4, 64>(a, s, nrhs);
   if (a.n == 8) return postm2l_t<8, 256>(a, s, nrhs);
   return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
@@ -561,7 +887,9 @@ int launch_l2l(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs)
   for (int level = kMinLev; level < a.maxlev; level++) {
     const int off = h_ploff[level], count = h_ploff[level + 1] - off;
     if (count <= 0) continue;
-    if (a.n == 4)
+    if (a.n == 4 && nrhs == 16)
+      hipLaunchKernelGGL(k_l2l_v16, dim3(count), dim3(1024), 0, s, a, off);
+    else if (a.n == 4)
       hipLaunchKernelGGL((k_l2l<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, off);
     else
       hipLaunchKernelGGL((k_l2l<8, 256>), dim3(count, nrhs), dim3(256), 0, s, a, off);
@@ -574,6 +902,12 @@ template <int NN>
 static void l2p_t(const PassArgs &a, hipStream_t s, int nrhs)
 {
   const int nr = vec_group(nrhs);
+#ifdef SINGLE
+  if (a.rows && NN == 4) {
+    hipLaunchKernelGGL(k_l2p_rows_mfma4, dim3((a.nleaf + 3) / 4), dim3(256), 0, s, a);
+    return;
+  }
+#endif
   if (a.rows)
     hipLaunchKernelGGL((k_l2p_leaf<NN, 16, true>), dim3(a.nleaf, 1), dim3(64), 0, s, a);
   else if (nr == 16)
