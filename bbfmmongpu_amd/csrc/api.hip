@@ -565,7 +565,15 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
 static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
 {
   hipStream_t s = p->stream;
-  const PassArgs a = pass_args(p, rows);
+  PassArgs a = pass_args(p, rows);
+  // the last downward level in one kernel (post-M2L of the leaves, L2L into
+  // them and L2P): the leaves' F is then not stored (the F fetch recomputes
+  // post-M2L from L)
+  const bool fuse = rows && p->n == 4 && p->r == 32 && p->complete && p->maxlev >= 3;
+  if (fuse) {
+    a.cell_end = p->h_levsta[p->maxlev];
+    a.l2l_end = p->maxlev - 1;
+  }
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2M), s));
   BB_TRY(launch_p2m(a, s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2M), s));
@@ -577,7 +585,10 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2L), s));
   BB_TRY(launch_l2l(a, p->h_ploff.data(), s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2P), s));
-  BB_TRY(launch_l2p(a, s, p->nrhs));
+  if (fuse)
+    BB_TRY(launch_down_leaf(a, p->h_ploff.data(), s));
+  else
+    BB_TRY(launch_l2p(a, s, p->nrhs));
   if (!rows)  // the particle-row L2P writes the far field itself
     BB_HIP(hipMemcpyAsync(p->d_farfield, p->d_phi, sizeof(real) * p->N, hipMemcpyDeviceToDevice, s));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2P), s));

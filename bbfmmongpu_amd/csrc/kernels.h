@@ -38,6 +38,10 @@ struct PassArgs {
   real *phit = nullptr;       // [Npad][16]
   const int *pad_of = nullptr;  // [N] leaf-order index -> padded index
   real *farfield = nullptr;   // [N] leaf order
+  // fused last downward level (launch_down_leaf): post-M2L skips cells from
+  // cell_end on and L2L stops before parents of level l2l_end
+  int cell_end = 0x7fffffff;
+  int l2l_end = 0x7fffffff;
 };
 
 int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
@@ -53,6 +57,10 @@ int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs);
 int launch_postm2l(const PassArgs &a, hipStream_t s, int nrhs);
 int launch_l2l(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs);
 int launch_l2p(const PassArgs &a, hipStream_t s, int nrhs);
+// particle rows, n = 4, complete tree, maxlev >= 3: post-M2L of the leaves,
+// the last L2L level and L2P in one kernel (set cell_end = first leaf cell
+// and l2l_end = maxlev - 1 for the other launches)
+int launch_down_leaf(const PassArgs &a, const int *h_ploff, hipStream_t s);
 
 // ---- M2L ----
 struct M2LArgs {

@@ -14,6 +14,8 @@
 #include "common.h"
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace bb {
 
 // S_n(x_m, x) for m = 0..NN-1 (bbfmm.c:1641-1679, Clenshaw recurrence).
@@ -31,6 +33,14 @@ __device__ __forceinline__ void cheb_row(const real *__restrict__ Tkz, real x, r
     }
     out[m] = x * b1 - b2 + (real)0.5 * Tkz[m * NN];
   }
+}
+
+// LDS writes of a wave visible to its other lanes
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __global__ void k_gather(int N, const int *__restrict__ perm, const real *__restrict__ src,
@@ -348,6 +358,129 @@ __global__ void __launch_bounds__(256) k_l2p_rows_mfma4(PassArgs a)
     }
   }
 }
+
+// Last level of the downward pass, fused (particle rows, n = 4, complete
+// tree): one block per parent P on level maxlev - 1 and a wave per child
+// leaf.  A leaf's F = scale Kw .* (U L) (post-M2L, on MFMA: A = U rows, B =
+// the leaf's L) plus the L2L interpolation of P's F (separable x and y passes
+// shared by the block, the z pass per wave in the accumulator layout); then
+// the L2P of the leaf's particles as in k_l2p_rows_mfma4.  The leaves' F
+// never goes to memory.
+__global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int off, int leaf0)
+{
+  constexpr int NN = 4, N3 = 64, N2 = 16, R = 32, LP = 17, UP = R + 1;
+  __shared__ float sU[N3 * UP];          // [node][coefficient], padded rows
+  __shared__ float cw[2 * N2];
+  __shared__ float fP[16][N3];           // parent F [vector][node]
+  __shared__ float fx[16][2 * N3];
+  __shared__ float fy[16][4 * N3 + 1];   // padded rows: the z pass reads across vectors
+  __shared__ float wbuf[8][N3 * LP];     // per wave: the leaf's L ([coefficient][vector]), then its F
+  const int P = a.plist[off + blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int ch = 8 * P + 1 + w;  // child w of P (complete tree)
+  for (int o = tid; o < N3 * R; o += 512) sU[(o / R) * UP + o % R] = a.Urm[o];
+  if (tid < 2 * N2) cw[tid] = a.Cw[tid];
+  for (int o = tid; o < 16 * N3; o += 512) fP[o / N3][o % N3] = a.F[((size_t)(o / N3) * a.ncell + P) * N3 + o % N3];
+  float *wb = wbuf[w];
+  for (int o = lane; o < 16 * R; o += 64) {
+    const int v = o / R, i = o % R;
+    wb[i * LP + v] = a.L[((size_t)v * a.ncell + ch) * R + i];
+  }
+  __syncthreads();
+  for (int o = tid; o < 16 * 2 * N3; o += 512) {  // x
+    const int v = o / (2 * N3), r = o % (2 * N3);
+    const int xb = r / N3, rem = r % N3, cx = rem / N2, yz = rem % N2;
+    float t = 0;
+#pragma unroll
+    for (int k = 0; k < NN; k++) t += fP[v][k * N2 + yz] * cw[xb * N2 + k * NN + cx];
+    fx[v][r] = t;
+  }
+  __syncthreads();
+  for (int o = tid; o < 16 * 4 * N3; o += 512) {  // y
+    const int v = o / (4 * N3), r = o % (4 * N3);
+    const int jj = r / N3, rem = r % N3, cx = rem / N2, cy = (rem / NN) % NN, pz = rem % NN;
+    float t = 0;
+#pragma unroll
+    for (int k = 0; k < NN; k++) t += fx[v][(jj >> 1) * N3 + cx * N2 + k * NN + pz] * cw[(jj & 1) * N2 + k * NN + cy];
+    fy[v][r] = t;
+  }
+  __syncthreads();
+  // post-M2L: acc[t][e] = (U L)[node 16t + 4kq + e][vector l16]
+  f32x4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < R / 4; kk++) {
+    const float b = wb[(4 * kk + kq) * LP + l16];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(sU[(16 * t + l16) * UP + 4 * kk + kq], b, acc[t], 0, 0, 0);
+  }
+  // + z pass of the L2L for child w: node = 16t + 4kq + e = (l1 = 4t + kq, cz = e)
+  const float sc = a.scale[a.maxlev];
+  float fv[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int node = 16 * t + 4 * kq + e;
+      float inc = 0;
+#pragma unroll
+      for (int k = 0; k < NN; k++) inc += fy[l16][(w >> 1) * N3 + (4 * t + kq) * NN + k] * cw[(w & 1) * N2 + k * NN + e];
+      fv[t][e] = sc * a.Kw[node] * acc[t][e] + a.prefac3 * inc;
+    }
+  wave_sync();  // every lane has read the leaf's L
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) wb[(16 * t + 4 * kq + e) * LP + l16] = fv[t][e];
+  wave_sync();
+  // L2P of leaf ch
+  const int leaf = ch - leaf0;
+  const int cnt = a.leafcnt[leaf];
+  if (cnt == 0) return;
+  const int sta = a.leafsta[leaf];
+  const int psta = a.pad_of[sta];
+  const real ih = a.leafihalf[leaf];
+  const real4 c = a.center[ch];
+  for (int tb = 0; tb < cnt; tb += 16) {
+    const int p = tb + l16;
+    const bool ok = p < cnt;
+    const real4 x = ok ? a.pos[sta + p] : c;
+    float sx[4], sy[4], sz[4];
+    cheb_row<4>(a.Tkz, ih * (x.x - c.x), sx);
+    cheb_row<4>(a.Tkz, ih * (x.y - c.y), sy);
+    cheb_row<4>(a.Tkz, ih * (x.z - c.z), sz);
+    const float zq = pick4(sz, kq);
+    f32x4 ph = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      const float wt = sx[kk >> 2] * sy[kk & 3] * zq;
+      ph = __builtin_amdgcn_mfma_f32_16x16x4f32(wt, wb[(4 * kk + kq) * LP + l16], ph, 0, 0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int i = tb + 4 * kq + e;
+      if (i < cnt) {
+        const size_t o = (size_t)(psta + i) * 16 + l16;
+        const float v = a.prefac3 * ph[e];
+        a.phit[o] = v;
+        if (l16 == 0) a.farfield[sta + i] = (a.potential ? 1.0f : a.qt[o]) * v;
+      }
+    }
+  }
+}
+
+int launch_down_leaf(const PassArgs &a, const int *h_ploff, hipStream_t s)
+{
+  const int lev = a.maxlev - 1;
+  const int off = h_ploff[lev], count = h_ploff[lev + 1] - off;
+  if (count <= 0) return BBFMM_OK;
+  hipLaunchKernelGGL(k_down_leaf_v16, dim3(count), dim3(512), 0, s, a, off, ((1 << (3 * a.maxlev)) - 1) / 7);
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
+}
 #endif
 
 // ---------------------------------------------------------------- 16 vectors per block, n = 4
@@ -355,14 +488,6 @@ __global__ void __launch_bounds__(256) k_l2p_rows_mfma4(PassArgs a)
 // inside one block per cell: the operator tables (U, Cw) are staged in LDS
 // once per block and every wave works in its private LDS region, so a cell
 // costs one round trip to memory for all vectors instead of one per vector.
-
-// LDS writes of a wave visible to its other lanes
-__device__ __forceinline__ void wave_sync()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // M2M (as k_m2m) for one level, 16 vectors: block = parent, wave = vector.
 __global__ void __launch_bounds__(1024) k_m2m_v16(PassArgs a, int off)
@@ -439,7 +564,7 @@ __global__ void __launch_bounds__(256) k_postm2l_v16(PassArgs a)
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int A = blockIdx.x * 4 + w;
   for (int o = threadIdx.x; o < N3 * R; o += 256) sU[o] = a.Ucm[o];
-  const bool act = A < a.ncell && a.celllev[A] >= kMinLev;
+  const bool act = A < a.ncell && A < a.cell_end && a.celllev[A] >= kMinLev;
   real *sl = wl[w];
   if (act) {
 #pragma unroll
@@ -802,7 +927,7 @@ static void p2m_t(const PassArgs &a, hipStream_t s, int nrhs)
 {
   const int nr = vec_group(nrhs);
 #ifdef SINGLE
-  if (a.rows && NN == 4) {
+  if (a.rows && NN == 4 && a.r == 32) {
     hipLaunchKernelGGL(k_p2m_rows_mfma4, dim3((a.nleaf + 3) / 4), dim3(256), 0, s, a);
     return;
   }
@@ -884,7 +1009,7 @@ int launch_l2l(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs)
   // L2L only (parents at levels minlev..maxlev-1)
   if (a.n != 4 && a.n != 8)
     return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
-  for (int level = kMinLev; level < a.maxlev; level++) {
+  for (int level = kMinLev; level < std::min(a.maxlev, a.l2l_end); level++) {
     const int off = h_ploff[level], count = h_ploff[level + 1] - off;
     if (count <= 0) continue;
     if (a.n == 4 && nrhs == 16)
