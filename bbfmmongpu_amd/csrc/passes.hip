@@ -236,7 +236,10 @@ __device__ __forceinline__ float pick4(const float *v, int k)
 __global__ void __launch_bounds__(256) k_p2m_rows_mfma4(PassArgs a)
 {
   constexpr int N3 = 64, LP = 17;  // padded LDS row of 16 vectors
-  __shared__ float sS[4][N3 * LP];
+  constexpr int CP = 13;           // LDS row of a particle's 12 Chebyshev values
+  constexpr int CH = 64;           // particles staged per chunk
+  // per wave: a chunk's Chebyshev values and charge rows, later Kw .* S
+  __shared__ float wst[4][CH * (CP + LP)];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int leaf = blockIdx.x * 4 + wave;
   const bool active = leaf < a.nleaf;
@@ -248,28 +251,51 @@ __global__ void __launch_bounds__(256) k_p2m_rows_mfma4(PassArgs a)
     sta = a.leafsta[leaf];
     psta = cnt > 0 ? a.pad_of[sta] : 0;
   }
+  float *sc = wst[wave], *sq = wst[wave] + CH * CP;
   f32x4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
   if (cnt > 0) {
     const real ih = a.leafihalf[leaf];
     const real4 c = a.center[A];
-    for (int k0 = 0; k0 < cnt; k0 += 4) {
-      const int p = k0 + kq;
-      const bool ok = p < cnt;
-      const real4 x = ok ? a.pos[sta + p] : c;
-      float sx[4], sy[4], sz[4];
-      cheb_row<4>(a.Tkz, ih * (x.x - c.x), sx);
-      cheb_row<4>(a.Tkz, ih * (x.y - c.y), sy);
-      cheb_row<4>(a.Tkz, ih * (x.z - c.z), sz);
-      const float syz = ok ? pick4(sy, l16 >> 2) * pick4(sz, l16 & 3) : 0.f;
-      const float bq = a.qt[(size_t)(psta + p) * 16 + l16];  // rows up to the padded count exist
+    for (int base = 0; base < cnt; base += CH) {
+      const int nk = min(CH, cnt - base);
+      // stage: lane j computes particle j's Chebyshev values; the chunk's
+      // charge rows are copied with coalesced loads
+      if (lane < nk) {
+        const real4 x = a.pos[sta + base + lane];
+        float t[4];
+        cheb_row<4>(a.Tkz, ih * (x.x - c.x), t);
 #pragma unroll
-      for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(sx[t] * syz, bq, acc[t], 0, 0, 0);
+        for (int m = 0; m < 4; m++) sc[lane * CP + m] = t[m];
+        cheb_row<4>(a.Tkz, ih * (x.y - c.y), t);
+#pragma unroll
+        for (int m = 0; m < 4; m++) sc[lane * CP + 4 + m] = t[m];
+        cheb_row<4>(a.Tkz, ih * (x.z - c.z), t);
+#pragma unroll
+        for (int m = 0; m < 4; m++) sc[lane * CP + 8 + m] = t[m];
+      }
+      const float *qr = a.qt + (size_t)(psta + base) * 16;
+      for (int o = lane; o < nk * 16; o += 64) sq[(o >> 4) * LP + (o & 15)] = qr[o];
+      wave_sync();
+      for (int k0 = 0; k0 < nk; k0 += 4) {
+        const int p = k0 + kq;
+        const bool ok = p < nk;
+        // rows past the chunk hold stale LDS contents: both operands masked
+        const float *cp = sc + (ok ? p : 0) * CP;
+        const float syz = ok ? cp[4 + (l16 >> 2)] * cp[8 + (l16 & 3)] : 0.f;
+        const float bq = ok ? sq[p * LP + l16] : 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const float wt = ok ? cp[t] * syz : 0.f;
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wt, bq, acc[t], 0, 0, 0);
+        }
+      }
+      wave_sync();  // the chunk's staging is read before the next one (or S) overwrites it
     }
   }
   // acc[t][v] = S[node 16t + 4 kq + v][vector l16]
-  float *sw = sS[wave];
+  float *sw = wst[wave];
   if (active) {
 #pragma unroll
     for (int t = 0; t < 4; t++) {
@@ -379,6 +405,11 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int off, int 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
   const int ch = 8 * P + 1 + w;  // child w of P (complete tree)
+  const int leaf = ch - leaf0;
+  const int cnt = a.leafcnt[leaf], sta = a.leafsta[leaf];
+  const real4 c = a.center[ch];
+  const real4 x0 = l16 < cnt ? a.pos[sta + l16] : c;
+  const real4 x1 = 16 + l16 < cnt ? a.pos[sta + 16 + l16] : c;
   for (int o = tid; o < N3 * R; o += 512) sU[(o / R) * UP + o % R] = a.Urm[o];
   if (tid < 2 * N2) cw[tid] = a.Cw[tid];
   for (int o = tid; o < 16 * N3; o += 512) fP[o / N3][o % N3] = a.F[((size_t)(o / N3) * a.ncell + P) * N3 + o % N3];
@@ -436,18 +467,14 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int off, int 
 #pragma unroll
     for (int e = 0; e < 4; e++) wb[(16 * t + 4 * kq + e) * LP + l16] = fv[t][e];
   wave_sync();
-  // L2P of leaf ch
-  const int leaf = ch - leaf0;
-  const int cnt = a.leafcnt[leaf];
+  // L2P of leaf ch (positions of its first two tiles loaded at the start)
   if (cnt == 0) return;
-  const int sta = a.leafsta[leaf];
   const int psta = a.pad_of[sta];
   const real ih = a.leafihalf[leaf];
-  const real4 c = a.center[ch];
   for (int tb = 0; tb < cnt; tb += 16) {
     const int p = tb + l16;
     const bool ok = p < cnt;
-    const real4 x = ok ? a.pos[sta + p] : c;
+    const real4 x = tb == 0 ? x0 : tb == 16 ? x1 : (ok ? a.pos[sta + p] : c);
     float sx[4], sy[4], sz[4];
     cheb_row<4>(a.Tkz, ih * (x.x - c.x), sx);
     cheb_row<4>(a.Tkz, ih * (x.y - c.y), sy);
