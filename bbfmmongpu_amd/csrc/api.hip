@@ -566,16 +566,21 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
 {
   hipStream_t s = p->stream;
   PassArgs a = pass_args(p, rows);
-  // the last downward level in one kernel (post-M2L of the leaves, L2L into
-  // them and L2P): the leaves' F is then not stored (the F fetch recomputes
+  // the first upward level (P2M, M2M into the leaves' parents) and the last
+  // downward level (post-M2L of the leaves, L2L into them, L2P) each in one
+  // kernel: the leaves' S and F are then not stored (the F fetch recomputes
   // post-M2L from L)
   const bool fuse = rows && p->n == 4 && p->r == 32 && p->complete && p->maxlev >= 3;
   if (fuse) {
     a.cell_end = p->h_levsta[p->maxlev];
     a.l2l_end = p->maxlev - 1;
+    a.m2m_start = p->maxlev - 2;
   }
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2M), s));
-  BB_TRY(launch_p2m(a, s, p->nrhs));
+  if (fuse)  // P2M and the first M2M level in one kernel
+    BB_TRY(launch_up_leaf(a, p->h_ploff.data(), s));
+  else
+    BB_TRY(launch_p2m(a, s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2M), s));
   BB_TRY(launch_m2m(a, p->h_ploff.data(), s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2L_PREP), s));

@@ -42,6 +42,8 @@ struct PassArgs {
   // cell_end on and L2L stops before parents of level l2l_end
   int cell_end = 0x7fffffff;
   int l2l_end = 0x7fffffff;
+  // fused first upward level (launch_up_leaf): M2M starts at parents of this level
+  int m2m_start = 0x7fffffff;
 };
 
 int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
@@ -61,6 +63,9 @@ int launch_l2p(const PassArgs &a, hipStream_t s, int nrhs);
 // the last L2L level and L2P in one kernel (set cell_end = first leaf cell
 // and l2l_end = maxlev - 1 for the other launches)
 int launch_down_leaf(const PassArgs &a, const int *h_ploff, hipStream_t s);
+// particle rows, n = 4, complete tree, maxlev >= 3: P2M and the M2M into the
+// leaves' parents in one kernel (set m2m_start = maxlev - 2 for the M2M)
+int launch_up_leaf(const PassArgs &a, const int *h_ploff, hipStream_t s);
 
 // ---- M2L ----
 struct M2LArgs {

@@ -499,6 +499,148 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int off, int 
   }
 }
 
+// First upward level fused (particle rows, n = 4, complete tree): one block
+// per parent P on level maxlev - 1 and a wave per child leaf.  Each wave runs
+// the leaf's P2M as k_p2m_rows_mfma4 (32 particles staged at a time) and
+// writes the leaf's M; the leaves' S stay in LDS, where the block forms P's S
+// by the M2M interpolation (z, y, x passes over the 16 vectors) and P's M.
+// The leaves' S never go to memory.
+__global__ void __launch_bounds__(512) k_up_leaf_v16(PassArgs a, int off, int leaf0)
+{
+  constexpr int NN = 4, N3 = 64, N2 = 16, R = 32, LP = 17, CP = 13, CH = 32;
+  constexpr int WB = N3 * LP;  // per wave: staging (CH * (CP + LP) floats), then the leaf's S
+  static_assert(CH * (CP + LP) <= WB, "staging fits the wave's buffer");
+  __shared__ float wbuf[8][WB];
+  __shared__ float szz[16][4 * N3];
+  __shared__ float cw[2 * N2];
+  const int P = a.plist[off + blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int ch = 8 * P + 1 + w;  // child w of P (complete tree)
+  const int leaf = ch - leaf0;
+  const int cnt = a.leafcnt[leaf], sta = a.leafsta[leaf];
+  if (tid < 2 * N2) cw[tid] = a.Cw[tid];
+  float *wb = wbuf[w];
+  float *sc = wb, *sq = wb + CH * CP;
+  f32x4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  if (cnt > 0) {
+    const int psta = a.pad_of[sta];
+    const real ih = a.leafihalf[leaf];
+    const real4 c = a.center[ch];
+    for (int base = 0; base < cnt; base += CH) {
+      const int nk = min(CH, cnt - base);
+      if (lane < nk) {
+        const real4 x = a.pos[sta + base + lane];
+        float t[4];
+        cheb_row<4>(a.Tkz, ih * (x.x - c.x), t);
+#pragma unroll
+        for (int m = 0; m < 4; m++) sc[lane * CP + m] = t[m];
+        cheb_row<4>(a.Tkz, ih * (x.y - c.y), t);
+#pragma unroll
+        for (int m = 0; m < 4; m++) sc[lane * CP + 4 + m] = t[m];
+        cheb_row<4>(a.Tkz, ih * (x.z - c.z), t);
+#pragma unroll
+        for (int m = 0; m < 4; m++) sc[lane * CP + 8 + m] = t[m];
+      }
+      const float *qr = a.qt + (size_t)(psta + base) * 16;
+      for (int o = lane; o < nk * 16; o += 64) sq[(o >> 4) * LP + (o & 15)] = qr[o];
+      wave_sync();
+      for (int k0 = 0; k0 < nk; k0 += 4) {
+        const int p = k0 + kq;
+        const bool ok = p < nk;
+        // rows past the chunk hold stale LDS contents: both operands masked
+        const float *cp = sc + (ok ? p : 0) * CP;
+        const float syz = ok ? cp[4 + (l16 >> 2)] * cp[8 + (l16 & 3)] : 0.f;
+        const float bq = ok ? sq[p * LP + l16] : 0.f;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const float wt = ok ? cp[t] * syz : 0.f;
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wt, bq, acc[t], 0, 0, 0);
+        }
+      }
+      wave_sync();
+    }
+  }
+  // the leaf's S (64 nodes x 16 vectors) into the wave's buffer
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) wb[(16 * t + 4 * kq + e) * LP + l16] = a.prefac3 * acc[t][e];
+  wave_sync();
+  {  // the leaf's M = U^T (Kw .* S)
+    f32x4 m[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll 4
+    for (int kk = 0; kk < 16; kk++) {
+      const int node = 4 * kk + kq;
+      const float b = wb[node * LP + l16] * a.Kw[node];
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+        m[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.Urm[(size_t)node * R + 16 * t + l16], b, m[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+      *reinterpret_cast<f32x4 *>(a.M + ((size_t)l16 * a.ncell + ch) * R + 16 * t + 4 * kq) = m[t];
+  }
+  __syncthreads();
+  // M2M of P (as k_m2m): z pass over the 8 leaves' S
+  for (int o = tid; o < 16 * 4 * N3; o += 512) {
+    const int v = o / (4 * N3), r = o % (4 * N3);
+    const int pair = r / N3, rem = r % N3, l1 = rem / NN, pz = rem % NN;
+    float t = 0;
+#pragma unroll
+    for (int zb = 0; zb < 2; zb++)
+#pragma unroll
+      for (int k = 0; k < NN; k++) t += wbuf[2 * pair + zb][(l1 * NN + k) * LP + v] * cw[zb * N2 + pz * NN + k];
+    szz[v][r] = t;
+  }
+  __syncthreads();
+  float *syy = &wbuf[0][0];       // [16][2 * N3], the leaves' S are consumed
+  float *sw = &wbuf[2][0];        // [16][N3]
+  for (int o = tid; o < 16 * 2 * N3; o += 512) {  // y
+    const int v = o / (2 * N3), r = o % (2 * N3);
+    const int xb = r / N3, rem = r % N3, cx = rem / N2, py = (rem / NN) % NN, pz = rem % NN;
+    float t = 0;
+#pragma unroll
+    for (int yb = 0; yb < 2; yb++)
+#pragma unroll
+      for (int k = 0; k < NN; k++) t += szz[v][(2 * xb + yb) * N3 + cx * N2 + k * NN + pz] * cw[yb * N2 + py * NN + k];
+    syy[v * 2 * N3 + r] = t;
+  }
+  __syncthreads();
+  for (int o = tid; o < 16 * N3; o += 512) {  // x
+    const int v = o / N3, l = o % N3;
+    const int px = l / N2, rem = l % N2;
+    float t = 0;
+#pragma unroll
+    for (int xb = 0; xb < 2; xb++)
+#pragma unroll
+      for (int k = 0; k < NN; k++) t += syy[v * 2 * N3 + xb * N3 + k * N2 + rem] * cw[xb * N2 + px * NN + k];
+    t *= a.prefac3;
+    a.S[((size_t)v * a.ncell + P) * N3 + l] = t;
+    sw[v * N3 + l] = t * a.Kw[l];
+  }
+  __syncthreads();
+  {  // P's M: a thread per (vector, coefficient)
+    const int v = tid / R, i = tid % R;
+    float m = 0;
+#pragma unroll 8
+    for (int l = 0; l < N3; l++) m += a.Urm[(size_t)l * R + i] * sw[v * N3 + l];
+    a.M[((size_t)v * a.ncell + P) * R + i] = m;
+  }
+}
+
+int launch_up_leaf(const PassArgs &a, const int *h_ploff, hipStream_t s)
+{
+  const int lev = a.maxlev - 1;
+  const int off = h_ploff[lev], count = h_ploff[lev + 1] - off;
+  if (count <= 0) return BBFMM_OK;
+  hipLaunchKernelGGL(k_up_leaf_v16, dim3(count), dim3(512), 0, s, a, off, ((1 << (3 * a.maxlev)) - 1) / 7);
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
+}
+
 int launch_down_leaf(const PassArgs &a, const int *h_ploff, hipStream_t s)
 {
   const int lev = a.maxlev - 1;
@@ -986,7 +1128,7 @@ int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs)
 {
   if (a.n != 4 && a.n != 8)
     return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
-  for (int level = a.maxlev - 1; level >= kMinLev; level--) {
+  for (int level = std::min(a.maxlev - 1, a.m2m_start); level >= kMinLev; level--) {
     const int off = h_ploff[level], count = h_ploff[level + 1] - off;
     if (count <= 0) continue;
     if (a.n == 4 && nrhs == 16)
