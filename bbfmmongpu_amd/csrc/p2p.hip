@@ -164,9 +164,8 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
     x[t] = P[psta + min(tb + 16 * t + ti, cnt - 1)];
     acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
-  for (int k = 0; k < pself; k += 4) {
-    const float4 z = P[psta + k + kq];
-    const float bq = Qt[(size_t)(psta + k) * 16 + qoff];
+  // the self leaf (r != 0 exclusion), loaded one group ahead
+  auto self_step = [&](const float4 &z, float bq) {
     float g[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) {
@@ -176,6 +175,21 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
     }
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(g[t], bq, acc[t], 0, 0, 0);
+  };
+  if (pself > 0) {
+    float4 zA = P[psta + kq];
+    float qA = Qt[(size_t)psta * 16 + qoff];
+    for (int k = 0; k < pself; k += 8) {
+      const int k1 = min(k + 4, pself - 4);
+      const float4 zB = P[psta + k1 + kq];
+      const float qB = Qt[(size_t)(psta + k1) * 16 + qoff];
+      self_step(zA, qA);
+      if (k + 4 >= pself) break;
+      const int k2 = min(k + 8, pself - 4);
+      zA = P[psta + k2 + kq];
+      qA = Qt[(size_t)(psta + k2) * 16 + qoff];
+      self_step(zB, qB);
+    }
   }
   if (T > 0) {
     int b = 1;
@@ -191,19 +205,30 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
         end = pos + __builtin_amdgcn_readlane(sg.y, b);
       }
     };
-    float4 yA = (P + pos)[kq];
-    float qA = (Qt + (size_t)pos * 16)[qoff];
-    advance();
-    for (int t = 0; t < T; t += 2) {
-      const float4 yB = (P + pos)[kq];
-      const float qB = (Qt + (size_t)pos * 16)[qoff];
+    // four groups in flight: a ring of four register sets, loaded three
+    // groups ahead (the loop is unrolled so that no set is copied)
+    float4 y0, y1, y2, y3;
+    float q0, q1, q2, q3;
+    auto load = [&](float4 &y, float &q) {
+      y = (P + pos)[kq];
+      q = (Qt + (size_t)pos * 16)[qoff];
       advance();
-      p2p16_step<NT>(yA, qA, x, acc);
+    };
+    load(y0, q0);
+    load(y1, q1);
+    load(y2, q2);
+    for (int t = 0; t < T; t += 4) {
+      load(y3, q3);
+      p2p16_step<NT>(y0, q0, x, acc);
       if (t + 1 >= T) break;
-      yA = (P + pos)[kq];
-      qA = (Qt + (size_t)pos * 16)[qoff];
-      advance();
-      p2p16_step<NT>(yB, qB, x, acc);
+      load(y0, q0);
+      p2p16_step<NT>(y1, q1, x, acc);
+      if (t + 2 >= T) break;
+      load(y1, q1);
+      p2p16_step<NT>(y2, q2, x, acc);
+      if (t + 3 >= T) break;
+      load(y2, q2);
+      p2p16_step<NT>(y3, q3, x, acc);
     }
   }
   if (a.phit) {
@@ -232,8 +257,8 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
 // G[target l%16 of each 16-target tile][source 4j + l/16] and feeds the
 // charges of that source for vector l%16 as the B operand.  The self leaf is
 // summed first with the r != 0 exclusion; the other neighbours form one
-// flattened sequence of 4-source groups, loaded one group ahead into two
-// alternating register sets.  Up to four target tiles share a pass.
+// flattened sequence of 4-source groups, loaded three groups ahead into a
+// ring of four register sets.  Up to four target tiles share a pass.
 __global__ void __launch_bounds__(64) k_p2p_seg16(P2PArgs a, int rhs0)
 {
   const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
