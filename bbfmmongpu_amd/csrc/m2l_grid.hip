@@ -218,6 +218,18 @@ __global__ void k_grid_scatter(GridArgs g, const real *__restrict__ M)
   }
 }
 
+// Blocks are dealt to the 8 XCDs round-robin (block b on XCD b % 8): the
+// logical block of launch block b gives each XCD a contiguous run of logical
+// blocks -- the 8 classes of a column group and neighbouring column groups,
+// which read the same shifted source tiles -- so they share its L2.  The
+// launch is rounded up to a multiple of 8; blocks past the end get -1.
+__device__ __forceinline__ int xcd_block(const GridArgs &g, int b)
+{
+  const int nb = g.blk0[g.maxlev + 1];
+  const int lb = (b & 7) * ((nb + 7) >> 3) + (b >> 3);
+  return lb < nb ? lb : -1;
+}
+
 // level / class / column-group of a block of the flattened launch
 __device__ __forceinline__ void decode_block(const GridArgs &g, int b, int &l, int &s, int &grp)
 {
@@ -237,8 +249,10 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(GridArgs g)
 {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  const int lb = xcd_block(g, blockIdx.x);
+  if (lb < 0) return;
   int l, s, grp;
-  decode_block(g, blockIdx.x, l, s, grp);
+  decode_block(g, lb, l, s, grp);
   const int P = 1 << (l - 1);
   const int PP = P + 2;
   const int ncols = P * P * P * g.nrhs;
@@ -675,8 +689,10 @@ template <int R>
 __global__ void __launch_bounds__(256) k_m2l_grid_plain(GridArgs g)
 {
   constexpr int CPB = 256 / R;
+  const int lb = xcd_block(g, blockIdx.x);
+  if (lb < 0) return;
   int l, s, grp;
-  decode_block(g, blockIdx.x, l, s, grp);
+  decode_block(g, lb, l, s, grp);
   const int P = 1 << (l - 1);
   const int PP = P + 2;
   const int ncols = P * P * P * g.nrhs;
@@ -1034,9 +1050,9 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
     if (a.r == 32)
-      hipLaunchKernelGGL(k_m2l_grid_plain<32>, dim3(g->nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_m2l_grid_plain<32>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else
-      hipLaunchKernelGGL(k_m2l_grid_plain<256>, dim3(g->nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL(k_m2l_grid_plain<256>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     BB_HIP(hipGetLastError());
     return BBFMM_OK;
 #ifdef SINGLE
@@ -1051,9 +1067,9 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     else if (a.ss == 2)
       hipLaunchKernelGGL((k_m2l_mfma_ss<2>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else if (a.wave_cols == 64)
-      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else
-      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 1>), dim3(g->nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 1>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     BB_HIP(hipGetLastError());
     return BBFMM_OK;
   case BBFMM_M2L_FP16X3:
@@ -1064,9 +1080,9 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
     if (a.wave_cols == 64)
-      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP16X3, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP16X3, 2>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else
-      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP16X3, 1>), dim3(g->nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP16X3, 1>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     BB_HIP(hipGetLastError());
     return BBFMM_OK;
   case BBFMM_M2L_FP32MFMA:
@@ -1076,9 +1092,9 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     if (a.r == 256)  // rounded up to whole rounds of the 8 XCDs; the extra blocks exit
       hipLaunchKernelGGL(k_m2l_fp32_r256, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else if (a.wave_cols == 64)
-      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP32MFMA, 2>), dim3(g->nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP32MFMA, 2>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else
-      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP32MFMA, 1>), dim3(g->nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_FP32MFMA, 1>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     BB_HIP(hipGetLastError());
     return BBFMM_OK;
 #endif

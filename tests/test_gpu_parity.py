@@ -104,17 +104,21 @@ def test_grid_m2l_against_oracle(prec, scheme, form, tol, nrhs, ops4, cuda):
         assert err < tol
 
 
-@pytest.mark.parametrize("N,lmod,nrhs", [(65536, 1, 16), (70000, 1, 16)])
-def test_bf16x2_many_vectors_against_oracle(N, lmod, nrhs, ops4, cuda):
+@pytest.mark.parametrize("N,lmod,nrhs,maxlev", [(65536, 1, 16, 5), (70000, 1, 16, 5), (65536, -1, 16, 3),
+                                                (20000, 0, 16, 3)])
+def test_bf16x2_many_vectors_against_oracle(N, lmod, nrhs, maxlev, ops4, cuda):
     """Split-BF16 M2L with many vectors (the source-stationary MFMA kernel at the
-    deepest levels) and the MFMA P2P, full solve against the oracle."""
+    deepest levels) and the MFMA P2P, full solve against the oracle; the
+    particle-row form with its fused first and last levels, from 2 to about
+    130 particles per leaf (several staging chunks, target tiles and P2P
+    passes per leaf)."""
     import torch
     import bbfmmongpu_amd as B
     U, K, _ = ops4
     pos, Q = O.uniform_inputs(N, seed=21, nrhs=nrhs)
     Q = Q.astype(np.float32)
     plan = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", lmod=lmod).set(scheme=3, form=1)
-    assert plan.maxlev == 5
+    assert plan.maxlev == maxlev
     phi = plan.solve(torch.from_numpy(Q).cuda()).cpu().numpy()
     tree = O.Tree(pos, lmod, "d")
     Kd, Ud = K.astype(np.float32).astype(np.float64), U.astype(np.float32).astype(np.float64)
