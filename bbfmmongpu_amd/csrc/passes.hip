@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(256) k_l2p_rows_mfma4(PassArgs a)
 // shared by the block, the z pass per wave in the accumulator layout); then
 // the L2P of the leaf's particles as in k_l2p_rows_mfma4.  The leaves' F
 // never goes to memory.
-__global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int off, int leaf0)
+__global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int leaf0)
 {
   constexpr int NN = 4, N3 = 64, N2 = 16, R = 32, LP = 17, UP = R + 1;
   __shared__ float sU[N3 * UP];          // [node][coefficient], padded rows
@@ -401,7 +401,7 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int off, int 
   __shared__ float fx[16][2 * N3];
   __shared__ float fy[16][4 * N3 + 1];   // padded rows: the z pass reads across vectors
   __shared__ float wbuf[8][N3 * LP];     // per wave: the leaf's L ([coefficient][vector]), then its F
-  const int P = a.plist[off + blockIdx.x];
+  const int P = p0 + blockIdx.x;  // complete tree: the parents of a level are consecutive
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
   const int ch = 8 * P + 1 + w;  // child w of P (complete tree)
@@ -505,7 +505,7 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int off, int 
 // writes the leaf's M; the leaves' S stay in LDS, where the block forms P's S
 // by the M2M interpolation (z, y, x passes over the 16 vectors) and P's M.
 // The leaves' S never go to memory.
-__global__ void __launch_bounds__(512) k_up_leaf_v16(PassArgs a, int off, int leaf0)
+__global__ void __launch_bounds__(512) k_up_leaf_v16(PassArgs a, int p0, int leaf0)
 {
   constexpr int NN = 4, N3 = 64, N2 = 16, R = 32, LP = 17, CP = 13, CH = 32;
   constexpr int WB = N3 * LP;  // per wave: staging (CH * (CP + LP) floats), then the leaf's S
@@ -513,7 +513,7 @@ __global__ void __launch_bounds__(512) k_up_leaf_v16(PassArgs a, int off, int le
   __shared__ float wbuf[8][WB];
   __shared__ float szz[16][4 * N3];
   __shared__ float cw[2 * N2];
-  const int P = a.plist[off + blockIdx.x];
+  const int P = p0 + blockIdx.x;  // complete tree: the parents of a level are consecutive
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
   const int ch = 8 * P + 1 + w;  // child w of P (complete tree)
@@ -636,7 +636,8 @@ int launch_up_leaf(const PassArgs &a, const int *h_ploff, hipStream_t s)
   const int lev = a.maxlev - 1;
   const int off = h_ploff[lev], count = h_ploff[lev + 1] - off;
   if (count <= 0) return BBFMM_OK;
-  hipLaunchKernelGGL(k_up_leaf_v16, dim3(count), dim3(512), 0, s, a, off, ((1 << (3 * a.maxlev)) - 1) / 7);
+  hipLaunchKernelGGL(k_up_leaf_v16, dim3(count), dim3(512), 0, s, a, ((1 << (3 * lev)) - 1) / 7,
+                     ((1 << (3 * a.maxlev)) - 1) / 7);
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }
@@ -646,7 +647,8 @@ int launch_down_leaf(const PassArgs &a, const int *h_ploff, hipStream_t s)
   const int lev = a.maxlev - 1;
   const int off = h_ploff[lev], count = h_ploff[lev + 1] - off;
   if (count <= 0) return BBFMM_OK;
-  hipLaunchKernelGGL(k_down_leaf_v16, dim3(count), dim3(512), 0, s, a, off, ((1 << (3 * a.maxlev)) - 1) / 7);
+  hipLaunchKernelGGL(k_down_leaf_v16, dim3(count), dim3(512), 0, s, a, ((1 << (3 * lev)) - 1) / 7,
+                     ((1 << (3 * a.maxlev)) - 1) / 7);
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }
