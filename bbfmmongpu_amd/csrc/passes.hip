@@ -394,13 +394,16 @@ __global__ void __launch_bounds__(256) k_l2p_rows_mfma4(PassArgs a)
 // never goes to memory.
 __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int leaf0)
 {
-  constexpr int NN = 4, N3 = 64, N2 = 16, R = 32, LP = 17, UP = R + 1;
-  __shared__ float sU[N3 * UP];          // [node][coefficient], padded rows
-  __shared__ float cw[2 * N2];
-  __shared__ float fP[16][N3];           // parent F [vector][node]
-  __shared__ float fx[16][2 * N3];
+  constexpr int NN = 4, N3 = 64, N2 = 16, R = 32, LP = 17;
+  // per wave: the leaf's L in rows 0..31 ([coefficient][vector]), later its F
+  // in all 64 rows; until then rows 32..63 of the waves hold P's F (waves 0,
+  // 1) and the x pass (waves 2..5).  3 blocks per CU.
+  __shared__ float wbuf[8][N3 * LP];
   __shared__ float fy[16][4 * N3 + 1];   // padded rows: the z pass reads across vectors
-  __shared__ float wbuf[8][N3 * LP];     // per wave: the leaf's L ([coefficient][vector]), then its F
+  __shared__ float cw[2 * N2];
+  constexpr int UPPER = 32 * LP;
+  auto fP = [&](int v) { return &wbuf[v >> 3][UPPER + (v & 7) * N3]; };
+  auto fx = [&](int v) { return &wbuf[2 + (v >> 2)][UPPER + (v & 3) * 2 * N3]; };
   const int P = p0 + blockIdx.x;  // complete tree: the parents of a level are consecutive
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
@@ -410,9 +413,8 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int l
   const real4 c = a.center[ch];
   const real4 x0 = l16 < cnt ? a.pos[sta + l16] : c;
   const real4 x1 = 16 + l16 < cnt ? a.pos[sta + 16 + l16] : c;
-  for (int o = tid; o < N3 * R; o += 512) sU[(o / R) * UP + o % R] = a.Urm[o];
   if (tid < 2 * N2) cw[tid] = a.Cw[tid];
-  for (int o = tid; o < 16 * N3; o += 512) fP[o / N3][o % N3] = a.F[((size_t)(o / N3) * a.ncell + P) * N3 + o % N3];
+  for (int o = tid; o < 16 * N3; o += 512) fP(o / N3)[o % N3] = a.F[((size_t)(o / N3) * a.ncell + P) * N3 + o % N3];
   float *wb = wbuf[w];
   for (int o = lane; o < 16 * R; o += 64) {
     const int v = o / R, i = o % R;
@@ -422,18 +424,20 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int l
   for (int o = tid; o < 16 * 2 * N3; o += 512) {  // x
     const int v = o / (2 * N3), r = o % (2 * N3);
     const int xb = r / N3, rem = r % N3, cx = rem / N2, yz = rem % N2;
+    const float *f = fP(v);
     float t = 0;
 #pragma unroll
-    for (int k = 0; k < NN; k++) t += fP[v][k * N2 + yz] * cw[xb * N2 + k * NN + cx];
-    fx[v][r] = t;
+    for (int k = 0; k < NN; k++) t += f[k * N2 + yz] * cw[xb * N2 + k * NN + cx];
+    fx(v)[r] = t;
   }
   __syncthreads();
   for (int o = tid; o < 16 * 4 * N3; o += 512) {  // y
     const int v = o / (4 * N3), r = o % (4 * N3);
     const int jj = r / N3, rem = r % N3, cx = rem / N2, cy = (rem / NN) % NN, pz = rem % NN;
+    const float *f = fx(v);
     float t = 0;
 #pragma unroll
-    for (int k = 0; k < NN; k++) t += fx[v][(jj >> 1) * N3 + cx * N2 + k * NN + pz] * cw[(jj & 1) * N2 + k * NN + cy];
+    for (int k = 0; k < NN; k++) t += f[(jj >> 1) * N3 + cx * N2 + k * NN + pz] * cw[(jj & 1) * N2 + k * NN + cy];
     fy[v][r] = t;
   }
   __syncthreads();
@@ -446,7 +450,8 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int l
     const float b = wb[(4 * kk + kq) * LP + l16];
 #pragma unroll
     for (int t = 0; t < 4; t++)
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(sU[(16 * t + l16) * UP + 4 * kk + kq], b, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.Ucm[(16 * t + l16) + (size_t)N3 * (4 * kk + kq)], b, acc[t], 0, 0,
+                                                    0);
   }
   // + z pass of the L2L for child w: node = 16t + 4kq + e = (l1 = 4t + kq, cz = e)
   const float sc = a.scale[a.maxlev];
