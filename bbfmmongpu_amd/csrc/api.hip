@@ -57,6 +57,11 @@ static bool g_verbose = getenv("BBFMM_AMD_VERBOSE") != nullptr;
 
 // ---------------------------------------------------------------- plan
 enum { ST_GATHER, ST_P2M, ST_M2M, ST_M2L_PREP, ST_M2L, ST_POST, ST_L2L, ST_L2P, ST_P2P, ST_SCATTER, ST_TOTAL };
+// per timed solve: the ST_TOTAL + 1 events of the plan's stream, then the
+// start and end of P2P on the side stream (the near field beside the far
+// field); stage_ms[ST_TOTAL + 1] is that P2P's own duration
+constexpr int kEvPerSet = ST_TOTAL + 3;
+constexpr int kEvSideA = ST_TOTAL + 1, kEvSideB = ST_TOTAL + 2;
 
 struct bbfmm_amd_plan {
   int N = 0, n = 0, n3 = 0, r = 0, nrhs = 1, maxlev = 0, ncell = 0, nleaf = 0, leaf0 = 0;
@@ -106,9 +111,17 @@ struct bbfmm_amd_plan {
   // per-stage timing: a ring of `nsets` event sets, one set per solve,
   // read (and averaged) only when stage times are requested
   int nsets = 0;
-  std::vector<hipEvent_t> ev;  // nsets * (ST_TOTAL + 1)
+  std::vector<hipEvent_t> ev;  // nsets * kEvPerSet
   long long solves = 0, timed_from = 0;
-  double stage_ms[ST_TOTAL + 1] = {};
+  double stage_ms[ST_TOTAL + 2] = {};
+  bool side_timed = false;     // the timed solves ran P2P on the side stream
+  // near field beside the far field (particle-row form, BBFMM_OPT_NEAR_STREAM):
+  // P2P on a side stream forked after the gather and joined before the
+  // scatter, into near-field rows of its own
+  int near_stream = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  real *d_phin = nullptr;
   bool solved = false;
   // whole-solve hipGraph (BBFMM_OPT_GRAPH): captured on a private stream for
   // one (q, phi) pointer pair, replayed on the plan's stream
@@ -166,7 +179,7 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_nbr, p->d_nnbr, p->d_iinter, p->d_inter, p->d_kidx, p->d_Ktable, p->d_levsta,
                   p->d_Ucm, p->d_Urm, p->d_Tkz, p->d_Kw, p->d_Cw, p->d_scale, p->d_q, p->d_phi,
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
-                  p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad, p->d_phit, p->d_ipad,
+                  p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad, p->d_phit, p->d_ipad, p->d_phin,
                   p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf, p->d_cls_cells,
                   p->d_vsrc};
   for (void *q : ptrs)
@@ -175,6 +188,11 @@ static void plan_free_device(bbfmm_amd_plan *p)
   p->gexec = nullptr;
   if (p->gstream) (void)hipStreamDestroy(p->gstream);
   p->gstream = nullptr;
+  if (p->side) (void)hipStreamDestroy(p->side);
+  p->side = nullptr;
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+  p->ev_fork = p->ev_join = nullptr;
   if (p->grid) grid_m2l_destroy(p->grid);
   p->grid = nullptr;
   if (p->list) list_m2l_destroy(p->list);
@@ -190,7 +208,7 @@ static int set_timing(bbfmm_amd_plan *p, int nsets)
   (void)hipStreamSynchronize(p->stream);
   for (auto &e : p->ev)
     if (e) (void)hipEventDestroy(e);
-  p->ev.assign((size_t)nsets * (ST_TOTAL + 1), nullptr);
+  p->ev.assign((size_t)nsets * kEvPerSet, nullptr);
   for (auto &e : p->ev) BB_HIP(hipEventCreate(&e));
   p->nsets = nsets;
   p->timed_from = p->solves;
@@ -206,7 +224,7 @@ static int read_timing(bbfmm_amd_plan *p)
   if (sets <= 0) return BBFMM_OK;
   BB_HIP(hipStreamSynchronize(p->stream));
   for (int k = 0; k < sets; k++) {
-    hipEvent_t *e = &p->ev[(size_t)k * (ST_TOTAL + 1)];
+    hipEvent_t *e = &p->ev[(size_t)k * kEvPerSet];
     if (p->graph_timed) {  // a graph launch: the total only
       float tot = 0;
       BB_HIP(hipEventElapsedTime(&tot, e[0], e[ST_TOTAL]));
@@ -221,13 +239,18 @@ static int read_timing(bbfmm_amd_plan *p)
     float tot = 0;
     BB_HIP(hipEventElapsedTime(&tot, e[0], e[ST_TOTAL]));
     p->stage_ms[ST_TOTAL] += tot / sets;
+    if (p->side_timed) {
+      float ms = 0;
+      BB_HIP(hipEventElapsedTime(&ms, e[kEvSideA], e[kEvSideB]));
+      p->stage_ms[ST_TOTAL + 1] += ms / sets;
+    }
   }
   return BBFMM_OK;
 }
 
 static inline hipEvent_t stage_event(bbfmm_amd_plan *p, int st)
 {
-  return p->ev[(size_t)(p->solves % p->nsets) * (ST_TOTAL + 1) + st];
+  return p->ev[(size_t)(p->solves % p->nsets) * kEvPerSet + st];
 }
 
 // Upload everything a solve needs from a forest (one complete tree in the
@@ -509,6 +532,12 @@ static int prepare_solve(bbfmm_amd_plan *p)
     BB_HIP(hipMemsetAsync(p->d_qt, 0, sizeof(real) * cnt, p->stream));
   }
   if (rows16(p) && !p->d_phit) BB_TRY(dalloc(&p->d_phit, (size_t)p->npad * 16));
+  if (rows16(p) && p->near_stream) {
+    if (!p->d_phin) BB_TRY(dalloc(&p->d_phin, (size_t)p->npad * 16));
+    if (!p->side) BB_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+    if (!p->ev_fork) BB_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+    if (!p->ev_join) BB_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+  }
 #endif
   return BBFMM_OK;
 }
@@ -562,10 +591,65 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
 // The solve pipeline on device arrays in leaf order (d_q -> d_phi).
 // rows: the charges are in d_qt as particle rows (the particle-row form);
 // otherwise in d_q [nrhs][N] leaf order, the potentials going to d_phi
+static P2PArgs p2p_args(const bbfmm_amd_plan *p)
+{
+  P2PArgs pp;
+  pp.N = p->N;
+  pp.nleaf = p->nleaf;
+  pp.nrhs = p->nrhs;
+  pp.pos = p->d_pos;
+  pp.leafsta = p->d_leafsta;
+  pp.leafcnt = p->d_leafcnt;
+  pp.nbr = p->d_nbr;
+  pp.nnbr = p->d_nnbr;
+  pp.q = p->d_q;
+  pp.phi = p->d_phi;
+  pp.use_mfma = p->form != BBFMM_M2L_PLAIN;
+  pp.potential = p->potential;
+  pp.ppos = p->d_ppos;
+  pp.seg = p->d_seg;
+  pp.nseg = p->d_nseg;
+  pp.pad_of = p->d_pad_of;
+  return pp;
+}
+
+// the particle-row form with P2P beside the far field
+static bool near_beside(const bbfmm_amd_plan *p, bool rows)
+{
+  return rows && p->near_stream && p->side && p->d_phin;
+}
+
 static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
 {
   hipStream_t s = p->stream;
   PassArgs a = pass_args(p, rows);
+  P2PArgs pp = p2p_args(p);
+#ifdef SINGLE
+  if (pp.use_mfma && p->nrhs >= 16) {
+    if (!p->d_qt) return set_error(BBFMM_E_ARG, "internal: solve not prepared");
+    pp.qt = p->d_qt;
+    if (rows) pp.phit = p->d_phit;
+  }
+#endif
+  const bool beside = near_beside(p, rows);  // particle rows: fp32 build only
+  // fork after the M2L: P2P (compute-bound) then shares the GPU with the
+  // latency-bound downward pass
+  auto fork_near = [&]() -> int {
+#ifdef SINGLE
+    if (beside) {
+      pp.phit = nullptr;
+      pp.phin = p->d_phin;
+      BB_HIP(hipEventRecord(p->ev_fork, s));
+      BB_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+      if (tm) BB_HIP(hipEventRecord(stage_event(p, kEvSideA), p->side));
+      BB_TRY(launch_p2p(pp, p->side));
+      if (tm) BB_HIP(hipEventRecord(stage_event(p, kEvSideB), p->side));
+      BB_HIP(hipEventRecord(p->ev_join, p->side));
+    }
+#endif
+    return BBFMM_OK;
+  };
+  if (tm) p->side_timed = beside;
   // the first upward level (P2M, M2M into the leaves' parents) and the last
   // downward level (post-M2L of the leaves, L2L into them, L2P) each in one
   // kernel: the leaves' S and F are then not stored (the F fetch recomputes
@@ -585,6 +669,7 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
   BB_TRY(launch_m2m(a, p->h_ploff.data(), s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2L_PREP), s));
   BB_TRY(run_m2l(p, p->d_M, p->d_L, tm ? stage_event(p, ST_M2L) : nullptr));
+  BB_TRY(fork_near());
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_POST), s));
   BB_TRY(launch_postm2l(a, s, p->nrhs));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2L), s));
@@ -596,32 +681,13 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
     BB_TRY(launch_l2p(a, s, p->nrhs));
   if (!rows)  // the particle-row L2P writes the far field itself
     BB_HIP(hipMemcpyAsync(p->d_farfield, p->d_phi, sizeof(real) * p->N, hipMemcpyDeviceToDevice, s));
+  // "p2p" stage: P2P itself, or with the near field beside the far field the
+  // wait for it after the far field is done (its own time: ST_TOTAL + 1)
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2P), s));
-  P2PArgs pp;
-  pp.N = p->N;
-  pp.nleaf = p->nleaf;
-  pp.nrhs = p->nrhs;
-  pp.pos = p->d_pos;
-  pp.leafsta = p->d_leafsta;
-  pp.leafcnt = p->d_leafcnt;
-  pp.nbr = p->d_nbr;
-  pp.nnbr = p->d_nnbr;
-  pp.q = p->d_q;
-  pp.phi = p->d_phi;
-  pp.use_mfma = p->form != BBFMM_M2L_PLAIN;
-  pp.potential = p->potential;
-  pp.ppos = p->d_ppos;
-  pp.seg = p->d_seg;
-  pp.nseg = p->d_nseg;
-  pp.pad_of = p->d_pad_of;
-#ifdef SINGLE
-  if (pp.use_mfma && p->nrhs >= 16) {
-    if (!p->d_qt) return set_error(BBFMM_E_ARG, "internal: solve not prepared");
-    pp.qt = p->d_qt;
-    if (rows) pp.phit = p->d_phit;
-  }
-#endif
-  BB_TRY(launch_p2p(pp, s));
+  if (beside)
+    BB_HIP(hipStreamWaitEvent(s, p->ev_join, 0));  // join
+  else
+    BB_TRY(launch_p2p(pp, s));
   return BBFMM_OK;
 }
 
@@ -701,8 +767,11 @@ static int solve_eager(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev, bool
     BB_TRY(launch_gather(p->d_perm, q_dev, p->d_q, p->N, p->nrhs, s));
   BB_TRY(solve_sorted(p, tm, rows));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_SCATTER), s));
-  if (rows)
-    BB_TRY(launch_scatter_rows16(p->d_ipad, p->d_phit, phi_dev, p->N, s));
+  if (rows) {
+    const bool beside = near_beside(p, rows);
+    BB_TRY(launch_scatter_rows16(p->d_ipad, p->d_phit, beside ? p->d_phin : nullptr, p->d_qt, p->potential, phi_dev,
+                                 p->N, s));
+  }
   else
     BB_TRY(launch_scatter(p->d_iperm, p->d_phi, phi_dev, p->N, p->nrhs, s));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_TOTAL), s));
@@ -820,6 +889,11 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
   case BBFMM_OPT_M2L_VARIANT:
     if (value < 0 || value > 5) return set_error(BBFMM_E_ARG, "M2L variant %d out of range", value);
     p->m2l_variant = value;
+    drop_graph(p);
+    return BBFMM_OK;
+  case BBFMM_OPT_NEAR_STREAM:
+    if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "near stream %d: 0 or 1", value);
+    p->near_stream = value;
     drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_LEVEL_SWITCH:
@@ -1082,7 +1156,7 @@ extern "C" int bbfmm_amd_stage_times(bbfmm_amd_plan *p, double *out, int nout)
 {
   if (!p || !out) return 0;
   if (read_timing(p) != BBFMM_OK) return 0;
-  const int m = nout < ST_TOTAL + 1 ? nout : ST_TOTAL + 1;
+  const int m = nout < ST_TOTAL + 2 ? nout : ST_TOTAL + 2;
   for (int k = 0; k < m; k++) out[k] = p->stage_ms[k];
   return m;
 }

@@ -52,7 +52,10 @@ int launch_scatter(const int *iperm, const real *src, real *dst, int N, int nrhs
 // particle-row form of 16 vectors: qt[ipad[j]][v] = q[v][j] and
 // phi[v][j] = phit[ipad[j]][v] (ipad: user index -> padded leaf-order index)
 int launch_gather_rows16(const int *ipad, const real *q, real *qt, int N, hipStream_t s);
-int launch_scatter_rows16(const int *ipad, const real *phit, real *phi, int N, hipStream_t s);
+// phin (may be null): near-field rows to add, then the output's q factor
+// (qt rows, unless potential); without phin the rows of phit are final
+int launch_scatter_rows16(const int *ipad, const real *phit, const real *phin, const real *qt, int potential,
+                          real *phi, int N, hipStream_t s);
 int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs);
 // h_ploff[l]: offset of level l's parents in a.plist (levels 0..maxlev)
 int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs);
@@ -128,6 +131,9 @@ struct P2PArgs {
   // particle-row form: the far-field potentials are in phit [Npad][16]; P2P
   // adds the near field and applies the output's q factor there (phi unused)
   float *phit = nullptr;
+  // particle-row form beside the far field: P2P stores the near-field
+  // potentials in rows of their own (the scatter combines them); phit unused
+  float *phin = nullptr;
 };
 int launch_p2p(const P2PArgs &a, hipStream_t s);
 

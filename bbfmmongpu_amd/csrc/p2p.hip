@@ -231,6 +231,17 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
       p2p16_step<NT>(y3, q3, x, acc);
     }
   }
+  if (a.phin) {
+    // particle rows of their own: the near-field potential
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = tb + 16 * t + 4 * kq + v;
+        if (i < cnt) a.phin[(size_t)(psta + i) * 16 + ti] = acc[t][v];
+      }
+    return;
+  }
   if (a.phit) {
     // particle rows: far field + near field, then the output's q factor
 #pragma unroll
@@ -312,7 +323,7 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
     if (left >= 16) {
 #ifdef SINGLE
       if (a.use_mfma && a.qt && a.ppos && a.seg) {
-        if (!a.phit) {  // particle rows arrive filled
+        if (!a.phit && !a.phin) {  // particle rows arrive filled
           hipLaunchKernelGGL(k_pad_charges16, dim3((a.N + 255) / 256), dim3(256), 0, s, a.N, a.pad_of,
                              a.q + (size_t)rhs * a.N, a.qt);
           BB_HIP(hipGetLastError());

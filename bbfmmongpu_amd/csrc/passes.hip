@@ -79,15 +79,26 @@ __global__ void k_gather_rows16(int N, const int *__restrict__ ipad, const real 
   for (int k = 0; k < 16; k++) row[k] = v[k];
 }
 
+// phin (not null): the near field arrived in rows of its own (P2P ran
+// beside the far-field passes); the scatter adds it and applies the output's
+// q factor (qt, unless potential).
 __global__ void k_scatter_rows16(int N, const int *__restrict__ ipad, const real *__restrict__ phit,
+                                 const real *__restrict__ phin, const real *__restrict__ qt, int potential,
                                  real *__restrict__ phi)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
-  const real *row = phit + (size_t)ipad[j] * 16;
+  const size_t r0 = (size_t)ipad[j] * 16;
   real v[16];
 #pragma unroll
-  for (int k = 0; k < 16; k++) v[k] = row[k];
+  for (int k = 0; k < 16; k++) v[k] = phit[r0 + k];
+  if (phin) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const real t = v[k] + phin[r0 + k];
+      v[k] = potential ? t : qt[r0 + k] * t;
+    }
+  }
 #pragma unroll
   for (int k = 0; k < 16; k++) phi[(size_t)k * N + j] = v[k];
 }
@@ -1084,10 +1095,12 @@ int launch_gather_rows16(const int *ipad, const real *q, real *qt, int N, hipStr
   return BBFMM_OK;
 }
 
-int launch_scatter_rows16(const int *ipad, const real *phit, real *phi, int N, hipStream_t s)
+int launch_scatter_rows16(const int *ipad, const real *phit, const real *phin, const real *qt, int potential,
+                          real *phi, int N, hipStream_t s)
 {
   if (N <= 0) return BBFMM_OK;
-  hipLaunchKernelGGL(k_scatter_rows16, dim3((N + 255) / 256), dim3(256), 0, s, N, ipad, phit, phi);
+  hipLaunchKernelGGL(k_scatter_rows16, dim3((N + 255) / 256), dim3(256), 0, s, N, ipad, phit, phin, qt, potential,
+                     phi);
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }

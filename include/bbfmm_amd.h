@@ -169,6 +169,11 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                     threads per block; IJ MFMA forms 0 automatic,
                                     1/4/5 source-stationary (split-BF16) with 8/4/2 field
                                     classes per wave, 2/3 one/two 32-column tiles per wave */
+#define BBFMM_OPT_NEAR_STREAM 8  /* 16-vector MFMA solves: 1 runs P2P on a side stream
+                                    beside the downward pass (forked after the M2L, joined
+                                    before the scatter), 0 (default) after it; the results
+                                    are the same bits.  At N = 1M the concurrent kernels
+                                    compete for the CUs and the solve is slower */
 #define BBFMM_OPT_LEVEL_SWITCH 7 /* CL/IJ schemes: first level on the dense grids, the
                                     levels below on the basic scheme (default 2) */
 
@@ -250,7 +255,9 @@ int bbfmm_amd_fetch(bbfmm_amd_plan *plan, int which, real *dst);
 /* Per-stage times (ms, HIP events on the plan's stream) averaged over the
    last min(solves since timing was set, k) timed solves: 0 gather, 1 P2M,
    2 M2M, 3 M2L preparation (dense-grid scatter), 4 M2L kernel, 5 post-M2L,
-   6 L2L, 7 L2P, 8 P2P, 9 scatter, 10 total.
+   6 L2L, 7 L2P, 8 P2P, 9 scatter, 10 total, 11 P2P on the side stream.
+   With P2P beside the far field (BBFMM_OPT_NEAR_STREAM) entry 8 is the wait
+   for it after the far field and entry 11 its own duration (0 otherwise).
    Synchronises the stream.  Returns the number of entries written. */
 int bbfmm_amd_stage_times(bbfmm_amd_plan *plan, double *out, int nout);
 

@@ -31,11 +31,14 @@ def _replays_match(plan, Q, cuda):
     plan.set(graph=False)
 
 
-def test_graph_single_plan_chosen_form(ops4, cuda):
+@pytest.mark.parametrize("near_stream", [False, True])
+def test_graph_single_plan_chosen_form(near_stream, ops4, cuda):
+    """near_stream: P2P on a side stream, forked and joined inside the graph."""
     import bbfmmongpu_amd as B
     U, K, _ = ops4
     pos, Q = O.uniform_inputs(65536, seed=8, nrhs=16)
-    plan = B.Plan(pos, 4, U, K, nrhs=16, prec="s", lmod=1).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2)
+    plan = B.Plan(pos, 4, U, K, nrhs=16, prec="s", lmod=1).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2,
+                                                               near_stream=near_stream)
     _replays_match(plan, Q.astype(np.float32), cuda)
     plan.close()
 

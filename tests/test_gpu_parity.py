@@ -127,3 +127,25 @@ def test_bf16x2_many_vectors_against_oracle(N, lmod, nrhs, maxlev, ops4, cuda):
         err = O.rel_l2(ref["phi"], phi[j])
         print(f"bf16x2 solve N={N} rhs={j}: {err:.2e}")
         assert err < 3e-5
+
+
+def test_near_field_beside_far_field_is_bit_identical(ops4, cuda):
+    """P2P on the side stream (near-field rows combined by the scatter) gives
+    the same bits as P2P after the far field (q * (far + near) either way),
+    for both outputs; the stage times report the side stream's P2P."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, Q = O.uniform_inputs(65536, seed=23, nrhs=16)
+    q = torch.from_numpy(Q.astype(np.float32)).to(cuda)
+    plan = B.Plan(pos, 4, U, K, nrhs=16, prec="s", lmod=1).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2)
+    for output in ("q_phi", "potential"):
+        plan.set(output=output, near_stream=True, timing=1)
+        beside = plan.solve(q).cpu().numpy()
+        st = plan.stage_times()
+        plan.set(near_stream=False)
+        after = plan.solve(q).cpu().numpy()
+        print(f"{output}: max abs diff {np.abs(beside - after).max():.3e}, stages {st}", flush=True)
+        assert np.array_equal(beside, after)
+        assert st["p2p_beside"] > 0
+    plan.close()
