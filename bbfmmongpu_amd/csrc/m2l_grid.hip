@@ -1167,9 +1167,16 @@ __global__ void __launch_bounds__(256) k_m2l_list_mfma(ListArgs g)
 {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  // XCD-aware order: the launch is a multiple of 8 blocks and hardware block b
+  // runs on XCD b % 8, which takes the contiguous range lb of columns so that
+  // neighbouring targets, which gather the same sources, share one L2.
+  const int nb = g.blk0[8];
+  const int b = blockIdx.x;
+  const int lb = (b & 7) * ((nb + 7) >> 3) + (b >> 3);
+  if (lb >= nb) return;
   int s = 0;
-  while (s < 7 && (int)blockIdx.x >= g.blk0[s + 1]) s++;
-  const int grp = blockIdx.x - g.blk0[s];
+  while (s < 7 && lb >= g.blk0[s + 1]) s++;
+  const int grp = lb - g.blk0[s];
   const int ncols = (g.cls_off[s + 1] - g.cls_off[s]) * g.nrhs;
   const int half = lane >> 5;
   // columns past the end compute a copy of the class's first column and are not stored
@@ -1184,19 +1191,41 @@ __global__ void __launch_bounds__(256) k_m2l_list_mfma(ListArgs g)
   f32x16 acc;
 #pragma unroll
   for (int e = 0; e < 16; e++) acc[e] = 0.0f;
-  int Sn = vs[0];
-  for (int v = 0; v < kOffsets; v++) {
-    const int S = Sn;
-    if (v + 1 < kOffsets) Sn = vs[v + 1];
+  // Software pipeline over the 189 offsets: the operator fragments and the
+  // gathered source column of offset v + 1 are in flight while the MFMAs of
+  // offset v run, and the source index of v + 2 one step before that (two
+  // register stages, the loop unrolled by two so that no stage is copied).
+  struct Stage {
+    f32x4 a[4], b[4];  // BF16X2: a[0..3] = hi0, lo0, hi1, lo1 of K_v; b likewise of the source
+  };
+  auto load = [&](int v, int S, Stage &t) {
     const int k = tab[v].x;
     if (FORM == BBFMM_M2L_BF16X2) {
-      const bf16x8 *Ka = (const bf16x8 *)g.Ka + (size_t)k * 4 * 64 + lane;
-      const bf16x8 a0h = Ka[0 * 64], a0l = Ka[1 * 64], a1h = Ka[2 * 64], a1l = Ka[3 * 64];
+      const f32x4 *Ka = (const f32x4 *)((const bf16x8 *)g.Ka + (size_t)k * 4 * 64 + lane);
+#pragma unroll
+      for (int q = 0; q < 4; q++) t.a[q] = Ka[q * 64];
       const size_t o = (rbase + S) * 32 + half * 8;
-      const bf16x8 b0h = *(const bf16x8 *)((const __bf16 *)g.M0 + o);
-      const bf16x8 b0l = *(const bf16x8 *)((const __bf16 *)g.M1 + o);
-      const bf16x8 b1h = *(const bf16x8 *)((const __bf16 *)g.M0 + o + 16);
-      const bf16x8 b1l = *(const bf16x8 *)((const __bf16 *)g.M1 + o + 16);
+      t.b[0] = *(const f32x4 *)((const __bf16 *)g.M0 + o);
+      t.b[1] = *(const f32x4 *)((const __bf16 *)g.M1 + o);
+      t.b[2] = *(const f32x4 *)((const __bf16 *)g.M0 + o + 16);
+      t.b[3] = *(const f32x4 *)((const __bf16 *)g.M1 + o + 16);
+    } else {
+      const f32x4 *Ka = (const f32x4 *)g.Ka + ((size_t)k * 64 + lane) * 4;
+      const f32x4 *bp = (const f32x4 *)((const float *)g.M0 + (rbase + S) * 32 + half * 16);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        t.a[q] = Ka[q];
+        t.b[q] = bp[q];
+      }
+    }
+  };
+  auto mma = [&](const Stage &t) {
+    if (FORM == BBFMM_M2L_BF16X2) {
+      typedef __attribute__((ext_vector_type(8))) __bf16 b8;
+      const b8 a0h = __builtin_bit_cast(b8, t.a[0]), a0l = __builtin_bit_cast(b8, t.a[1]);
+      const b8 a1h = __builtin_bit_cast(b8, t.a[2]), a1l = __builtin_bit_cast(b8, t.a[3]);
+      const b8 b0h = __builtin_bit_cast(b8, t.b[0]), b0l = __builtin_bit_cast(b8, t.b[1]);
+      const b8 b1h = __builtin_bit_cast(b8, t.b[2]), b1l = __builtin_bit_cast(b8, t.b[3]);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0h, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0l, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, b0h, acc, 0, 0, 0);
@@ -1204,20 +1233,27 @@ __global__ void __launch_bounds__(256) k_m2l_list_mfma(ListArgs g)
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1l, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, b1h, acc, 0, 0, 0);
     } else {
-      const f32x4 *Ka = (const f32x4 *)g.Ka + ((size_t)k * 64 + lane) * 4;
-      const f32x4 *bp = (const f32x4 *)((const float *)g.M0 + (rbase + S) * 32 + half * 16);
-      f32x4 a[4], b[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        a[q] = Ka[q];
-        b[q] = bp[q];
-      }
 #pragma unroll
       for (int q = 0; q < 4; q++)
 #pragma unroll
-        for (int w = 0; w < 4; w++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q][w], b[q][w], acc, 0, 0, 0);
+        for (int w = 0; w < 4; w++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(t.a[q][w], t.b[q][w], acc, 0, 0, 0);
     }
+  };
+  static_assert(kOffsets % 2 == 1 && kOffsets >= 3, "the unrolled pipeline below");
+  Stage X, Y;
+  load(0, vs[0], X);
+  load(1, vs[1], Y);
+  int S2 = vs[2];
+  // offsets v (stage X) and v + 1 (stage Y) per iteration
+  for (int v = 0; v + 2 < kOffsets; v += 2) {
+    const int S3 = vs[v + 3 < kOffsets ? v + 3 : kOffsets - 1];
+    mma(X);
+    load(v + 2, S2, X);
+    S2 = vs[v + 4 < kOffsets ? v + 4 : kOffsets - 1];
+    mma(Y);
+    if (v + 3 < kOffsets) load(v + 3, S3, Y);
   }
+  mma(X);  // offset kOffsets - 1
   if (!valid) return;
   float *Lp = (float *)g.L + ((size_t)rh * g.ncell + F) * 32 + 4 * half;
 #pragma unroll
@@ -1360,12 +1396,12 @@ int list_m2l_run(ListM2L *g, const real *M, real *L, hipStream_t s, hipEvent_t m
     hipLaunchKernelGGL(k_m2l_list_prep<BBFMM_M2L_BF16X2>, dim3(pb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (g->nblocks) hipLaunchKernelGGL(k_m2l_list_mfma<BBFMM_M2L_BF16X2>, dim3(g->nblocks), dim3(256), 0, s, a);
+    if (g->nblocks) hipLaunchKernelGGL(k_m2l_list_mfma<BBFMM_M2L_BF16X2>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
   } else {
     hipLaunchKernelGGL(k_m2l_list_prep<BBFMM_M2L_FP32MFMA>, dim3(pb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (g->nblocks) hipLaunchKernelGGL(k_m2l_list_mfma<BBFMM_M2L_FP32MFMA>, dim3(g->nblocks), dim3(256), 0, s, a);
+    if (g->nblocks) hipLaunchKernelGGL(k_m2l_list_mfma<BBFMM_M2L_FP32MFMA>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
   }
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
