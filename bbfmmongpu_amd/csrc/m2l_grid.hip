@@ -1117,10 +1117,21 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
 struct ListArgs {
   int ncell, nrhs;
   int cls_off[9];   // class c cells: cls_cells[cls_off[c] .. cls_off[c + 1])
-  int blk0[9];      // first block of each class
+  int blk0[9];      // block count of class c: blk0[c + 1] - blk0[c]
+  // logical block -> 8 * (block within its class) + class: the classes are
+  // interleaved in proportion to their sizes, so that the blocks running at
+  // one time (and the contiguous range each XCD takes) cover one region of
+  // the forest for all 8 classes and share its source cells in L2
+  const int *blk_tab;
   const int *cls_cells;
   const int *vsrc;  // [ncell * 189]
   const int4 *tab;  // [8][189] (operator, ...)
+  // per wave (4 per block): bit v of words 0..2 set when some column of the
+  // wave has a non-empty source at offset v
+  const unsigned long long *mask;  // [4 * blocks][4]
+  // per wave, the source cells of its 32 columns at each offset, so that one
+  // offset's indices are one 128-byte line (ncell for columns past the end)
+  const int *vsw;  // [4 * blocks][189][32]
   const void *Ka;
   const void *M0, *M1;
   real *L;
@@ -1130,8 +1141,45 @@ struct ListM2L {
   int form = 0, nrhs = 1, ncell = 0, nblocks = 0;
   ListArgs args{};
   int4 *d_tab = nullptr;
+  int *d_blk = nullptr;
+  unsigned long long *d_mask = nullptr;
+  int *d_vsw = nullptr;
   void *d_K = nullptr, *d_M0 = nullptr, *d_M1 = nullptr;
 };
+
+// Columns of wave w of block b (in the list kernel's logical block order):
+// class s and the column range.  Shared by the kernel and its mask set-up.
+__device__ __forceinline__ int2 list_block(const ListArgs &g, int lb)
+{
+  const int code = g.blk_tab[lb];
+  return make_int2(code & 7, code >> 3);  // class, block within the class
+}
+
+// One wave per wave of the list kernel: the union over its valid columns of
+// the non-empty offsets (vsrc != ncell), as a 189-bit mask, and the wave's
+// column-contiguous copy of the source table.
+__global__ void __launch_bounds__(64) k_m2l_list_mask(ListArgs g, unsigned long long *mask, int *vsw)
+{
+  const int gw = blockIdx.x;  // 4 * logical block + wave
+  const int lb = gw >> 2, wave = gw & 3;
+  const int2 sb = list_block(g, lb);
+  const int s = sb.x, grp = sb.y;
+  const int ncols = (g.cls_off[s + 1] - g.cls_off[s]) * g.nrhs;
+  const int lane = threadIdx.x;
+  const int col = (grp * 4 + wave) * 32 + (lane & 31);
+  const bool valid = lane < 32 && col < ncols;
+  const int F = valid ? g.cls_cells[g.cls_off[s] + col / g.nrhs] : 0;
+  const int *vs = g.vsrc + (size_t)F * kOffsets;
+  unsigned long long m[3] = {0, 0, 0};
+  int *out = vsw + (size_t)gw * kOffsets * 32 + lane;
+  for (int v = 0; v < kOffsets; v++) {
+    const int S = valid ? vs[v] : g.ncell;
+    const bool any = __ballot(S != g.ncell) != 0;
+    if (any) m[v >> 6] |= 1ull << (v & 63);
+    if (lane < 32) out[v * 32] = S;
+  }
+  if (lane < 4) mask[(size_t)gw * 4 + lane] = lane < 3 ? m[lane] : 0ull;
+}
 
 template <int FORM>
 __global__ void __launch_bounds__(256) k_m2l_list_prep(ListArgs g, const real *__restrict__ M)
@@ -1168,15 +1216,15 @@ __global__ void __launch_bounds__(256) k_m2l_list_mfma(ListArgs g)
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   // XCD-aware order: the launch is a multiple of 8 blocks and hardware block b
-  // runs on XCD b % 8, which takes the contiguous range lb of columns so that
-  // neighbouring targets, which gather the same sources, share one L2.
+  // runs on XCD b % 8, which takes a contiguous range of logical blocks lb
+  // (one region of the forest, all classes) so that neighbouring targets,
+  // which gather the same sources, share one L2.
   const int nb = g.blk0[8];
   const int b = blockIdx.x;
   const int lb = (b & 7) * ((nb + 7) >> 3) + (b >> 3);
   if (lb >= nb) return;
-  int s = 0;
-  while (s < 7 && lb >= g.blk0[s + 1]) s++;
-  const int grp = lb - g.blk0[s];
+  const int2 sb = list_block(g, lb);
+  const int s = sb.x, grp = sb.y;
   const int ncols = (g.cls_off[s + 1] - g.cls_off[s]) * g.nrhs;
   const int half = lane >> 5;
   // columns past the end compute a copy of the class's first column and are not stored
@@ -1185,16 +1233,21 @@ __global__ void __launch_bounds__(256) k_m2l_list_mfma(ListArgs g)
   const int cc = valid ? col : 0;
   const int F = g.cls_cells[g.cls_off[s] + cc / g.nrhs];
   const int rh = cc % g.nrhs;
-  const int *vs = g.vsrc + (size_t)F * kOffsets;
   const int4 *tab = g.tab + s * kOffsets;
   const size_t rbase = (size_t)rh * (g.ncell + 1);
+  const int gw = __builtin_amdgcn_readfirstlane(lb * 4 + wave);
+  const int *vsl = g.vsw + (size_t)gw * kOffsets * 32 + (lane & 31);
+  auto vs = [&](int v) { return vsl[v * 32]; };
   f32x16 acc;
 #pragma unroll
   for (int e = 0; e < 16; e++) acc[e] = 0.0f;
-  // Software pipeline over the 189 offsets: the operator fragments and the
-  // gathered source column of offset v + 1 are in flight while the MFMAs of
-  // offset v run, and the source index of v + 2 one step before that (two
-  // register stages, the loop unrolled by two so that no stage is copied).
+  // Only the offsets at which some column of the wave has a non-empty source
+  // are visited (the others would add K_v times the zero cell); they come in
+  // increasing order from the wave's 189-bit mask.  Software pipeline: the
+  // operator fragments and the gathered source column of the next offset are
+  // in flight while the MFMAs of the current one run, and the source index
+  // of the one after that a step earlier (two register stages, the loop
+  // unrolled by two so that no stage is copied).
   struct Stage {
     f32x4 a[4], b[4];  // BF16X2: a[0..3] = hi0, lo0, hi1, lo1 of K_v; b likewise of the source
   };
@@ -1239,21 +1292,50 @@ __global__ void __launch_bounds__(256) k_m2l_list_mfma(ListArgs g)
         for (int w = 0; w < 4; w++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(t.a[q][w], t.b[q][w], acc, 0, 0, 0);
     }
   };
-  static_assert(kOffsets % 2 == 1 && kOffsets >= 3, "the unrolled pipeline below");
+  unsigned long long m0 = g.mask[(size_t)gw * 4], m1 = g.mask[(size_t)gw * 4 + 1], m2 = g.mask[(size_t)gw * 4 + 2];
+  // next visited offset, or -1 past the last (scalar selects, no branches)
+  auto pop = [&]() -> int {
+    const bool a0 = m0 != 0, a1 = m1 != 0, a2 = m2 != 0;
+    const int t0 = a0 ? __builtin_ctzll(m0) : 0, t1 = a1 ? __builtin_ctzll(m1) : 0,
+              t2 = a2 ? __builtin_ctzll(m2) : 0;
+    const int v = a0 ? t0 : a1 ? 64 + t1 : a2 ? 128 + t2 : -1;
+    m0 = a0 ? m0 & (m0 - 1) : m0;
+    m1 = !a0 && a1 ? m1 & (m1 - 1) : m1;
+    m2 = !a0 && !a1 ? m2 & (m2 - 1) : m2;
+    return v;
+  };
+  // Loads are unconditional (offset 0 stands in past the last) so that no
+  // branch separates a load from the MFMAs that wait for it; the source
+  // indices are read an iteration (two offsets) before their columns.
   Stage X, Y;
-  load(0, vs[0], X);
-  load(1, vs[1], Y);
-  int S2 = vs[2];
-  // offsets v (stage X) and v + 1 (stage Y) per iteration
-  for (int v = 0; v + 2 < kOffsets; v += 2) {
-    const int S3 = vs[v + 3 < kOffsets ? v + 3 : kOffsets - 1];
-    mma(X);
-    load(v + 2, S2, X);
-    S2 = vs[v + 4 < kOffsets ? v + 4 : kOffsets - 1];
-    mma(Y);
-    if (v + 3 < kOffsets) load(v + 3, S3, Y);
+  int vy, vz, vw;
+  const int vx = pop();
+  vy = pop();
+  vz = pop();
+  vw = pop();
+  if (vx >= 0) {
+    load(vx, vs(vx), X);
+    load(max(vy, 0), vs(max(vy, 0)), Y);
+    int Sz = vs(max(vz, 0)), Sw = vs(max(vw, 0));
+    // X holds an offset, Y the next one (vy, when vy >= 0); vz and vw follow,
+    // with sources Sz and Sw
+    for (;;) {
+      const int vu = pop(), vt = pop();
+      const int Su = vs(max(vu, 0)), St = vs(max(vt, 0));
+      __builtin_amdgcn_sched_barrier(0);
+      mma(X);
+      load(max(vz, 0), Sz, X);
+      if (vy < 0) break;
+      mma(Y);
+      load(max(vw, 0), Sw, Y);
+      if (vz < 0) break;
+      vy = vw;
+      vz = vu;
+      vw = vt;
+      Sz = Su;
+      Sw = St;
+    }
   }
-  mma(X);  // offset kOffsets - 1
   if (!valid) return;
   float *Lp = (float *)g.L + ((size_t)rh * g.ncell + F) * 32 + 4 * half;
 #pragma unroll
@@ -1316,6 +1398,17 @@ int list_m2l_create(ListM2L **out, int r, int nrhs, int form, const real *K, int
   g->nblocks = nb;
   std::vector<int4> tab;
   offset_table(tab);
+  // proportional interleave of the classes' blocks (stable: ties by class)
+  std::vector<std::pair<double, int>> order;
+  order.reserve(nb);
+  for (int c = 0; c < 8; c++) {
+    const int n = a.blk0[c + 1] - a.blk0[c];
+    for (int k = 0; k < n; k++) order.push_back({(k + 0.5) / n, 8 * k + c});
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<double, int> &x, const std::pair<double, int> &y) { return x.first < y.first; });
+  std::vector<int> blk(nb);
+  for (int i = 0; i < nb; i++) blk[i] = order[i].second;
   int rc = BBFMM_OK;
   auto fail = [&](hipError_t e, const char *what) {
     rc = set_error(BBFMM_E_HIP, "list M2L: %s: %s", what, hipGetErrorString(e));
@@ -1324,6 +1417,11 @@ int list_m2l_create(ListM2L **out, int r, int nrhs, int form, const real *K, int
   if ((e = hipMalloc(&g->d_tab, sizeof(int4) * tab.size())) != hipSuccess) fail(e, "hipMalloc");
   if (rc == BBFMM_OK && (e = hipMemcpy(g->d_tab, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice)) != hipSuccess)
     fail(e, "copy");
+  if (rc == BBFMM_OK && nb > 0) {
+    if ((e = hipMalloc(&g->d_blk, sizeof(int) * (size_t)nb)) != hipSuccess) fail(e, "hipMalloc");
+    else if ((e = hipMemcpy(g->d_blk, blk.data(), sizeof(int) * (size_t)nb, hipMemcpyHostToDevice)) != hipSuccess)
+      fail(e, "copy");
+  }
   if (rc == BBFMM_OK) {
     if (form == BBFMM_M2L_BF16X2) {
       std::vector<__bf16> Kb((size_t)kNumOps * 2 * 2 * 64 * 8);
@@ -1369,9 +1467,25 @@ int list_m2l_create(ListM2L **out, int r, int nrhs, int form, const real *K, int
   a.cls_cells = d_cls_cells;
   a.vsrc = d_vsrc;
   a.tab = g->d_tab;
+  a.blk_tab = g->d_blk;
   a.Ka = g->d_K;
   a.M0 = g->d_M0;
   a.M1 = g->d_M1;
+  if (nb > 0) {
+    if ((e = hipMalloc(&g->d_mask, sizeof(unsigned long long) * 16 * (size_t)nb)) != hipSuccess ||
+        (e = hipMalloc(&g->d_vsw, sizeof(int) * 4 * (size_t)nb * kOffsets * 32)) != hipSuccess) {
+      fail(e, "hipMalloc");
+    } else {
+      hipLaunchKernelGGL(k_m2l_list_mask, dim3(4 * nb), dim3(64), 0, s, a, g->d_mask, g->d_vsw);
+      if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess) fail(e, "mask set-up");
+    }
+    if (rc != BBFMM_OK) {
+      list_m2l_destroy(g);
+      return rc;
+    }
+  }
+  a.mask = g->d_mask;
+  a.vsw = g->d_vsw;
   *out = g;
   return BBFMM_OK;
 #endif
@@ -1380,7 +1494,7 @@ int list_m2l_create(ListM2L **out, int r, int nrhs, int form, const real *K, int
 void list_m2l_destroy(ListM2L *g)
 {
   if (!g) return;
-  for (void *p : {(void *)g->d_tab, g->d_K, g->d_M0, g->d_M1})
+  for (void *p : {(void *)g->d_tab, (void *)g->d_blk, (void *)g->d_mask, (void *)g->d_vsw, g->d_K, g->d_M0, g->d_M1})
     if (p) (void)hipFree(p);
   delete g;
 }
