@@ -196,9 +196,11 @@ def run_c4(args):
     """64 independent instances, N log-uniform in [4096, 262144]; half Plummer
     (a = 0.1, r <= 1, mapped into the unit cube), half on a sphere of radius
     0.4; charges uniform [0, 1]; each instance on its own sparse octree with
-    leaves of at most 32 points; n = 4, r = 32, fp32.  One plan, M2L
-    configuration by the autotuner, the whole solve replayed as one hipGraph.
-    Errors against the FP64 direct sum on sampled targets of every instance."""
+    leaves of at most 32 points; n = 4, r = 32, fp32 data with the split-BF16
+    M2L (FP32 accumulation).  One plan, M2L configuration by the autotuner,
+    the whole solve replayed as one hipGraph.  Errors against the FP64 direct
+    sum on sampled targets of every instance; the FP32 MFMA list kernel is
+    timed on the same plan beside it."""
     import torch
     import bbfmmongpu_amd as B
     from bbfmmongpu_amd import workloads as W
@@ -210,10 +212,11 @@ def run_c4(args):
     Ntot = sum(p.shape[0] for p in pos_list)
     U, K, _ = B.operators(4, 32, prec="s")
     t0 = time.time()
-    # fp32 arithmetic throughout; the FP32 MFMA form lets the tuner choose
-    # between the basic kernel and the class-grouped list MFMA kernel
+    # the n = 4 precision form (split-BF16 M2L, FP32 accumulation; everything
+    # else fp32); the tuner chooses between the basic kernel and the
+    # class-grouped list MFMA kernel
     plan = B.Plan.batch(pos_list, 4, U, K, nrhs=1, prec="s", stream=stream.cuda_stream).set(
-        form=B.FORM_FP32MFMA)
+        form=B.FORM_BF16X2)
     t_setup = time.time() - t0
     depths = plan.depths
     log(f"c4: {len(pos_list)} instances, N total {Ntot}, depths {min(depths)}..{max(depths)}, "
@@ -228,16 +231,28 @@ def run_c4(args):
     plan.set(graph=True)
     ms_graph, _ = timed_solves(plan, q, phi, args.steps, args.warmup, stream)
     log(f"c4: eager {ms_eager:.3f} ms/solve, graph {ms_graph:.3f} ms/solve, stages {st}")
-    out = phi.cpu().numpy()[0]
-    errs = []
-    o = 0
+    direct = []
     for i, (p, qi) in enumerate(zip(pos_list, q_list)):
         n = p.shape[0]
         tg = np.sort(np.random.default_rng(100 + i).choice(n, min(n, 256), replace=False)).astype(np.int32)
-        d = B.direct(p, qi, tg)
-        errs.append(rel_l2(d, out[o + tg]))
-        o += n
+        direct.append((tg, B.direct(p, qi, tg)))
+
+    def errors(out):
+        errs, o = [], 0
+        for p, (tg, d) in zip(pos_list, direct):
+            errs.append(rel_l2(d, out[o + tg]))
+            o += p.shape[0]
+        return errs
+
+    errs = errors(phi.cpu().numpy()[0])
     log(f"c4: errors vs FP64 direct: max {max(errs):.3e} median {float(np.median(errs)):.3e}")
+    # the same solve with the FP32 MFMA list kernel (eager), for comparison
+    plan.set(graph=False).set(scheme=B.SCHEME_IJ, form=B.FORM_FP32MFMA)
+    plan.solve(q, phi)
+    ms_f32, st_f32 = timed_solves(plan, q, phi, args.steps, args.warmup, stream)
+    errs_f32 = errors(phi.cpu().numpy()[0])
+    log(f"c4: FP32 MFMA list M2L: {ms_f32:.3f} ms/solve (m2l {st_f32['m2l']:.3f} ms), "
+        f"errors max {max(errs_f32):.3e} median {float(np.median(errs_f32)):.3e}")
     flop_m2l = 32 * 63 * plan.m2l_interactions
     result = {
         "metric": "FMM s/solve for 64 uneven instances in one hipGraph (n=4, r=32), rel L2 err vs FP64 direct",
@@ -247,7 +262,7 @@ def run_c4(args):
         "config": {"workload": f"64 instances, N log-uniform [4096, 262144] (total {Ntot}), half Plummer "
                                f"a=0.1, half sphere surface r=0.4, leaves <= 32 points, n=4, r=32",
                    "instances": len(pos_list), "N_total": Ntot, "depth_min": min(depths),
-                   "depth_max": max(depths), "m2l_form": "FP32 MFMA allowed (fp32 arithmetic)",
+                   "depth_max": max(depths), "m2l_form": "split-BF16 (3 products, FP32 accumulation)",
                    "m2l_config": {"scheme": int(best[0]), "variant": int(best[1]), "level_switch": int(best[2])}},
         "particles_per_s": Ntot / (ms_graph * 1e-3),
         "ms_eager": ms_eager, "ms_graph": ms_graph,
@@ -255,6 +270,8 @@ def run_c4(args):
         "p2p_ginteractions_per_s": plan.p2p_pairs / (st["p2p"] * 1e-3) / 1e9,
         "stage_ms": st, "tuning_table": table,
         "rel_l2_err": {"max": max(errs), "median": float(np.median(errs)), "targets_per_instance": 256},
+        "other_forms": {"fp32_mfma": {"ms_eager": ms_f32, "m2l_ms": st_f32["m2l"],
+                                      "rel_l2_err": {"max": max(errs_f32), "median": float(np.median(errs_f32))}}},
         "setup_s": t_setup,
     }
     plan.close()
