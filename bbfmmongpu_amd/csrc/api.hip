@@ -479,6 +479,9 @@ static PassArgs pass_args(const bbfmm_amd_plan *p, bool rows = false)
   a.L = p->d_L;
   a.F = p->d_F;
   a.potential = p->potential;
+#ifdef SINGLE
+  a.cols = !p->complete && p->n == 4 && p->r == 32;
+#endif
   if (rows) {
     a.rows = 1;
     a.qt = p->d_qt;
@@ -610,6 +613,8 @@ static P2PArgs p2p_args(const bbfmm_amd_plan *p)
   pp.seg = p->d_seg;
   pp.nseg = p->d_nseg;
   pp.pad_of = p->d_pad_of;
+  // sparse forests with a few particles per leaf: a lane per target particle
+  if (!p->complete && p->N < 12LL * p->nleaf) pp.leafof = p->d_leafof;
   return pp;
 }
 

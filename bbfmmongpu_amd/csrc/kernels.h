@@ -44,6 +44,9 @@ struct PassArgs {
   int l2l_end = 0x7fffffff;
   // fused first upward level (launch_up_leaf): M2M starts at parents of this level
   int m2m_start = 0x7fffffff;
+  // sparse forests, n = 4, r = 32, fp32: P2M and post-M2L take 32 cells
+  // (leaves) per wave as the columns of MFMA products
+  int cols = 0;
 };
 
 int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
@@ -134,6 +137,9 @@ struct P2PArgs {
   // particle-row form beside the far field: P2P stores the near-field
   // potentials in rows of their own (the scatter combines them); phit unused
   float *phin = nullptr;
+  // sparse forests (a few particles per leaf): a lane per target particle,
+  // leafof[i] its leaf; used when set
+  const int *leafof = nullptr;
 };
 int launch_p2p(const P2PArgs &a, hipStream_t s);
 

@@ -109,6 +109,44 @@ __global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
   }
 }
 
+// Sparse forests, where leaves hold a few particles: a lane per target
+// particle, sweeping its own leaf's neighbour list (the neighbour leaves'
+// particles straight from memory; lanes of one leaf read the same source at
+// the same time).  Same sums as above; the r != 0 test is applied to every
+// neighbour leaf, which changes nothing since coincident particles share a
+// leaf.
+template <int NR>
+__global__ void __launch_bounds__(256) k_p2p_lane(P2PArgs a, int rhs0)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.N) return;
+  const int lf = a.leafof[i];
+  const real4 x = a.pos[i];
+  const int nn = a.nnbr[lf];
+  const int *nb = a.nbr + (size_t)lf * 27;
+  const size_t N = a.N;
+  real acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) acc[r] = 0;
+  for (int b = 0; b < nn; b++) {
+    const int B = nb[b];
+    const int j0 = a.leafsta[B], j1 = j0 + a.leafcnt[B];
+    for (int j = j0; j < j1; j++) {
+      const real4 y = a.pos[j];
+      const real dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z;
+      const real rr = dx * dx + dy * dy + dz * dz;
+      const real w = rr != (real)0 ? rsqrt_r(rr) : (real)0;
+#pragma unroll
+      for (int r = 0; r < NR; r++) acc[r] += w * a.q[(size_t)(rhs0 + r) * N + j];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const size_t o = (size_t)(rhs0 + r) * N + i;
+    a.phi[o] += (a.potential ? (real)1 : a.q[o]) * acc[r];
+  }
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (each with its own L2):
 // give XCD x the x-th contiguous eighth of the leaves (Morton order), so that
 // neighbouring leaves, which read the same sources, share an L2.
@@ -318,6 +356,21 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
 {
   if (a.nleaf <= 0 || a.N <= 0) return BBFMM_OK;
   int rhs = 0;
+  if (a.leafof && !a.phit && !a.phin) {
+    const int nb = (a.N + 255) / 256;
+    while (rhs < a.nrhs) {
+      const int left = a.nrhs - rhs;
+      if (left >= 4) {
+        hipLaunchKernelGGL(k_p2p_lane<4>, dim3(nb), dim3(256), 0, s, a, rhs);
+        rhs += 4;
+      } else {
+        hipLaunchKernelGGL(k_p2p_lane<1>, dim3(nb), dim3(256), 0, s, a, rhs);
+        rhs += 1;
+      }
+      BB_HIP(hipGetLastError());
+    }
+    return BBFMM_OK;
+  }
   while (rhs < a.nrhs) {
     const int left = a.nrhs - rhs;
     if (left >= 16) {

@@ -226,6 +226,7 @@ __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
 
 #ifdef SINGLE
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // v[k] for a lane-dependent k in 0..3 without a dynamically indexed array
 __device__ __forceinline__ float pick4(const float *v, int k)
@@ -1111,6 +1112,115 @@ static int vec_group(int nrhs)
   return nrhs % 16 == 0 ? 16 : nrhs % 4 == 0 ? 4 : 1;
 }
 
+#ifdef SINGLE
+// P2M of 32 leaves per wave (n = 4, r = 32; sparse forests, where leaves hold
+// a few particles): lane l takes leaf j = l % 32 of the wave and the nodes
+// 2m + h (h = l / 32, m = 0..31) and sums S over its leaf's particles
+// (anotherUpwardPass.c:61-330, as k_p2m); then M = U^T (Kw .* S) on the matrix
+// unit: 32 k-steps of v_mfma_f32_32x32x2_f32 with A = U^T (coefficient l % 32,
+// node 2s + h) and B = (Kw .* S)[node 2s + h][leaf j], the lane's entry m = s.
+__global__ void __launch_bounds__(256) k_p2m_cols(PassArgs a)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 31, h = lane >> 5;
+  const int leaf = (blockIdx.x * 4 + wave) * 32 + j;
+  const int rhs = blockIdx.y;
+  const bool valid = leaf < a.nleaf;
+  float acc[32];
+#pragma unroll
+  for (int m = 0; m < 32; m++) acc[m] = 0.0f;
+  int A = 0;
+  if (valid) {
+    A = a.leafcell[leaf];
+    const float ih = a.leafihalf[leaf];
+    const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
+    const float4 c = a.center[A];
+    const float *q = a.q + (size_t)rhs * a.N + sta;
+    for (int p = 0; p < cnt; p++) {
+      const float4 x = a.pos[sta + p];
+      const float qv = q[p];
+      float tx[4], ty[4], tz[4];
+      cheb_row<4>(a.Tkz, ih * (x.x - c.x), tx);
+      cheb_row<4>(a.Tkz, ih * (x.y - c.y), ty);
+      cheb_row<4>(a.Tkz, ih * (x.z - c.z), tz);
+      float P[16];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) P[4 * u + v] = qv * tx[u] * ty[v];
+      // node 2m + h: x index m / 8, y index (m / 2) % 4, z index 2 (m % 2) + h
+      const float z0 = h ? tz[1] : tz[0], z1 = h ? tz[3] : tz[2];
+#pragma unroll
+      for (int m = 0; m < 32; m++) acc[m] += P[4 * (m / 8) + (m / 2) % 4] * ((m & 1) ? z1 : z0);
+    }
+    float *Sp = a.S + ((size_t)rhs * a.ncell + A) * 64 + h;
+#pragma unroll
+    for (int m = 0; m < 32; m++) Sp[2 * m] = a.prefac3 * acc[m];
+  }
+  f32x16 mc;
+#pragma unroll
+  for (int e = 0; e < 16; e++) mc[e] = 0.0f;
+#pragma unroll
+  for (int m = 0; m < 32; m++) {
+    const float u = a.Urm[(2 * m + h) * 32 + j];
+    const float b = a.prefac3 * acc[m] * a.Kw[2 * m + h];
+    mc = __builtin_amdgcn_mfma_f32_32x32x2f32(u, b, mc, 0, 0, 0);
+  }
+  if (!valid) return;
+  // lane: rows 8 (e / 4) + 4 h + e % 4 of its leaf's column
+  float *Mp = a.M + ((size_t)rhs * a.ncell + A) * 32 + 4 * h;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    *reinterpret_cast<f32x4 *>(Mp + 8 * q) = f32x4{mc[4 * q], mc[4 * q + 1], mc[4 * q + 2], mc[4 * q + 3]};
+}
+
+// post-M2L of 32 cells per wave (n = 4, r = 32; sparse forests):
+// F = scale_l Kw .* (U L) (auxAnotherFMMInteraction.c:398-416, as k_postm2l)
+// is U (64 nodes x 32) times the 32 cells' L columns: two row tiles of 32
+// nodes, 16 k-steps of v_mfma_f32_32x32x2_f32.  Lane l: cell j = l % 32 of the
+// wave, h = l / 32; A = U[32 t + j][2 s + h], B = L[cell j][2 s + h].
+__global__ void __launch_bounds__(256) k_postm2l_cols(PassArgs a)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 31, h = lane >> 5;
+  const int A = (blockIdx.x * 4 + wave) * 32 + j;
+  const int rhs = blockIdx.y;
+  const int level = A < a.ncell && A < a.cell_end ? a.celllev[A] : -1;
+  const bool act = level >= kMinLev;
+  // the lane's coefficients 2 k + h (cells without F read cell 0, unused)
+  const float *Lr = a.L + ((size_t)rhs * a.ncell + (act ? A : 0)) * 32 + h;
+  float lv[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) lv[k] = Lr[2 * k];
+  f32x16 d0, d1;
+#pragma unroll
+  for (int e = 0; e < 16; e++) d0[e] = d1[e] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const float b = lv[k];
+    const float *Uc = a.Ucm + 64 * (2 * k + h);
+    d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Uc[j], b, d0, 0, 0, 0);
+    d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Uc[32 + j], b, d1, 0, 0, 0);
+  }
+  if (!act) return;
+  const float sc = a.scale[level];
+  float *Fp = a.F + ((size_t)rhs * a.ncell + A) * 64 + 4 * h;
+  // lane: rows 8 (e / 4) + 4 h + e % 4 of its cell's column, per row tile
+  auto store = [&](const f32x16 d, int t) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int l = 32 * t + 8 * q + 4 * h;
+      f32x4 o;
+#pragma unroll
+      for (int e = 0; e < 4; e++) o[e] = sc * a.Kw[l + e] * d[4 * q + e];
+      *reinterpret_cast<f32x4 *>(Fp + 32 * t + 8 * q) = o;
+    }
+  };
+  store(d0, 0);
+  store(d1, 1);
+}
+#endif
+
 template <int NN, int BLOCK>
 static void p2m_t(const PassArgs &a, hipStream_t s, int nrhs)
 {
@@ -1118,6 +1228,10 @@ static void p2m_t(const PassArgs &a, hipStream_t s, int nrhs)
 #ifdef SINGLE
   if (a.rows && NN == 4 && a.r == 32) {
     hipLaunchKernelGGL(k_p2m_rows_mfma4, dim3((a.nleaf + 3) / 4), dim3(256), 0, s, a);
+    return;
+  }
+  if (a.cols && NN == 4 && a.r == 32) {
+    hipLaunchKernelGGL(k_p2m_cols, dim3((a.nleaf + 127) / 128, nrhs), dim3(256), 0, s, a);
     return;
   }
 #endif
@@ -1182,6 +1296,14 @@ static int postm2l_t(const PassArgs &a, hipStream_t s, int nrhs)
 
 int launch_postm2l(const PassArgs &a, hipStream_t s, int nrhs)
 {
+#ifdef SINGLE
+  if (a.cols && a.n == 4 && a.r == 32 && nrhs != 16) {
+    if (a.ncell <= 0) return BBFMM_OK;
+    hipLaunchKernelGGL(k_postm2l_cols, dim3((a.ncell + 127) / 128, nrhs), dim3(256), 0, s, a);
+    BB_HIP(hipGetLastError());
+    return BBFMM_OK;
+  }
+#endif
   if (a.n == 4 && nrhs == 16 && a.r == 32) {
     if (a.ncell <= 0) return BBFMM_OK;
     hipLaunchKernelGGL(k_postm2l_v16, dim3((a.ncell + 3) / 4), dim3(256), 0, s, a);

@@ -47,6 +47,50 @@ def test_batch_against_oracle_per_instance(prec, nrhs, ops4, cuda):
     plan.close()
 
 
+def _sparse_instances():
+    from bbfmmongpu_amd import workloads as W
+    rng = np.random.default_rng(7)
+    return [(W.sphere_surface(6000, rng), 5), (W.plummer(4000, rng), 4), (W.sphere_surface(2000, rng), 4),
+            (W.uniform(1500, rng), 4)]
+
+
+def _leaves(inst):
+    total = 0
+    for p, d in inst:
+        c = np.clip(np.floor((p + 0.5) * (1 << d)).astype(np.int64), 0, (1 << d) - 1)
+        total += np.unique((c[:, 0] << (2 * d)) + (c[:, 1] << d) + c[:, 2]).size
+    return total
+
+
+@pytest.mark.parametrize("prec,nrhs", [("s", 1), ("d", 1), ("s", 5)])
+def test_batch_sparse_leaves_against_oracle(prec, nrhs, ops4, cuda):
+    """About 3 points per leaf: P2P runs a lane per target particle (one and
+    four vectors at a time); per instance against the oracle."""
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    inst = _sparse_instances()
+    pos_list = [p for p, _ in inst]
+    depths = [d for _, d in inst]
+    Ntot = sum(p.shape[0] for p in pos_list)
+    assert Ntot < 12 * _leaves(inst)  # the lane-per-particle P2P is chosen below 12 points per leaf
+    Q = np.random.default_rng(8).uniform(-1.0, 1.0, size=(nrhs, Ntot))
+    dt = np.float32 if prec == "s" else np.float64
+    plan = B.Plan.batch(pos_list, 4, U, K, nrhs=nrhs, depths=depths, prec=prec)
+    phi = plan.solve_host(Q.astype(dt)).reshape(nrhs, Ntot)
+    tol = 1e-5 if prec == "s" else 1e-12
+    o = 0
+    for (p, d) in inst:
+        n = p.shape[0]
+        tree = O.Tree(p, 0, prec, maxlev=d)
+        for j in sorted({0, nrhs - 1}):
+            ref = O.fmm(tree, Q[j, o:o + n].astype(dt), 4, U.astype(dt), K.astype(dt), scheme="basic")
+            err = O.rel_l2(ref["phi"], phi[j, o:o + n])
+            print(f"sparse instance N={n} depth={d} rhs={j}: {err:.2e}", flush=True)
+            assert err < tol
+        o += n
+    plan.close()
+
+
 def test_batch_auto_depth_against_direct(ops4, cuda):
     import bbfmmongpu_amd as B
     from bbfmmongpu_amd import workloads as W
