@@ -106,6 +106,12 @@ struct bbfmm_amd_plan {
   // and the per-form operator layout (created on first use)
   int h_cls_off[9] = {};
   int *d_cls_cells = nullptr, *d_vsrc = nullptr;
+  // downward pass of sparse forests on MFMA (n = 4, r = 32, fp32): the cells
+  // of levels >= 2 grouped by (level, class) as (cell, parent), waves of up
+  // to 32 of them per level, and the operators Kw .* U and the 8 L2L maps
+  int2 *d_dcp = nullptr, *d_dwv = nullptr;
+  real *d_KU = nullptr, *d_T8 = nullptr;
+  std::vector<int> h_dwv_off;  // [maxlev + 2] first wave of each level
   bb::ListM2L *list = nullptr;
   int list_form = -1;
   // per-stage timing: a ring of `nsets` event sets, one set per solve,
@@ -181,7 +187,7 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
                   p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad, p->d_phit, p->d_ipad, p->d_phin,
                   p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf, p->d_cls_cells,
-                  p->d_vsrc};
+                  p->d_vsrc, p->d_dcp, p->d_dwv, p->d_KU, p->d_T8};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
@@ -406,6 +412,56 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
     for (int i = 0; i < N; i++) ipad[p->h_perm[i]] = pad_of[i];
     BB_TRY(upload(&p->d_ipad, ipad.data(), ipad.size(), stream));
   }
+  // sparse forests, n = 4, r = 32: tables of the fused post-M2L + L2L kernel
+  std::vector<int2> dcp, dwv;
+  std::vector<real> KU, T8;
+#ifdef SINGLE
+  if (!f.complete && n == 4 && r == 32) {
+    std::vector<int> parent((size_t)ncell, -1), cls((size_t)ncell, 0);
+    for (int P = 0; P < ncell; P++)
+      for (int c = 0; c < 8; c++) {
+        const int ch = f.child[(size_t)8 * P + c];
+        if (ch >= 0) {
+          parent[ch] = P;
+          cls[ch] = c;
+        }
+      }
+    const int nl = f.maxdepth + 1;
+    std::vector<std::vector<int>> bucket((size_t)nl * 8);
+    for (int A = 0; A < ncell; A++)
+      if (f.celllev[A] >= kMinLev) bucket[(size_t)f.celllev[A] * 8 + cls[A]].push_back(A);
+    p->h_dwv_off.assign(nl + 1, 0);
+    for (int l = 0; l < nl; l++) {
+      p->h_dwv_off[l] = (int)dwv.size();
+      for (int c = 0; c < 8; c++) {
+        const std::vector<int> &b = bucket[(size_t)l * 8 + c];
+        const int start = (int)dcp.size();
+        for (int A : b) dcp.push_back(make_int2(A, parent[A]));
+        for (int k = 0; k < (int)b.size(); k += 32)
+          dwv.push_back(make_int2(start + k, (c << 8) | std::min(32, (int)b.size() - k)));
+      }
+    }
+    p->h_dwv_off[nl] = (int)dwv.size();
+    KU.resize((size_t)n3 * r);
+    for (int i = 0; i < r; i++)
+      for (int c = 0; c < n3; c++) KU[(size_t)c + (size_t)n3 * i] = Kw[c] * U[(size_t)c + (size_t)n3 * i];
+    const real pf = (real)2 / (real)n, pf3 = pf * pf * pf;
+    T8.resize((size_t)8 * n3 * n3);
+    for (int c8 = 0; c8 < 8; c8++) {
+      const int xb = c8 >> 2, yb = (c8 >> 1) & 1, zb = c8 & 1;
+      for (int pn = 0; pn < n3; pn++)
+        for (int cn = 0; cn < n3; cn++) {
+          const int px = pn / 16, py = (pn / 4) % 4, pz = pn % 4, cx = cn / 16, cy = (cn / 4) % 4, cz = cn % 4;
+          T8[((size_t)c8 * n3 + pn) * n3 + cn] =
+              pf3 * Cw[xb * 16 + px * 4 + cx] * Cw[yb * 16 + py * 4 + cy] * Cw[zb * 16 + pz * 4 + cz];
+        }
+    }
+    BB_TRY(upload(&p->d_dcp, dcp.data(), dcp.size(), stream));
+    BB_TRY(upload(&p->d_dwv, dwv.data(), dwv.size(), stream));
+    BB_TRY(upload(&p->d_KU, KU.data(), KU.size(), stream));
+    BB_TRY(upload(&p->d_T8, T8.data(), T8.size(), stream));
+  }
+#endif
   const size_t NR = (size_t)N * nrhs;
   BB_TRY(dalloc(&p->d_q, NR));
   BB_TRY(dalloc(&p->d_phi, NR));
@@ -481,6 +537,10 @@ static PassArgs pass_args(const bbfmm_amd_plan *p, bool rows = false)
   a.potential = p->potential;
 #ifdef SINGLE
   a.cols = !p->complete && p->n == 4 && p->r == 32;
+  a.dcp = p->d_dcp;
+  a.dwv = p->d_dwv;
+  a.KU = p->d_KU;
+  a.T8 = p->d_T8;
 #endif
   if (rows) {
     a.rows = 1;
@@ -676,9 +736,15 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
   BB_TRY(run_m2l(p, p->d_M, p->d_L, tm ? stage_event(p, ST_M2L) : nullptr));
   BB_TRY(fork_near());
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_POST), s));
-  BB_TRY(launch_postm2l(a, s, p->nrhs));
-  if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2L), s));
-  BB_TRY(launch_l2l(a, p->h_ploff.data(), s, p->nrhs));
+  if (a.dcp) {
+    // sparse forests: post-M2L and L2L level by level in one product per cell
+    BB_TRY(launch_down_cols(a, p->h_dwv_off.data(), s, p->nrhs));
+    if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2L), s));
+  } else {
+    BB_TRY(launch_postm2l(a, s, p->nrhs));
+    if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2L), s));
+    BB_TRY(launch_l2l(a, p->h_ploff.data(), s, p->nrhs));
+  }
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2P), s));
   if (fuse)
     BB_TRY(launch_down_leaf(a, p->h_ploff.data(), s));

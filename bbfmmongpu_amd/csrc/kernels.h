@@ -47,6 +47,12 @@ struct PassArgs {
   // sparse forests, n = 4, r = 32, fp32: P2M and post-M2L take 32 cells
   // (leaves) per wave as the columns of MFMA products
   int cols = 0;
+  // sparse forests, n = 4, r = 32, fp32: post-M2L and L2L fused level by
+  // level (launch_down_cols); null otherwise
+  const int2 *dcp = nullptr;  // cells of levels >= 2 by (level, class): (cell, parent)
+  const int2 *dwv = nullptr;  // waves: (first entry of dcp, class << 8 | count <= 32)
+  const real *KU = nullptr;   // Kw .* U, column-major (64 x 32)
+  const real *T8 = nullptr;   // (2/n)^3 L2L maps of the 8 child classes, [class][parent node][child node]
 };
 
 int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
@@ -64,6 +70,10 @@ int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs);
 int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs);
 int launch_postm2l(const PassArgs &a, hipStream_t s, int nrhs);
 int launch_l2l(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs);
+// sparse forests (a.dcp set): per level l >= 2, F = scale_l Kw .* (U L) plus
+// the L2L of the parent's F, for every cell of the level; h_dwv_off[l] is the
+// first wave of level l
+int launch_down_cols(const PassArgs &a, const int *h_dwv_off, hipStream_t s, int nrhs);
 int launch_l2p(const PassArgs &a, hipStream_t s, int nrhs);
 // particle rows, n = 4, complete tree, maxlev >= 3: post-M2L of the leaves,
 // the last L2L level and L2P in one kernel (set cell_end = first leaf cell

@@ -1219,7 +1219,81 @@ __global__ void __launch_bounds__(256) k_postm2l_cols(PassArgs a)
   store(d0, 0);
   store(d1, 1);
 }
+
+// Downward pass of sparse forests, one level at a time (n = 4, r = 32): for
+// the 32 cells of one child class c in a wave,
+//   F = scale_l Kw .* (U L) + (2/n)^3 T_c F_parent
+// (post-M2L, auxAnotherFMMInteraction.c:398-416, and L2L,
+// anotherDownwardPassX.c:40-211; T_c = Cx (x) Cy (x) Cz of the class) as one
+// product [Kw .* U | T_c] (64 x 96) times the columns [scale_l L ; F_parent]:
+// two row tiles of 32 nodes, 16 + 32 k-steps of v_mfma_f32_32x32x2_f32 (no
+// parent term at level 2).  Lane l: cell j = l % 32, h = l / 32; the parents'
+// F are final since the levels run in order.
+__global__ void __launch_bounds__(256) k_down_cols(PassArgs a, int w0, int w1, int level)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 31, h = lane >> 5;
+  const int w = w0 + blockIdx.x * 4 + wave;
+  if (w >= w1) return;  // whole wave; no block-wide barrier below
+  const int2 wv = a.dwv[w];
+  const int c = wv.y >> 8, cnt = wv.y & 255;
+  const bool valid = j < cnt;
+  const int2 cp = a.dcp[wv.x + (valid ? j : 0)];
+  const int rhs = blockIdx.y;
+  const float sc = a.scale[level];
+  const float *Lr = a.L + ((size_t)rhs * a.ncell + cp.x) * 32 + h;
+  float lv[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) lv[k] = sc * Lr[2 * k];
+  f32x16 d0, d1;
+#pragma unroll
+  for (int e = 0; e < 16; e++) d0[e] = d1[e] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const float *Uc = a.KU + 64 * (2 * k + h);
+    d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Uc[j], lv[k], d0, 0, 0, 0);
+    d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Uc[32 + j], lv[k], d1, 0, 0, 0);
+  }
+  if (level > kMinLev) {
+    const float *Fr = a.F + ((size_t)rhs * a.ncell + cp.y) * 64 + h;
+    const float *T = a.T8 + (size_t)c * 64 * 64;
+    float fv[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) fv[k] = Fr[2 * k];
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      const float *Tc = T + 64 * (2 * k + h);
+      d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Tc[j], fv[k], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Tc[32 + j], fv[k], d1, 0, 0, 0);
+    }
+  }
+  if (!valid) return;
+  float *Fp = a.F + ((size_t)rhs * a.ncell + cp.x) * 64 + 4 * h;
+  // lane: rows 8 (e / 4) + 4 h + e % 4 of its cell's column, per row tile
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    *reinterpret_cast<f32x4 *>(Fp + 8 * q) = f32x4{d0[4 * q], d0[4 * q + 1], d0[4 * q + 2], d0[4 * q + 3]};
+    *reinterpret_cast<f32x4 *>(Fp + 32 + 8 * q) = f32x4{d1[4 * q], d1[4 * q + 1], d1[4 * q + 2], d1[4 * q + 3]};
+  }
+}
 #endif
+
+int launch_down_cols(const PassArgs &a, const int *h_dwv_off, hipStream_t s, int nrhs)
+{
+#ifdef SINGLE
+  if (!a.dcp || a.n != 4 || a.r != 32) return set_error(BBFMM_E_ARG, "internal: no sparse downward tables");
+  for (int level = kMinLev; level <= a.maxlev; level++) {
+    const int w0 = h_dwv_off[level], w1 = h_dwv_off[level + 1];
+    if (w1 <= w0) continue;
+    hipLaunchKernelGGL(k_down_cols, dim3((w1 - w0 + 3) / 4, nrhs), dim3(256), 0, s, a, w0, w1, level);
+    BB_HIP(hipGetLastError());
+  }
+  return BBFMM_OK;
+#else
+  (void)a; (void)h_dwv_off; (void)s; (void)nrhs;
+  return set_error(BBFMM_E_ARG, "the sparse downward pass on MFMA needs the fp32 build");
+#endif
+}
 
 template <int NN, int BLOCK>
 static void p2m_t(const PassArgs &a, hipStream_t s, int nrhs)
