@@ -1369,6 +1369,119 @@ int class_slot(int s, int vx, int vy, int vz)
   return slot[s * 343 + 49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
 }
 
+// Split-BF16 list M2L with the operator shared by the block: the 4 waves of
+// a block visit the union of their offset masks in step, and each offset's
+// operator fragments (4 KB) are loaded once per block (one 1 KB fragment per
+// wave, through registers into LDS, double-buffered) instead of once per
+// wave; the vector-memory pipe, which the per-wave kernel saturates with
+// 8 KB per offset and wave, then moves 5 KB.  Per step: the next offset's
+// fragment and source column are loaded while the current offset's six
+// products run; one barrier per step.  Same columns, order of products and
+// results as k_m2l_list_mfma<BBFMM_M2L_BF16X2> (the extra offsets a wave
+// visits add K_v times its zero sources).
+__global__ void __launch_bounds__(256) k_m2l_list_lds(ListArgs g)
+{
+  __shared__ bf16x8 sK[2][4][64];  // [buffer][fragment: k-step x hi/lo][lane]
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nb = g.blk0[8];
+  const int b = blockIdx.x;
+  const int lb = (b & 7) * ((nb + 7) >> 3) + (b >> 3);
+  if (lb >= nb) return;  // whole block
+  const int2 sb = list_block(g, lb);
+  const int s = sb.x, grp = sb.y;
+  const int ncols = (g.cls_off[s + 1] - g.cls_off[s]) * g.nrhs;
+  const int half = lane >> 5;
+  const int col = (grp * 4 + wave) * 32 + (lane & 31);
+  const bool valid = col < ncols;
+  const int cc = valid ? col : 0;
+  const int F = g.cls_cells[g.cls_off[s] + cc / g.nrhs];
+  const int rh = cc % g.nrhs;
+  const int4 *tab = g.tab + s * kOffsets;
+  const size_t rbase = (size_t)rh * (g.ncell + 1);
+  const int gw = __builtin_amdgcn_readfirstlane(lb * 4 + wave);
+  const int *vsl = g.vsw + (size_t)gw * kOffsets * 32 + (lane & 31);
+  auto vs = [&](int v) { return vsl[v * 32]; };
+  // the block's union of the 4 waves' masks
+  const unsigned long long *mk = g.mask + (size_t)lb * 16;
+  unsigned long long m0 = mk[0] | mk[4] | mk[8] | mk[12];
+  unsigned long long m1 = mk[1] | mk[5] | mk[9] | mk[13];
+  unsigned long long m2 = mk[2] | mk[6] | mk[10] | mk[14];
+  auto pop = [&]() -> int {
+    const bool a0 = m0 != 0, a1 = m1 != 0, a2 = m2 != 0;
+    const int t0 = a0 ? __builtin_ctzll(m0) : 0, t1 = a1 ? __builtin_ctzll(m1) : 0,
+              t2 = a2 ? __builtin_ctzll(m2) : 0;
+    const int v = a0 ? t0 : a1 ? 64 + t1 : a2 ? 128 + t2 : -1;
+    m0 = a0 ? m0 & (m0 - 1) : m0;
+    m1 = !a0 && a1 ? m1 & (m1 - 1) : m1;
+    m2 = !a0 && !a1 ? m2 & (m2 - 1) : m2;
+    return v;
+  };
+  // this wave's fragment of offset v's operator, and a column's source
+  auto load_k = [&](int v) {
+    const int k = tab[v].x;
+    return *((const f32x4 *)((const bf16x8 *)g.Ka + ((size_t)k * 4 + wave) * 64 + lane));
+  };
+  auto load_b = [&](int S, f32x4 *t) {
+    const size_t o = (rbase + S) * 32 + half * 8;
+    t[0] = *(const f32x4 *)((const __bf16 *)g.M0 + o);
+    t[1] = *(const f32x4 *)((const __bf16 *)g.M1 + o);
+    t[2] = *(const f32x4 *)((const __bf16 *)g.M0 + o + 16);
+    t[3] = *(const f32x4 *)((const __bf16 *)g.M1 + o + 16);
+  };
+  typedef __attribute__((ext_vector_type(8))) __bf16 b8;
+  f32x16 acc;
+#pragma unroll
+  for (int e = 0; e < 16; e++) acc[e] = 0.0f;
+  int v = pop();
+  if (v >= 0) {  // uniform over the block
+    f32x4 bc[4];
+    {
+      const f32x4 kf = load_k(v);
+      load_b(vs(v), bc);
+      *reinterpret_cast<f32x4 *>(&sK[0][wave][lane]) = kf;
+    }
+    int vn = pop();
+    int Sn = vs(max(vn, 0));
+    __syncthreads();
+    for (int buf = 0;; buf ^= 1) {
+      const b8 a0h = __builtin_bit_cast(b8, sK[buf][0][lane]), a0l = __builtin_bit_cast(b8, sK[buf][1][lane]);
+      const b8 a1h = __builtin_bit_cast(b8, sK[buf][2][lane]), a1l = __builtin_bit_cast(b8, sK[buf][3][lane]);
+      // next offset (offset 0 stands in past the last): operator fragment and
+      // source column, issued before this offset's products (kept there)
+      const f32x4 kn = load_k(max(vn, 0));
+      f32x4 bn[4];
+      load_b(Sn, bn);
+      const int vnn = pop();
+      const int Snn = vs(max(vnn, 0));
+      __builtin_amdgcn_sched_barrier(0);
+      const b8 b0h = __builtin_bit_cast(b8, bc[0]), b0l = __builtin_bit_cast(b8, bc[1]);
+      const b8 b1h = __builtin_bit_cast(b8, bc[2]), b1l = __builtin_bit_cast(b8, bc[3]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0h, b0l, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0l, b0h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1h, b1l, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1l, b1h, acc, 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (vn < 0) break;  // uniform over the block
+      *reinterpret_cast<f32x4 *>(&sK[buf ^ 1][wave][lane]) = kn;
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 4; q++) bc[q] = bn[q];
+      vn = vnn;
+      Sn = Snn;
+    }
+  }
+  if (!valid) return;
+  float *Lp = (float *)g.L + ((size_t)rh * g.ncell + F) * 32 + 4 * half;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    f32x4 o = {acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+    *reinterpret_cast<f32x4 *>(Lp + 8 * q) = o;
+  }
+}
+
 int list_m2l_create(ListM2L **out, int r, int nrhs, int form, const real *K, int ncell, const int *cls_off,
                     const int *d_cls_cells, const int *d_vsrc, hipStream_t s)
 {
@@ -1510,7 +1623,7 @@ int list_m2l_run(ListM2L *g, const real *M, real *L, hipStream_t s, hipEvent_t m
     hipLaunchKernelGGL(k_m2l_list_prep<BBFMM_M2L_BF16X2>, dim3(pb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (g->nblocks) hipLaunchKernelGGL(k_m2l_list_mfma<BBFMM_M2L_BF16X2>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+    if (g->nblocks) hipLaunchKernelGGL(k_m2l_list_lds, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
   } else {
     hipLaunchKernelGGL(k_m2l_list_prep<BBFMM_M2L_FP32MFMA>, dim3(pb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
