@@ -110,7 +110,8 @@ struct bbfmm_amd_plan {
   // of levels >= 2 grouped by (level, class) as (cell, parent), waves of up
   // to 32 of them per level, and the operators Kw .* U and the 8 L2L maps
   int2 *d_dcp = nullptr, *d_dwv = nullptr;
-  real *d_KU = nullptr, *d_T8 = nullptr;
+  real *d_KU = nullptr, *d_T8 = nullptr, *d_T8m = nullptr;
+  int *d_mplist = nullptr;  // parents of each level ordered by their set of children (M2M on MFMA)
   std::vector<int> h_dwv_off;  // [maxlev + 2] first wave of each level
   bb::ListM2L *list = nullptr;
   int list_form = -1;
@@ -187,7 +188,7 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
                   p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad, p->d_phit, p->d_ipad, p->d_phin,
                   p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf, p->d_cls_cells,
-                  p->d_vsrc, p->d_dcp, p->d_dwv, p->d_KU, p->d_T8};
+                  p->d_vsrc, p->d_dcp, p->d_dwv, p->d_KU, p->d_T8, p->d_T8m, p->d_mplist};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
@@ -414,7 +415,8 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
   }
   // sparse forests, n = 4, r = 32: tables of the fused post-M2L + L2L kernel
   std::vector<int2> dcp, dwv;
-  std::vector<real> KU, T8;
+  std::vector<real> KU, T8, T8m;
+  std::vector<int> mplist;
 #ifdef SINGLE
   if (!f.complete && n == 4 && r == 32) {
     std::vector<int> parent((size_t)ncell, -1), cls((size_t)ncell, 0);
@@ -460,6 +462,26 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
     BB_TRY(upload(&p->d_dwv, dwv.data(), dwv.size(), stream));
     BB_TRY(upload(&p->d_KU, KU.data(), KU.size(), stream));
     BB_TRY(upload(&p->d_T8, T8.data(), T8.size(), stream));
+    // M2M: the maps transposed per class ([class][child node][parent node]),
+    // and each level's parents ordered by their set of children, so that the
+    // 32 parents of a wave mostly share it and the wave skips the classes
+    // none of them has
+    T8m.resize(T8.size());
+    for (int c8 = 0; c8 < 8; c8++)
+      for (int pn = 0; pn < n3; pn++)
+        for (int cn = 0; cn < n3; cn++)
+          T8m[((size_t)c8 * n3 + cn) * n3 + pn] = T8[((size_t)c8 * n3 + pn) * n3 + cn];
+    BB_TRY(upload(&p->d_T8m, T8m.data(), T8m.size(), stream));
+    mplist = f.plist;
+    auto cmask = [&](int P) {
+      int m = 0;
+      for (int c = 0; c < 8; c++) m |= (f.child[(size_t)8 * P + c] >= 0) << c;
+      return m;
+    };
+    for (size_t l = 0; l + 1 < f.ploff.size(); l++)
+      std::stable_sort(mplist.begin() + f.ploff[l], mplist.begin() + f.ploff[l + 1],
+                       [&](int x, int y) { return cmask(x) < cmask(y); });
+    if (!mplist.empty()) BB_TRY(upload(&p->d_mplist, mplist.data(), mplist.size(), stream));
   }
 #endif
   const size_t NR = (size_t)N * nrhs;
@@ -541,6 +563,8 @@ static PassArgs pass_args(const bbfmm_amd_plan *p, bool rows = false)
   a.dwv = p->d_dwv;
   a.KU = p->d_KU;
   a.T8 = p->d_T8;
+  a.T8m = p->d_T8m;
+  a.mplist = p->d_mplist;
 #endif
   if (rows) {
     a.rows = 1;

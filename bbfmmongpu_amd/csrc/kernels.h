@@ -53,6 +53,9 @@ struct PassArgs {
   const int2 *dwv = nullptr;  // waves: (first entry of dcp, class << 8 | count <= 32)
   const real *KU = nullptr;   // Kw .* U, column-major (64 x 32)
   const real *T8 = nullptr;   // (2/n)^3 L2L maps of the 8 child classes, [class][parent node][child node]
+  // sparse forests, n = 4, r = 32, fp32: M2M on MFMA (k_m2m_cols); null otherwise
+  const real *T8m = nullptr;    // the maps as [class][child node][parent node]
+  const int *mplist = nullptr;  // plist with each level's parents ordered by their set of children
 };
 
 int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);

@@ -1276,6 +1276,72 @@ __global__ void __launch_bounds__(256) k_down_cols(PassArgs a, int w0, int w1, i
     *reinterpret_cast<f32x4 *>(Fp + 32 + 8 * q) = f32x4{d1[4 * q], d1[4 * q + 1], d1[4 * q + 2], d1[4 * q + 3]};
   }
 }
+
+// M2M of sparse forests, one level (n = 4, r = 32): 32 parents per wave, in
+// the order of mplist (parents with the same set of children together);
+// S_parent = sum_c T_c^T S_child(c) (anotherUpwardPass.c, as k_m2m) as
+// products over the child classes some parent of the wave has
+// (v_mfma_f32_32x32x2_f32, A = T_c^T, B = the children's S, zero for absent
+// children), then M = U^T (Kw .* S_parent) on the matrix unit with the
+// contraction ordered as the accumulators hold the nodes: lane (j, h) owns
+// nodes 32 t + 8 q + 4 h + e of parent j, so k-step (t, q, e) pairs node
+// 32 t + 8 q + e (h = 0) with node 32 t + 8 q + 4 + e (h = 1).
+__global__ void __launch_bounds__(256) k_m2m_cols(PassArgs a, int off, int count)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane & 31, h = lane >> 5;
+  const int base = (blockIdx.x * 4 + wave) * 32;
+  if (base >= count) return;  // whole wave; no block-wide barrier below
+  const bool valid = base + j < count;
+  const int P = a.mplist[off + (valid ? base + j : base)];
+  const size_t rs = (size_t)blockIdx.y * a.ncell;
+  f32x16 d0, d1;
+#pragma unroll
+  for (int e = 0; e < 16; e++) d0[e] = d1[e] = 0.0f;
+  for (int c = 0; c < 8; c++) {
+    const int ch = valid ? a.child[(size_t)8 * P + c] : -1;
+    if (__ballot(ch >= 0) == 0) continue;  // no parent of the wave has this child
+    const float *Sr = a.S + (rs + (ch >= 0 ? ch : P)) * 64 + h;
+    float sv[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      const float v = Sr[2 * k];
+      sv[k] = ch >= 0 ? v : 0.0f;
+    }
+    const float *Tm = a.T8m + (size_t)c * 64 * 64;
+#pragma unroll
+    for (int k = 0; k < 32; k++) {
+      const float *Tc = Tm + 64 * (2 * k + h);
+      d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Tc[j], sv[k], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Tc[32 + j], sv[k], d1, 0, 0, 0);
+    }
+  }
+  if (valid) {
+    float *Sp = a.S + (rs + P) * 64 + 4 * h;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      *reinterpret_cast<f32x4 *>(Sp + 8 * q) = f32x4{d0[4 * q], d0[4 * q + 1], d0[4 * q + 2], d0[4 * q + 3]};
+      *reinterpret_cast<f32x4 *>(Sp + 32 + 8 * q) = f32x4{d1[4 * q], d1[4 * q + 1], d1[4 * q + 2], d1[4 * q + 3]};
+    }
+  }
+  f32x16 mc;
+#pragma unroll
+  for (int e = 0; e < 16; e++) mc[e] = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int node = 8 * q + 4 * h + e;
+      mc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.Urm[node * 32 + j], a.Kw[node] * d0[4 * q + e], mc, 0, 0, 0);
+      mc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.Urm[(32 + node) * 32 + j], a.Kw[32 + node] * d1[4 * q + e], mc,
+                                                0, 0, 0);
+    }
+  if (!valid) return;
+  float *Mp = a.M + (rs + P) * 32 + 4 * h;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    *reinterpret_cast<f32x4 *>(Mp + 8 * q) = f32x4{mc[4 * q], mc[4 * q + 1], mc[4 * q + 2], mc[4 * q + 3]};
+}
 #endif
 
 int launch_down_cols(const PassArgs &a, const int *h_dwv_off, hipStream_t s, int nrhs)
@@ -1341,6 +1407,10 @@ int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs)
     if (count <= 0) continue;
     if (a.n == 4 && nrhs == 16)
       hipLaunchKernelGGL(k_m2m_v16, dim3(count), dim3(1024), 0, s, a, off);
+#ifdef SINGLE
+    else if (a.mplist && a.T8m && a.n == 4 && a.r == 32)
+      hipLaunchKernelGGL(k_m2m_cols, dim3((count + 127) / 128, nrhs), dim3(256), 0, s, a, off, count);
+#endif
     else if (a.n == 4)
       hipLaunchKernelGGL((k_m2m<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, off);
     else
