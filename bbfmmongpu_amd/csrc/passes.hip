@@ -1241,28 +1241,38 @@ __global__ void __launch_bounds__(256) k_down_cols(PassArgs a, int w0, int w1, i
   const int2 cp = a.dcp[wv.x + (valid ? j : 0)];
   const int rhs = blockIdx.y;
   const float sc = a.scale[level];
-  const float *Lr = a.L + ((size_t)rhs * a.ncell + cp.x) * 32 + h;
+  // k-step k pairs coefficient k (h = 0) with 16 + k (h = 1), and parent node
+  // k with 32 + k, so that a lane's operands are contiguous in memory
+  const f32x4 *Lr = reinterpret_cast<const f32x4 *>(a.L + ((size_t)rhs * a.ncell + cp.x) * 32 + 16 * h);
   float lv[16];
 #pragma unroll
-  for (int k = 0; k < 16; k++) lv[k] = sc * Lr[2 * k];
+  for (int q = 0; q < 4; q++) {
+    const f32x4 v = Lr[q];
+#pragma unroll
+    for (int e = 0; e < 4; e++) lv[4 * q + e] = sc * v[e];
+  }
   f32x16 d0, d1;
 #pragma unroll
   for (int e = 0; e < 16; e++) d0[e] = d1[e] = 0.0f;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    const float *Uc = a.KU + 64 * (2 * k + h);
+    const float *Uc = a.KU + 64 * (16 * h + k);
     d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Uc[j], lv[k], d0, 0, 0, 0);
     d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Uc[32 + j], lv[k], d1, 0, 0, 0);
   }
   if (level > kMinLev) {
-    const float *Fr = a.F + ((size_t)rhs * a.ncell + cp.y) * 64 + h;
+    const f32x4 *Fr = reinterpret_cast<const f32x4 *>(a.F + ((size_t)rhs * a.ncell + cp.y) * 64 + 32 * h);
     const float *T = a.T8 + (size_t)c * 64 * 64;
     float fv[32];
 #pragma unroll
-    for (int k = 0; k < 32; k++) fv[k] = Fr[2 * k];
+    for (int q = 0; q < 8; q++) {
+      const f32x4 v = Fr[q];
+#pragma unroll
+      for (int e = 0; e < 4; e++) fv[4 * q + e] = v[e];
+    }
 #pragma unroll
     for (int k = 0; k < 32; k++) {
-      const float *Tc = T + 64 * (2 * k + h);
+      const float *Tc = T + 64 * (32 * h + k);
       d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Tc[j], fv[k], d0, 0, 0, 0);
       d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Tc[32 + j], fv[k], d1, 0, 0, 0);
     }
@@ -1282,7 +1292,7 @@ __global__ void __launch_bounds__(256) k_down_cols(PassArgs a, int w0, int w1, i
 // S_parent = sum_c T_c^T S_child(c) (anotherUpwardPass.c, as k_m2m) as
 // products over the child classes some parent of the wave has
 // (v_mfma_f32_32x32x2_f32, A = T_c^T, B = the children's S, zero for absent
-// children), then M = U^T (Kw .* S_parent) on the matrix unit with the
+// children; k-step k pairs child nodes k and 32 + k), then M = U^T (Kw .* S_parent) on the matrix unit with the
 // contraction ordered as the accumulators hold the nodes: lane (j, h) owns
 // nodes 32 t + 8 q + 4 h + e of parent j, so k-step (t, q, e) pairs node
 // 32 t + 8 q + e (h = 0) with node 32 t + 8 q + 4 + e (h = 1).
@@ -1301,17 +1311,19 @@ __global__ void __launch_bounds__(256) k_m2m_cols(PassArgs a, int off, int count
   for (int c = 0; c < 8; c++) {
     const int ch = valid ? a.child[(size_t)8 * P + c] : -1;
     if (__ballot(ch >= 0) == 0) continue;  // no parent of the wave has this child
-    const float *Sr = a.S + (rs + (ch >= 0 ? ch : P)) * 64 + h;
+    // k-step k pairs child node k (h = 0) with 32 + k (h = 1): contiguous rows
+    const f32x4 *Sr = reinterpret_cast<const f32x4 *>(a.S + (rs + (ch >= 0 ? ch : P)) * 64 + 32 * h);
     float sv[32];
 #pragma unroll
-    for (int k = 0; k < 32; k++) {
-      const float v = Sr[2 * k];
-      sv[k] = ch >= 0 ? v : 0.0f;
+    for (int q = 0; q < 8; q++) {
+      const f32x4 v = Sr[q];
+#pragma unroll
+      for (int e = 0; e < 4; e++) sv[4 * q + e] = ch >= 0 ? v[e] : 0.0f;
     }
     const float *Tm = a.T8m + (size_t)c * 64 * 64;
 #pragma unroll
     for (int k = 0; k < 32; k++) {
-      const float *Tc = Tm + 64 * (2 * k + h);
+      const float *Tc = Tm + 64 * (32 * h + k);
       d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Tc[j], sv[k], d0, 0, 0, 0);
       d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Tc[32 + j], sv[k], d1, 0, 0, 0);
     }
