@@ -1115,10 +1115,10 @@ static int vec_group(int nrhs)
 #ifdef SINGLE
 // P2M of 32 leaves per wave (n = 4, r = 32; sparse forests, where leaves hold
 // a few particles): lane l takes leaf j = l % 32 of the wave and the nodes
-// 2m + h (h = l / 32, m = 0..31) and sums S over its leaf's particles
+// 32h + m (h = l / 32, m = 0..31) and sums S over its leaf's particles
 // (anotherUpwardPass.c:61-330, as k_p2m); then M = U^T (Kw .* S) on the matrix
 // unit: 32 k-steps of v_mfma_f32_32x32x2_f32 with A = U^T (coefficient l % 32,
-// node 2s + h) and B = (Kw .* S)[node 2s + h][leaf j], the lane's entry m = s.
+// node 32h + s) and B = (Kw .* S)[node 32h + s][leaf j], the lane's entry m = s.
 __global__ void __launch_bounds__(256) k_p2m_cols(PassArgs a)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1143,27 +1143,30 @@ __global__ void __launch_bounds__(256) k_p2m_cols(PassArgs a)
       cheb_row<4>(a.Tkz, ih * (x.x - c.x), tx);
       cheb_row<4>(a.Tkz, ih * (x.y - c.y), ty);
       cheb_row<4>(a.Tkz, ih * (x.z - c.z), tz);
-      float P[16];
+      // node 32h + m: x index 2h + m / 16, y index (m / 4) % 4, z index m % 4
+      const float x0 = h ? tx[2] : tx[0], x1 = h ? tx[3] : tx[1];
+      float P[8];
 #pragma unroll
-      for (int u = 0; u < 4; u++)
+      for (int v = 0; v < 4; v++) {
+        P[v] = qv * x0 * ty[v];
+        P[4 + v] = qv * x1 * ty[v];
+      }
 #pragma unroll
-        for (int v = 0; v < 4; v++) P[4 * u + v] = qv * tx[u] * ty[v];
-      // node 2m + h: x index m / 8, y index (m / 2) % 4, z index 2 (m % 2) + h
-      const float z0 = h ? tz[1] : tz[0], z1 = h ? tz[3] : tz[2];
-#pragma unroll
-      for (int m = 0; m < 32; m++) acc[m] += P[4 * (m / 8) + (m / 2) % 4] * ((m & 1) ? z1 : z0);
+      for (int m = 0; m < 32; m++) acc[m] += P[4 * (m / 16) + (m / 4) % 4] * tz[m % 4];
     }
-    float *Sp = a.S + ((size_t)rhs * a.ncell + A) * 64 + h;
+    f32x4 *Sp = reinterpret_cast<f32x4 *>(a.S + ((size_t)rhs * a.ncell + A) * 64 + 32 * h);
 #pragma unroll
-    for (int m = 0; m < 32; m++) Sp[2 * m] = a.prefac3 * acc[m];
+    for (int q = 0; q < 8; q++)
+      Sp[q] = f32x4{a.prefac3 * acc[4 * q], a.prefac3 * acc[4 * q + 1], a.prefac3 * acc[4 * q + 2],
+                    a.prefac3 * acc[4 * q + 3]};
   }
   f32x16 mc;
 #pragma unroll
   for (int e = 0; e < 16; e++) mc[e] = 0.0f;
 #pragma unroll
   for (int m = 0; m < 32; m++) {
-    const float u = a.Urm[(2 * m + h) * 32 + j];
-    const float b = a.prefac3 * acc[m] * a.Kw[2 * m + h];
+    const float u = a.Urm[(32 * h + m) * 32 + j];
+    const float b = a.prefac3 * acc[m] * a.Kw[32 * h + m];
     mc = __builtin_amdgcn_mfma_f32_32x32x2f32(u, b, mc, 0, 0, 0);
   }
   if (!valid) return;
@@ -1178,7 +1181,7 @@ __global__ void __launch_bounds__(256) k_p2m_cols(PassArgs a)
 // F = scale_l Kw .* (U L) (auxAnotherFMMInteraction.c:398-416, as k_postm2l)
 // is U (64 nodes x 32) times the 32 cells' L columns: two row tiles of 32
 // nodes, 16 k-steps of v_mfma_f32_32x32x2_f32.  Lane l: cell j = l % 32 of the
-// wave, h = l / 32; A = U[32 t + j][2 s + h], B = L[cell j][2 s + h].
+// wave, h = l / 32; A = U[32 t + j][16 h + s], B = L[cell j][16 h + s].
 __global__ void __launch_bounds__(256) k_postm2l_cols(PassArgs a)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1187,18 +1190,23 @@ __global__ void __launch_bounds__(256) k_postm2l_cols(PassArgs a)
   const int rhs = blockIdx.y;
   const int level = A < a.ncell && A < a.cell_end ? a.celllev[A] : -1;
   const bool act = level >= kMinLev;
-  // the lane's coefficients 2 k + h (cells without F read cell 0, unused)
-  const float *Lr = a.L + ((size_t)rhs * a.ncell + (act ? A : 0)) * 32 + h;
+  // k-step k pairs coefficient k (h = 0) with 16 + k (h = 1): the lane's
+  // coefficients are contiguous (cells without F read cell 0, unused)
+  const f32x4 *Lr = reinterpret_cast<const f32x4 *>(a.L + ((size_t)rhs * a.ncell + (act ? A : 0)) * 32 + 16 * h);
   float lv[16];
 #pragma unroll
-  for (int k = 0; k < 16; k++) lv[k] = Lr[2 * k];
+  for (int q = 0; q < 4; q++) {
+    const f32x4 v = Lr[q];
+#pragma unroll
+    for (int e = 0; e < 4; e++) lv[4 * q + e] = v[e];
+  }
   f32x16 d0, d1;
 #pragma unroll
   for (int e = 0; e < 16; e++) d0[e] = d1[e] = 0.0f;
 #pragma unroll
   for (int k = 0; k < 16; k++) {
     const float b = lv[k];
-    const float *Uc = a.Ucm + 64 * (2 * k + h);
+    const float *Uc = a.Ucm + 64 * (16 * h + k);
     d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(Uc[j], b, d0, 0, 0, 0);
     d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(Uc[32 + j], b, d1, 0, 0, 0);
   }

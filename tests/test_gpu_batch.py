@@ -91,6 +91,29 @@ def test_batch_sparse_leaves_against_oracle(prec, nrhs, ops4, cuda):
     plan.close()
 
 
+def test_batch_sparse_fetch_matches_fp64(ops4, cuda):
+    """Sparse forests in fp32 run P2M, M2M and post-M2L on MFMA over 32 cells
+    per wave; the multipoles and the post-M2L F they fetch match the fp64
+    plan's (per-cell kernels) on the same instances."""
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    inst = _sparse_instances()
+    pos_list = [p for p, _ in inst]
+    depths = [d for _, d in inst]
+    Ntot = sum(p.shape[0] for p in pos_list)
+    q = np.random.default_rng(9).uniform(-1.0, 1.0, size=Ntot)
+    out = {}
+    for prec, dt in (("s", np.float32), ("d", np.float64)):
+        plan = B.Plan.batch(pos_list, 4, U, K, nrhs=1, depths=depths, prec=prec)
+        plan.solve_host(q.astype(dt))
+        out[prec] = {w: plan.fetch(w).astype(np.float64) for w in ("M", "L", "F")}
+        plan.close()
+    for w in ("M", "L", "F"):
+        err = O.rel_l2(out["d"][w], out["s"][w])
+        print(f"sparse fetch {w}: {err:.2e}", flush=True)
+        assert err < 1e-5
+
+
 def test_batch_auto_depth_against_direct(ops4, cuda):
     import bbfmmongpu_amd as B
     from bbfmmongpu_amd import workloads as W
