@@ -76,6 +76,7 @@ struct bbfmm_amd_plan {
   double m2l_interactions = 0, p2p_pairs = 0;
   // device
   real4 *d_pos = nullptr, *d_center = nullptr, *d_K4 = nullptr;
+  real4 *d_pq = nullptr;  // sparse forests: positions with one vector's charge in w (lane-per-particle P2P)
   int *d_perm = nullptr, *d_leafsta = nullptr, *d_leafcnt = nullptr, *d_leafof = nullptr;
   int *d_iperm = nullptr;  // original index -> leaf order
   int *d_celllev = nullptr, *d_child = nullptr, *d_plist = nullptr, *d_leafcell = nullptr;
@@ -188,7 +189,7 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
                   p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad, p->d_phit, p->d_ipad, p->d_phin,
                   p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf, p->d_cls_cells,
-                  p->d_vsrc, p->d_dcp, p->d_dwv, p->d_KU, p->d_T8, p->d_T8m, p->d_mplist};
+                  p->d_vsrc, p->d_dcp, p->d_dwv, p->d_KU, p->d_T8, p->d_T8m, p->d_mplist, p->d_pq};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
@@ -484,6 +485,7 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
     if (!mplist.empty()) BB_TRY(upload(&p->d_mplist, mplist.data(), mplist.size(), stream));
   }
 #endif
+  if (!f.complete && (long long)N < 12LL * nleaf) BB_TRY(dalloc(&p->d_pq, (size_t)N));
   const size_t NR = (size_t)N * nrhs;
   BB_TRY(dalloc(&p->d_q, NR));
   BB_TRY(dalloc(&p->d_phi, NR));
@@ -698,7 +700,10 @@ static P2PArgs p2p_args(const bbfmm_amd_plan *p)
   pp.nseg = p->d_nseg;
   pp.pad_of = p->d_pad_of;
   // sparse forests with a few particles per leaf: a lane per target particle
-  if (!p->complete && p->N < 12LL * p->nleaf) pp.leafof = p->d_leafof;
+  if (!p->complete && p->N < 12LL * p->nleaf) {
+    pp.leafof = p->d_leafof;
+    pp.pq = p->d_pq;
+  }
   return pp;
 }
 

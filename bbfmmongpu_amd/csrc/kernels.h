@@ -153,6 +153,7 @@ struct P2PArgs {
   // sparse forests (a few particles per leaf): a lane per target particle,
   // leafof[i] its leaf; used when set
   const int *leafof = nullptr;
+  real4 *pq = nullptr;  // [N] scratch: positions with one vector's charge in w (one load per source)
 };
 int launch_p2p(const P2PArgs &a, hipStream_t s);
 

@@ -147,6 +147,41 @@ __global__ void __launch_bounds__(256) k_p2p_lane(P2PArgs a, int rhs0)
   }
 }
 
+// The same for one vector, the sources read as (x, y, z, q) in one load.
+__global__ void __launch_bounds__(256) k_p2p_lane_pq(P2PArgs a, int rhs)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.N) return;
+  const int lf = a.leafof[i];
+  const real4 x = a.pq[i];
+  const int nn = a.nnbr[lf];
+  const int *nb = a.nbr + (size_t)lf * 27;
+  real acc = 0;
+  for (int b = 0; b < nn; b++) {
+    const int B = nb[b];
+    const int j0 = a.leafsta[B], j1 = j0 + a.leafcnt[B];
+    for (int j = j0; j < j1; j++) {
+      const real4 y = a.pq[j];
+      const real dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z;
+      const real rr = dx * dx + dy * dy + dz * dz;
+      const real w = rr != (real)0 ? rsqrt_r(rr) : (real)0;
+      acc += w * y.w;
+    }
+  }
+  const size_t o = (size_t)rhs * a.N + i;
+  a.phi[o] += (a.potential ? (real)1 : x.w) * acc;
+}
+
+// pq[i] = (pos[i].xyz, q[rhs][i])
+__global__ void k_pack_pq(int N, const real4 *__restrict__ pos, const real *__restrict__ q, real4 *__restrict__ pq)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  real4 v = pos[i];
+  v.w = q[i];
+  pq[i] = v;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (each with its own L2):
 // give XCD x the x-th contiguous eighth of the leaves (Morton order), so that
 // neighbouring leaves, which read the same sources, share an L2.
@@ -363,6 +398,11 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
       if (left >= 4) {
         hipLaunchKernelGGL(k_p2p_lane<4>, dim3(nb), dim3(256), 0, s, a, rhs);
         rhs += 4;
+      } else if (a.pq) {
+        hipLaunchKernelGGL(k_pack_pq, dim3(nb), dim3(256), 0, s, a.N, a.pos, a.q + (size_t)rhs * a.N, a.pq);
+        BB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_p2p_lane_pq, dim3(nb), dim3(256), 0, s, a, rhs);
+        rhs += 1;
       } else {
         hipLaunchKernelGGL(k_p2p_lane<1>, dim3(nb), dim3(256), 0, s, a, rhs);
         rhs += 1;
