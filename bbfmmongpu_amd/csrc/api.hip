@@ -699,6 +699,7 @@ static P2PArgs p2p_args(const bbfmm_amd_plan *p)
   pp.seg = p->d_seg;
   pp.nseg = p->d_nseg;
   pp.pad_of = p->d_pad_of;
+  pp.npad = p->npad;
   // sparse forests with a few particles per leaf: a lane per target particle
   if (!p->complete && p->N < 12LL * p->nleaf) {
     pp.leafof = p->d_leafof;

@@ -143,6 +143,7 @@ struct P2PArgs {
   const int2 *seg = nullptr;    // [nleaf][27] (padded start, padded count), self first, empties dropped
   const int *nseg = nullptr;    // [nleaf]
   const int *pad_of = nullptr;  // [N] leaf-order index -> padded index
+  int npad = 0;                 // entries of ppos (and rows of qt)
   float *qt = nullptr;          // [Npad][16] charges of 16 vectors, source-major
   // particle-row form: the far-field potentials are in phit [Npad][16]; P2P
   // adds the near field and applies the output's q factor there (phi unused)

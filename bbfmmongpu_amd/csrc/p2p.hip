@@ -221,7 +221,10 @@ __device__ __forceinline__ void p2p16_step(const float4 &z, float bq, const floa
 
 // One pass over all sources for NT tiles of 16 targets starting at target
 // tb of the leaf; results added into phi (q * potential, or the potential).
-template <int NT>
+// BUF: the source positions and charge rows are read with buffer loads whose
+// group offset is a scalar register (the padded arrays then stay below 2 GB),
+// so no vector instructions go into the per-group addresses.
+template <int NT, bool BUF>
 __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, int nseg, int T, int sta,
                                            int cnt, int psta, int pself, int tb)
 {
@@ -230,6 +233,17 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
   const float4 *P = reinterpret_cast<const float4 *>(a.ppos);
   const float *Qt = a.qt;
   const int qoff = kq * 16 + ti;
+  const __amdgpu_buffer_rsrc_t rP = __builtin_amdgcn_make_buffer_rsrc((void *)a.ppos, 0, a.npad * 16, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rQ = __builtin_amdgcn_make_buffer_rsrc((void *)a.qt, 0, a.npad * 64, 0x00020000);
+  // source group at padded index p: lane l reads the position of source p + l / 16 and its charge of vector l % 16
+  auto src_pos = [&](int p) -> float4 {
+    if (BUF) return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rP, kq * 16, p * 16, 0));
+    return P[p + kq];
+  };
+  auto src_q = [&](int p) -> float {
+    if (BUF) return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rQ, qoff * 4, p * 64, 0));
+    return Qt[(size_t)p * 16 + qoff];
+  };
   float4 x[NT];
   f32x4 acc[NT];
 #pragma unroll
@@ -250,17 +264,17 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
     for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(g[t], bq, acc[t], 0, 0, 0);
   };
   if (pself > 0) {
-    float4 zA = P[psta + kq];
-    float qA = Qt[(size_t)psta * 16 + qoff];
+    float4 zA = src_pos(psta);
+    float qA = src_q(psta);
     for (int k = 0; k < pself; k += 8) {
       const int k1 = min(k + 4, pself - 4);
-      const float4 zB = P[psta + k1 + kq];
-      const float qB = Qt[(size_t)(psta + k1) * 16 + qoff];
+      const float4 zB = src_pos(psta + k1);
+      const float qB = src_q(psta + k1);
       self_step(zA, qA);
       if (k + 4 >= pself) break;
       const int k2 = min(k + 8, pself - 4);
-      zA = P[psta + k2 + kq];
-      qA = Qt[(size_t)(psta + k2) * 16 + qoff];
+      zA = src_pos(psta + k2);
+      qA = src_q(psta + k2);
       self_step(zB, qB);
     }
   }
@@ -283,8 +297,8 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
     float4 y0, y1, y2, y3;
     float q0, q1, q2, q3;
     auto load = [&](float4 &y, float &q) {
-      y = (P + pos)[kq];
-      q = (Qt + (size_t)pos * 16)[qoff];
+      y = src_pos(pos);
+      q = src_q(pos);
       advance();
     };
     load(y0, q0);
@@ -343,6 +357,7 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
 // summed first with the r != 0 exclusion; the other neighbours form one
 // flattened sequence of 4-source groups, loaded three groups ahead into a
 // ring of four register sets.  Up to four target tiles share a pass.
+template <bool BUF>
 __global__ void __launch_bounds__(64) k_p2p_seg16(P2PArgs a, int rhs0)
 {
   const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
@@ -358,16 +373,16 @@ __global__ void __launch_bounds__(64) k_p2p_seg16(P2PArgs a, int rhs0)
   for (int tb = 0; tb < cnt;) {
     const int tiles = (cnt - tb + 15) >> 4;
     if (tiles >= 4) {
-      p2p16_pass<4>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      p2p16_pass<4, BUF>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
       tb += 64;
     } else if (tiles == 3) {
-      p2p16_pass<3>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      p2p16_pass<3, BUF>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
       tb += 48;
     } else if (tiles == 2) {
-      p2p16_pass<2>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      p2p16_pass<2, BUF>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
       tb += 32;
     } else {
-      p2p16_pass<1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      p2p16_pass<1, BUF>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
       tb += 16;
     }
   }
@@ -421,7 +436,10 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
                              a.q + (size_t)rhs * a.N, a.qt);
           BB_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_p2p_seg16, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+        if (a.npad > 0 && a.npad < (1 << 25))  // charge rows below 2 GB: scalar-offset buffer loads
+          hipLaunchKernelGGL(k_p2p_seg16<true>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+        else
+          hipLaunchKernelGGL(k_p2p_seg16<false>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
       } else
 #endif
         hipLaunchKernelGGL(k_p2p<16>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
