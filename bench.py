@@ -278,6 +278,89 @@ def run_c4(args):
     print(json.dumps(result), flush=True)
 
 
+def run_c5(args):
+    """Differentiation workload: N = 262,144 uniform in [-1/2, 1/2)^3 (seed
+    0), depth 4, n = 4, r = 32, fp32, 8 charge vectors uniform [-1, 1]; the
+    loss 1/2 ||phi - phi_t||^2 with phi_t the FP64 direct-sum potential of a
+    second charge set (seed 2).  A step is the forward solve of the 8 vectors
+    plus the adjoint solve (bbfmm_amd_gradient: 16 vectors [q; g], returning
+    dLoss/dq and dLoss/dpositions) on plans built once, data resident on the
+    GPU; the torch.autograd.Function path, which builds its plans (tree and
+    lists on the host) inside every call, is timed beside it."""
+    import torch
+    import bbfmmongpu_amd as B
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    N, m = 262144, 8
+    rng = np.random.default_rng(0)
+    pos = rng.random((N, 3)) - 0.5
+    Q = rng.uniform(-1.0, 1.0, size=(m, N))
+    Q2 = np.random.default_rng(2).uniform(-1.0, 1.0, size=(m, N))
+    t_np, _ = B.direct_field(pos, Q2)
+    U, K, _ = B.operators(4, 32, prec="s")
+    form = B.FORM_PLAIN  # what B.potential picks for 8 vectors
+    fwd = B.Plan(pos, 4, U, K, nrhs=m, prec="s", stream=stream.cuda_stream).set(
+        scheme=B.SCHEME_IJ, form=form, output="potential")
+    bwd = B.Plan(pos, 4, U, K, nrhs=2 * m, prec="s", stream=stream.cuda_stream).set(scheme=B.SCHEME_IJ, form=form)
+    q = torch.from_numpy(Q.astype(np.float32)).to(dev).contiguous()
+    t = torch.from_numpy(t_np.astype(np.float32)).to(dev).contiguous()
+    phi = torch.empty_like(q)
+
+    def step():
+        fwd.solve(q, phi)
+        g = (phi - t).contiguous()
+        return bwd.gradient(q, g)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        dq, dx = step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    ms_f, _ = timed_solves(fwd, q, phi, args.steps, args.warmup, stream)
+    # the autograd path (plans rebuilt per call), a few steps
+    pos_t = torch.tensor(pos, dtype=torch.float32, device=dev, requires_grad=True)
+    q_t = q.clone().requires_grad_(True)
+    n_ag = 3
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(n_ag):
+        pos_t.grad = None
+        q_t.grad = None
+        ph = B.potential(pos_t, q_t, 4, U, K, prec="s")
+        loss = 0.5 * ((ph - t) ** 2).sum()
+        loss.backward()
+    torch.cuda.synchronize()
+    s_ag = (time.time() - t0) / n_ag
+    # accuracy of the potentials against the FP64 direct sum (sampled)
+    tg = np.sort(np.random.default_rng(1).choice(N, size=min(args.targets, N), replace=False)).astype(np.int32)
+    ex, _ = B.direct_field(pos, Q[:2], tg)
+    errs = [rel_l2(ex[j], phi[j].cpu().numpy()[tg]) for j in range(2)]
+    agree = float((q_t.grad - dq).norm() / dq.norm())
+    log(f"c5: forward + adjoint {ms:.3f} ms/step (forward {ms_f:.3f} ms), autograd step with plan builds "
+        f"{s_ag:.3f} s, potential errors {errs}, autograd vs plan dq {agree:.2e}")
+    result = {
+        "metric": "forward + adjoint (dLoss/dq, dLoss/dpos) s/step at N=262K, n=4, 8 vectors",
+        "value": 1e3 / ms, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "N=262144 uniform, depth 4, n=4, r=32, 8 charge vectors, loss 1/2||phi - phi_t||^2 "
+                               "(phi_t: FP64 direct sum of a second charge set)", "N": N, "vectors": m,
+                   "m2l_form": "plain fp32 (IJ)"},
+        "ms_forward": ms_f, "s_autograd_step_with_plan_builds": s_ag,
+        "rel_l2_err_potential": errs, "autograd_vs_plan_dq": agree,
+    }
+    fwd.close()
+    bwd.close()
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,9 +373,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for HBM byte counters)")
-    ap.add_argument("--config", choices=["c2", "c3", "c4"], default="c2",
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: N=1M n=4 chosen vs plain (default); c3: N=4M n=8 r=256 depth 6, "
-                         "16 vectors, FP32 MFMA M2L; c4: 64 uneven instances in one plan and one hipGraph")
+                         "16 vectors, FP32 MFMA M2L; c4: 64 uneven instances in one plan and one hipGraph; "
+                         "c5: N=262K forward + adjoint (gradients) of 8 vectors")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note only "
@@ -303,6 +387,9 @@ def main():
         return
     if args.config == "c4":
         run_c4(args)
+        return
+    if args.config == "c5":
+        run_c5(args)
         return
 
     import torch
