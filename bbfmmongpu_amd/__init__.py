@@ -441,6 +441,6 @@ def potential(pos, q, n: int, U, K, prec: str = "s", lmod: int = 0, scheme: int 
               form: int | None = None):
     """Differentiable FMM potentials (torch): pos N x 3, q m x N on the GPU,
     returns m x N potentials; gradients flow to pos and q."""
-    if form is None:
-        form = FORM_BF16X2 if (prec == "s" and n == 4 and q.shape[0] >= 16) else FORM_PLAIN
+    if form is None:  # the n = 4 precision form once the M2L has several columns per cell
+        form = FORM_BF16X2 if (prec == "s" and n == 4 and q.shape[0] >= 8) else FORM_PLAIN
     return __getattr__("FMMPotential").apply(pos, q, (n, U, K, prec, lmod, scheme, form))

@@ -285,8 +285,9 @@ def run_c5(args):
     second charge set (seed 2).  A step is the forward solve of the 8 vectors
     plus the adjoint solve (bbfmm_amd_gradient: 16 vectors [q; g], returning
     dLoss/dq and dLoss/dpositions) on plans built once, data resident on the
-    GPU; the torch.autograd.Function path, which builds its plans (tree and
-    lists on the host) inside every call, is timed beside it."""
+    GPU, with the split-BF16 M2L; the plain fp32 forms and the
+    torch.autograd.Function path (which builds its plans, tree and lists on
+    the host, inside every call) are timed beside it."""
     import torch
     import bbfmmongpu_amd as B
 
@@ -300,30 +301,45 @@ def run_c5(args):
     Q2 = np.random.default_rng(2).uniform(-1.0, 1.0, size=(m, N))
     t_np, _ = B.direct_field(pos, Q2)
     U, K, _ = B.operators(4, 32, prec="s")
-    form = B.FORM_PLAIN  # what B.potential picks for 8 vectors
-    fwd = B.Plan(pos, 4, U, K, nrhs=m, prec="s", stream=stream.cuda_stream).set(
-        scheme=B.SCHEME_IJ, form=form, output="potential")
-    bwd = B.Plan(pos, 4, U, K, nrhs=2 * m, prec="s", stream=stream.cuda_stream).set(scheme=B.SCHEME_IJ, form=form)
     q = torch.from_numpy(Q.astype(np.float32)).to(dev).contiguous()
     t = torch.from_numpy(t_np.astype(np.float32)).to(dev).contiguous()
-    phi = torch.empty_like(q)
+    tg = np.sort(np.random.default_rng(1).choice(N, size=min(args.targets, N), replace=False)).astype(np.int32)
+    ex, _ = B.direct_field(pos, Q[:2], tg)
 
-    def step():
-        fwd.solve(q, phi)
-        g = (phi - t).contiguous()
-        return bwd.gradient(q, g)
+    def measure(form):
+        fwd = B.Plan(pos, 4, U, K, nrhs=m, prec="s", stream=stream.cuda_stream).set(
+            scheme=B.SCHEME_IJ, form=form, output="potential")
+        bwd = B.Plan(pos, 4, U, K, nrhs=2 * m, prec="s", stream=stream.cuda_stream).set(
+            scheme=B.SCHEME_IJ, form=form)
+        phi = torch.empty_like(q)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        dq, dx = step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
-    ms_f, _ = timed_solves(fwd, q, phi, args.steps, args.warmup, stream)
+        def step():
+            fwd.solve(q, phi)
+            g = (phi - t).contiguous()
+            return bwd.gradient(q, g)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            dq, dx = step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        ms_f, _ = timed_solves(fwd, q, phi, args.steps, args.warmup, stream)
+        errs = [rel_l2(ex[j], phi[j].cpu().numpy()[tg]) for j in range(2)]
+        fwd.close()
+        bwd.close()
+        return ms, ms_f, errs, dq, dx
+
+    ms, ms_f, errs, dq, dx = measure(B.FORM_BF16X2)
+    ms_p, ms_fp, errs_p, dq_p, dx_p = measure(B.FORM_PLAIN)
+    d_dq = float((dq - dq_p).norm() / dq_p.norm())
+    d_dx = float((dx - dx_p).norm() / dx_p.norm())
+    log(f"c5: plain fp32 forms: {ms_p:.3f} ms/step (forward {ms_fp:.3f} ms), errors {errs_p}; "
+        f"split-BF16 vs plain gradients: dq {d_dq:.2e} dx {d_dx:.2e}")
     # the autograd path (plans rebuilt per call), a few steps
     pos_t = torch.tensor(pos, dtype=torch.float32, device=dev, requires_grad=True)
     q_t = q.clone().requires_grad_(True)
@@ -338,10 +354,6 @@ def run_c5(args):
         loss.backward()
     torch.cuda.synchronize()
     s_ag = (time.time() - t0) / n_ag
-    # accuracy of the potentials against the FP64 direct sum (sampled)
-    tg = np.sort(np.random.default_rng(1).choice(N, size=min(args.targets, N), replace=False)).astype(np.int32)
-    ex, _ = B.direct_field(pos, Q[:2], tg)
-    errs = [rel_l2(ex[j], phi[j].cpu().numpy()[tg]) for j in range(2)]
     agree = float((q_t.grad - dq).norm() / dq.norm())
     log(f"c5: forward + adjoint {ms:.3f} ms/step (forward {ms_f:.3f} ms), autograd step with plan builds "
         f"{s_ag:.3f} s, potential errors {errs}, autograd vs plan dq {agree:.2e}")
@@ -352,12 +364,12 @@ def run_c5(args):
         "data": "synthetic",
         "config": {"workload": "N=262144 uniform, depth 4, n=4, r=32, 8 charge vectors, loss 1/2||phi - phi_t||^2 "
                                "(phi_t: FP64 direct sum of a second charge set)", "N": N, "vectors": m,
-                   "m2l_form": "plain fp32 (IJ)"},
+                   "m2l_form": "split-BF16 (IJ), the n = 4 precision form"},
         "ms_forward": ms_f, "s_autograd_step_with_plan_builds": s_ag,
         "rel_l2_err_potential": errs, "autograd_vs_plan_dq": agree,
+        "other_forms": {"plain_fp32": {"ms_per_step": ms_p, "ms_forward": ms_fp, "rel_l2_err_potential": errs_p,
+                                       "gradients_vs_split_bf16": {"dq": d_dq, "dx": d_dx}}},
     }
-    fwd.close()
-    bwd.close()
     print(json.dumps(result), flush=True)
 
 
