@@ -128,9 +128,16 @@ __global__ void __launch_bounds__(256) k_p2p_lane(P2PArgs a, int rhs0)
   real acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) acc[r] = 0;
+  // the next neighbour's range is fetched while this one's sources are summed
+  int Bn = nb[0];  // nn >= 1: the leaf itself
+  int j0n = a.leafsta[Bn], cn = a.leafcnt[Bn];
   for (int b = 0; b < nn; b++) {
-    const int B = nb[b];
-    const int j0 = a.leafsta[B], j1 = j0 + a.leafcnt[B];
+    const int j0 = j0n, j1 = j0 + cn;
+    if (b + 1 < nn) {
+      Bn = nb[b + 1];
+      j0n = a.leafsta[Bn];
+      cn = a.leafcnt[Bn];
+    }
     for (int j = j0; j < j1; j++) {
       const real4 y = a.pos[j];
       const real dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z;
@@ -157,9 +164,16 @@ __global__ void __launch_bounds__(256) k_p2p_lane_pq(P2PArgs a, int rhs)
   const int nn = a.nnbr[lf];
   const int *nb = a.nbr + (size_t)lf * 27;
   real acc = 0;
+  // the next neighbour's range is fetched while this one's sources are summed
+  int Bn = nb[0];  // nn >= 1: the leaf itself
+  int j0n = a.leafsta[Bn], cn = a.leafcnt[Bn];
   for (int b = 0; b < nn; b++) {
-    const int B = nb[b];
-    const int j0 = a.leafsta[B], j1 = j0 + a.leafcnt[B];
+    const int j0 = j0n, j1 = j0 + cn;
+    if (b + 1 < nn) {
+      Bn = nb[b + 1];
+      j0n = a.leafsta[Bn];
+      cn = a.leafcnt[Bn];
+    }
     for (int j = j0; j < j1; j++) {
       const real4 y = a.pq[j];
       const real dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z;
