@@ -135,7 +135,7 @@ def test_batch_auto_depth_against_direct(ops4, cuda):
     plan.close()
 
 
-@pytest.mark.parametrize("form,nrhs,tol", [(2, 1, 1e-5), (2, 3, 1e-5), (1, 16, 3e-5)])
+@pytest.mark.parametrize("form,nrhs,tol", [(2, 1, 1e-5), (2, 3, 1e-5), (1, 1, 3e-5), (1, 3, 3e-5), (1, 16, 3e-5)])
 def test_batch_list_mfma_against_oracle(form, nrhs, tol, ops4, cuda):
     """The IJ blocking on sparse trees (class-grouped gathered columns on
     MFMA): FP32 MFMA and split BF16, per instance against the oracle."""

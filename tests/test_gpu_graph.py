@@ -43,12 +43,18 @@ def test_graph_single_plan_chosen_form(near_stream, ops4, cuda):
     plan.close()
 
 
-def test_graph_batch_plan(ops4, cuda):
+@pytest.mark.parametrize("form", [None, 1])
+def test_graph_batch_plan(form, ops4, cuda):
+    """A batch of sparse instances (a few points per leaf): the lane P2P and
+    the MFMA pass kernels of sparse forests, and with form 1 the split-BF16
+    list M2L, replayed from the graph."""
     import bbfmmongpu_amd as B
     from bbfmmongpu_amd import workloads as W
     U, K, _ = ops4
     pos_list, q_list = W.uneven_instances(count=6, nmin=4096, nmax=16384, seed=5)
     plan = B.Plan.batch(pos_list, 4, U, K, nrhs=1, prec="s")
+    if form is not None:
+        plan.set(scheme=B.SCHEME_IJ, form=form)
     Q = np.concatenate(q_list).astype(np.float32)[None, :]
     _replays_match(plan, Q, cuda)
     plan.close()
