@@ -1129,8 +1129,10 @@ struct ListArgs {
   // per wave (4 per block): bit v of words 0..2 set when some column of the
   // wave has a non-empty source at offset v
   const unsigned long long *mask;  // [4 * blocks][4]
-  // per wave, the source cells of its 32 columns at each offset, so that one
-  // offset's indices are one 128-byte line (ncell for columns past the end)
+  // one vector per cell (nrhs = 1): per wave, the source cells of its 32
+  // columns at each offset, so that one offset's indices are one 128-byte
+  // line (ncell for columns past the end); null otherwise (the columns of a
+  // wave then hold at most 16 cells, read from vsrc)
   const int *vsw;  // [4 * blocks][189][32]
   const void *Ka;
   const void *M0, *M1;
@@ -1171,12 +1173,12 @@ __global__ void __launch_bounds__(64) k_m2l_list_mask(ListArgs g, unsigned long 
   const int F = valid ? g.cls_cells[g.cls_off[s] + col / g.nrhs] : 0;
   const int *vs = g.vsrc + (size_t)F * kOffsets;
   unsigned long long m[3] = {0, 0, 0};
-  int *out = vsw + (size_t)gw * kOffsets * 32 + lane;
+  int *out = vsw ? vsw + (size_t)gw * kOffsets * 32 + lane : nullptr;
   for (int v = 0; v < kOffsets; v++) {
     const int S = valid ? vs[v] : g.ncell;
     const bool any = __ballot(S != g.ncell) != 0;
     if (any) m[v >> 6] |= 1ull << (v & 63);
-    if (lane < 32) out[v * 32] = S;
+    if (out && lane < 32) out[v * 32] = S;
   }
   if (lane < 4) mask[(size_t)gw * 4 + lane] = lane < 3 ? m[lane] : 0ull;
 }
@@ -1236,8 +1238,10 @@ __global__ void __launch_bounds__(256) k_m2l_list_mfma(ListArgs g)
   const int4 *tab = g.tab + s * kOffsets;
   const size_t rbase = (size_t)rh * (g.ncell + 1);
   const int gw = __builtin_amdgcn_readfirstlane(lb * 4 + wave);
-  const int *vsl = g.vsw + (size_t)gw * kOffsets * 32 + (lane & 31);
-  auto vs = [&](int v) { return vsl[v * 32]; };
+  // source of column (lane % 32) at offset v: the wave-major copy, or vsrc
+  const int *vsl = g.vsw ? g.vsw + (size_t)gw * kOffsets * 32 + (lane & 31) : g.vsrc + (size_t)F * kOffsets;
+  const int vst = g.vsw ? 32 : 1;
+  auto vs = [&](int v) { return vsl[v * vst]; };
   f32x16 acc;
 #pragma unroll
   for (int e = 0; e < 16; e++) acc[e] = 0.0f;
@@ -1400,8 +1404,10 @@ __global__ void __launch_bounds__(256) k_m2l_list_lds(ListArgs g)
   const int4 *tab = g.tab + s * kOffsets;
   const size_t rbase = (size_t)rh * (g.ncell + 1);
   const int gw = __builtin_amdgcn_readfirstlane(lb * 4 + wave);
-  const int *vsl = g.vsw + (size_t)gw * kOffsets * 32 + (lane & 31);
-  auto vs = [&](int v) { return vsl[v * 32]; };
+  // source of column (lane % 32) at offset v: the wave-major copy, or vsrc
+  const int *vsl = g.vsw ? g.vsw + (size_t)gw * kOffsets * 32 + (lane & 31) : g.vsrc + (size_t)F * kOffsets;
+  const int vst = g.vsw ? 32 : 1;
+  auto vs = [&](int v) { return vsl[v * vst]; };
   // the block's union of the 4 waves' masks
   const unsigned long long *mk = g.mask + (size_t)lb * 16;
   unsigned long long m0 = mk[0] | mk[4] | mk[8] | mk[12];
@@ -1586,7 +1592,7 @@ int list_m2l_create(ListM2L **out, int r, int nrhs, int form, const real *K, int
   a.M1 = g->d_M1;
   if (nb > 0) {
     if ((e = hipMalloc(&g->d_mask, sizeof(unsigned long long) * 16 * (size_t)nb)) != hipSuccess ||
-        (e = hipMalloc(&g->d_vsw, sizeof(int) * 4 * (size_t)nb * kOffsets * 32)) != hipSuccess) {
+        (nrhs == 1 && (e = hipMalloc(&g->d_vsw, sizeof(int) * 4 * (size_t)nb * kOffsets * 32)) != hipSuccess)) {
       fail(e, "hipMalloc");
     } else {
       hipLaunchKernelGGL(k_m2l_list_mask, dim3(4 * nb), dim3(64), 0, s, a, g->d_mask, g->d_vsw);
