@@ -65,6 +65,8 @@ struct GridArgs {
   const int *steps;  // [class group][208][kStepWords] step records (step_table)
   int nsteps[8];     // steps of each class group
   int ss;            // source-stationary kernel: field classes per wave (8, 4 or 2); 0: not used
+  int gbytes;        // bytes of one grid array when below 2 GB (scalar-offset buffer loads), else 0
+  int kbytes;        // bytes of the operator array
 };
 
 constexpr int kSteps = 208;  // 8 source classes x 26 non-zero parent shifts
@@ -395,7 +397,7 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(GridArgs g)
 // blocks (class groups), which evens out the work per CU when the column
 // groups do not fill whole rounds of the chip, and drops the steps in which
 // every class of the group has a near-field offset.  Split-BF16 (fp32 build).
-template <int CLS>
+template <int CLS, bool BUF>
 __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
 {
   constexpr int WAVES = 4;  // one k-part (k-step x hi/lo) of the operators per wave
@@ -447,18 +449,53 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
   // the MFMAs of the current step run; the source tile of the next step is
   // loaded into registers at the same time.  Records are read two steps ahead.
   typedef __attribute__((address_space(3))) void lds_void;
+  // BUF: buffer loads whose per-step part is a scalar offset (the operator's
+  // index, the step's shift) and whose per-lane part is fixed, so that no
+  // vector instructions go into the step's addresses (grids below 2 GB)
+  const __amdgpu_buffer_rsrc_t rK = __builtin_amdgcn_make_buffer_rsrc((void *)g.Ka, 0, g.kbytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rMh = __builtin_amdgcn_make_buffer_rsrc((void *)g.Mg0, 0, g.gbytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rMl = __builtin_amdgcn_make_buffer_rsrc((void *)g.Mg1, 0, g.gbytes, 0x00020000);
   auto load_k = [&](const int *e, int buf) {
 #pragma unroll
     for (int i = 0; i < CLS; i++) {
       const int k = (e[1 + (i >> 1)] >> (16 * (i & 1))) & 0xffff;
-      __builtin_amdgcn_global_load_lds((const void *)(Kl + (size_t)k * 256), (lds_void *)&sK[buf][i][wave][0], 16,
-                                       0, 0);
+      if (BUF)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rK, (lds_void *)&sK[buf][i][wave][0], 16, (wave * 64 + lane) * 16,
+                                                 k * 256 * 16, 0, 0);
+      else
+        __builtin_amdgcn_global_load_lds((const void *)(Kl + (size_t)k * 256), (lds_void *)&sK[buf][i][wave][0], 16,
+                                         0, 0);
     }
   };
   auto src_off = [&](const int *e) {
     const int sc = e[0] & 7, pk = e[0] >> 3;
     const int dx = ((pk >> 4) & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = (pk & 3) - 1;
     return ((size_t)(slot0 + sc * cls + (dx * PP + dy) * PP + dz + base) * g.nrhs + rh) * 32 + half * 8;
+  };
+  // the scalar offset is unsigned: the shift's most negative value,
+  // -(PP^2 + PP + 1), moves to the lane part (base >= PP^2 + PP + 1)
+  const int sh0 = PP * PP + PP + 1;
+  auto src_soff = [&](const int *e) -> int {
+    const int sc = e[0] & 7, pk = e[0] >> 3;
+    const int dx = ((pk >> 4) & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = (pk & 3) - 1;
+    return (slot0 + sc * cls + (dx * PP + dy) * PP + dz + sh0) * g.nrhs * 64;
+  };
+  const int voff = (((base - sh0) * g.nrhs + rh) * 32 + half * 8) * 2;
+  // source tile of step e: hi and lo of k-steps 0 and 1
+  auto load_src = [&](const int *e, bf16x8 &h0, bf16x8 &l0, bf16x8 &h1, bf16x8 &l1) {
+    if (BUF) {
+      const int so = src_soff(e);
+      h0 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMh, voff, so, 0));
+      l0 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMl, voff, so, 0));
+      h1 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMh, voff + 32, so, 0));
+      l1 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMl, voff + 32, so, 0));
+    } else {
+      const size_t o = src_off(e);
+      h0 = *(const bf16x8 *)(Mh + o);
+      l0 = *(const bf16x8 *)(Ml + o);
+      h1 = *(const bf16x8 *)(Mh + o + 16);
+      l1 = *(const bf16x8 *)(Ml + o + 16);
+    }
   };
   const int nst = g.nsteps[cg];
   const int *S = g.steps + (size_t)cg * kSteps * kStepWords;
@@ -472,11 +509,7 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
 #pragma unroll
     for (int i = 0; i < 5; i++) e1[i] = S[s1 * kStepWords + i];
     load_k(e0, 0);
-    const size_t o = src_off(e0);
-    bh0 = *(const bf16x8 *)(Mh + o);
-    bl0 = *(const bf16x8 *)(Ml + o);
-    bh1 = *(const bf16x8 *)(Mh + o + 16);
-    bl1 = *(const bf16x8 *)(Ml + o + 16);
+    load_src(e0, bh0, bl0, bh1, bl1);
   }
   __builtin_amdgcn_s_waitcnt(kWaitVm0);
   __syncthreads();
@@ -510,11 +543,8 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
     }
     __builtin_amdgcn_sched_barrier(0);
     load_k(e1, b ^ 1);
-    const size_t o = src_off(e1);
-    const bf16x8 nh0 = *(const bf16x8 *)(Mh + o);
-    const bf16x8 nl0 = *(const bf16x8 *)(Ml + o);
-    const bf16x8 nh1 = *(const bf16x8 *)(Mh + o + 16);
-    const bf16x8 nl1 = *(const bf16x8 *)(Ml + o + 16);
+    bf16x8 nh0, nl0, nh1, nl1;
+    load_src(e1, nh0, nl0, nh1, nl1);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int c = 0; c < CLS; c++) {
@@ -844,6 +874,10 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     slots += 8 * PP * PP * PP;
   }
   g->nslots = slots;
+  {
+    const size_t gb = slots * (size_t)nrhs * r * 2;  // split-BF16 grids: 2 bytes per value
+    a.gbytes = gb < ((size_t)1 << 31) ? (int)gb : 0;
+  }
   int tpw = 1, cpb;
   // variant 0 (automatic): source-stationary blocking once the deepest level
   // has enough columns (parents x vectors) to give every SIMD at least a wave;
@@ -926,6 +960,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
       if ((e = hipMalloc(&g->d_K, sizeof(__bf16) * Kb.size())) != hipSuccess) fail(e, "hipMalloc");
       else if ((e = hipMemcpyAsync(g->d_K, Kb.data(), sizeof(__bf16) * Kb.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
         fail(e, "copy");
+      a.kbytes = (int)(sizeof(__bf16) * Kb.size());
       const size_t nb2 = sizeof(__bf16) * slots * nrhs * r;
       if (rc == BBFMM_OK && (e = hipMalloc(&g->d_Mg0, nb2)) != hipSuccess) fail(e, "hipMalloc");
       if (rc == BBFMM_OK && (e = hipMalloc(&g->d_Mg1, nb2)) != hipSuccess) fail(e, "hipMalloc");
@@ -1037,6 +1072,19 @@ void grid_m2l_destroy(GridM2L *g)
   delete g;
 }
 
+#ifdef SINGLE
+// grids below 2 GB take the scalar-offset buffer-load form of the kernel
+template <int CLS>
+static void launch_ss(const GridM2L *g, const GridArgs &a, hipStream_t s)
+{
+  const dim3 grid((g->nblocks + 7) & ~7);
+  if (a.gbytes > 0)
+    hipLaunchKernelGGL((k_m2l_mfma_ss<CLS, true>), grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_m2l_mfma_ss<CLS, false>), grid, dim3(256), 0, s, a);
+}
+#endif
+
 int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, hipEvent_t mid)
 {
   GridArgs a = g->args;
@@ -1061,11 +1109,11 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
     if (a.ss == 8)  // rounded up to whole rounds of the 8 XCDs; the extra blocks exit
-      hipLaunchKernelGGL((k_m2l_mfma_ss<8>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+      launch_ss<8>(g, a, s);
     else if (a.ss == 4)
-      hipLaunchKernelGGL((k_m2l_mfma_ss<4>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+      launch_ss<4>(g, a, s);
     else if (a.ss == 2)
-      hipLaunchKernelGGL((k_m2l_mfma_ss<2>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+      launch_ss<2>(g, a, s);
     else if (a.wave_cols == 64)
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else
