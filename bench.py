@@ -501,12 +501,12 @@ def main():
     # x 6 products of 32x32x16 (zero operator included)
     groups = sum(((1 << (3 * (lev - 1))) * nrhs + 127) // 128 for lev in range(2, chosen.maxlev + 1))
     issued = sum(ss_group_steps(4)) * 4 * 6 * (2 * 32 * 32 * 16) * groups * 4
-    roof_m2l = {"kernel": "k_m2l_mfma_ss<4>", "bound": "mfma", "unit": "TFLOP/s",
+    roof_m2l = {"kernel": "k_m2l_mfma_ss<4,true>", "bound": "mfma", "unit": "TFLOP/s",
                 "achieved": flop_m2l * nrhs / (st_chosen["m2l"] * 1e-3) / 1e12, "peak": PEAK_BF16_TFLOPS,
                 "launch_ms": st_chosen["m2l"], "algorithmic_flop_per_launch": flop_m2l * nrhs,
                 "issued_mfma_flop_per_launch": issued,
                 "issued_mfma_frac": issued / (st_chosen["m2l"] * 1e-3) / (PEAK_BF16_TFLOPS * 1e12),
-                "traffic": measured_traffic("k_m2l_mfma_ss<4>")}
+                "traffic": measured_traffic("k_m2l_mfma_ss<4,true>")}
     roof_m2l["frac"] = roof_m2l["achieved"] / roof_m2l["peak"]
     roof_p2p = {"kernel": "k_p2p_seg16<true>", "bound": "valu", "unit": "TFLOP/s",
                 "achieved": p2p_flop / (st_chosen["p2p"] * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS,
