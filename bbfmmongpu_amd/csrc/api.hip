@@ -124,10 +124,14 @@ struct bbfmm_amd_plan {
   double stage_ms[ST_TOTAL + 2] = {};
   bool side_timed = false;     // the timed solves ran P2P on the side stream
   // near field beside the far field (particle-row form, BBFMM_OPT_NEAR_STREAM):
-  // P2P on a side stream forked after the gather and joined before the
+  // P2P on a side stream forked after the M2L and joined before the
   // scatter, into near-field rows of its own
   int near_stream = 0;
   hipStream_t side = nullptr;
+  // BBFMM_OPT_P2P_FORM: split-BF16 accumulation of the 16-vector P2P, and
+  // the hi / lo charge parts it reads ([Npad][16] bf16 each)
+  int p2p_split = 0;
+  void *d_qsh = nullptr, *d_qsl = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   real *d_phin = nullptr;
   bool solved = false;
@@ -189,7 +193,8 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
                   p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad, p->d_phit, p->d_ipad, p->d_phin,
                   p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf, p->d_cls_cells,
-                  p->d_vsrc, p->d_dcp, p->d_dwv, p->d_KU, p->d_T8, p->d_T8m, p->d_mplist, p->d_pq};
+                  p->d_vsrc, p->d_dcp, p->d_dwv, p->d_KU, p->d_T8, p->d_T8m, p->d_mplist, p->d_pq,
+                  p->d_qsh,   p->d_qsl};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
@@ -621,6 +626,11 @@ static int prepare_solve(bbfmm_amd_plan *p)
     BB_HIP(hipMemsetAsync(p->d_qt, 0, sizeof(real) * cnt, p->stream));
   }
   if (rows16(p) && !p->d_phit) BB_TRY(dalloc(&p->d_phit, (size_t)p->npad * 16));
+  if (p->p2p_split && p->form != BBFMM_M2L_PLAIN && p->nrhs >= 16 && p->npad > 0 && !p->d_qsh) {
+    const size_t bytes = sizeof(__bf16) * (size_t)p->npad * 16;
+    BB_HIP(hipMalloc(&p->d_qsh, bytes));
+    BB_HIP(hipMalloc(&p->d_qsl, bytes));
+  }
   if (rows16(p) && p->near_stream) {
     if (!p->d_phin) BB_TRY(dalloc(&p->d_phin, (size_t)p->npad * 16));
     if (!p->side) BB_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
@@ -700,6 +710,10 @@ static P2PArgs p2p_args(const bbfmm_amd_plan *p)
   pp.nseg = p->d_nseg;
   pp.pad_of = p->d_pad_of;
   pp.npad = p->npad;
+  if (p->p2p_split && p->d_qsh) {
+    pp.qsh = p->d_qsh;
+    pp.qsl = p->d_qsl;
+  }
   // sparse forests with a few particles per leaf: a lane per target particle
   if (!p->complete && p->N < 12LL * p->nleaf) {
     pp.leafof = p->d_leafof;
@@ -995,6 +1009,11 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
   case BBFMM_OPT_NEAR_STREAM:
     if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "near stream %d: 0 or 1", value);
     p->near_stream = value;
+    drop_graph(p);
+    return BBFMM_OK;
+  case BBFMM_OPT_P2P_FORM:
+    if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "P2P form %d: 0 (fp32) or 1 (split-BF16)", value);
+    p->p2p_split = value;
     drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_LEVEL_SWITCH:

@@ -155,6 +155,10 @@ struct P2PArgs {
   // leafof[i] its leaf; used when set
   const int *leafof = nullptr;
   real4 *pq = nullptr;  // [N] scratch: positions with one vector's charge in w (one load per source)
+  // split-BF16 accumulation (16 vectors, padded copy below 2^25 entries):
+  // hi and lo bf16 parts of qt at [padded source / 4][vector][4 sources],
+  // written from qt before the pass; used when set
+  void *qsh = nullptr, *qsl = nullptr;
 };
 int launch_p2p(const P2PArgs &a, hipStream_t s);
 

@@ -233,6 +233,49 @@ __device__ __forceinline__ void p2p16_step(const float4 &z, float bq, const floa
   for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(g[t], bq, acc[t], 0, 0, 0);
 }
 
+// results of NT target tiles (accumulator layout of the 16x16 MFMA: lane l
+// holds targets tb + 16 t + 4 (l / 16) + v, vector l % 16) added into phi
+// (q * potential, or the potential), the particle rows, or the near rows
+template <int NT>
+__device__ __forceinline__ void p2p16_store(const P2PArgs &a, int rhs0, int sta, int cnt, int psta, int tb,
+                                            const f32x4 *acc)
+{
+  const int l = threadIdx.x;
+  const int ti = l & 15, kq = l >> 4;
+  const float *Qt = a.qt;
+  if (a.phin) {
+    // particle rows of their own: the near-field potential
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = tb + 16 * t + 4 * kq + v;
+        if (i < cnt) a.phin[(size_t)(psta + i) * 16 + ti] = acc[t][v];
+      }
+    return;
+  }
+  if (a.phit) {
+    // particle rows: far field + near field, then the output's q factor
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = tb + 16 * t + 4 * kq + v;
+        const size_t o = (size_t)(psta + i) * 16 + ti;
+        if (i < cnt) a.phit[o] = (a.potential ? 1.0f : Qt[o]) * (a.phit[o] + acc[t][v]);
+      }
+    return;
+  }
+  const size_t row = (size_t)(rhs0 + ti) * a.N + sta;
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = tb + 16 * t + 4 * kq + v;
+      if (i < cnt) a.phi[row + i] += (a.potential ? 1.0f : a.q[row + i]) * acc[t][v];
+    }
+}
+
 // One pass over all sources for NT tiles of 16 targets starting at target
 // tb of the leaf; results added into phi (q * potential, or the potential).
 // BUF: the source positions and charge rows are read with buffer loads whose
@@ -332,37 +375,7 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
       p2p16_step<NT>(y3, q3, x, acc);
     }
   }
-  if (a.phin) {
-    // particle rows of their own: the near-field potential
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int i = tb + 16 * t + 4 * kq + v;
-        if (i < cnt) a.phin[(size_t)(psta + i) * 16 + ti] = acc[t][v];
-      }
-    return;
-  }
-  if (a.phit) {
-    // particle rows: far field + near field, then the output's q factor
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int i = tb + 16 * t + 4 * kq + v;
-        const size_t o = (size_t)(psta + i) * 16 + ti;
-        if (i < cnt) a.phit[o] = (a.potential ? 1.0f : Qt[o]) * (a.phit[o] + acc[t][v]);
-      }
-    return;
-  }
-  const size_t row = (size_t)(rhs0 + ti) * a.N + sta;
-#pragma unroll
-  for (int t = 0; t < NT; t++)
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int i = tb + 16 * t + 4 * kq + v;
-      if (i < cnt) a.phi[row + i] += (a.potential ? 1.0f : a.q[row + i]) * acc[t][v];
-    }
+  p2p16_store<NT>(a, rhs0, sta, cnt, psta, tb, acc);
 }
 
 // Sixteen vectors on MFMA; qt is [padded N][16].  Lane l computes
@@ -401,6 +414,206 @@ __global__ void __launch_bounds__(64) k_p2p_seg16(P2PArgs a, int rhs0)
     }
   }
 }
+
+// ------------------------------------------------ split-BF16 accumulation
+// On gfx950 the FP32 MFMA and the FP32 vector ALU of a SIMD do not run at the
+// same time, so in k_p2p_seg16 the 1/r work and the MFMA accumulation (32
+// cycles per 64 pairs) add up.  Here G = 1/r (fp32) and the charges are each
+// split into bf16 hi + lo parts and the sum over sources takes three
+// products (hi*hi + hi*lo + lo*hi) of v_mfma_f32_16x16x32_bf16: 16 targets x
+// 32 sources x 16 vectors per product, fp32 accumulation.  The relative
+// error of one term is below 3 * 2^-16 (the dropped lo*lo and the rounding of
+// the lo parts).
+#ifdef SINGLE
+typedef __bf16 bf16x8p __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4p __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2p __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x3 __attribute__((ext_vector_type(3)));
+
+// two fp32 values -> their bf16 hi parts (round to nearest even) and the
+// bf16 of the remainders, each pair in one conversion
+__device__ __forceinline__ void split2(float g0, float g1, unsigned &h, unsigned &lo)
+{
+  h = __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{g0, g1}, bf16x2p));
+  const float h0 = __builtin_bit_cast(float, h << 16), h1 = __builtin_bit_cast(float, h & 0xffff0000u);
+  lo = __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{g0 - h0, g1 - h1}, bf16x2p));
+}
+
+// qt [Npad][16] -> hi and lo parts at [padded source / 4][vector][4 sources]
+__global__ void k_split_q16(int ngroups, const float *__restrict__ qt, bf16x4p *__restrict__ qh,
+                            bf16x4p *__restrict__ ql)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (group of 4 sources, vector)
+  if (i >= ngroups * 16) return;
+  const int g = i >> 4, v = i & 15;
+  bf16x4p h, lo;
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const float x = qt[(size_t)(4 * g + e) * 16 + v];
+    h[e] = (__bf16)x;
+    lo[e] = (__bf16)(x - (float)h[e]);
+  }
+  qh[i] = h;
+  ql[i] = lo;
+}
+
+// One pass over all sources for NT tiles of 16 targets starting at target tb.
+// A step takes 8 groups of 4 sources: lane l (target l % 16 of each tile)
+// computes G for the two groups 2 (l / 16) and 2 (l / 16) + 1 of the step,
+// which are also the k-slots of its A and B operands.  The self leaf's
+// groups are contiguous; the neighbours' groups follow the segment table in
+// the order of k_p2p_seg16, walked with scalar registers.  Groups past the
+// end take the last group's position with zero charges.
+template <int NT>
+__device__ __forceinline__ void p2p32_pass(const P2PArgs &a, int rhs0, int2 sg, int nseg, int T, int sta, int cnt,
+                                           int psta, int pself, int tb)
+{
+  const int l = threadIdx.x;
+  const int ti = l & 15, kq = l >> 4;
+  const float4 *P = reinterpret_cast<const float4 *>(a.ppos);
+  const __amdgpu_buffer_rsrc_t rP = __builtin_amdgcn_make_buffer_rsrc((void *)a.ppos, 0, a.npad * 16, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rH = __builtin_amdgcn_make_buffer_rsrc(a.qsh, 0, a.npad * 32, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rL = __builtin_amdgcn_make_buffer_rsrc(a.qsl, 0, a.npad * 32, 0x00020000);
+  float4 x[NT];
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    x[t] = P[psta + min(tb + 16 * t + ti, cnt - 1)];
+    acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  // the lane's two groups at padded positions pa, pb; va / vb: they count
+  struct Src {
+    f32x3 y[8];
+    bf16x4p h[2], lo[2];
+  };
+  auto load = [&](Src &u, int pa, int pb, bool va, bool vb) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      u.y[e] = __builtin_bit_cast(f32x3, __builtin_amdgcn_raw_buffer_load_b96(rP, (pa + e) * 16, 0, 0));
+      u.y[4 + e] = __builtin_bit_cast(f32x3, __builtin_amdgcn_raw_buffer_load_b96(rP, (pb + e) * 16, 0, 0));
+    }
+    const int oa = ((pa >> 2) * 16 + ti) * 8, ob = ((pb >> 2) * 16 + ti) * 8;
+    u.h[0] = __builtin_bit_cast(bf16x4p, __builtin_amdgcn_raw_buffer_load_b64(rH, oa, 0, 0));
+    u.lo[0] = __builtin_bit_cast(bf16x4p, __builtin_amdgcn_raw_buffer_load_b64(rL, oa, 0, 0));
+    u.h[1] = __builtin_bit_cast(bf16x4p, __builtin_amdgcn_raw_buffer_load_b64(rH, ob, 0, 0));
+    u.lo[1] = __builtin_bit_cast(bf16x4p, __builtin_amdgcn_raw_buffer_load_b64(rL, ob, 0, 0));
+    const bf16x4p z = {};
+    if (!va) u.h[0] = u.lo[0] = z;
+    if (!vb) u.h[1] = u.lo[1] = z;
+  };
+  auto step = [&](const Src &u, bool self) {
+    bf16x8p bh, bl;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      bh[e] = u.h[0][e];
+      bh[4 + e] = u.h[1][e];
+      bl[e] = u.lo[0][e];
+      bl[4 + e] = u.lo[1][e];
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      float g[8];
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float dx = u.y[e].x - x[t].x, dy = u.y[e].y - x[t].y, dz = u.y[e].z - x[t].z;
+        const float rr = dx * dx + dy * dy + dz * dz;
+        g[e] = __builtin_amdgcn_rsqf(rr);
+        if (self) g[e] = rr != 0.f ? g[e] : 0.f;
+      }
+      unsigned hw[4], lw[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) split2(g[2 * e], g[2 * e + 1], hw[e], lw[e]);
+      typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+      const bf16x8p gh = __builtin_bit_cast(bf16x8p, u32x4{hw[0], hw[1], hw[2], hw[3]});
+      const bf16x8p gl = __builtin_bit_cast(bf16x8p, u32x4{lw[0], lw[1], lw[2], lw[3]});
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gh, bh, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gh, bl, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gl, bh, acc[t], 0, 0, 0);
+    }
+  };
+  // self leaf: groups 0 .. pself/4 - 1 from psta
+  const int gs = pself >> 2;
+  for (int g0 = 0; g0 < gs; g0 += 8) {
+    const int ga = g0 + 2 * kq, gb = ga + 1;
+    Src u;
+    load(u, psta + 4 * min(ga, gs - 1), psta + 4 * min(gb, gs - 1), ga < gs, gb < gs);
+    step(u, true);
+  }
+  if (T > 0) {
+    int b = 1;
+    int pos = __builtin_amdgcn_readlane(sg.x, 1);
+    int end = pos + __builtin_amdgcn_readlane(sg.y, 1);
+    // next group; stays on the last group at the end
+    auto advance = [&]() {
+      if (pos + 4 < end) {
+        pos += 4;
+      } else if (b + 1 < nseg) {
+        b++;
+        pos = __builtin_amdgcn_readlane(sg.x, b);
+        end = pos + __builtin_amdgcn_readlane(sg.y, b);
+      }
+    };
+    // the 8 group positions of the next step, the lane's two picked out
+    auto next = [&](Src &u, int g0) {
+      int p[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        p[j] = pos;
+        advance();
+      }
+      const int pa = kq == 0 ? p[0] : kq == 1 ? p[2] : kq == 2 ? p[4] : p[6];
+      const int pb = kq == 0 ? p[1] : kq == 1 ? p[3] : kq == 2 ? p[5] : p[7];
+      load(u, pa, pb, g0 + 2 * kq < T, g0 + 2 * kq + 1 < T);
+    };
+    Src u0, u1;
+    next(u0, 0);
+    for (int g0 = 0; g0 < T; g0 += 16) {
+      if (g0 + 8 < T) next(u1, g0 + 8);
+      step(u0, false);
+      if (g0 + 8 >= T) break;
+      if (g0 + 16 < T) next(u0, g0 + 16);
+      step(u1, false);
+    }
+  }
+  p2p16_store<NT>(a, rhs0, sta, cnt, psta, tb, acc);
+}
+
+// k_p2p_seg16 with the split-BF16 accumulation; qsh / qsl filled from qt.
+// At most NTMAX target tiles per pass: the registers of a kernel are those of
+// its largest pass, and with NTMAX = 3 four waves fit on a SIMD (leaves of up
+// to 48 particles still take one pass).
+template <int NTMAX>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NTMAX <= 3 ? 4 : 3, 8))) k_p2p_seg32x(P2PArgs a, int rhs0)
+{
+  const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
+  const int l = threadIdx.x;
+  const int sta = a.leafsta[leaf], cnt = a.leafcnt[leaf];
+  if (cnt == 0) return;
+  const int nseg = a.nseg[leaf];
+  const int2 sg = l < nseg ? a.seg[(size_t)leaf * 27 + l] : make_int2(0, 0);
+  const int psta = __builtin_amdgcn_readfirstlane(sg.x);
+  const int pself = __builtin_amdgcn_readfirstlane(sg.y);
+  int T = 0;
+  for (int b = 1; b < nseg; b++) T += __builtin_amdgcn_readlane(sg.y, b) >> 2;
+  for (int tb = 0; tb < cnt;) {
+    const int tiles = min((cnt - tb + 15) >> 4, NTMAX);
+    if (NTMAX >= 4 && tiles >= 4) {
+      p2p32_pass<NTMAX >= 4 ? 4 : 1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      tb += 64;
+    } else if (NTMAX >= 3 && tiles == 3) {
+      p2p32_pass<NTMAX >= 3 ? 3 : 1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      tb += 48;
+    } else if (tiles == 2) {
+      p2p32_pass<2>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      tb += 32;
+    } else {
+      p2p32_pass<1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      tb += 16;
+    }
+  }
+}
+#endif
 
 // charges of vectors rhs0..rhs0+15, leaf order [16][N] -> padded [Npad][16]
 __global__ void k_pad_charges16(int N, const int *__restrict__ pad_of, const float *__restrict__ q,
@@ -450,7 +663,16 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
                              a.q + (size_t)rhs * a.N, a.qt);
           BB_HIP(hipGetLastError());
         }
-        if (a.npad > 0 && a.npad < (1 << 25))  // charge rows below 2 GB: scalar-offset buffer loads
+        if (a.qsh && a.qsl && a.npad > 0 && a.npad < (1 << 25)) {  // split-BF16 accumulation
+          const int ng = a.npad >> 2;
+          hipLaunchKernelGGL(k_split_q16, dim3((ng * 16 + 255) / 256), dim3(256), 0, s, ng, a.qt,
+                             (bf16x4p *)a.qsh, (bf16x4p *)a.qsl);
+          BB_HIP(hipGetLastError());
+          if (a.N <= 40LL * a.nleaf)  // up to about 40 particles per leaf: passes of 1 to 3 tiles
+            hipLaunchKernelGGL(k_p2p_seg32x<3>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+          else
+            hipLaunchKernelGGL(k_p2p_seg32x<4>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+        } else if (a.npad > 0 && a.npad < (1 << 25))  // charge rows below 2 GB: scalar-offset buffer loads
           hipLaunchKernelGGL(k_p2p_seg16<true>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
         else
           hipLaunchKernelGGL(k_p2p_seg16<false>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);

@@ -428,8 +428,8 @@ def main():
     t0 = time.time()
     plain = B.Plan(pos, 4, U, K, nrhs=1, prec="s", stream=stream.cuda_stream).set(scheme=B.SCHEME_IJ,
                                                                                      form=B.FORM_PLAIN)
-    chosen = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", stream=stream.cuda_stream).set(scheme=B.SCHEME_IJ,
-                                                                                       form=B.FORM_BF16X2)
+    chosen = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", stream=stream.cuda_stream).set(
+        scheme=B.SCHEME_IJ, form=B.FORM_BF16X2, p2p_form=B.P2P_BF16X2)
     t_setup = time.time() - t0
     log(f"maxlev={chosen.maxlev} ncell={chosen.ncell} interactions={chosen.m2l_interactions:.0f} "
         f"pairs={chosen.p2p_pairs:.4g} setup={t_setup:.2f}s")
@@ -469,15 +469,17 @@ def main():
     log(f"errors vs FP64 direct: plain {err_plain:.3e} chosen {errs_chosen} fp64-plain {err_fp64:.3e}")
 
     # the other arithmetic forms of the 16-vector solve on the same inputs:
-    # FP32 MFMA and the FP16 3-term split (both on the dense grids)
+    # the chosen M2L with the fp32 P2P, and the FP32 MFMA and FP16 3-term
+    # split M2L forms (dense grids, fp32 P2P)
     variants = {}
-    for name, form in (("fp32_mfma", B.FORM_FP32MFMA), ("fp16x3", B.FORM_FP16X3)):
+    for name, form, p2p in (("p2p_fp32", B.FORM_BF16X2, B.P2P_FP32), ("fp32_mfma", B.FORM_FP32MFMA, B.P2P_FP32),
+                            ("fp16x3", B.FORM_FP16X3, B.P2P_FP32)):
         vp = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", stream=stream.cuda_stream).set(scheme=B.SCHEME_IJ,
-                                                                                        form=form)
+                                                                                        form=form, p2p_form=p2p)
         phiv = torch.empty_like(qm)
         ms_v, st_v = timed_solves(vp, qm, phiv, max(args.steps // 2, 1), 1, stream)
         pv = phiv.cpu().numpy()
-        variants[name] = {"ms_per_step": ms_v, "m2l_ms": st_v["m2l"],
+        variants[name] = {"ms_per_step": ms_v, "m2l_ms": st_v["m2l"], "p2p_ms": st_v["p2p"],
                           "rel_l2_err_q_phi": [rel_l2(direct_ref[j], pv[j, tg]) for j in range(nerr)],
                           "vs_fp64_plain": rel_l2(phi64, pv[0])}
         vp.close()
@@ -508,10 +510,10 @@ def main():
                 "issued_mfma_frac": issued / (st_chosen["m2l"] * 1e-3) / (PEAK_BF16_TFLOPS * 1e12),
                 "traffic": measured_traffic("k_m2l_mfma_ss<4,true>")}
     roof_m2l["frac"] = roof_m2l["achieved"] / roof_m2l["peak"]
-    roof_p2p = {"kernel": "k_p2p_seg16<true>", "bound": "valu", "unit": "TFLOP/s",
+    roof_p2p = {"kernel": "k_p2p_seg32x<3>", "bound": "valu", "unit": "TFLOP/s",
                 "achieved": p2p_flop / (st_chosen["p2p"] * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS,
                 "launch_ms": st_chosen["p2p"], "algorithmic_flop_per_launch": p2p_flop,
-                "traffic": measured_traffic("k_p2p_seg16<true>")}
+                "traffic": measured_traffic("k_p2p_seg32x<3>")}
     roof_p2p["frac"] = roof_p2p["achieved"] / roof_p2p["peak"]
     dominant = roof_m2l if st_chosen["m2l"] >= st_chosen["p2p"] else roof_p2p
 
@@ -532,6 +534,7 @@ def main():
                                f"{nrhs} charge vectors per solve",
                    "N": N, "n": 4, "r": 32, "maxlev": chosen.maxlev, "nrhs": nrhs,
                    "m2l_scheme": "dense grid (IJ)", "m2l_form": "split-BF16 MFMA (hi+lo, 3 products), fp32 accumulate",
+                   "p2p_form": "1/r in fp32; sum over sources as split-BF16 MFMA (hi+lo, 3 products), fp32 accumulate",
                    "plain_form": "fp32 drop-in, dense-grid M2L in fp32 FMAs, one vector"},
         "s_per_solve": s_per_solve_chosen,
         "m2l_gflops": m2l_gflops_chosen,

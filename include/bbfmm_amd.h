@@ -174,6 +174,12 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                     before the scatter), 0 (default) after it; the results
                                     are the same bits.  At N = 1M the concurrent kernels
                                     compete for the CUs and the solve is slower */
+#define BBFMM_OPT_P2P_FORM 9  /* 16-vector MFMA solves (fp32 build): 0 (default) the fp32
+                                 P2P (1/r and the sum over sources in fp32); 1 1/r in fp32,
+                                 the sum over sources as three split-BF16 products (hi*hi +
+                                 hi*lo + lo*hi) with fp32 accumulation, relative error of a
+                                 term below 3 * 2^-16.  No counterpart in the reference
+                                 (its P2P is plain fp32) */
 #define BBFMM_OPT_LEVEL_SWITCH 7 /* CL/IJ schemes: first level on the dense grids, the
                                     levels below on the basic scheme (default 2) */
 

@@ -104,20 +104,21 @@ def test_grid_m2l_against_oracle(prec, scheme, form, tol, nrhs, ops4, cuda):
         assert err < tol
 
 
+@pytest.mark.parametrize("p2p", [0, 1])
 @pytest.mark.parametrize("N,lmod,nrhs,maxlev", [(65536, 1, 16, 5), (70000, 1, 16, 5), (65536, -1, 16, 3),
                                                 (20000, 0, 16, 3)])
-def test_bf16x2_many_vectors_against_oracle(N, lmod, nrhs, maxlev, ops4, cuda):
+def test_bf16x2_many_vectors_against_oracle(N, lmod, nrhs, maxlev, p2p, ops4, cuda):
     """Split-BF16 M2L with many vectors (the source-stationary MFMA kernel at the
-    deepest levels) and the MFMA P2P, full solve against the oracle; the
-    particle-row form with its fused first and last levels, from 2 to about
-    130 particles per leaf (several staging chunks, target tiles and P2P
-    passes per leaf)."""
+    deepest levels) and the MFMA P2P (fp32, or the split-BF16 sum over
+    sources), full solve against the oracle; the particle-row form with its
+    fused first and last levels, from 2 to about 130 particles per leaf
+    (several staging chunks, target tiles and P2P passes per leaf)."""
     import torch
     import bbfmmongpu_amd as B
     U, K, _ = ops4
     pos, Q = O.uniform_inputs(N, seed=21, nrhs=nrhs)
     Q = Q.astype(np.float32)
-    plan = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", lmod=lmod).set(scheme=3, form=1)
+    plan = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", lmod=lmod).set(scheme=3, form=1, p2p_form=p2p)
     assert plan.maxlev == maxlev
     phi = plan.solve(torch.from_numpy(Q).cuda()).cpu().numpy()
     tree = O.Tree(pos, lmod, "d")
@@ -127,6 +128,47 @@ def test_bf16x2_many_vectors_against_oracle(N, lmod, nrhs, maxlev, ops4, cuda):
         err = O.rel_l2(ref["phi"], phi[j])
         print(f"bf16x2 solve N={N} rhs={j}: {err:.2e}")
         assert err < 3e-5
+
+
+@pytest.mark.parametrize("N,lmod", [(65536, 1), (65536, -1), (20000, 0), (30000, 0)])
+def test_split_bf16_p2p_within_its_error_bound(N, lmod, ops4, cuda):
+    """The split-BF16 P2P changes only the near field, and each term G q of
+    it by less than 3 * 2^-16 relative (hi*hi + hi*lo + lo*hi of bf16 parts,
+    fp32 accumulation): per target and vector, |phi_split - phi_fp32| stays
+    below 3 * 2^-16 * sum_j |q_j| / r_ij (plus fp32 slack), the sum taken from
+    the oracle's near field with |q| as charges.  A wrong source, vector or
+    target index, or a dropped product, breaks the bound by orders of
+    magnitude.  2 to 130 particles per leaf (1 to 4 target tiles, several
+    passes, partial source steps)."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, Q = O.uniform_inputs(N, seed=31, nrhs=16)
+    Q = Q.astype(np.float32)
+    q = torch.from_numpy(Q).to(cuda)
+    plan = B.Plan(pos, 4, U, K, nrhs=16, prec="s", lmod=lmod).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2,
+                                                                   output="potential")
+    plan.set(p2p_form=B.P2P_FP32)
+    ref = plan.solve(q).cpu().numpy().astype(np.float64)
+    plan.set(p2p_form=B.P2P_BF16X2)
+    spl = plan.solve(q).cpu().numpy().astype(np.float64)
+    again = plan.solve(q).cpu().numpy().astype(np.float64)
+    plan.close()
+    assert np.array_equal(spl, again)
+    tree = O.Tree(pos, lmod, "d")
+    Kd, Ud = K.astype(np.float64), U.astype(np.float64)
+    for j in (0, 7, 15):
+        qa = np.abs(Q[j]).astype(np.float64)
+        out = O.fmm(tree, qa, 4, Ud, Kd, scheme="sibling")
+        absnear = (out["phi"] - out["phi_far"]) / qa  # sum_j |q_j| / r_ij per target
+        bound = 3 * 2.0 ** -16 * absnear + 1e-6 * absnear + 1e-6
+        d = np.abs(spl[j] - ref[j])
+        worst = (d / bound).max()
+        near_rel = np.linalg.norm(d) / np.linalg.norm(absnear)
+        print(f"split P2P N={N} lmod={lmod} rhs={j}: max |diff| / bound = {worst:.3f}, "
+              f"rel to the near field {near_rel:.2e}", flush=True)
+        assert worst < 1.0
+        assert near_rel > 0  # the two forms differ (the split form ran)
 
 
 def test_near_field_beside_far_field_is_bit_identical(ops4, cuda):
