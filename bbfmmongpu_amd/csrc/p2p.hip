@@ -420,12 +420,11 @@ __global__ void __launch_bounds__(64) k_p2p_seg16(P2PArgs a, int rhs0)
 // same time, so in k_p2p_seg16 the 1/r work and the MFMA accumulation (32
 // cycles per 64 pairs) add up.  Here G = 1/r (fp32) and the charges are each
 // split into bf16 hi + lo parts and the sum over sources takes three
-// products (hi*hi + hi*lo + lo*hi) of v_mfma_f32_16x16x32_bf16: 16 targets x
-// 32 sources x 16 vectors per product, fp32 accumulation.  The relative
+// products (hi*hi + hi*lo + lo*hi) of v_mfma_f32_16x16x16_bf16: 16 targets x
+// 16 sources x 16 vectors per product, fp32 accumulation.  The relative
 // error of one term is below 3 * 2^-16 (the dropped lo*lo and the rounding of
 // the lo parts).
 #ifdef SINGLE
-typedef __bf16 bf16x8p __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4p __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2p __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -458,16 +457,19 @@ __global__ void k_split_q16(int ngroups, const float *__restrict__ qt, bf16x4p *
   ql[i] = lo;
 }
 
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+
 // One pass over all sources for NT tiles of 16 targets starting at target tb.
-// A step takes 8 groups of 4 sources: lane l (target l % 16 of each tile)
-// computes G for the two groups 2 (l / 16) and 2 (l / 16) + 1 of the step,
-// which are also the k-slots of its A and B operands.  The self leaf's
-// groups are contiguous; the neighbours' groups follow the segment table in
-// the order of k_p2p_seg16, walked with scalar registers.  Groups past the
-// end take the last group's position with zero charges.
+// A step takes 4 groups of 4 sources: lane l (target l % 16 of each tile)
+// computes G for group l / 16 of the step, whose 4 sources are also its
+// k-slots in the A and B operands, so no data moves between lanes.  The self
+// leaf's groups are contiguous; the neighbours' groups follow the segment
+// table in the order of k_p2p_seg16, walked with scalar registers, two steps
+// in flight.  Groups past the end take the last group's position with zero
+// charges.
 template <int NT>
-__device__ __forceinline__ void p2p32_pass(const P2PArgs &a, int rhs0, int2 sg, int nseg, int T, int sta, int cnt,
-                                           int psta, int pself, int tb)
+__device__ __forceinline__ void p2p16x_pass(const P2PArgs &a, int rhs0, int2 sg, int nseg, int T, int sta, int cnt,
+                                            int psta, int pself, int tb)
 {
   const int l = threadIdx.x;
   const int ti = l & 15, kq = l >> 4;
@@ -482,62 +484,50 @@ __device__ __forceinline__ void p2p32_pass(const P2PArgs &a, int rhs0, int2 sg, 
     x[t] = P[psta + min(tb + 16 * t + ti, cnt - 1)];
     acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
-  // the lane's two groups at padded positions pa, pb; va / vb: they count
+  // the lane's group at padded position pa (x, y, z of its 4 sources and
+  // their hi / lo charges of vector ti); va: it counts
   struct Src {
-    f32x3 y[8];
-    bf16x4p h[2], lo[2];
+    f32x3 y[4];
+    i16x4 h, lo;
   };
-  auto load = [&](Src &u, int pa, int pb, bool va, bool vb) {
+  auto load = [&](Src &u, int pa, bool va) {
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
+    for (int e = 0; e < 4; e++)
       u.y[e] = __builtin_bit_cast(f32x3, __builtin_amdgcn_raw_buffer_load_b96(rP, (pa + e) * 16, 0, 0));
-      u.y[4 + e] = __builtin_bit_cast(f32x3, __builtin_amdgcn_raw_buffer_load_b96(rP, (pb + e) * 16, 0, 0));
-    }
-    const int oa = ((pa >> 2) * 16 + ti) * 8, ob = ((pb >> 2) * 16 + ti) * 8;
-    u.h[0] = __builtin_bit_cast(bf16x4p, __builtin_amdgcn_raw_buffer_load_b64(rH, oa, 0, 0));
-    u.lo[0] = __builtin_bit_cast(bf16x4p, __builtin_amdgcn_raw_buffer_load_b64(rL, oa, 0, 0));
-    u.h[1] = __builtin_bit_cast(bf16x4p, __builtin_amdgcn_raw_buffer_load_b64(rH, ob, 0, 0));
-    u.lo[1] = __builtin_bit_cast(bf16x4p, __builtin_amdgcn_raw_buffer_load_b64(rL, ob, 0, 0));
-    const bf16x4p z = {};
-    if (!va) u.h[0] = u.lo[0] = z;
-    if (!vb) u.h[1] = u.lo[1] = z;
+    const int oa = ((pa >> 2) * 16 + ti) * 8;
+    u.h = __builtin_bit_cast(i16x4, __builtin_amdgcn_raw_buffer_load_b64(rH, oa, 0, 0));
+    u.lo = __builtin_bit_cast(i16x4, __builtin_amdgcn_raw_buffer_load_b64(rL, oa, 0, 0));
+    const i16x4 z = {};
+    if (!va) u.h = u.lo = z;
   };
   auto step = [&](const Src &u, bool self) {
-    bf16x8p bh, bl;
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      bh[e] = u.h[0][e];
-      bh[4 + e] = u.h[1][e];
-      bl[e] = u.lo[0][e];
-      bl[4 + e] = u.lo[1][e];
-    }
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-      float g[8];
+      float g[4];
 #pragma unroll
-      for (int e = 0; e < 8; e++) {
+      for (int e = 0; e < 4; e++) {
         const float dx = u.y[e].x - x[t].x, dy = u.y[e].y - x[t].y, dz = u.y[e].z - x[t].z;
         const float rr = dx * dx + dy * dy + dz * dz;
         g[e] = __builtin_amdgcn_rsqf(rr);
         if (self) g[e] = rr != 0.f ? g[e] : 0.f;
       }
-      unsigned hw[4], lw[4];
-#pragma unroll
-      for (int e = 0; e < 4; e++) split2(g[2 * e], g[2 * e + 1], hw[e], lw[e]);
-      typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-      const bf16x8p gh = __builtin_bit_cast(bf16x8p, u32x4{hw[0], hw[1], hw[2], hw[3]});
-      const bf16x8p gl = __builtin_bit_cast(bf16x8p, u32x4{lw[0], lw[1], lw[2], lw[3]});
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gh, bh, acc[t], 0, 0, 0);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gh, bl, acc[t], 0, 0, 0);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gl, bh, acc[t], 0, 0, 0);
+      unsigned hw[2], lw[2];
+      split2(g[0], g[1], hw[0], lw[0]);
+      split2(g[2], g[3], hw[1], lw[1]);
+      typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+      const i16x4 gh = __builtin_bit_cast(i16x4, u32x2{hw[0], hw[1]});
+      const i16x4 gl = __builtin_bit_cast(i16x4, u32x2{lw[0], lw[1]});
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gh, u.h, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gh, u.lo, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gl, u.h, acc[t], 0, 0, 0);
     }
   };
   // self leaf: groups 0 .. pself/4 - 1 from psta
   const int gs = pself >> 2;
-  for (int g0 = 0; g0 < gs; g0 += 8) {
-    const int ga = g0 + 2 * kq, gb = ga + 1;
+  for (int g0 = 0; g0 < gs; g0 += 4) {
+    const int ga = g0 + kq;
     Src u;
-    load(u, psta + 4 * min(ga, gs - 1), psta + 4 * min(gb, gs - 1), ga < gs, gb < gs);
+    load(u, psta + 4 * min(ga, gs - 1), ga < gs);
     step(u, true);
   }
   if (T > 0) {
@@ -554,25 +544,24 @@ __device__ __forceinline__ void p2p32_pass(const P2PArgs &a, int rhs0, int2 sg, 
         end = pos + __builtin_amdgcn_readlane(sg.y, b);
       }
     };
-    // the 8 group positions of the next step, the lane's two picked out
+    // the 4 group positions of the next step, the lane's picked out
     auto next = [&](Src &u, int g0) {
-      int p[8];
+      int p[4];
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
+      for (int j = 0; j < 4; j++) {
         p[j] = pos;
         advance();
       }
-      const int pa = kq == 0 ? p[0] : kq == 1 ? p[2] : kq == 2 ? p[4] : p[6];
-      const int pb = kq == 0 ? p[1] : kq == 1 ? p[3] : kq == 2 ? p[5] : p[7];
-      load(u, pa, pb, g0 + 2 * kq < T, g0 + 2 * kq + 1 < T);
+      const int pa = kq == 0 ? p[0] : kq == 1 ? p[1] : kq == 2 ? p[2] : p[3];
+      load(u, pa, g0 + kq < T);
     };
     Src u0, u1;
     next(u0, 0);
-    for (int g0 = 0; g0 < T; g0 += 16) {
-      if (g0 + 8 < T) next(u1, g0 + 8);
+    for (int g0 = 0; g0 < T; g0 += 8) {
+      if (g0 + 4 < T) next(u1, g0 + 4);
       step(u0, false);
-      if (g0 + 8 >= T) break;
-      if (g0 + 16 < T) next(u0, g0 + 16);
+      if (g0 + 4 >= T) break;
+      if (g0 + 8 < T) next(u0, g0 + 8);
       step(u1, false);
     }
   }
@@ -581,10 +570,9 @@ __device__ __forceinline__ void p2p32_pass(const P2PArgs &a, int rhs0, int2 sg, 
 
 // k_p2p_seg16 with the split-BF16 accumulation; qsh / qsl filled from qt.
 // At most NTMAX target tiles per pass: the registers of a kernel are those of
-// its largest pass, and with NTMAX = 3 four waves fit on a SIMD (leaves of up
-// to 48 particles still take one pass).
+// its largest pass (NTMAX = 3: 80 VGPRs, 6 waves per SIMD; NTMAX = 4: 100).
 template <int NTMAX>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NTMAX <= 3 ? 4 : 3, 8))) k_p2p_seg32x(P2PArgs a, int rhs0)
+__global__ void __launch_bounds__(64) k_p2p_seg16x(P2PArgs a, int rhs0)
 {
   const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
   const int l = threadIdx.x;
@@ -599,16 +587,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NTMAX <
   for (int tb = 0; tb < cnt;) {
     const int tiles = min((cnt - tb + 15) >> 4, NTMAX);
     if (NTMAX >= 4 && tiles >= 4) {
-      p2p32_pass<NTMAX >= 4 ? 4 : 1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      p2p16x_pass<NTMAX >= 4 ? 4 : 1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
       tb += 64;
     } else if (NTMAX >= 3 && tiles == 3) {
-      p2p32_pass<NTMAX >= 3 ? 3 : 1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      p2p16x_pass<NTMAX >= 3 ? 3 : 1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
       tb += 48;
     } else if (tiles == 2) {
-      p2p32_pass<2>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      p2p16x_pass<2>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
       tb += 32;
     } else {
-      p2p32_pass<1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
+      p2p16x_pass<1>(a, rhs0, sg, nseg, T, sta, cnt, psta, pself, tb);
       tb += 16;
     }
   }
@@ -669,9 +657,9 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
                              (bf16x4p *)a.qsh, (bf16x4p *)a.qsl);
           BB_HIP(hipGetLastError());
           if (a.N <= 40LL * a.nleaf)  // up to about 40 particles per leaf: passes of 1 to 3 tiles
-            hipLaunchKernelGGL(k_p2p_seg32x<3>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+            hipLaunchKernelGGL(k_p2p_seg16x<3>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
           else
-            hipLaunchKernelGGL(k_p2p_seg32x<4>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
+            hipLaunchKernelGGL(k_p2p_seg16x<4>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
         } else if (a.npad > 0 && a.npad < (1 << 25))  // charge rows below 2 GB: scalar-offset buffer loads
           hipLaunchKernelGGL(k_p2p_seg16<true>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
         else

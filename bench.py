@@ -510,10 +510,10 @@ def main():
                 "issued_mfma_frac": issued / (st_chosen["m2l"] * 1e-3) / (PEAK_BF16_TFLOPS * 1e12),
                 "traffic": measured_traffic("k_m2l_mfma_ss<4,true>")}
     roof_m2l["frac"] = roof_m2l["achieved"] / roof_m2l["peak"]
-    roof_p2p = {"kernel": "k_p2p_seg32x<3>", "bound": "valu", "unit": "TFLOP/s",
+    roof_p2p = {"kernel": "k_p2p_seg16x<3>", "bound": "valu", "unit": "TFLOP/s",
                 "achieved": p2p_flop / (st_chosen["p2p"] * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS,
                 "launch_ms": st_chosen["p2p"], "algorithmic_flop_per_launch": p2p_flop,
-                "traffic": measured_traffic("k_p2p_seg32x<3>")}
+                "traffic": measured_traffic("k_p2p_seg16x<3>")}
     roof_p2p["frac"] = roof_p2p["achieved"] / roof_p2p["peak"]
     dominant = roof_m2l if st_chosen["m2l"] >= st_chosen["p2p"] else roof_p2p
 
