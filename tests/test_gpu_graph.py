@@ -31,14 +31,15 @@ def _replays_match(plan, Q, cuda):
     plan.set(graph=False)
 
 
-@pytest.mark.parametrize("near_stream", [False, True])
-def test_graph_single_plan_chosen_form(near_stream, ops4, cuda):
-    """near_stream: P2P on a side stream, forked and joined inside the graph."""
+@pytest.mark.parametrize("near_stream,p2p", [(False, 0), (True, 0), (False, 1), (True, 1)])
+def test_graph_single_plan_chosen_form(near_stream, p2p, ops4, cuda):
+    """near_stream: P2P on a side stream, forked and joined inside the graph;
+    p2p 1: the split-BF16 P2P (its charge split kernel inside the graph)."""
     import bbfmmongpu_amd as B
     U, K, _ = ops4
     pos, Q = O.uniform_inputs(65536, seed=8, nrhs=16)
     plan = B.Plan(pos, 4, U, K, nrhs=16, prec="s", lmod=1).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2,
-                                                               near_stream=near_stream)
+                                                               near_stream=near_stream, p2p_form=p2p)
     _replays_match(plan, Q.astype(np.float32), cuda)
     plan.close()
 
