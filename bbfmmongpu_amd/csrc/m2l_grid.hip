@@ -1185,6 +1185,8 @@ struct ListArgs {
   const void *Ka;
   const void *M0, *M1;
   real *L;
+  int kbytes;  // bytes of Ka
+  int mbytes;  // bytes of M0 (and M1) when below 2 GB (scalar-offset buffer loads), else 0
 };
 
 struct ListM2L {
@@ -1431,6 +1433,7 @@ int class_slot(int s, int vx, int vy, int vz)
 // products run; one barrier per step.  Same columns, order of products and
 // results as k_m2l_list_mfma<BBFMM_M2L_BF16X2> (the extra offsets a wave
 // visits add K_v times its zero sources).
+template <bool BUF>
 __global__ void __launch_bounds__(256) k_m2l_list_lds(ListArgs g)
 {
   __shared__ bf16x8 sK[2][4][64];  // [buffer][fragment: k-step x hi/lo][lane]
@@ -1471,12 +1474,28 @@ __global__ void __launch_bounds__(256) k_m2l_list_lds(ListArgs g)
     m2 = !a0 && !a1 ? m2 & (m2 - 1) : m2;
     return v;
   };
-  // this wave's fragment of offset v's operator, and a column's source
+  // this wave's fragment of offset v's operator, and a column's source.
+  // BUF: buffer loads, the operator's index a scalar offset over a fixed
+  // per-lane part, the source's offset one 32-bit lane value (grids below
+  // 2 GB), so that no 64-bit address arithmetic goes into a step
+  const __amdgpu_buffer_rsrc_t rK = __builtin_amdgcn_make_buffer_rsrc((void *)g.Ka, 0, g.kbytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rM0 = __builtin_amdgcn_make_buffer_rsrc((void *)g.M0, 0, g.mbytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rM1 = __builtin_amdgcn_make_buffer_rsrc((void *)g.M1, 0, g.mbytes, 0x00020000);
   auto load_k = [&](int v) {
     const int k = tab[v].x;
+    if (BUF)
+      return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rK, (wave * 64 + lane) * 16, k * 4096, 0));
     return *((const f32x4 *)((const bf16x8 *)g.Ka + ((size_t)k * 4 + wave) * 64 + lane));
   };
   auto load_b = [&](int S, f32x4 *t) {
+    if (BUF) {
+      const int o = (((int)rbase + S) * 32 + half * 8) * 2;
+      t[0] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rM0, o, 0, 0));
+      t[1] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rM1, o, 0, 0));
+      t[2] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rM0, o + 32, 0, 0));
+      t[3] = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rM1, o + 32, 0, 0));
+      return;
+    }
     const size_t o = (rbase + S) * 32 + half * 8;
     t[0] = *(const f32x4 *)((const __bf16 *)g.M0 + o);
     t[1] = *(const f32x4 *)((const __bf16 *)g.M1 + o);
@@ -1606,6 +1625,8 @@ int list_m2l_create(ListM2L **out, int r, int nrhs, int form, const real *K, int
       else if ((e = hipMemcpy(g->d_K, Kb.data(), sizeof(__bf16) * Kb.size(), hipMemcpyHostToDevice)) != hipSuccess)
         fail(e, "copy");
       const size_t nb2 = sizeof(__bf16) * (size_t)nrhs * (ncell + 1) * 32;
+      a.kbytes = (int)(sizeof(__bf16) * Kb.size());
+      a.mbytes = nb2 < ((size_t)1 << 31) ? (int)nb2 : 0;
       if (rc == BBFMM_OK && (e = hipMalloc(&g->d_M0, nb2)) != hipSuccess) fail(e, "hipMalloc");
       if (rc == BBFMM_OK && (e = hipMalloc(&g->d_M1, nb2)) != hipSuccess) fail(e, "hipMalloc");
       if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_M0, 0, nb2, s)) != hipSuccess) fail(e, "memset");
@@ -1677,7 +1698,10 @@ int list_m2l_run(ListM2L *g, const real *M, real *L, hipStream_t s, hipEvent_t m
     hipLaunchKernelGGL(k_m2l_list_prep<BBFMM_M2L_BF16X2>, dim3(pb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (g->nblocks) hipLaunchKernelGGL(k_m2l_list_lds, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+    if (g->nblocks && a.mbytes > 0)  // grids below 2 GB: scalar-offset buffer loads
+      hipLaunchKernelGGL(k_m2l_list_lds<true>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+    else if (g->nblocks)
+      hipLaunchKernelGGL(k_m2l_list_lds<false>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
   } else {
     hipLaunchKernelGGL(k_m2l_list_prep<BBFMM_M2L_FP32MFMA>, dim3(pb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
