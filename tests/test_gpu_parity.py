@@ -171,6 +171,41 @@ def test_split_bf16_p2p_within_its_error_bound(N, lmod, ops4, cuda):
         assert near_rel > 0  # the two forms differ (the split form ran)
 
 
+@pytest.mark.parametrize("output", ["q_phi", "potential"])
+def test_split_bf16_p2p_two_blocks_of_16_vectors(output, ops4, cuda):
+    """32 vectors: the P2P runs once per block of 16 (charges padded per
+    block, results added into phi, not the particle rows); per target and
+    vector the split form stays within 3 * 2^-16 * sum_j |q_j| / r_ij of the
+    fp32 form (times |q_i| for the q * phi output)."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    N = 20000
+    pos, Q = O.uniform_inputs(N, seed=33, nrhs=32)
+    Q = Q.astype(np.float32)
+    q = torch.from_numpy(Q).to(cuda)
+    plan = B.Plan(pos, 4, U, K, nrhs=32, prec="s", lmod=0).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2,
+                                                                output=output)
+    plan.set(p2p_form=B.P2P_FP32)
+    ref = plan.solve(q).cpu().numpy().astype(np.float64)
+    plan.set(p2p_form=B.P2P_BF16X2)
+    spl = plan.solve(q).cpu().numpy().astype(np.float64)
+    plan.close()
+    tree = O.Tree(pos, 0, "d")
+    Kd, Ud = K.astype(np.float64), U.astype(np.float64)
+    for j in (0, 15, 16, 31):
+        qa = np.abs(Q[j]).astype(np.float64)
+        out = O.fmm(tree, qa, 4, Ud, Kd, scheme="sibling")
+        absnear = (out["phi"] - out["phi_far"]) / qa
+        scale = qa if output == "q_phi" else 1.0
+        bound = (3 * 2.0 ** -16 * absnear + 1e-6 * absnear + 1e-6) * scale
+        d = np.abs(spl[j] - ref[j])
+        worst = (d / bound).max()
+        print(f"split P2P 32 vectors output={output} rhs={j}: max |diff| / bound = {worst:.3f}", flush=True)
+        assert worst < 1.0
+        assert d.max() > 0
+
+
 def test_near_field_beside_far_field_is_bit_identical(ops4, cuda):
     """P2P on the side stream (near-field rows combined by the scatter) gives
     the same bits as P2P after the far field (q * (far + near) either way),
