@@ -1174,7 +1174,7 @@ extern "C" int bbfmm_amd_plan_update_positions(bbfmm_amd_plan *p, const real3 *p
     const real inv = p->h_leafinv[k];
     const int ncpd = 1 << p->h_leafdepth[k];
     int ix = (int)((x.x + half) * inv), iy = (int)((x.y + half) * inv), iz = (int)((x.z + half) * inv);
-    if (ix < 0 || iy < 0 || iz < 0 || !(x.x <= half) || !(x.y <= half) || !(x.z <= half))
+    if (!(x.x >= -half && x.x <= half && x.y >= -half && x.y <= half && x.z >= -half && x.z <= half))
       return set_error(BBFMM_E_RANGE, "a particle lies outside the root cell [-L/2, L/2]^3");
     ix = ix < ncpd - 1 ? ix : ncpd - 1;
     iy = iy < ncpd - 1 ? iy : ncpd - 1;

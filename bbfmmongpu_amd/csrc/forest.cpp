@@ -144,7 +144,7 @@ int bucket(const real3 *pos, int n, int d, real L, std::vector<uint32_t> &key)
   for (int i = 0; i < n; i++) {
     const real x = pos[i].x, y = pos[i].y, z = pos[i].z;
     int ix = (int)((x + half) * inv), iy = (int)((y + half) * inv), iz = (int)((z + half) * inv);
-    if (ix < 0 || iy < 0 || iz < 0 || !(x <= half) || !(y <= half) || !(z <= half))
+    if (!(x >= -half && x <= half && y >= -half && y <= half && z >= -half && z <= half))
       return set_error(BBFMM_E_RANGE, "a particle lies outside the root cell [-L/2, L/2]^3");
     ix = std::min(ix, ncpd - 1);
     iy = std::min(iy, ncpd - 1);

@@ -78,8 +78,8 @@ __global__ void __launch_bounds__(64) k_l2p_grad(GradArgs a)
     real qk[MMAX], gk[MMAX];
 #pragma unroll
     for (int v = 0; v < MMAX; v++) {
-      qk[v] = v < a.m ? a.qg[(size_t)v * a.N + i] : (real)0;
-      gk[v] = v < a.m ? a.qg[(size_t)(a.m + v) * a.N + i] : (real)0;
+      qk[v] = v < a.mb ? a.qg[(size_t)(a.v0 + v) * a.N + i] : (real)0;
+      gk[v] = v < a.mb ? a.qg[(size_t)(a.m + a.v0 + v) * a.N + i] : (real)0;
     }
     real gx = 0, gy = 0, gz = 0;
 #pragma unroll 1
@@ -93,9 +93,9 @@ __global__ void __launch_bounds__(64) k_l2p_grad(GradArgs a)
           real f = 0;
 #pragma unroll
           for (int v = 0; v < MMAX; v++)
-            if (v < a.m)
-              f += gk[v] * a.F[((size_t)v * a.ncell + A) * N3 + l] +
-                   qk[v] * a.F[((size_t)(a.m + v) * a.ncell + A) * N3 + l];
+            if (v < a.mb)
+              f += gk[v] * a.F[((size_t)(a.v0 + v) * a.ncell + A) * N3 + l] +
+                   qk[v] * a.F[((size_t)(a.m + a.v0 + v) * a.ncell + A) * N3 + l];
           in_s += sz[l3] * f;
           in_d += dsz[l3] * f;
         }
@@ -104,9 +104,15 @@ __global__ void __launch_bounds__(64) k_l2p_grad(GradArgs a)
         gz += sx[l1] * sy[l2] * in_d;
       }
     const real sc = a.prefac3 * ih;
-    a.dx[i] = sc * gx;
-    a.dx[(size_t)a.N + i] = sc * gy;
-    a.dx[(size_t)2 * a.N + i] = sc * gz;
+    if (a.v0 > 0) {  // a later block of vectors: add to the earlier blocks' sum
+      a.dx[i] += sc * gx;
+      a.dx[(size_t)a.N + i] += sc * gy;
+      a.dx[(size_t)2 * a.N + i] += sc * gz;
+    } else {
+      a.dx[i] = sc * gx;
+      a.dx[(size_t)a.N + i] = sc * gy;
+      a.dx[(size_t)2 * a.N + i] = sc * gz;
+    }
   }
 }
 
@@ -132,8 +138,8 @@ __global__ void __launch_bounds__(64) k_p2p_grad(GradArgs a)
     real qk[MMAX], gk[MMAX];
 #pragma unroll
     for (int v = 0; v < MMAX; v++) {
-      qk[v] = v < a.m ? a.qg[(size_t)v * a.N + i] : (real)0;
-      gk[v] = v < a.m ? a.qg[(size_t)(a.m + v) * a.N + i] : (real)0;
+      qk[v] = v < a.mb ? a.qg[(size_t)(a.v0 + v) * a.N + i] : (real)0;
+      gk[v] = v < a.mb ? a.qg[(size_t)(a.m + a.v0 + v) * a.N + i] : (real)0;
     }
     real ax = 0, ay = 0, az = 0;
     for (int b = 0; b < nn; b++) {
@@ -147,8 +153,8 @@ __global__ void __launch_bounds__(64) k_p2p_grad(GradArgs a)
           sp[threadIdx.x] = a.pos[j];
 #pragma unroll
           for (int v = 0; v < MMAX; v++) {
-            sqg[threadIdx.x][v] = v < a.m ? a.qg[(size_t)v * a.N + j] : (real)0;
-            sqg[threadIdx.x][MMAX + v] = v < a.m ? a.qg[(size_t)(a.m + v) * a.N + j] : (real)0;
+            sqg[threadIdx.x][v] = v < a.mb ? a.qg[(size_t)(a.v0 + v) * a.N + j] : (real)0;
+            sqg[threadIdx.x][MMAX + v] = v < a.mb ? a.qg[(size_t)(a.m + a.v0 + v) * a.N + j] : (real)0;
           }
         }
         __syncthreads();
@@ -193,33 +199,37 @@ static void grad_t(const GradArgs &a, hipStream_t s)
   hipLaunchKernelGGL((k_p2p_grad<MMAX>), dim3(a.nleaf), dim3(64), 0, s, a);
 }
 
+// one block of a.mb <= 16 vectors (a.v0 the first)
 template <int NN>
-static int grad_n(const GradArgs &a, hipStream_t s)
+static void grad_n(const GradArgs &a, hipStream_t s)
 {
-  if (a.m <= 1)
+  if (a.mb <= 1)
     grad_t<NN, 1>(a, s);
-  else if (a.m <= 4)
+  else if (a.mb <= 4)
     grad_t<NN, 4>(a, s);
-  else if (a.m <= 8)
+  else if (a.mb <= 8)
     grad_t<NN, 8>(a, s);
-  else if (a.m <= 16)
-    grad_t<NN, 16>(a, s);
   else
-    return set_error(BBFMM_E_ARG, "gradient: at most 16 charge vectors (got %d)", a.m);
-  return BBFMM_OK;
+    grad_t<NN, 16>(a, s);
 }
 
-int launch_gradient(const GradArgs &a, const int *iperm, real *dx_out, hipStream_t s)
+int launch_gradient(const GradArgs &a0, const int *iperm, real *dx_out, hipStream_t s)
 {
-  if (a.nleaf <= 0 || a.N <= 0) return BBFMM_OK;
-  if (a.n == 4)
-    BB_TRY(grad_n<4>(a, s));
-  else if (a.n == 8)
-    BB_TRY(grad_n<8>(a, s));
-  else
-    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+  if (a0.nleaf <= 0 || a0.N <= 0) return BBFMM_OK;
+  if (a0.n != 4 && a0.n != 8)
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a0.n);
+  // blocks of at most 16 vectors, summed into dx in order
+  for (int v0 = 0; v0 < a0.m; v0 += 16) {
+    GradArgs a = a0;
+    a.v0 = v0;
+    a.mb = a0.m - v0 < 16 ? a0.m - v0 : 16;
+    if (a.n == 4)
+      grad_n<4>(a, s);
+    else
+      grad_n<8>(a, s);
+  }
   BB_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_scatter_vec3, dim3((a.N + 255) / 256), dim3(256), 0, s, a.N, iperm, a.dx, dx_out);
+  hipLaunchKernelGGL(k_scatter_vec3, dim3((a0.N + 255) / 256), dim3(256), 0, s, a0.N, iperm, a0.dx, dx_out);
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }

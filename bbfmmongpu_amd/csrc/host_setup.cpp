@@ -109,7 +109,7 @@ int build_tree(anotherTree *t, real L, int maxlev, int N, const real3 *pos)
   for (int i = 0; i < N; i++) {
     const real x = pos[i].x, y = pos[i].y, z = pos[i].z;
     int ix = (int)((x + half) * inv), iy = (int)((y + half) * inv), iz = (int)((z + half) * inv);
-    if (ix < 0 || iy < 0 || iz < 0 || !(x <= half) || !(y <= half) || !(z <= half)) {
+    if (!(x >= -half && x <= half && y >= -half && y <= half && z >= -half && z <= half)) {
       rc = BBFMM_E_RANGE;
       ix = iy = iz = 0;
     }

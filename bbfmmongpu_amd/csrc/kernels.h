@@ -92,7 +92,8 @@ struct M2LArgs {
   const int *h_levsta;     // host [maxlev+2]; NULL for a batch of sparse instances (basic scheme only)
   // basic / sibling schemes
   const int *iinter;       // [ncell]
-  const int *inter;        // [ncell*192]
+  const int *inter;        // [ncell*inter_pitch]
+  int inter_pitch = kPitchInteraction;
   const int *kidx;         // [ncell*192]
   const real4 *K4;         // [316][r/4][r] four consecutive columns of row i
   const int *Ktable;       // [343] (device)
@@ -165,6 +166,7 @@ int launch_p2p(const P2PArgs &a, hipStream_t s);
 // ---- adjoint (position gradient) ----
 struct GradArgs {
   int N, n, nleaf, ncell, m;  // m vectors: charges q in rows 0..m-1 of qg, dLoss/dphi in m..2m-1
+  int v0 = 0, mb = 0;         // the block of vectors v0..v0+mb-1 a launch handles (mb <= 16)
   real prefac3;
   const int *leafcell;        // [nleaf]
   const real *leafihalf;      // [nleaf]

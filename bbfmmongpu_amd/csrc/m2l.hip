@@ -15,7 +15,7 @@ namespace bb {
 
 template <int R, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_m2l_basic(int F0, int nF, int ncell, const int *__restrict__ iinter,
-                                                   const int *__restrict__ inter,
+                                                   const int *__restrict__ inter, int ipitch,
                                                    const int *__restrict__ kidx,
                                                    const real4 *__restrict__ K4,
                                                    const real *__restrict__ M, real *__restrict__ L)
@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(BLOCK) k_m2l_basic(int F0, int nF, int ncell, 
   if (F >= F0 + nF) return;  // no barrier below
   const size_t roff = (size_t)blockIdx.y * ncell * R;
   const int ni = iinter[F];
-  const int *il = inter + (size_t)kPitchInteraction * F;
+  const int *il = inter + (size_t)ipitch * F;
   const int *kl = kidx + (size_t)kPitchInteraction * F;
   real acc0 = 0, acc1 = 0;
   for (int t = 0; t < ni; t++) {
@@ -151,14 +151,14 @@ static int basic_t(const M2LArgs &a, hipStream_t s)
     // no level table (a batch of sparse instances): one launch over every
     // cell; cells without a list (levels 0-1) write L = 0
     hipLaunchKernelGGL((k_m2l_basic<R, BLOCK>), dim3((a.ncell + CPB - 1) / CPB, a.nrhs), dim3(BLOCK), 0, s, 0,
-                       a.ncell, a.ncell, a.iinter, a.inter, a.kidx, a.K4, a.M, a.L);
+                       a.ncell, a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.K4, a.M, a.L);
     BB_HIP(hipGetLastError());
     return BBFMM_OK;
   }
   for (int level = a.minlev; level <= a.maxlev; level++) {
     const int F0 = a.h_levsta[level], nF = a.h_levsta[level + 1] - F0;
     hipLaunchKernelGGL((k_m2l_basic<R, BLOCK>), dim3((nF + CPB - 1) / CPB, a.nrhs), dim3(BLOCK), 0, s, F0, nF,
-                       a.ncell, a.iinter, a.inter, a.kidx, a.K4, a.M, a.L);
+                       a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.K4, a.M, a.L);
     BB_HIP(hipGetLastError());
   }
   return BBFMM_OK;

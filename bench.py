@@ -8,21 +8,25 @@ depth 5 by the reference's level rule.  A step is one full solve (P2M, M2M,
 M2L, post-M2L, L2L, L2P, P2P) on data already resident in HBM:
 
   chosen form : 16 charge vectors per solve on one geometry, M2L on MFMA with
-                split-BF16 operands (hi + lo, 3 products) and fp32 accumulate;
+                split-BF16 operands (hi + lo, 3 products) and fp32 accumulate,
+                P2P in fp32 (1/r and the sum over sources);
   plain form  : the fp32 drop-in (one vector, dense-grid M2L in fp32 FMAs).
 
 value = per-vector solves per second of the chosen form (whole job).  Errors
 are relative L2 of q*phi against the FP64 direct sum (GPU) on 4,096 sampled
 targets.  cpu_baseline times the reference's own CPU path (CPU8A build,
-oracle/_ref) on the host on the same input.
+oracle/_ref) on the host on the same input: one thread per physical core
+available to the process, one warm-up solve, the median of 5.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import itertools
 import json
 import os
 import re
+import subprocess
 import sys
 import time
 
@@ -33,7 +37,12 @@ PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
 
 
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_c2.txt")
+ROOT = os.path.dirname(os.path.abspath(__file__))
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c2.txt")
+CSRC = os.path.join(ROOT, "bbfmmongpu_amd", "csrc")
+# source file of each kernel whose HBM traffic the bench reports
+KERNEL_SOURCE = {"k_m2l_mfma_ss<4,true>": "m2l_grid.hip", "k_p2p_seg16<true>": "p2p.hip",
+                 "k_p2p_seg16x<3>": "p2p.hip"}
 
 
 def ss_group_steps(cls: int) -> list[int]:
@@ -56,17 +65,62 @@ def ss_group_steps(cls: int) -> list[int]:
     return out
 
 
+def source_sha(kernel: str):
+    """First 12 hex digits of the SHA-256 of the source file of `kernel`."""
+    name = KERNEL_SOURCE.get(kernel)
+    if not name:
+        return None
+    with open(os.path.join(CSRC, name), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:12]
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor), or None."""
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor) and where they
+    came from.  The summary records the SHA-256 of the kernel's source file
+    when it was measured; if the source has changed since, the bytes are stale
+    and None is returned instead."""
+    src = {"file": os.path.relpath(TRAFFIC_FILE, ROOT), "src_sha": None, "current_src_sha": source_sha(kernel)}
     try:
         for line in open(TRAFFIC_FILE):
             f = dict(kv.split("=", 1) for kv in line.split() if "=" in kv)
             if f.get("kernel") == kernel and "traffic_bytes" in f:
-                return float(f["traffic_bytes"])
+                src["src_sha"] = f.get("src_sha")
+                if src["src_sha"] and src["src_sha"] == src["current_src_sha"]:
+                    return float(f["traffic_bytes"]), src
+                src["stale"] = True
+                return None, src
     except OSError:
         pass
-    return None
+    return None, src
+
+
+def host_cores():
+    """Physical cores this process may run on: the (socket, core) pairs that
+    lscpu lists for the CPUs of the affinity mask, capped by the cgroup CPU
+    quota.  Returns (threads, detail)."""
+    aff = sorted(os.sched_getaffinity(0))
+    detail = {"affinity_cpus": len(aff), "lscpu_physical_cores": None, "cgroup_quota_cpus": None}
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,SOCKET,CORE"], capture_output=True, text=True, check=True).stdout
+        rows = [l.split(",") for l in out.splitlines() if l and not l.startswith("#")]
+        detail["lscpu_physical_cores"] = len({(r[1], r[2]) for r in rows})
+        allowed = set(aff)
+        phys = len({(r[1], r[2]) for r in rows if int(r[0]) in allowed})
+        detail["physical_cores_in_affinity"] = phys
+    except (OSError, subprocess.CalledProcessError, ValueError, IndexError):
+        pass
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            detail["cgroup_quota_cpus"] = int(quota) / int(period)
+    except (OSError, ValueError):
+        pass
+    threads = phys or len(aff)
+    if detail["cgroup_quota_cpus"]:
+        threads = max(1, min(threads, int(detail["cgroup_quota_cpus"])))
+    return threads, detail
 
 
 def log(msg):
@@ -99,32 +153,40 @@ def timed_solves(plan, q, phi, steps, warmup, stream):
     return ms, stages
 
 
-def cpu_baseline(pos, q, U, K, threads):
-    """Reference CPU path (oracle/_ref, CPU8A build) on the same input."""
+def cpu_baseline(pos, q, U, K, threads, reps=5):
+    """Reference CPU path (oracle/_ref, CPU8A build) on the same input: one
+    warm-up solve, then the median of `reps` solves on the same tree."""
     import oracle as O
     if O.ref_available("s", fast=True):
         t0 = time.time()
-        out = O.ref_run(pos, q, 4, U, K, prec="s", dump=False, fast=True, threads=threads)
+        out = O.ref_run(pos, q, 4, U, K, prec="s", dump=False, fast=True, threads=threads, reps=reps + 1,
+                        bind=True)
         wall = time.time() - t0
         info = out["info"]
-        m = re.search(r"calc_performance: kernel = ([0-9.]+)", info["stderr"])
-        p = re.search(r"num_pairwise_interactions_per_sec = ([0-9.]+)", info["stderr"])
+        # the reference prints its own M2L Gflop/s and P2P rate per solve:
+        # the median over the timed solves (the first line is the warm-up)
+        m = [float(x) for x in re.findall(r"calc_performance: kernel = ([0-9.]+)", info["stderr"])][1:]
+        p = [float(x) for x in re.findall(r"num_pairwise_interactions_per_sec = ([0-9.]+)", info["stderr"])][1:]
         return {
             "kind": "reference",
             "s_per_solve": info["t_solve"],
-            "m2l_gflops": float(m.group(1)) if m else None,
-            "p2p_ginteractions_per_s": float(p.group(1)) if p else None,
+            "s_per_solve_all": info.get("t_solve_all"),
+            "m2l_gflops": float(np.median(m)) if m else None,
+            "p2p_ginteractions_per_s": float(np.median(p)) if p else None,
             "t_upward": info["t_upward"], "t_interaction": info["t_interaction"],
             "t_downwardx": info["t_downwardx"], "t_nearfield": info["t_nearfield"],
             "wall_s": wall, "phi": out["phi"],
         }
     # restated port (oracle C build) when the reference build is absent
     T = O.Tree(pos, 0, "s")
-    t0 = time.time()
-    res = O.fmm(T, q.astype(np.float32), 4, U, K, scheme="sibling")
-    t = time.time() - t0
-    return {"kind": "port", "s_per_solve": t, "m2l_gflops": None, "p2p_ginteractions_per_s": None,
-            "wall_s": t, "phi": res["phi"]}
+    ts = []
+    for _ in range(reps + 1):
+        t0 = time.time()
+        res = O.fmm(T, q.astype(np.float32), 4, U, K, scheme="sibling")
+        ts.append(time.time() - t0)
+    t = float(np.median(ts[1:]))
+    return {"kind": "port", "s_per_solve": t, "s_per_solve_all": ts, "m2l_gflops": None,
+            "p2p_ginteractions_per_s": None, "wall_s": sum(ts), "phi": res["phi"]}
 
 
 def run_c3(args):
@@ -382,7 +444,9 @@ def main():
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--targets", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (default: one per physical core available)")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU baseline solves (median; after 1 warm-up)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for HBM byte counters)")
     ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
@@ -429,7 +493,7 @@ def main():
     plain = B.Plan(pos, 4, U, K, nrhs=1, prec="s", stream=stream.cuda_stream).set(scheme=B.SCHEME_IJ,
                                                                                      form=B.FORM_PLAIN)
     chosen = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", stream=stream.cuda_stream).set(
-        scheme=B.SCHEME_IJ, form=B.FORM_BF16X2, p2p_form=B.P2P_BF16X2)
+        scheme=B.SCHEME_IJ, form=B.FORM_BF16X2, p2p_form=B.P2P_FP32)
     t_setup = time.time() - t0
     log(f"maxlev={chosen.maxlev} ncell={chosen.ncell} interactions={chosen.m2l_interactions:.0f} "
         f"pairs={chosen.p2p_pairs:.4g} setup={t_setup:.2f}s")
@@ -469,11 +533,12 @@ def main():
     log(f"errors vs FP64 direct: plain {err_plain:.3e} chosen {errs_chosen} fp64-plain {err_fp64:.3e}")
 
     # the other arithmetic forms of the 16-vector solve on the same inputs:
-    # the chosen M2L with the fp32 P2P, and the FP32 MFMA and FP16 3-term
-    # split M2L forms (dense grids, fp32 P2P)
+    # the chosen M2L with the split-BF16 P2P sum (narrower than the fp32 P2P
+    # of the stated configuration), and the FP32 MFMA and FP16 3-term split
+    # M2L forms (dense grids, fp32 P2P)
     variants = {}
-    for name, form, p2p in (("p2p_fp32", B.FORM_BF16X2, B.P2P_FP32), ("fp32_mfma", B.FORM_FP32MFMA, B.P2P_FP32),
-                            ("fp16x3", B.FORM_FP16X3, B.P2P_FP32)):
+    for name, form, p2p in (("p2p_bf16x2", B.FORM_BF16X2, B.P2P_BF16X2),
+                            ("fp32_mfma", B.FORM_FP32MFMA, B.P2P_FP32), ("fp16x3", B.FORM_FP16X3, B.P2P_FP32)):
         vp = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", stream=stream.cuda_stream).set(scheme=B.SCHEME_IJ,
                                                                                         form=form, p2p_form=p2p)
         phiv = torch.empty_like(qm)
@@ -498,23 +563,31 @@ def main():
 
     # roofline of the dominant kernel of the chosen form
     p2p_flop = pairs * (18 + 2 * nrhs)  # geometry (20-flop convention) shared by the vectors
+    p2p_valu_flop = pairs * 18          # the 1/r part alone (the sum over sources runs on MFMA)
     # issued MFMA flops of the source-stationary kernel (4 field classes per
     # wave): per 32-column tile and class group, the group's steps x 4 classes
     # x 6 products of 32x32x16 (zero operator included)
     groups = sum(((1 << (3 * (lev - 1))) * nrhs + 127) // 128 for lev in range(2, chosen.maxlev + 1))
     issued = sum(ss_group_steps(4)) * 4 * 6 * (2 * 32 * 32 * 16) * groups * 4
-    roof_m2l = {"kernel": "k_m2l_mfma_ss<4,true>", "bound": "mfma", "unit": "TFLOP/s",
+    m2l_kernel = "k_m2l_mfma_ss<4,true>"
+    tr_m2l, tr_m2l_src = measured_traffic(m2l_kernel)
+    roof_m2l = {"kernel": m2l_kernel, "bound": "mfma", "unit": "TFLOP/s",
                 "achieved": flop_m2l * nrhs / (st_chosen["m2l"] * 1e-3) / 1e12, "peak": PEAK_BF16_TFLOPS,
                 "launch_ms": st_chosen["m2l"], "algorithmic_flop_per_launch": flop_m2l * nrhs,
                 "issued_mfma_flop_per_launch": issued,
                 "issued_mfma_frac": issued / (st_chosen["m2l"] * 1e-3) / (PEAK_BF16_TFLOPS * 1e12),
-                "traffic": measured_traffic("k_m2l_mfma_ss<4,true>")}
+                "traffic": tr_m2l, "traffic_source": tr_m2l_src}
     roof_m2l["frac"] = roof_m2l["achieved"] / roof_m2l["peak"]
-    roof_p2p = {"kernel": "k_p2p_seg16x<3>", "bound": "valu", "unit": "TFLOP/s",
+    p2p_kernel = "k_p2p_seg16<true>"
+    tr_p2p, tr_p2p_src = measured_traffic(p2p_kernel)
+    roof_p2p = {"kernel": p2p_kernel, "bound": "valu", "unit": "TFLOP/s",
                 "achieved": p2p_flop / (st_chosen["p2p"] * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS,
                 "launch_ms": st_chosen["p2p"], "algorithmic_flop_per_launch": p2p_flop,
-                "traffic": measured_traffic("k_p2p_seg16x<3>")}
+                "flop_convention": "(18 + 2 * vectors) per pair: 1/r once, one FMA per vector",
+                "valu_only_achieved": p2p_valu_flop / (st_chosen["p2p"] * 1e-3) / 1e12,
+                "traffic": tr_p2p, "traffic_source": tr_p2p_src}
     roof_p2p["frac"] = roof_p2p["achieved"] / roof_p2p["peak"]
+    roof_p2p["valu_only_frac"] = roof_p2p["valu_only_achieved"] / roof_p2p["peak"]
     dominant = roof_m2l if st_chosen["m2l"] >= st_chosen["p2p"] else roof_p2p
 
     result = {
@@ -534,7 +607,7 @@ def main():
                                f"{nrhs} charge vectors per solve",
                    "N": N, "n": 4, "r": 32, "maxlev": chosen.maxlev, "nrhs": nrhs,
                    "m2l_scheme": "dense grid (IJ)", "m2l_form": "split-BF16 MFMA (hi+lo, 3 products), fp32 accumulate",
-                   "p2p_form": "1/r in fp32; sum over sources as split-BF16 MFMA (hi+lo, 3 products), fp32 accumulate",
+                   "p2p_form": "fp32 (1/r on the vector ALUs, the sum over sources on FP32 MFMA)",
                    "plain_form": "fp32 drop-in, dense-grid M2L in fp32 FMAs, one vector"},
         "s_per_solve": s_per_solve_chosen,
         "m2l_gflops": m2l_gflops_chosen,
@@ -544,6 +617,13 @@ def main():
                        "plain_q_phi": err_plain, "fp64_plain_q_phi": err_fp64,
                        "chosen_vs_fp64_plain": err_chosen_vs_fp64, "targets": int(tg.size),
                        "vectors_checked": nerr},
+        # the metric's bar (rel L2 <= 1e-4 against the FP64 direct sum) and
+        # the floor the algorithm itself sets at n = 4, r = 32: the fp64
+        # plain FMM's own error
+        "err_bar": 1e-4,
+        "meets_err_bar": bool(max(errs_pot) <= 1e-4),
+        "err_floor_fp64_plain": err_fp64,
+        "err_bar_reachable_at_n4": bool(err_fp64 <= 1e-4),
         "plain": {"s_per_solve": s_per_solve_plain, "m2l_gflops": m2l_gflops_plain,
                   "p2p_ginteractions_per_s": p2p_gips_plain, "stage_ms": st_plain},
         "chosen_stage_ms": st_chosen,
@@ -555,18 +635,23 @@ def main():
     }
 
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: reference CPU path, {threads} threads, full N={N} solve of one vector")
-        cb = cpu_baseline(pos, Q[0], U, K, threads)
+        threads, cores = host_cores()
+        if args.cpu_threads:
+            threads = args.cpu_threads
+        log(f"cpu baseline: reference CPU path, {threads} threads ({cores}), full N={N} solves of one vector")
+        cb = cpu_baseline(pos, Q[0], U, K, threads, reps=args.cpu_reps)
         phi_cpu = cb.pop("phi")
         cb_err = rel_l2(direct_ref[0], phi_cpu[tg])
         result["cpu_baseline"] = {
             "value": 1.0 / cb["s_per_solve"], "unit": "solve/s (per charge vector)", "cores": threads,
             "kind": cb["kind"],
-            "sample": f"one full solve of vector 0 at N={N} (upward + interaction + downward + near field; "
-                      f"tree build excluded), OMP_NUM_THREADS={threads}",
-            "s_per_solve": cb["s_per_solve"], "m2l_gflops": cb["m2l_gflops"],
+            "sample": f"full solves of vector 0 at N={N} (upward + interaction + downward + near field; "
+                      f"tree build excluded) on one tree: 1 warm-up, median of {args.cpu_reps}; "
+                      f"OMP_NUM_THREADS={threads} (one per physical core), OMP_PROC_BIND=close, OMP_PLACES=cores",
+            "s_per_solve": cb["s_per_solve"], "s_per_solve_all": cb["s_per_solve_all"],
+            "m2l_gflops": cb["m2l_gflops"],
             "p2p_ginteractions_per_s": cb["p2p_ginteractions_per_s"], "rel_l2_err_q_phi": cb_err,
+            "host": cores,
             "detail": {k: v for k, v in cb.items() if k.startswith("t_") or k == "wall_s"},
         }
     plain.close()

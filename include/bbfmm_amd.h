@@ -245,7 +245,8 @@ int bbfmm_amd_plan_update_positions(bbfmm_amd_plan *plan, const real3 *pos);
    dx_dev = dLoss/dpositions (N x 3, original order).  The expansions are
    recomputed by one solve of the 2m vectors [q; g]; the far part of dx
    differentiates the L2P interpolant (Chebyshev derivative) and the near
-   part the 1/r kernel over the P2P neighbourhoods.  At most 16 vectors. */
+   part the 1/r kernel over the P2P neighbourhoods (in blocks of 16 vectors
+   when m > 16). */
 int bbfmm_amd_gradient(bbfmm_amd_plan *plan, const real *q_dev, const real *g_dev, real *dq_dev,
                        real *dx_dev);
 

@@ -279,8 +279,13 @@ def write_operator_files(dirname: str, prec: str, Uf: np.ndarray, K: np.ndarray,
         f.write(np.ascontiguousarray(K, dt).tobytes())
 
 
-def ref_run(pos, q, n, Uf, K, prec="s", lmod=0, dump=True, fast=False, threads=None):
-    """Run the compiled reference CPU path (oracle/_ref) on these inputs."""
+def ref_run(pos, q, n, Uf, K, prec="s", lmod=0, dump=True, fast=False, threads=None, reps=1,
+            bind=False):
+    """Run the compiled reference CPU path (oracle/_ref) on these inputs.
+
+    reps > 1: the solve runs reps times on one tree, the first a warm-up;
+    info["t_solve"] is the median of the others.  bind: OMP_PROC_BIND=close,
+    OMP_PLACES=cores (the threads pinned one per core)."""
     exe = os.path.join(REF_DIR, f"ref_fmm_{'fast_' if fast else ''}{prec}")
     dt = dtype(prec)
     N = pos.shape[0]
@@ -289,11 +294,14 @@ def ref_run(pos, q, n, Uf, K, prec="s", lmod=0, dump=True, fast=False, threads=N
     env = dict(os.environ)
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
+    if bind:
+        env["OMP_PROC_BIND"] = "close"
+        env["OMP_PLACES"] = "cores"
     with tempfile.TemporaryDirectory() as d:
         np.ascontiguousarray(pos, dt).tofile(os.path.join(d, "pos.bin"))
         np.ascontiguousarray(q, dt).tofile(os.path.join(d, "q.bin"))
         write_operator_files(d, prec, Uf, K, n)
-        res = subprocess.run([exe, d, str(N), str(n), str(lmod), "1" if dump else "0"],
+        res = subprocess.run([exe, d, str(N), str(n), str(lmod), "1" if dump else "0", str(int(reps))],
                              capture_output=True, text=True, env=env)
         if res.returncode != 0:
             raise RuntimeError(res.stderr[-2000:])

@@ -7,16 +7,20 @@
  * files (the reference reads K/U from files as well, bbfmm.c:216-254), so no
  * LAPACK routine of the reference is reached.
  *
- * usage: ref_fmm DIR N n lmod [dump]
+ * usage: ref_fmm DIR N n lmod [dump [reps]]
  *   DIR/pos.bin  N*3 real     DIR/q.bin  N real
  *   DIR/K.bin    316*r*r real (reference K-file format)
  *   DIR/U.bin    int r, n^3*r real (reference U-file format)
  * writes DIR/phi.bin (q*phi, the reference's output convention) and, with
  * dump=1, the tree, lists, M (proxysval), F (fieldval) and the far-field phi.
- * Prints one line of timings (seconds) to stdout.
+ * Prints one line of timings (seconds) to stdout.  With reps > 1 the solve
+ * (upward pass .. near field, on the tree built once) runs reps times; the
+ * first run is a warm-up, t_solve is the median of the others and t_solve_all
+ * lists every run.  phi.bin holds the last run's output.
  */
 #include "bbfmm.h"
 #include <malloc.h>
+#include <string.h>
 #include <time.h>
 
 static double now(void)
@@ -50,12 +54,15 @@ static void write_file(const char *dir, const char *name, const void *p, size_t 
 
 int main(int argc, char **argv)
 {
-  if (argc < 5) { fprintf(stderr, "usage: %s DIR N n lmod [dump]\n", argv[0]); return 2; }
+  if (argc < 5) { fprintf(stderr, "usage: %s DIR N n lmod [dump [reps]]\n", argv[0]); return 2; }
   const char *dir = argv[1];
   const int N = atoi(argv[2]);
   const int n = atoi(argv[3]);
   int l = atoi(argv[4]);
   const int dump = argc > 5 ? atoi(argv[5]) : 0;
+  int reps = argc > 6 ? atoi(argv[6]) : 1;
+  if (reps < 1) reps = 1;
+  if (reps > 64) reps = 64;
   const int dof = 1, n3 = n * n * n;
   const real L = 1;
 
@@ -114,22 +121,48 @@ int main(int argc, char **argv)
     write_file(dir, "ref_neighbors.bin", c->neighbors, sizeof(int) * ncell * c->pitch_neighbors);
     write_file(dir, "ref_interaction.bin", c->interaction, sizeof(int) * ncell * c->pitch_interaction);
   }
-  anotherUpwardPass(&atree, field, Cweights, Tkz, q, U, Kweights, cutoff, n, dof);
-  double t2 = now();
-  if (dump) write_file(dir, "ref_M.bin", atree->proxysval, sizeof(real) * ncell * cutoff);
-  anotherFMMInteraction(&atree, K, Ktable, U, Kweights, n, dof, cutoff, homogen);
-  double t3 = now();
-  if (dump) write_file(dir, "ref_F.bin", atree->fieldval, sizeof(real) * ncell * n3);
-  anotherDownwardPassX(&atree, field, Cweights, Tkz, q, U, cutoff, n, dof, homogen, phi);
-  double t4 = now();
-  if (dump) write_file(dir, "ref_phifar.bin", phi, sizeof(real) * N);
-  anotherNearField(&atree, dof, homogen, field, field, phi, q);
-  double t5 = now();
+  double tup = 0, tint = 0, tdown = 0, tnear = 0, tsolve[64];
+  for (int rep = 0; rep < reps; rep++) {
+    const int last = rep == reps - 1;
+    memset(phi, 0, sizeof(real) * N);
+    const double s1 = now();
+    anotherUpwardPass(&atree, field, Cweights, Tkz, q, U, Kweights, cutoff, n, dof);
+    const double s2 = now();
+    if (dump && last) write_file(dir, "ref_M.bin", atree->proxysval, sizeof(real) * ncell * cutoff);
+    anotherFMMInteraction(&atree, K, Ktable, U, Kweights, n, dof, cutoff, homogen);
+    const double s3 = now();
+    if (dump && last) write_file(dir, "ref_F.bin", atree->fieldval, sizeof(real) * ncell * n3);
+    anotherDownwardPassX(&atree, field, Cweights, Tkz, q, U, cutoff, n, dof, homogen, phi);
+    const double s4 = now();
+    if (dump && last) write_file(dir, "ref_phifar.bin", phi, sizeof(real) * N);
+    anotherNearField(&atree, dof, homogen, field, field, phi, q);
+    const double s5 = now();
+    tsolve[rep] = s5 - s1;
+    if (last) {
+      tup = s2 - s1;
+      tint = s3 - s2;
+      tdown = s4 - s3;
+      tnear = s5 - s4;
+    }
+  }
   write_file(dir, "phi.bin", phi, sizeof(real) * N);
   anotherFMMCleanup(&atree);
-  printf("{\"maxlev\": %d, \"cutoff\": %d, \"t_tree\": %.6e, \"t_upward\": %.6e, "
+  /* median of the runs after the warm-up (all runs when reps == 1) */
+  const int first = reps > 1 ? 1 : 0, cnt = reps - first;
+  double sorted[64];
+  for (int k = 0; k < cnt; k++) sorted[k] = tsolve[first + k];
+  for (int a = 1; a < cnt; a++)
+    for (int b = a; b > 0 && sorted[b - 1] > sorted[b]; b--) {
+      const double x = sorted[b];
+      sorted[b] = sorted[b - 1];
+      sorted[b - 1] = x;
+    }
+  const double med = cnt % 2 ? sorted[cnt / 2] : 0.5 * (sorted[cnt / 2 - 1] + sorted[cnt / 2]);
+  printf("{\"maxlev\": %d, \"cutoff\": %d, \"reps\": %d, \"t_tree\": %.6e, \"t_upward\": %.6e, "
          "\"t_interaction\": %.6e, \"t_downwardx\": %.6e, \"t_nearfield\": %.6e, "
-         "\"t_solve\": %.6e}\n",
-         l, cutoff, t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4, t5 - t1);
+         "\"t_solve\": %.6e, \"t_solve_all\": [",
+         l, cutoff, reps, t1 - t0, tup, tint, tdown, tnear, med);
+  for (int k = 0; k < reps; k++) printf("%s%.6e", k ? ", " : "", tsolve[k]);
+  printf("]}\n");
   return 0;
 }

@@ -18,6 +18,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"^\s*#.*$", "", text, flags=re.M)  # preprocessor lines
     names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", text))
     names.discard("if")
     return sorted(names)
@@ -137,6 +138,38 @@ def test_error_paths_without_gpu():
     assert B.m2l_drv(32, Lv, K, Lv, 2, 2, 1.0, ctr, z, z, z, z, 7, prec="s") == 1
     assert b"scheme" in lib.bbfmm_amd_last_error_string()
     assert lib.bbfmm_amd_set_default_scheme(9) == 1
+
+
+@pytest.mark.parametrize("prec", ["s", "d"])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_particle_just_below_root_cell_is_rejected(prec, axis):
+    """A coordinate in (-L/2 - leaf length, -L/2) truncates to leaf index 0
+    (the reference's bucketing, anotherFMMDistribute.c:267-269), so only an
+    explicit lower-bound check rejects it; it must not be accepted and
+    extrapolated by P2M/L2P."""
+    lib = B.load(prec)
+    dt = O.dtype(prec)
+    U, K, _ = B.operators(4, 32, prec=prec)
+    pos, _ = O.uniform_inputs(100, seed=3)
+    pos = pos.astype(dt)
+    pos[17, axis] = -0.5 - 1e-3  # inside the first leaf's width below the root cell
+    h = C.c_void_p()
+    rc = lib.bbfmm_amd_plan_create(C.byref(h), 100, O._p(pos), 1.0, 4, 0, 32, O._p(K), O._p(U), 1, None)
+    assert rc == 5, lib.bbfmm_amd_last_error_string()
+    t = C.c_void_p()
+    one = 1.0
+    lib.anotherBuildFMMHierachy(one, 4, 1, 2, 32, C.byref(t), 100, 100, O._p(pos), O._p(pos))
+    assert lib.bbfmm_amd_last_error() == 5
+    lib.anotherFMMCleanup(C.byref(t))
+    Ns = np.array([50, 50], np.int32)
+    rc = lib.bbfmm_amd_batch_create(C.byref(h), 2, O._p(Ns), O._p(pos), None, 32, one, 4, 32, O._p(K), O._p(U), 1,
+                                    None, None)
+    assert rc == 5, lib.bbfmm_amd_last_error_string()
+    # exactly -L/2 is inside (leaf 0), as in the reference
+    pos[17, axis] = -0.5
+    lib.anotherBuildFMMHierachy(one, 4, 1, 2, 32, C.byref(t), 100, 100, O._p(pos), O._p(pos))
+    assert lib.bbfmm_amd_last_error() == 0
+    lib.anotherFMMCleanup(C.byref(t))
 
 
 def test_no_gpu_is_reported_not_faked():
