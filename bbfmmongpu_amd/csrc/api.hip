@@ -76,6 +76,7 @@ struct bbfmm_amd_plan {
   double m2l_interactions = 0, p2p_pairs = 0;
   // device
   real4 *d_pos = nullptr, *d_center = nullptr, *d_K4 = nullptr;
+  real *d_Kc = nullptr;  // column-major operators (ranks other than 32 and 256)
   real4 *d_pq = nullptr;  // sparse forests: positions with one vector's charge in w (lane-per-particle P2P)
   int *d_perm = nullptr, *d_leafsta = nullptr, *d_leafcnt = nullptr, *d_leafof = nullptr;
   int *d_iperm = nullptr;  // original index -> leaf order
@@ -96,6 +97,7 @@ struct bbfmm_amd_plan {
   int *d_nseg = nullptr, *d_pad_of = nullptr;
   int npad = 0;
   int potential = 0;           // output: 0 = q * potential (reference), 1 = potential
+  int l2p_ref = 0;             // n % 4 == 3: L2P with the reference's z weights (anotherDownwardPassX.c:304)
   // host copies for moving particles inside their leaves (update_positions)
   std::vector<int> h_leafof, h_pad_of, h_leafxyz, h_leafdepth;  // [N], [N], [nleaf*3], [nleaf]
   std::vector<real> h_leafinv;                                   // [nleaf] 1 / leaf length
@@ -187,7 +189,7 @@ int check_device()
 
 static void plan_free_device(bbfmm_amd_plan *p)
 {
-  void *ptrs[] = {p->d_pos, p->d_center, p->d_K4, p->d_perm, p->d_leafsta, p->d_leafcnt, p->d_leafof,
+  void *ptrs[] = {p->d_pos, p->d_center, p->d_K4, p->d_Kc, p->d_perm, p->d_leafsta, p->d_leafcnt, p->d_leafof,
                   p->d_nbr, p->d_nnbr, p->d_iinter, p->d_inter, p->d_kidx, p->d_Ktable, p->d_levsta,
                   p->d_Ucm, p->d_Urm, p->d_Tkz, p->d_Kw, p->d_Cw, p->d_scale, p->d_q, p->d_phi,
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
@@ -275,9 +277,9 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
                             real homogen, int nrhs, hipStream_t stream)
 {
   BB_TRY(check_device());
-  if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
-  if (r != 32 && r != 256) return set_error(BBFMM_E_ARG, "SVD rank r=%d not supported (r = 32 or 256)", r);
-  if (r > n * n * n) return set_error(BBFMM_E_ARG, "rank r=%d exceeds n^3=%d", r, n * n * n);
+  if (n < kMinOrder || n > kMaxOrder)
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = %d..%d)", n, kMinOrder, kMaxOrder);
+  if (r < 1 || r > n * n * n) return set_error(BBFMM_E_ARG, "rank r=%d outside 1..n^3=%d", r, n * n * n);
   if (nrhs < 1) return set_error(BBFMM_E_ARG, "nrhs must be >= 1");
   if (f.maxdepth < kMinLev) return set_error(BBFMM_E_ARG, "maxlev=%d < %d", f.maxdepth, kMinLev);
   p->complete = f.complete;
@@ -309,9 +311,12 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
   p->m2l_interactions = f.ninter;
   int Kt[343];
   ktable(Kt);
-  // operators: K4 (four consecutive columns of a row), U both ways
-  std::vector<real4> K4((size_t)kNumOps * r * r / 4);
-  for (int k = 0; k < kNumOps; k++)
+  // operators: K4 (four consecutive columns of a row; r = 32 and 256, the
+  // ranks with compiled-in kernels), column-major as given (other ranks), U
+  // both ways
+  const bool k4 = r == 32 || r == 256;
+  std::vector<real4> K4(k4 ? (size_t)kNumOps * r * r / 4 : 0);
+  for (int k = 0; k < (k4 ? kNumOps : 0); k++)
     for (int jb = 0; jb < r / 4; jb++)
       for (int i = 0; i < r; i++) {
         const real *Kk = K + (size_t)k * r * r;
@@ -357,7 +362,10 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
     BB_TRY(upload(&p->d_cls_cells, f.cls_cells.data(), f.cls_cells.size(), stream));
     BB_TRY(upload(&p->d_vsrc, f.vsrc.data(), f.vsrc.size(), stream));
   }
-  BB_TRY(upload(&p->d_K4, K4.data(), K4.size(), stream));
+  if (k4)
+    BB_TRY(upload(&p->d_K4, K4.data(), K4.size(), stream));
+  else
+    BB_TRY(upload(&p->d_Kc, K, (size_t)kNumOps * r * r, stream));
   BB_TRY(upload(&p->d_Ucm, U, (size_t)n3 * r, stream));
   BB_TRY(upload(&p->d_Urm, Urm.data(), Urm.size(), stream));
   BB_TRY(upload(&p->d_Tkz, Tkz, (size_t)n * n, stream));
@@ -564,6 +572,7 @@ static PassArgs pass_args(const bbfmm_amd_plan *p, bool rows = false)
   a.L = p->d_L;
   a.F = p->d_F;
   a.potential = p->potential;
+  a.l2p_ref = p->l2p_ref;
 #ifdef SINGLE
   a.cols = !p->complete && p->n == 4 && p->r == 32;
   a.dcp = p->d_dcp;
@@ -654,6 +663,7 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
   m.inter = p->d_inter;
   m.kidx = p->d_kidx;
   m.K4 = p->d_K4;
+  m.Kc = p->d_Kc;
   m.Ktable = p->d_Ktable;
   m.M = M;
   m.L = L;
@@ -911,7 +921,8 @@ extern "C" int bbfmm_amd_plan_create(bbfmm_amd_plan **plan, int N, const real3 *
   clear_error();
   if (!plan || (!pos && N > 0) || !K || !U || N < 0) return set_error(BBFMM_E_ARG, "null argument");
   *plan = nullptr;
-  if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
+  if (n < kMinOrder || n > kMaxOrder)
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = %d..%d)", n, kMinOrder, kMaxOrder);
   anotherTree *t = (anotherTree *)calloc(1, sizeof(anotherTree));
   const int maxlev = maxlev_rule(N > 0 ? N : 1, lmod);
   int rc = build_tree(t, L, maxlev, N, pos);
@@ -941,7 +952,8 @@ extern "C" int bbfmm_amd_batch_create(bbfmm_amd_plan **plan, int ninst, const in
   clear_error();
   if (!plan || !N || !pos || !K || !U || ninst < 1) return set_error(BBFMM_E_ARG, "null argument");
   *plan = nullptr;
-  if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
+  if (n < kMinOrder || n > kMaxOrder)
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = %d..%d)", n, kMinOrder, kMaxOrder);
   HostForest f;
   std::vector<int> dep;
   BB_TRY(forest_from_points(f, ninst, N, pos, depth, leafmax, L, &dep));
@@ -999,6 +1011,11 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
   case BBFMM_OPT_OUTPUT:
     if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "output %d: 0 (q .* potential) or 1 (potential)", value);
     p->potential = value;
+    drop_graph(p);
+    return BBFMM_OK;
+  case BBFMM_OPT_L2P_REFERENCE:
+    if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "L2P form %d: 0 (interpolant) or 1 (reference)", value);
+    p->l2p_ref = value;
     drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_M2L_VARIANT:
@@ -1229,6 +1246,7 @@ extern "C" int bbfmm_amd_gradient(bbfmm_amd_plan *p, const real *q_dev, const re
   ga.nbr = p->d_nbr;
   ga.nnbr = p->d_nnbr;
   ga.Tkz = p->d_Tkz;
+  ga.l2p_ref = p->l2p_ref;
   ga.F = p->d_F;
   ga.qg = p->d_q;
   ga.dx = p->d_grad;
@@ -1284,7 +1302,8 @@ extern "C" int bbfmm_amd_stage_times(bbfmm_amd_plan *p, double *out, int nout)
 extern "C" int bbfmm_amd_operators(int n, int r, real *U, real *K, double *sigma)
 {
   clear_error();
-  if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
+  if (n < kMinOrder || n > kMaxOrder)
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = %d..%d)", n, kMinOrder, kMaxOrder);
   const int n3 = n * n * n;
   if (r < 1 || r > n3) return set_error(BBFMM_E_ARG, "rank r=%d out of range", r);
   std::vector<double> Ud((size_t)n3 * r), Kd((size_t)kNumOps * r * r), sg((size_t)n3);
@@ -1299,7 +1318,8 @@ extern "C" int bbfmm_amd_operators(int n, int r, real *U, real *K, double *sigma
 extern "C" int bbfmm_amd_operators_device(int n, int r, real *U, real *K, double *sigma)
 {
   clear_error();
-  if (n != 4 && n != 8) return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", n);
+  if (n < kMinOrder || n > kMaxOrder)
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = %d..%d)", n, kMinOrder, kMaxOrder);
   const int n3 = n * n * n;
   if (r < 1 || r > n3) return set_error(BBFMM_E_ARG, "rank r=%d out of range", r);
   BB_TRY(check_device());
@@ -1442,6 +1462,7 @@ int dropin_plan(DropInPlan &d, anotherTree *t, const real3 *field, real *K, real
   if (rc != BBFMM_OK) return rc;
   d.p->scheme = g_default_scheme;
   d.p->form = BBFMM_M2L_PLAIN;
+  d.p->l2p_ref = 1;  // the drop-in reproduces the reference's field output, quirks included
   return BBFMM_OK;
 }
 
@@ -1589,8 +1610,8 @@ extern "C" int m2l_drv(int r, real *L, real *K, real *M, int minlev, int maxlev,
   (void)pitch_neighbors;
   (void)neighbors;
   clear_error();
-  if (r != 32 && r != 256) {
-    set_error(BBFMM_E_ARG, "m2l_drv: r=%d not supported (r = 32 or 256)", r);
+  if (r < 1 || r > kMaxRank) {
+    set_error(BBFMM_E_ARG, "m2l_drv: r=%d outside 1..%d", r, kMaxRank);
     return M2L_EXIT_FAIL;
   }
   if (scheme < M2L_SCHEME_BA || scheme > M2L_SCHEME_IJ) {
@@ -1641,8 +1662,10 @@ extern "C" int m2l_drv(int r, real *L, real *K, real *M, int minlev, int maxlev,
     tv.celeng = cl.data();
     kindex(&tv, Kt, kidx.data());
   }
-  std::vector<real4> K4((size_t)kNumOps * r * r / 4);
-  for (int k = 0; k < kNumOps; k++)
+  // K4 layout for the compiled-in ranks, the caller's column-major K otherwise
+  const bool k4 = r == 32 || r == 256;
+  std::vector<real4> K4(k4 ? (size_t)kNumOps * r * r / 4 : 0);
+  for (int k = 0; k < (k4 ? kNumOps : 0); k++)
     for (int jb = 0; jb < r / 4; jb++)
       for (int i = 0; i < r; i++) {
         const real *Kk = K + (size_t)k * r * r;
@@ -1661,7 +1684,8 @@ extern "C" int m2l_drv(int r, real *L, real *K, real *M, int minlev, int maxlev,
     if ((rc = upload(&p.d_inter, inter.data(), inter.size(), nullptr)) != BBFMM_OK) break;
     if ((rc = upload(&p.d_kidx, kidx.data(), kidx.size(), nullptr)) != BBFMM_OK) break;
     if ((rc = upload(&p.d_Ktable, Kt, (size_t)343, nullptr)) != BBFMM_OK) break;
-    if ((rc = upload(&p.d_K4, K4.data(), K4.size(), nullptr)) != BBFMM_OK) break;
+    if (k4 && (rc = upload(&p.d_K4, K4.data(), K4.size(), nullptr)) != BBFMM_OK) break;
+    if (!k4 && (rc = upload(&p.d_Kc, K, (size_t)kNumOps * r * r, nullptr)) != BBFMM_OK) break;
     if ((rc = upload(&p.d_M, M, (size_t)ncell * r, nullptr)) != BBFMM_OK) break;
     if ((rc = dalloc(&dL, (size_t)ncell * r)) != BBFMM_OK) break;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -1680,6 +1704,7 @@ extern "C" int m2l_drv(int r, real *L, real *K, real *M, int minlev, int maxlev,
     m.inter = p.d_inter;
     m.kidx = p.d_kidx;
     m.K4 = p.d_K4;
+    m.Kc = p.d_Kc;
     m.Ktable = p.d_Ktable;
     m.M = p.d_M;
     m.L = dL;

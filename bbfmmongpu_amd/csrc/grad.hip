@@ -75,6 +75,10 @@ __global__ void __launch_bounds__(64) k_l2p_grad(GradArgs a)
     cheb_val_der<NN>(a.Tkz, ih * (x.x - c.x), sx, dsx);
     cheb_val_der<NN>(a.Tkz, ih * (x.y - c.y), sy, dsy);
     cheb_val_der<NN>(a.Tkz, ih * (x.z - c.z), sz, dsz);
+    if (NN % 4 == 3 && a.l2p_ref) {  // the reference's L2P z weights (anotherDownwardPassX.c:304)
+      sz[NN - 1] = sz[NN - 2];
+      dsz[NN - 1] = dsz[NN - 2];
+    }
     real qk[MMAX], gk[MMAX];
 #pragma unroll
     for (int v = 0; v < MMAX; v++) {
@@ -216,17 +220,12 @@ static void grad_n(const GradArgs &a, hipStream_t s)
 int launch_gradient(const GradArgs &a0, const int *iperm, real *dx_out, hipStream_t s)
 {
   if (a0.nleaf <= 0 || a0.N <= 0) return BBFMM_OK;
-  if (a0.n != 4 && a0.n != 8)
-    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a0.n);
   // blocks of at most 16 vectors, summed into dx in order
   for (int v0 = 0; v0 < a0.m; v0 += 16) {
     GradArgs a = a0;
     a.v0 = v0;
     a.mb = a0.m - v0 < 16 ? a0.m - v0 : 16;
-    if (a.n == 4)
-      grad_n<4>(a, s);
-    else
-      grad_n<8>(a, s);
+    BB_WITH_ORDER(a.n, grad_n<NN>(a, s));
   }
   BB_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_scatter_vec3, dim3((a0.N + 255) / 256), dim3(256), 0, s, a0.N, iperm, a0.dx, dx_out);

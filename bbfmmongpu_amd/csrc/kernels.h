@@ -29,6 +29,7 @@ struct PassArgs {
   real *phi;            // [rhs][N] leaf order
   real *S, *M, *L, *F;
   int potential = 0;    // 1: L2P writes the potential, 0: q * potential (the reference's output)
+  int l2p_ref = 0;      // n % 4 == 3: L2P restates the reference's z weights (anotherDownwardPassX.c:304)
   // particle-row form of 16 vectors (rows = 1): charges and potentials as
   // rows of 16 in the padded leaf order of the MFMA P2P; P2M reads qt, L2P
   // writes the far-field potential into phit and vector 0's far field (with
@@ -95,7 +96,8 @@ struct M2LArgs {
   const int *inter;        // [ncell*inter_pitch]
   int inter_pitch = kPitchInteraction;
   const int *kidx;         // [ncell*192]
-  const real4 *K4;         // [316][r/4][r] four consecutive columns of row i
+  const real4 *K4;         // [316][r/4][r] four consecutive columns of row i (r = 32, 256)
+  const real *Kc = nullptr;  // [316][r][r] column-major (the reference's layout; other ranks)
   const int *Ktable;       // [343] (device)
   const real *M;           // [rhs][ncell][r]
   real *L;                 // [rhs][ncell][r]
@@ -173,6 +175,7 @@ struct GradArgs {
   const real4 *pos, *center;         // leaf order
   const int *leafsta, *leafcnt, *nbr, *nnbr;
   const real *Tkz;
+  int l2p_ref = 0;                   // n % 4 == 3: the reference's L2P z weights (as PassArgs)
   const real *F;                     // [2m][ncell][n^3] local values after L2L
   const real *qg;                    // [2m][N] leaf order
   real *dx;                          // [3][N] leaf order (scratch)

@@ -5,6 +5,8 @@
 //            source siblings of each neighbouring cluster staged in LDS
 //            (the blocking of m2l_kern_sibling_blocking.cu / CPU7A).
 // Both write L (not accumulate) for every cell of levels minlev..maxlev.
+// r = 32 and 256 have kernels with the rank fixed at compile time (K in the
+// K4 layout below); every other rank runs the *_any kernels (K column-major).
 //
 // K4 layout: K4[k][j/4][i] = (K_k(i,4j'), K_k(i,4j'+1), K_k(i,4j'+2), K_k(i,4j'+3))
 // so that the lanes of a row group read consecutive 16-byte (fp32) words.
@@ -143,6 +145,175 @@ __global__ void __launch_bounds__(R < 64 ? 64 : R) k_m2l_sibling(int level, int 
   }
 }
 
+// ---------------------------------------------------------------- any rank
+// Ranks other than 32 and 256 (the reference reaches them through its
+// generic-r path, m2l_host_ij_blocking.cu:13-17,419-428, and rejects them in
+// the basic and sibling hosts; r = n^3/2 for n = 3..8 and n^3/2 - 1 under
+// DISABLE_INCREASE_CUTOFF_BY_ONE, bbfmm.c:1340-1343).  K is read in the
+// reference's own column-major layout (K_k(i, j) at k r^2 + j r + i): the r
+// rows of a field cell are consecutive lanes, so each column's loads are
+// coalesced, and the source's M entries are the same address for all of
+// them.  A block of BLOCK threads holds BLOCK / r cells (r <= BLOCK) or one
+// cell with ceil(r / BLOCK) rows per thread.
+template <int BLOCK>
+__device__ __forceinline__ void any_rank_lane(int r, int &rt, int &cpb, int &cl, int &i0)
+{
+  rt = r < BLOCK ? r : BLOCK;
+  cpb = BLOCK / rt;
+  cl = (int)threadIdx.x / rt;
+  i0 = (int)threadIdx.x - cl * rt;
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_m2l_basic_any(int r, int F0, int nF, int ncell,
+                                                       const int *__restrict__ iinter,
+                                                       const int *__restrict__ inter, int ipitch,
+                                                       const int *__restrict__ kidx,
+                                                       const real *__restrict__ Kc,
+                                                       const real *__restrict__ M, real *__restrict__ L)
+{
+  constexpr int RPT = (kMaxRank + BLOCK - 1) / BLOCK;  // rows per thread at most
+  int rt, cpb, cl, i0;
+  any_rank_lane<BLOCK>(r, rt, cpb, cl, i0);
+  const int F = F0 + blockIdx.x * cpb + cl;
+  if (cl >= cpb || F >= F0 + nF) return;  // no barrier below
+  const size_t roff = (size_t)blockIdx.y * ncell * r;
+  const int ni = iinter[F];
+  const int *il = inter + (size_t)ipitch * F;
+  const int *kl = kidx + (size_t)kPitchInteraction * F;
+  real acc[RPT];
+#pragma unroll
+  for (int u = 0; u < RPT; u++) acc[u] = 0;
+  for (int t = 0; t < ni; t++) {
+    const real *Kk = Kc + (size_t)kl[t] * r * r + i0;
+    const real *Ms = M + roff + (size_t)il[t] * r;
+    for (int j = 0; j < r; j++) {
+      const real m = Ms[j];
+#pragma unroll
+      for (int u = 0; u < RPT; u++)
+        if (i0 + u * rt < r) acc[u] += Kk[(size_t)j * r + u * rt] * m;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < RPT; u++)
+    if (i0 + u * rt < r) L[roff + (size_t)F * r + i0 + u * rt] = acc[u];
+}
+
+// Sibling blocking at any rank: as k_m2l_sibling, a cluster's r threads
+// accumulate the rows of its 8 field siblings; the 8 source siblings of each
+// neighbouring cluster are staged in LDS ([cluster][sibling][r], dynamic).
+__global__ void __launch_bounds__(256) k_m2l_sibling_any(int r, int level, int plev0, int lev0, int ncell,
+                                                        const int *__restrict__ Ktable,
+                                                        const real *__restrict__ Kc,
+                                                        const real *__restrict__ M, real *__restrict__ L)
+{
+  constexpr int RPT = (kMaxRank + 255) / 256;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  real *sm_all = reinterpret_cast<real *>(smem_raw);
+  __shared__ int skt[343];
+  int rt, cpb, cl, i0;
+  any_rank_lane<256>(r, rt, cpb, cl, i0);
+  const int plevel = level - 1;
+  const int npar = 1 << (3 * plevel);
+  const int pidx = blockIdx.x * cpb + cl;  // local parent index
+  const bool active = cl < cpb && pidx < npar;
+  real *sm = sm_all + (size_t)(cl < cpb ? cl : 0) * 8 * r;
+  const size_t roff = (size_t)blockIdx.y * ncell * r;
+  for (int o = threadIdx.x; o < 343; o += 256) skt[o] = Ktable[o];
+  int px = 0, py = 0, pz = 0;
+  if (active) morton_decode(pidx, plevel, px, py, pz);
+  const int G = 1 << plevel;
+  real acc[8][RPT];
+#pragma unroll
+  for (int f = 0; f < 8; f++)
+#pragma unroll
+    for (int u = 0; u < RPT; u++) acc[f][u] = 0;
+  for (int d = 0; d < 27; d++) {
+    const int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+    const int sx = px + dx, sy = py + dy, sz = pz + dz;
+    const bool ok = active && (dx | dy | dz) != 0 && sx >= 0 && sy >= 0 && sz >= 0 && sx < G && sy < G && sz < G;
+    __syncthreads();
+    if (ok) {
+      const int SC = plev0 + morton_encode(sx, sy, sz, plevel);
+      const real *src = M + roff + (size_t)(8 * SC + 1) * r;
+      for (int o = i0; o < 8 * r; o += rt) sm[o] = src[o];
+    }
+    __syncthreads();
+    if (!ok) continue;
+#pragma unroll
+    for (int f = 0; f < 8; f++) {
+      const int fx = (f >> 2) & 1, fy = (f >> 1) & 1, fz = f & 1;
+      for (int s = 0; s < 8; s++) {
+        const int vx = 2 * dx + ((s >> 2) & 1) - fx;
+        const int vy = 2 * dy + ((s >> 1) & 1) - fy;
+        const int vz = 2 * dz + (s & 1) - fz;
+        const int k = skt[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
+        if (k < 0) continue;
+        const real *Kk = Kc + (size_t)k * r * r + i0;
+        const real *ms = sm + s * r;
+        for (int j = 0; j < r; j++) {
+          const real m = ms[j];
+#pragma unroll
+          for (int u = 0; u < RPT; u++)
+            if (i0 + u * rt < r) acc[f][u] += Kk[(size_t)j * r + u * rt] * m;
+        }
+      }
+    }
+  }
+  if (active) {
+    const int A0 = lev0 + 8 * pidx;
+#pragma unroll
+    for (int f = 0; f < 8; f++)
+#pragma unroll
+      for (int u = 0; u < RPT; u++)
+        if (i0 + u * rt < r) L[roff + (size_t)(A0 + f) * r + i0 + u * rt] = acc[f][u];
+  }
+}
+
+template <int BLOCK>
+static int basic_any_t(const M2LArgs &a, hipStream_t s)
+{
+  const int rt = a.r < BLOCK ? a.r : BLOCK, cpb = BLOCK / rt;
+  if (!a.h_levsta) {
+    hipLaunchKernelGGL((k_m2l_basic_any<BLOCK>), dim3((a.ncell + cpb - 1) / cpb, a.nrhs), dim3(BLOCK), 0, s, a.r,
+                       0, a.ncell, a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.Kc, a.M, a.L);
+    BB_HIP(hipGetLastError());
+    return BBFMM_OK;
+  }
+  for (int level = a.minlev; level <= a.maxlev; level++) {
+    const int F0 = a.h_levsta[level], nF = a.h_levsta[level + 1] - F0;
+    hipLaunchKernelGGL((k_m2l_basic_any<BLOCK>), dim3((nF + cpb - 1) / cpb, a.nrhs), dim3(BLOCK), 0, s, a.r, F0,
+                       nF, a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.Kc, a.M, a.L);
+    BB_HIP(hipGetLastError());
+  }
+  return BBFMM_OK;
+}
+
+static int basic_any(const M2LArgs &a, hipStream_t s)
+{
+  if (!a.Kc) return set_error(BBFMM_E_ARG, "M2L at r=%d needs the column-major operators", a.r);
+  switch (a.variant) {
+  case 0: return basic_any_t<256>(a, s);
+  case 1: return basic_any_t<128>(a, s);
+  case 2: return basic_any_t<512>(a, s);
+  default: return set_error(BBFMM_E_ARG, "M2L basic: variant %d", a.variant);
+  }
+}
+
+static int sibling_any(const M2LArgs &a, hipStream_t s)
+{
+  if (!a.Kc) return set_error(BBFMM_E_ARG, "M2L at r=%d needs the column-major operators", a.r);
+  const int rt = a.r < 256 ? a.r : 256, cpb = 256 / rt;
+  const size_t sh = sizeof(real) * (size_t)cpb * 8 * a.r;
+  for (int level = a.minlev; level <= a.maxlev; level++) {
+    const int npar = 1 << (3 * (level - 1));
+    hipLaunchKernelGGL(k_m2l_sibling_any, dim3((npar + cpb - 1) / cpb, a.nrhs), dim3(256), sh, s, a.r, level,
+                       a.h_levsta[level - 1], a.h_levsta[level], a.ncell, a.Ktable, a.Kc, a.M, a.L);
+    BB_HIP(hipGetLastError());
+  }
+  return BBFMM_OK;
+}
+
 template <int R, int BLOCK>
 static int basic_t(const M2LArgs &a, hipStream_t s)
 {
@@ -177,9 +348,10 @@ static int basic_r(const M2LArgs &a, hipStream_t s)
 
 int launch_m2l_basic(const M2LArgs &a, hipStream_t s)
 {
-  if (a.r == 32) return basic_r<32>(a, s);
-  if (a.r == 256) return basic_r<256>(a, s);
-  return set_error(BBFMM_E_ARG, "M2L: r=%d not supported (r = 32 or 256)", a.r);
+  if (a.r < 1 || a.r > kMaxRank) return set_error(BBFMM_E_ARG, "M2L: r=%d outside 1..%d", a.r, kMaxRank);
+  if (a.r == 32 && a.K4) return basic_r<32>(a, s);
+  if (a.r == 256 && a.K4) return basic_r<256>(a, s);
+  return basic_any(a, s);
 }
 
 template <int R>
@@ -199,9 +371,10 @@ static int sibling_t(const M2LArgs &a, hipStream_t s)
 int launch_m2l_sibling(const M2LArgs &a, hipStream_t s)
 {
   if (a.minlev < 2) return set_error(BBFMM_E_ARG, "M2L sibling scheme needs minlev >= 2");
-  if (a.r == 32) return sibling_t<32>(a, s);
-  if (a.r == 256) return sibling_t<256>(a, s);
-  return set_error(BBFMM_E_ARG, "M2L: r=%d not supported (r = 32 or 256)", a.r);
+  if (a.r < 1 || a.r > kMaxRank) return set_error(BBFMM_E_ARG, "M2L: r=%d outside 1..%d", a.r, kMaxRank);
+  if (a.r == 32 && a.K4) return sibling_t<32>(a, s);
+  if (a.r == 256 && a.K4) return sibling_t<256>(a, s);
+  return sibling_any(a, s);
 }
 
 } // namespace bb

@@ -30,7 +30,9 @@
 //           largest entry (measured on the device each solve); all sources
 //           of a field cell lie on its level, so the product is unscaled
 //           exactly in the epilogue.
-// MFMA forms use r = 32 (one 32 x 32 output tile per 32 columns).
+// MFMA forms use r = 32 (one 32 x 32 output tile per 32 columns); FP32 also
+// r = 256.  PLAIN takes any rank 1..512 (r = 32 and 256 with the rank fixed
+// at compile time).
 #include "common.h"
 #include "kernels.h"
 
@@ -169,10 +171,11 @@ __global__ void __launch_bounds__(256) k_level_absmax(GridArgs g, const real *__
 // ---------------------------------------------------------------- scatter
 // M [rhs][ncell][r] -> per-class grids [slot][rhs][r]; one thread per
 // (cell, rhs, 8 consecutive coefficients).
-template <int FORM>
+template <int FORM, int CW = 8>
 __global__ void k_grid_scatter(GridArgs g, const real *__restrict__ M)
 {
-  const int chunks = g.r / 8;
+  static_assert(CW == 8 || FORM == BBFMM_M2L_PLAIN, "the MFMA grids take 8 coefficients per thread");
+  const int chunks = g.r / CW;
   const long long total = (long long)(g.levsta[g.maxlev + 1] - g.levsta[g.minlev]) * g.nrhs * chunks;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
@@ -186,8 +189,8 @@ __global__ void k_grid_scatter(GridArgs g, const real *__restrict__ M)
   demorton3(A - g.levsta[l], l, x, y, z);
   const int c = ((x & 1) << 2) | ((y & 1) << 1) | (z & 1);
   const size_t slot = grid_slot(g, l, c, x >> 1, y >> 1, z >> 1);
-  const real *src = M + ((size_t)rhs * g.ncell + A) * g.r + ch * 8;
-  const size_t dst = (slot * g.nrhs + rhs) * g.r + ch * 8;
+  const real *src = M + ((size_t)rhs * g.ncell + A) * g.r + ch * CW;
+  const size_t dst = (slot * g.nrhs + rhs) * g.r + ch * CW;
   if (FORM == BBFMM_M2L_BF16X2) {
     bf16x8 hi, lo;
 #pragma unroll
@@ -216,7 +219,7 @@ __global__ void k_grid_scatter(GridArgs g, const real *__restrict__ M)
   } else {
     real *d = (real *)g.Mg0 + dst;
 #pragma unroll
-    for (int e = 0; e < 8; e++) d[e] = src[e];
+    for (int e = 0; e < CW; e++) d[e] = src[e];
   }
 }
 
@@ -762,6 +765,55 @@ __global__ void __launch_bounds__(256) k_m2l_grid_plain(GridArgs g)
   g.L[((size_t)rh * g.ncell + A) * R + i] = acc0 + acc1;
 }
 
+// Any rank (r other than 32 and 256): as k_m2l_grid_plain with K in the
+// reference's column-major layout (K_k(i, j) at k r^2 + j r + i) and scalar
+// loads; 256 / r columns per block (r <= 256) or one column with
+// ceil(r / 256) rows per thread.
+__global__ void __launch_bounds__(256) k_m2l_grid_plain_any(GridArgs g)
+{
+  constexpr int RPT = (kMaxRank + 255) / 256;
+  const int r = g.r;
+  const int rt = r < 256 ? r : 256, cpb = 256 / rt;
+  const int lb = xcd_block(g, blockIdx.x);
+  if (lb < 0) return;
+  int l, s, grp;
+  decode_block(g, lb, l, s, grp);
+  const int P = 1 << (l - 1);
+  const int PP = P + 2;
+  const int ncols = P * P * P * g.nrhs;
+  const int cl = (int)threadIdx.x / rt, i0 = (int)threadIdx.x - cl * rt;
+  const int col = grp * cpb + cl;
+  if (cl >= cpb || col >= ncols) return;  // no barrier in this kernel
+  const int pl = col / g.nrhs, rh = col - (col / g.nrhs) * g.nrhs;
+  const int pz = pl % P, py = (pl / P) % P, px = pl / (P * P);
+  const int base = ((px + 1) * PP + py + 1) * PP + pz + 1;
+  const int4 *tab = g.tab + s * kOffsets;
+  const real *Kc = (const real *)g.Ka;
+  const real *Mg = (const real *)g.Mg0;
+  const int slot0 = g.lvl_slot0[l];
+  real acc[RPT];
+#pragma unroll
+  for (int u = 0; u < RPT; u++) acc[u] = 0;
+  for (int v = 0; v < kOffsets; v++) {
+    const int4 e = tab[v];
+    const int dx = ((e.z >> 4) & 3) - 1, dy = ((e.z >> 2) & 3) - 1, dz = (e.z & 3) - 1;
+    const size_t slot = (size_t)slot0 + e.y * PP * PP * PP + (dx * PP + dy) * PP + dz + base;
+    const real *Mp = Mg + (slot * g.nrhs + rh) * r;
+    const real *Kk = Kc + (size_t)e.x * r * r + i0;
+    for (int j = 0; j < r; j++) {
+      const real m = Mp[j];
+#pragma unroll
+      for (int u = 0; u < RPT; u++)
+        if (i0 + u * rt < r) acc[u] += Kk[(size_t)j * r + u * rt] * m;
+    }
+  }
+  const int fx = 2 * px + ((s >> 2) & 1), fy = 2 * py + ((s >> 1) & 1), fz = 2 * pz + (s & 1);
+  const int A = g.levsta[l] + morton3(fx, fy, fz, l);
+#pragma unroll
+  for (int u = 0; u < RPT; u++)
+    if (i0 + u * rt < r) g.L[((size_t)rh * g.ncell + A) * r + i0 + u * rt] = acc[u];
+}
+
 // ---------------------------------------------------------------- host side
 static void offset_table(std::vector<int4> &tab)
 {
@@ -844,7 +896,9 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     return set_error(BBFMM_E_ARG, "dense-grid M2L: the source-stationary variant is split-BF16 only");
   if (variant >= 2 && (form == BBFMM_M2L_PLAIN || r != 32))
     return set_error(BBFMM_E_ARG, "dense-grid M2L: tile variants are for the r = 32 MFMA forms");
-  if (r != 32 && r != 256) return set_error(BBFMM_E_ARG, "dense-grid M2L: r=%d not supported", r);
+  if (r < 1 || r > kMaxRank) return set_error(BBFMM_E_ARG, "dense-grid M2L: r=%d outside 1..%d", r, kMaxRank);
+  if (form != BBFMM_M2L_PLAIN && r != 32 && r != 256)
+    return set_error(BBFMM_E_ARG, "dense-grid M2L: the MFMA forms need r = 32 or 256 (got r=%d)", r);
 #ifdef SINGLE
   if (form == BBFMM_M2L_BF16X2 && r != 32)
     return set_error(BBFMM_E_ARG, "the split-BF16 M2L form is built for r = 32 (got r=%d)", r);
@@ -891,7 +945,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     if (variant == 5) a.ss = 2;
   }
   if (form == BBFMM_M2L_PLAIN) {
-    cpb = 256 / r;
+    cpb = 256 / (r < 256 ? r : 256);
   } else if (a.ss) {
     cpb = 4 * 32;  // 4 waves x 32 columns, one class group per block
   } else {
@@ -926,7 +980,16 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     fail(e, "copy");
   // operators in the form's layout
   if (rc == BBFMM_OK) {
-    if (form == BBFMM_M2L_PLAIN) {
+    if (form == BBFMM_M2L_PLAIN && r != 32 && r != 256) {
+      // any rank: the caller's column-major operators as they are
+      if ((e = hipMalloc(&g->d_K, sizeof(real) * (size_t)kNumOps * r * r)) != hipSuccess) fail(e, "hipMalloc");
+      else if ((e = hipMemcpyAsync(g->d_K, K, sizeof(real) * (size_t)kNumOps * r * r, hipMemcpyHostToDevice, s)) !=
+               hipSuccess)
+        fail(e, "copy");
+      if (rc == BBFMM_OK && (e = hipStreamSynchronize(s)) != hipSuccess) fail(e, "sync");
+      if (rc == BBFMM_OK && (e = hipMalloc(&g->d_Mg0, sizeof(real) * slots * nrhs * r)) != hipSuccess) fail(e, "hipMalloc");
+      if (rc == BBFMM_OK && (e = hipMemsetAsync(g->d_Mg0, 0, sizeof(real) * slots * nrhs * r, s)) != hipSuccess) fail(e, "memset");
+    } else if (form == BBFMM_M2L_PLAIN) {
       std::vector<real4> K4((size_t)kNumOps * r * r / 4);
       for (int k = 0; k < kNumOps; k++)
         for (int jb = 0; jb < r / 4; jb++)
@@ -1090,14 +1153,20 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
   GridArgs a = g->args;
   if (ncell != a.ncell) return set_error(BBFMM_E_ARG, "dense-grid M2L: ncell %d != %d", ncell, a.ncell);
   a.L = L;
-  const long long scat = (long long)(a.levsta[a.maxlev + 1] - a.levsta[a.minlev]) * a.nrhs * (a.r / 8);
-  const int sb = (int)((scat + 255) / 256);
+  const long long cells = (long long)(a.levsta[a.maxlev + 1] - a.levsta[a.minlev]) * a.nrhs;
+  const int sb = (int)((cells * (a.r / 8) + 255) / 256);
   switch (g->form) {
   case BBFMM_M2L_PLAIN:
-    hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_PLAIN>, dim3(sb), dim3(256), 0, s, a, M);
+    if (a.r % 8 == 0)
+      hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_PLAIN>, dim3(sb), dim3(256), 0, s, a, M);
+    else
+      hipLaunchKernelGGL((k_grid_scatter<BBFMM_M2L_PLAIN, 1>), dim3((int)((cells * a.r + 255) / 256)), dim3(256), 0,
+                         s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (a.r == 32)
+    if (a.r != 32 && a.r != 256)
+      hipLaunchKernelGGL(k_m2l_grid_plain_any, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+    else if (a.r == 32)
       hipLaunchKernelGGL(k_m2l_grid_plain<32>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else
       hipLaunchKernelGGL(k_m2l_grid_plain<256>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);

@@ -9,8 +9,9 @@
 // Cells are numbered as the reference numbers them (cell.h:27-28): the
 // children of A are 8A+1..8A+8, so the children of a cell are contiguous.
 //
-// The Chebyshev order is a template parameter (n = 4 or 8, r = n^3/2),
-// the orders the reference's GPU kernels support (m2l_host_basic.cu:82-97).
+// The Chebyshev order is a template parameter, instantiated for n = 3..8
+// (the reference takes n at run time, main.c:58); the rank r is a run-time
+// argument of every kernel here except the n = 4, r = 32 MFMA kernels.
 #include "common.h"
 #include "kernels.h"
 
@@ -33,6 +34,16 @@ __device__ __forceinline__ void cheb_row(const real *__restrict__ Tkz, real x, r
     }
     out[m] = x * b1 - b2 + (real)0.5 * Tkz[m * NN];
   }
+}
+
+// The reference's L2P for n % 4 == 3 weights the last z node's value with the
+// second-to-last node's interpolation weight (anotherDownwardPassX.c:304, a
+// row reused in its remainder loop); with `on` the z weights are changed the
+// same way, otherwise the interpolant is evaluated as it is.
+template <int NN>
+__device__ __forceinline__ void l2p_reference_z(real *sz, int on)
+{
+  if (NN % 4 == 3 && on) sz[NN - 1] = sz[NN - 2];
 }
 
 // LDS writes of a wave visible to its other lanes
@@ -135,10 +146,9 @@ __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
 {
   static_assert(!ROWS || NR == 16, "particle rows hold 16 vectors");
   constexpr int N3 = NN * NN * NN;
-  constexpr int NPT = N3 / BLOCK;
+  constexpr int NPT = (N3 + BLOCK - 1) / BLOCK;  // nodes per thread (the last may be past N3)
   constexpr int CH = BLOCK;
   constexpr int QS = row_stride<NR>();
-  static_assert(N3 % BLOCK == 0, "nodes per thread");
   __shared__ real sx[NN][CH], sy[NN][CH], sz[NN][CH];
   __shared__ __align__(16) real sq[CH * QS];
   __shared__ __align__(16) real sw[N3 * NR];
@@ -184,6 +194,7 @@ __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
       const int l = threadIdx.x + p * BLOCK;
+      if (N3 % BLOCK != 0 && l >= N3) break;
       const int l1 = l / (NN * NN), l2 = (l / NN) % NN, l3 = l % NN;
       for (int j = 0; j < nk; j++) {
         const real w = sx[l1][j] * sy[l2][j] * sz[l3][j];
@@ -196,6 +207,7 @@ __global__ void __launch_bounds__(BLOCK) k_p2m(PassArgs a)
 #pragma unroll
   for (int p = 0; p < NPT; p++) {
     const int l = threadIdx.x + p * BLOCK;
+    if (N3 % BLOCK != 0 && l >= N3) break;
     const real kw = a.Kw[l];
 #pragma unroll
     for (int k = 0; k < NR; k++) {
@@ -988,6 +1000,7 @@ __global__ void __launch_bounds__(256) k_l2p(PassArgs a)
   cheb_row<NN>(a.Tkz, ih * (x.x - c.x), sxv);
   cheb_row<NN>(a.Tkz, ih * (x.y - c.y), syv);
   cheb_row<NN>(a.Tkz, ih * (x.z - c.z), szv);
+  l2p_reference_z<NN>(szv, a.l2p_ref);
   const real *FA = a.F + (size_t)rhs * a.ncell * N3 + (size_t)A * N3;
   real sum = 0;
 #pragma unroll
@@ -1034,6 +1047,7 @@ __global__ void __launch_bounds__(64) k_l2p_leaf(PassArgs a)
     cheb_row<NN>(a.Tkz, ih * (x.x - c.x), sxv);
     cheb_row<NN>(a.Tkz, ih * (x.y - c.y), syv);
     cheb_row<NN>(a.Tkz, ih * (x.z - c.z), szv);
+    l2p_reference_z<NN>(szv, a.l2p_ref);
     real sum[RS];
 #pragma unroll
     for (int k = 0; k < RS; k++) sum[k] = 0;
@@ -1405,36 +1419,37 @@ static void p2m_t(const PassArgs &a, hipStream_t s, int nrhs)
     hipLaunchKernelGGL((k_p2m<NN, BLOCK, 1>), dim3(a.nleaf, nrhs), dim3(BLOCK), 0, s, a);
 }
 
+// threads per block of the per-cell pass kernels: one per Chebyshev node up
+// to 256 (n = 7, 8: two nodes per thread in P2M)
+__host__ __device__ constexpr int node_block(int nn)
+{
+  return nn <= 4 ? 64 : nn == 5 ? 128 : 256;
+}
+
 int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs)
 {
   if (a.nleaf <= 0) return BBFMM_OK;
-  if (a.n == 4)
-    p2m_t<4, 64>(a, s, nrhs);
-  else if (a.n == 8)
-    p2m_t<8, 256>(a, s, nrhs);
-  else
-    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+  BB_WITH_ORDER(a.n, p2m_t<NN, node_block(NN)>(a, s, nrhs));
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }
 
 int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs)
 {
-  if (a.n != 4 && a.n != 8)
-    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+  if (a.n < kMinOrder || a.n > kMaxOrder)
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = %d..%d)", a.n, kMinOrder, kMaxOrder);
   for (int level = std::min(a.maxlev - 1, a.m2m_start); level >= kMinLev; level--) {
     const int off = h_ploff[level], count = h_ploff[level + 1] - off;
     if (count <= 0) continue;
-    if (a.n == 4 && nrhs == 16)
+    if (a.n == 4 && a.r == 32 && nrhs == 16)
       hipLaunchKernelGGL(k_m2m_v16, dim3(count), dim3(1024), 0, s, a, off);
 #ifdef SINGLE
     else if (a.mplist && a.T8m && a.n == 4 && a.r == 32)
       hipLaunchKernelGGL(k_m2m_cols, dim3((count + 127) / 128, nrhs), dim3(256), 0, s, a, off, count);
 #endif
-    else if (a.n == 4)
-      hipLaunchKernelGGL((k_m2m<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, off);
     else
-      hipLaunchKernelGGL((k_m2m<8, 256>), dim3(count, nrhs), dim3(256), 0, s, a, off);
+      BB_WITH_ORDER(a.n, hipLaunchKernelGGL((k_m2m<NN, node_block(NN)>), dim3(count, nrhs), dim3(node_block(NN)),
+                                            0, s, a, off));
     BB_HIP(hipGetLastError());
   }
   return BBFMM_OK;
@@ -1474,25 +1489,23 @@ int launch_postm2l(const PassArgs &a, hipStream_t s, int nrhs)
     BB_HIP(hipGetLastError());
     return BBFMM_OK;
   }
-  if (a.n == 4) return postm2l_t<4, 64>(a, s, nrhs);
-  if (a.n == 8) return postm2l_t<8, 256>(a, s, nrhs);
-  return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+  BB_WITH_ORDER(a.n, return postm2l_t<NN, node_block(NN)>(a, s, nrhs));
+  return BBFMM_OK;
 }
 
 int launch_l2l(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs)
 {
   // L2L only (parents at levels minlev..maxlev-1)
-  if (a.n != 4 && a.n != 8)
-    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+  if (a.n < kMinOrder || a.n > kMaxOrder)
+    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = %d..%d)", a.n, kMinOrder, kMaxOrder);
   for (int level = kMinLev; level < std::min(a.maxlev, a.l2l_end); level++) {
     const int off = h_ploff[level], count = h_ploff[level + 1] - off;
     if (count <= 0) continue;
     if (a.n == 4 && nrhs == 16)
       hipLaunchKernelGGL(k_l2l_v16, dim3(count), dim3(1024), 0, s, a, off);
-    else if (a.n == 4)
-      hipLaunchKernelGGL((k_l2l<4, 64>), dim3(count, nrhs), dim3(64), 0, s, a, off);
     else
-      hipLaunchKernelGGL((k_l2l<8, 256>), dim3(count, nrhs), dim3(256), 0, s, a, off);
+      BB_WITH_ORDER(a.n, hipLaunchKernelGGL((k_l2l<NN, node_block(NN)>), dim3(count, nrhs), dim3(node_block(NN)),
+                                            0, s, a, off));
     BB_HIP(hipGetLastError());
   }
   return BBFMM_OK;
@@ -1521,12 +1534,7 @@ static void l2p_t(const PassArgs &a, hipStream_t s, int nrhs)
 int launch_l2p(const PassArgs &a, hipStream_t s, int nrhs)
 {
   if (a.N <= 0) return BBFMM_OK;
-  if (a.n == 4)
-    l2p_t<4>(a, s, nrhs);
-  else if (a.n == 8)
-    l2p_t<8>(a, s, nrhs);
-  else
-    return set_error(BBFMM_E_ARG, "Chebyshev order n=%d not supported (n = 4 or 8)", a.n);
+  BB_WITH_ORDER(a.n, l2p_t<NN>(a, s, nrhs));
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }

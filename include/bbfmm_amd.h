@@ -182,6 +182,13 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                  (its P2P is plain fp32) */
 #define BBFMM_OPT_LEVEL_SWITCH 7 /* CL/IJ schemes: first level on the dense grids, the
                                     levels below on the basic scheme (default 2) */
+#define BBFMM_OPT_L2P_REFERENCE 10 /* orders n % 4 == 3 (n = 3, 7): 1 evaluates L2P as the
+                                      reference does, weighting the last z node with the
+                                      second-to-last node's interpolation weight
+                                      (anotherDownwardPassX.c:304); 0 (default) evaluates the
+                                      Chebyshev interpolant.  The drop-in entry points
+                                      (anotherFMMCompute, anotherDownwardPass) always take 1.
+                                      Other orders: no effect */
 
 /* Create a plan for N particles at host positions pos (N real3, inside the
    root cell [-L/2,L/2]^3), Chebyshev order n, level modifier lmod (the -d

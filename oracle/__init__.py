@@ -175,9 +175,11 @@ def operators(n: int, r: int | None = None):
 
 
 def fmm(tree: Tree, q: np.ndarray, n: int, Uf: np.ndarray, K: np.ndarray,
-        scheme: str = "basic"):
+        scheme: str = "basic", l2p_exact: bool = False):
     """Full oracle solve (anotherFMMCompute.c): returns a dict with M, L, F,
-    phi_far and phi (= q*phi, the reference's output convention)."""
+    phi_far and phi (= q*phi, the reference's output convention).
+    l2p_exact: at n % 4 == 3 evaluate the L2P interpolant itself instead of
+    restating the reference's z weights (anotherDownwardPassX.c:304)."""
     prec = tree.prec
     dt = dtype(prec)
     lb = lib(prec)
@@ -204,9 +206,13 @@ def fmm(tree: Tree, q: np.ndarray, n: int, Uf: np.ndarray, K: np.ndarray,
                    _p(Uf), _p(Kw), _p(Lv), _p(F))
     F0 = F.copy()
     phi = np.zeros(tree.N, dt)
-    lb.orc_downward(C.c_int(tree.maxlev), _r(prec, tree.L), C.c_int(n), _p(tree.center),
-                    _p(tree.list), _p(tree.sta), _p(tree.end), _p(tree.pos), _p(q),
-                    _p(Tkz), _p(Cw), _p(F), _p(phi))
+    lb.orc_set_l2p_exact(C.c_int(int(bool(l2p_exact))))
+    try:
+        lb.orc_downward(C.c_int(tree.maxlev), _r(prec, tree.L), C.c_int(n), _p(tree.center),
+                        _p(tree.list), _p(tree.sta), _p(tree.end), _p(tree.pos), _p(q),
+                        _p(Tkz), _p(Cw), _p(F), _p(phi))
+    finally:
+        lb.orc_set_l2p_exact(C.c_int(0))
     phi_far = phi.copy()
     lb.orc_nearfield(C.c_int(tree.maxlev), C.c_int(tree.N), _p(tree.list), _p(tree.sta),
                      _p(tree.end), _p(tree.ineigh), _p(tree.neighbors), _p(tree.pos), _p(q),

@@ -759,6 +759,11 @@ void orc_postm2l(int maxlev, real L, int n, int r, real homogen, const real *U,
 /* ------------------------------------------------------------------ */
 /* Downward pass: anotherDownwardPassX.c:40-211 (L2L), :226-424 (L2P)  */
 /* ------------------------------------------------------------------ */
+/* L2P of orders n % 4 == 3: 0 (default) restates the reference's reuse of
+   its second-to-last z row; 1 evaluates the Chebyshev interpolant itself. */
+static int orc_l2p_exact = 0;
+void orc_set_l2p_exact(int on) { orc_l2p_exact = on != 0; }
+
 void orc_downward(int maxlev, real L, int n, const orc_real3 *center,
                   const int *list, const int *sta, const int *end,
                   const orc_real3 *pos, const real *q, const real *Tkz,
@@ -859,10 +864,12 @@ void orc_downward(int maxlev, real L, int n, const orc_real3 *center,
             for (int i = 0; i < Nf; i++) sum[i] += x1[i] * y2[i] * (z0[i] * FA[l] + z1[i] * FA[l + 1]);
             l += 2;
           } else if (n % 4 == 3) {
-            /* the reference uses its second z row twice here (anotherDownwardPassX.c:304) */
+            /* the reference uses its second z row twice here (anotherDownwardPassX.c:304);
+               orc_set_l2p_exact(1) takes the last row instead (the interpolant itself) */
             const real *z0 = &Sz[(n - 3) * Nf], *z1 = &Sz[(n - 2) * Nf];
+            const real *z2 = orc_l2p_exact ? &Sz[(n - 1) * Nf] : z1;
             for (int i = 0; i < Nf; i++)
-              sum[i] += x1[i] * y2[i] * (z0[i] * FA[l] + z1[i] * FA[l + 1] + z1[i] * FA[l + 2]);
+              sum[i] += x1[i] * y2[i] * (z0[i] * FA[l] + z1[i] * FA[l + 1] + z2[i] * FA[l + 2]);
             l += 3;
           }
         }

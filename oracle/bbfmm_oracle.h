@@ -66,6 +66,9 @@ void orc_m2l_sibling(int r, int minlev, int maxlev, const real *K, const real *M
                      const int *sourceclusters, const int *Ktable, real *Lv);
 void orc_postm2l(int maxlev, real L, int n, int r, real homogen, const real *U,
                  const real *Kweights, const real *Lv, real *F);
+/* L2P at n % 4 == 3: 0 restates the reference (anotherDownwardPassX.c:304),
+   1 evaluates the interpolant exactly */
+void orc_set_l2p_exact(int on);
 void orc_downward(int maxlev, real L, int n, const orc_real3 *center,
                   const int *list, const int *sta, const int *end,
                   const orc_real3 *pos, const real *q, const real *Tkz,

@@ -34,3 +34,29 @@ def cuda():
     if not torch.cuda.is_available():
         pytest.skip("no GPU visible")
     return torch.device("cuda:0")
+
+
+def truncated_operators(U, K, n, r):
+    """Rank-r operators from a set of rank >= r: U's first r columns and the
+    leading r x r block of every E_k = U^T K0_k U (column-major)."""
+    import numpy as np
+    n3 = n ** 3
+    r0 = U.size // n3
+    assert r <= r0, (r, r0)
+    Ur = np.ascontiguousarray(U.reshape(r0, n3)[:r]).ravel()
+    Kr = np.ascontiguousarray(K.reshape(316, r0, r0)[:, :r, :r]).ravel()
+    return Ur, Kr
+
+
+_ops_rank = {}
+
+
+def operators_upto(n, r):
+    """Oracle operators of order n at rank r (from one cached set of rank
+    max(r, n^3/2) per order)."""
+    import oracle as O
+    rmax = max(r, n ** 3 // 2)
+    if n not in _ops_rank or _ops_rank[n][0].size // n ** 3 < rmax:
+        _ops_rank[n] = O.operators(n, rmax)
+    U, K, _ = _ops_rank[n]
+    return truncated_operators(U, K, n, r)

@@ -121,9 +121,10 @@ def test_error_paths_without_gpu():
     U, K, _ = B.operators(4, 32, prec="s")
     pos = np.zeros((10, 3), np.float32)
     h = C.c_void_p()
-    # unsupported order: rejected before any device call
-    rc = lib.bbfmm_amd_plan_create(C.byref(h), 10, O._p(pos), 1.0, 5, 0, 32, O._p(K), O._p(U), 1, None)
-    assert rc == 1 and b"not supported" in lib.bbfmm_amd_last_error_string()
+    # unsupported orders (outside 3..8): rejected before any device call
+    for n in (2, 9):
+        rc = lib.bbfmm_amd_plan_create(C.byref(h), 10, O._p(pos), 1.0, n, 0, 32, O._p(K), O._p(U), 1, None)
+        assert rc == 1 and b"not supported" in lib.bbfmm_amd_last_error_string()
     # particle outside the root cell
     bad = np.full((10, 3), 0.9, np.float32)
     rc = lib.bbfmm_amd_plan_create(C.byref(h), 10, O._p(bad), 1.0, 4, 0, 32, O._p(K), O._p(U), 1, None)
@@ -133,8 +134,9 @@ def test_error_paths_without_gpu():
     z = np.zeros(nc * 192, np.int32)
     Lv = np.zeros(nc * 32, np.float32)
     ctr = np.zeros((nc, 3), np.float32)
-    assert B.m2l_drv(48, Lv, K, Lv, 2, 2, 1.0, ctr, z, z, z, z, 0, prec="s") == 1
-    assert b"r=48" in lib.bbfmm_amd_last_error_string()
+    for r in (0, 513):  # any rank 1..512 runs (the any-rank kernels)
+        assert B.m2l_drv(r, Lv, K, Lv, 2, 2, 1.0, ctr, z, z, z, z, 0, prec="s") == 1
+        assert f"r={r}".encode() in lib.bbfmm_amd_last_error_string()
     assert B.m2l_drv(32, Lv, K, Lv, 2, 2, 1.0, ctr, z, z, z, z, 7, prec="s") == 1
     assert b"scheme" in lib.bbfmm_amd_last_error_string()
     assert lib.bbfmm_amd_set_default_scheme(9) == 1
