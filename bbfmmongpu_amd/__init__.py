@@ -27,6 +27,7 @@ OPT_SCHEME, OPT_FORM, OPT_TIMING, OPT_OUTPUT, OPT_GRAPH, OPT_M2L_VARIANT, OPT_LE
 OPT_NEAR_STREAM = 8
 OPT_P2P_FORM = 9
 OPT_L2P_REFERENCE = 10
+OPT_ADDRESSING = 11
 P2P_FP32, P2P_BF16X2 = 0, 1
 # "p2p_beside": P2P's own time when it runs on the side stream beside the far
 # field ("p2p" is then the wait for it after the far field)
@@ -260,10 +261,12 @@ class Plan:
     def set(self, scheme: int | None = None, form: int | None = None, timing: int | None = None,
             output: str | None = None, graph: bool | None = None, variant: int | None = None,
             level_switch: int | None = None, near_stream: bool | None = None, p2p_form: int | None = None,
-            l2p_reference: bool | None = None):
+            l2p_reference: bool | None = None, flat_loads: bool | None = None):
         """output: "q_phi" (q .* potential, the reference's output) or "potential";
         l2p_reference: orders n = 3, 7 evaluate L2P with the reference's z
         weights (anotherDownwardPassX.c:304) instead of the interpolant;
+        flat_loads: kernels with buffer-load and pointer forms take the
+        pointer form (their form for arrays of 2 GB and more) at any size;
         graph: capture the whole solve into one hipGraph and replay it;
         near_stream: 16-vector MFMA solves run P2P beside the far field;
         p2p_form: 16-vector MFMA solves, P2P_FP32 (default) or P2P_BF16X2
@@ -286,6 +289,8 @@ class Plan:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_NEAR_STREAM, int(bool(near_stream))))
         if p2p_form is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_P2P_FORM, int(p2p_form)))
+        if flat_loads is not None:
+            _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_ADDRESSING, int(bool(flat_loads))))
         if l2p_reference is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_L2P_REFERENCE, int(bool(l2p_reference))))
         if graph is not None:

@@ -98,6 +98,7 @@ struct bbfmm_amd_plan {
   int npad = 0;
   int potential = 0;           // output: 0 = q * potential (reference), 1 = potential
   int l2p_ref = 0;             // n % 4 == 3: L2P with the reference's z weights (anotherDownwardPassX.c:304)
+  int flat_loads = 0;          // BBFMM_OPT_ADDRESSING: pointer-form loads in every kernel that has both forms
   // host copies for moving particles inside their leaves (update_positions)
   std::vector<int> h_leafof, h_pad_of, h_leafxyz, h_leafdepth;  // [N], [N], [nleaf*3], [nleaf]
   std::vector<real> h_leafinv;                                   // [nleaf] 1 / leaf length
@@ -607,6 +608,7 @@ static int ensure_grid(bbfmm_amd_plan *p)
                              p->d_cls_cells, p->d_vsrc, p->stream));
       p->list_form = p->form;
     }
+    list_m2l_set_flat(p->list, p->flat_loads);
     return BBFMM_OK;
   }
   const int form = p->scheme == M2L_SCHEME_CL ? BBFMM_M2L_PLAIN : p->form;
@@ -620,6 +622,7 @@ static int ensure_grid(bbfmm_amd_plan *p)
     p->grid_minlev = minlev;
     p->grid_variant = variant;
   }
+  grid_m2l_set_flat(p->grid, p->flat_loads);
   return BBFMM_OK;
 }
 
@@ -720,6 +723,7 @@ static P2PArgs p2p_args(const bbfmm_amd_plan *p)
   pp.nseg = p->d_nseg;
   pp.pad_of = p->d_pad_of;
   pp.npad = p->npad;
+  pp.flat = p->flat_loads;
   if (p->p2p_split && p->d_qsh) {
     pp.qsh = p->d_qsh;
     pp.qsl = p->d_qsl;
@@ -1011,6 +1015,11 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
   case BBFMM_OPT_OUTPUT:
     if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "output %d: 0 (q .* potential) or 1 (potential)", value);
     p->potential = value;
+    drop_graph(p);
+    return BBFMM_OK;
+  case BBFMM_OPT_ADDRESSING:
+    if (value != 0 && value != 1) return set_error(BBFMM_E_ARG, "addressing %d: 0 (automatic) or 1 (pointers)", value);
+    p->flat_loads = value;
     drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_L2P_REFERENCE:

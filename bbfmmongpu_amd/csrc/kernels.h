@@ -114,6 +114,9 @@ struct GridM2L;  // defined in m2l_grid.hip
 int grid_m2l_create(GridM2L **g, int r, int maxlev, int nrhs, int form, const real *K_colmajor_host,
                     hipStream_t s, int minlev, int variant);
 void grid_m2l_destroy(GridM2L *g);
+// flat = 1: the split-BF16 source-stationary kernel reads through pointers
+// (its form for grids of 2 GB and more) whatever the grid size
+void grid_m2l_set_flat(GridM2L *g, int flat);
 // records `mid` (if not null) between the scatter into the grids and the M2L kernel
 int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, hipEvent_t mid);
 
@@ -125,6 +128,8 @@ struct ListM2L;  // defined in m2l_grid.hip
 int list_m2l_create(ListM2L **g, int r, int nrhs, int form, const real *K_colmajor_host, int ncell,
                     const int *cls_off, const int *d_cls_cells, const int *d_vsrc, hipStream_t s);
 void list_m2l_destroy(ListM2L *g);
+// flat = 1: pointer-form loads (the form for arrays of 2 GB and more) always
+void list_m2l_set_flat(ListM2L *g, int flat);
 int list_m2l_run(ListM2L *g, const real *M, real *L, hipStream_t s, hipEvent_t mid);
 // slot of offset (vx, vy, vz) among the 189 M2L offsets of sibling class s, or -1
 int class_slot(int s, int vx, int vy, int vz);
@@ -162,6 +167,9 @@ struct P2PArgs {
   // hi and lo bf16 parts of qt at [padded source / 4][vector][4 sources],
   // written from qt before the pass; used when set
   void *qsh = nullptr, *qsl = nullptr;
+  // 1: the fp32 16-vector P2P reads through pointers (its form at 2^25 padded
+  // particles and more) whatever the size
+  int flat = 0;
 };
 int launch_p2p(const P2PArgs &a, hipStream_t s);
 

@@ -79,6 +79,7 @@ constexpr int kWaitVm0 = 0x0F70;
 
 struct GridM2L {
   int r = 0, maxlev = 0, nrhs = 1, form = 0, ncell_cap = 0, minlev = kMinLev, variant = 0;
+  int flat = 0;  // 1: pointer-form loads even on grids below 2 GB
   GridArgs args{};
   int nblocks = 0;
   size_t nslots = 0;
@@ -765,6 +766,85 @@ __global__ void __launch_bounds__(256) k_m2l_grid_plain(GridArgs g)
   g.L[((size_t)rh * g.ncell + A) * R + i] = acc0 + acc1;
 }
 
+// r = 32, real precision: a block is 64 columns (lanes) of one class at one
+// level and wave w computes rows 8w..8w+7 of all of them.  K_v (column-major,
+// the reference's layout) is the same for every lane of a wave, so its
+// entries arrive by scalar loads into SGPRs: each FMA reads one SGPR (K) and
+// one VGPR (a coefficient of the lane's source column), no LDS or vector
+// load is spent on K, and the only vector loads are the lane's 32-coefficient
+// source column per offset, issued one offset ahead.
+__global__ void __launch_bounds__(256) k_m2l_grid_plain32(GridArgs g)
+{
+  const int lb = xcd_block(g, blockIdx.x);
+  if (lb < 0) return;
+  int l, s, grp;
+  decode_block(g, lb, l, s, grp);
+  const int P = 1 << (l - 1);
+  const int PP = P + 2;
+  const int ncols = P * P * P * g.nrhs;
+  const int lane = threadIdx.x & 63;
+  const int rg = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);  // row group
+  const int col = grp * 64 + lane;
+  const bool valid = col < ncols;
+  const int cc = valid ? col : 0;  // columns past the end compute column 0 and are not stored
+  const int pl = cc / g.nrhs, rh = cc - (cc / g.nrhs) * g.nrhs;
+  const int pz = pl % P, py = (pl / P) % P, px = pl / (P * P);
+  const int base = ((px + 1) * PP + py + 1) * PP + pz + 1;
+  const int4 *__restrict__ tab = g.tab + s * kOffsets;
+  const real *__restrict__ Kc = (const real *)g.Ka + rg * 8;
+  const real *__restrict__ Mg = (const real *)g.Mg0;
+  const size_t slot0 = (size_t)g.lvl_slot0[l] + base;
+  const int cls = PP * PP * PP;
+  auto src = [&](int v) {
+    const int4 e = tab[v];
+    const int dx = ((e.z >> 4) & 3) - 1, dy = ((e.z >> 2) & 3) - 1, dz = (e.z & 3) - 1;
+    const size_t slot = slot0 + e.y * cls + (dx * PP + dy) * PP + dz;
+    return reinterpret_cast<const real4 *>(Mg + (slot * g.nrhs + rh) * 32);
+  };
+  real acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) acc[i] = 0;
+  auto load = [&](int v, real4 *m) {
+    const real4 *p = src(v);
+#pragma unroll
+    for (int q = 0; q < 8; q++) m[q] = p[q];
+  };
+  auto mac = [&](int v, const real4 *m) {
+    const real *__restrict__ Kk = Kc + (size_t)tab[v].x * 1024;
+#pragma unroll
+    for (int jq = 0; jq < 8; jq++) {
+      const real mj[4] = {m[jq].x, m[jq].y, m[jq].z, m[jq].w};
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const real *__restrict__ kc = Kk + (4 * jq + t) * 32;  // column j: rows 8 rg .. 8 rg + 7
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc[i] += kc[i] * mj[t];
+      }
+    }
+  };
+  // two register sets in turn: the next offset's column is in flight while
+  // the current one is multiplied (no copy between the sets, so no wait for
+  // the loads at the end of an iteration)
+  real4 ma[8], mb[8];
+  load(0, ma);
+  static_assert(kOffsets % 2 == 1, "the loop below ends on an odd offset count");
+  for (int v = 0; v + 1 < kOffsets; v += 2) {
+    load(v + 1, mb);
+    __builtin_amdgcn_sched_barrier(0);  // loads issued before the products
+    mac(v, ma);
+    load(v + 2, ma);
+    __builtin_amdgcn_sched_barrier(0);
+    mac(v + 1, mb);
+  }
+  mac(kOffsets - 1, ma);
+  if (!valid) return;
+  const int fx = 2 * px + ((s >> 2) & 1), fy = 2 * py + ((s >> 1) & 1), fz = 2 * pz + (s & 1);
+  const int A = g.levsta[l] + morton3(fx, fy, fz, l);
+  real *Lp = g.L + ((size_t)rh * g.ncell + A) * 32 + rg * 8;
+#pragma unroll
+  for (int i = 0; i < 8; i++) Lp[i] = acc[i];
+}
+
 // Any rank (r other than 32 and 256): as k_m2l_grid_plain with K in the
 // reference's column-major layout (K_k(i, j) at k r^2 + j r + i) and scalar
 // loads; 256 / r columns per block (r <= 256) or one column with
@@ -945,7 +1025,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     if (variant == 5) a.ss = 2;
   }
   if (form == BBFMM_M2L_PLAIN) {
-    cpb = 256 / (r < 256 ? r : 256);
+    cpb = r == 32 ? 64 : 256 / (r < 256 ? r : 256);  // k_m2l_grid_plain32: 64 columns per block
   } else if (a.ss) {
     cpb = 4 * 32;  // 4 waves x 32 columns, one class group per block
   } else {
@@ -980,8 +1060,8 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     fail(e, "copy");
   // operators in the form's layout
   if (rc == BBFMM_OK) {
-    if (form == BBFMM_M2L_PLAIN && r != 32 && r != 256) {
-      // any rank: the caller's column-major operators as they are
+    if (form == BBFMM_M2L_PLAIN && r != 256) {
+      // r = 32 and any rank: the caller's column-major operators as they are
       if ((e = hipMalloc(&g->d_K, sizeof(real) * (size_t)kNumOps * r * r)) != hipSuccess) fail(e, "hipMalloc");
       else if ((e = hipMemcpyAsync(g->d_K, K, sizeof(real) * (size_t)kNumOps * r * r, hipMemcpyHostToDevice, s)) !=
                hipSuccess)
@@ -1122,6 +1202,11 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   return BBFMM_OK;
 }
 
+void grid_m2l_set_flat(GridM2L *g, int flat)
+{
+  if (g) g->flat = flat != 0;
+}
+
 void grid_m2l_destroy(GridM2L *g)
 {
   if (!g) return;
@@ -1141,7 +1226,7 @@ template <int CLS>
 static void launch_ss(const GridM2L *g, const GridArgs &a, hipStream_t s)
 {
   const dim3 grid((g->nblocks + 7) & ~7);
-  if (a.gbytes > 0)
+  if (a.gbytes > 0 && !g->flat)
     hipLaunchKernelGGL((k_m2l_mfma_ss<CLS, true>), grid, dim3(256), 0, s, a);
   else
     hipLaunchKernelGGL((k_m2l_mfma_ss<CLS, false>), grid, dim3(256), 0, s, a);
@@ -1164,10 +1249,10 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
                          s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (a.r != 32 && a.r != 256)
+    if (a.r == 32)
+      hipLaunchKernelGGL(k_m2l_grid_plain32, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
+    else if (a.r != 256)
       hipLaunchKernelGGL(k_m2l_grid_plain_any, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
-    else if (a.r == 32)
-      hipLaunchKernelGGL(k_m2l_grid_plain<32>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else
       hipLaunchKernelGGL(k_m2l_grid_plain<256>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     BB_HIP(hipGetLastError());
@@ -1260,6 +1345,7 @@ struct ListArgs {
 
 struct ListM2L {
   int form = 0, nrhs = 1, ncell = 0, nblocks = 0;
+  int flat = 0;  // 1: pointer-form loads even on arrays below 2 GB
   ListArgs args{};
   int4 *d_tab = nullptr;
   int *d_blk = nullptr;
@@ -1748,6 +1834,11 @@ int list_m2l_create(ListM2L **out, int r, int nrhs, int form, const real *K, int
 #endif
 }
 
+void list_m2l_set_flat(ListM2L *g, int flat)
+{
+  if (g) g->flat = flat != 0;
+}
+
 void list_m2l_destroy(ListM2L *g)
 {
   if (!g) return;
@@ -1767,7 +1858,7 @@ int list_m2l_run(ListM2L *g, const real *M, real *L, hipStream_t s, hipEvent_t m
     hipLaunchKernelGGL(k_m2l_list_prep<BBFMM_M2L_BF16X2>, dim3(pb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (g->nblocks && a.mbytes > 0)  // grids below 2 GB: scalar-offset buffer loads
+    if (g->nblocks && a.mbytes > 0 && !g->flat)  // grids below 2 GB: scalar-offset buffer loads
       hipLaunchKernelGGL(k_m2l_list_lds<true>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else if (g->nblocks)
       hipLaunchKernelGGL(k_m2l_list_lds<false>, dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);

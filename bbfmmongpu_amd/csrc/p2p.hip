@@ -660,7 +660,7 @@ int launch_p2p(const P2PArgs &a, hipStream_t s)
             hipLaunchKernelGGL(k_p2p_seg16x<3>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
           else
             hipLaunchKernelGGL(k_p2p_seg16x<4>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
-        } else if (a.npad > 0 && a.npad < (1 << 25))  // charge rows below 2 GB: scalar-offset buffer loads
+        } else if (!a.flat && a.npad > 0 && a.npad < (1 << 25))  // charge rows below 2 GB: scalar-offset buffer loads
           hipLaunchKernelGGL(k_p2p_seg16<true>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);
         else
           hipLaunchKernelGGL(k_p2p_seg16<false>, dim3(a.nleaf), dim3(64), 0, s, a, rhs);

@@ -189,6 +189,11 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                       Chebyshev interpolant.  The drop-in entry points
                                       (anotherFMMCompute, anotherDownwardPass) always take 1.
                                       Other orders: no effect */
+#define BBFMM_OPT_ADDRESSING 11 /* 0 (default): kernels with two forms read through
+                                   scalar-offset buffer loads while their arrays are
+                                   below 2 GB and through pointers beyond; 1: through
+                                   pointers always (the split-BF16 source-stationary
+                                   and list M2L, the fp32 16-vector P2P).  Same results */
 
 /* Create a plan for N particles at host positions pos (N real3, inside the
    root cell [-L/2,L/2]^3), Chebyshev order n, level modifier lmod (the -d
@@ -279,6 +284,7 @@ int bbfmm_amd_stage_times(bbfmm_amd_plan *plan, double *out, int nout);
    out[t] = q_i * sum_{j: x_j != x_i} q_j / |x_i - x_j| for i = targets[t]
    (all particles when targets is NULL, nt = N).  pos is N x 3 doubles. */
 int bbfmm_amd_direct(int N, const double *pos, const double *q, int nt, const int *targets, double *out);
+
 /* FP64 direct-sum potentials and their gradients for nvec charge vectors
    (q: nvec x N): for target i = targets[t], pot[v*nt + t] =
    sum_{j: x_j != x_i} q_vj / r_ij and grad[(v*nt + t)*3 + c] = the c-th

@@ -38,7 +38,7 @@ def _oracle(n, r, prec, l2p_exact):
     key = (n, r, prec, l2p_exact)
     if key not in _ref_cache:
         pos, q, U, K = _inputs(n, r, prec)
-        T = O.Tree(pos, 1, prec)  # lmod = 1: maxlev 3, so the dense grids have two levels
+        T = O.Tree(pos, 1, prec)  # lmod = 1: maxlev 3 or 4, so the dense grids have two levels or more
         _ref_cache[key] = (T, O.fmm(T, q, n, U, K, scheme="sibling", l2p_exact=l2p_exact))
     return _ref_cache[key]
 
@@ -51,7 +51,7 @@ def test_plan_orders_all_schemes(n, r, prec, cuda):
     pos, q, U, K = _inputs(n, r, prec)
     T, ref = _oracle(n, r, prec, True)
     plan = B.Plan(pos, n, U, K, nrhs=1, lmod=1, prec=prec)
-    assert plan.maxlev == T.maxlev == 3 and plan.r == r
+    assert plan.maxlev == T.maxlev >= 3 and plan.r == r
     qd = torch.from_numpy(q).to(cuda)
     for scheme in (B.SCHEME_BA, B.SCHEME_SI, B.SCHEME_CL, B.SCHEME_IJ):
         plan.set(scheme=scheme, form=B.FORM_PLAIN)
