@@ -13,8 +13,10 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstdlib>
+#include <thread>
 
 namespace bb {
 
@@ -123,7 +125,7 @@ int forest_from_tree(HostForest &f, const anotherTree *t)
 namespace {
 struct Level {
   std::vector<uint32_t> key;  // sorted Morton keys of the stored cells
-  int first = 0;              // global index of the first cell
+  int first = 0;              // index of the first cell (instance-local)
   int find(uint32_t k) const
   {
     const auto it = std::lower_bound(key.begin(), key.end(), k);
@@ -131,27 +133,227 @@ struct Level {
   }
 };
 
-// leaf keys of n points at depth d (bucketing of anotherFMMDistribute.c:258-298)
-int bucket(const real3 *pos, int n, int d, real L, std::vector<uint32_t> &key)
+// One instance built on its own (instance-local cell, leaf and particle
+// numbering); forest_from_points concatenates them.
+struct Inst {
+  int n = 0, d = 0, ncell = 0, nleaf = 0, rc = BBFMM_OK;
+  std::vector<real4> center;
+  std::vector<int> celllev, child, cls;           // [ncell], [ncell * 8], [ncell]
+  std::vector<std::vector<int>> plev;             // cells with children per level 0..d-1
+  std::vector<int> perm, leafof;                  // [n]
+  std::vector<int> leafcell, leafsta, leafcnt;    // [nleaf]
+  std::vector<int> nbr, nnbr;                     // [nleaf * 27], [nleaf]
+  std::vector<int> iinter, inter, kidx, vsrc;     // [ncell], [ncell * 192] x 2, [ncell * 189]
+  double pairs = 0, ninter = 0;
+};
+
+// Leaf keys of the reference's bucketing (anotherFMMDistribute.c:258-298) at
+// depth kMaxForestDepth.  The cell length at depth d is L / 2^d (halved
+// exactly level by level), so (x + L/2) / L_d = 2^(D - d) times its value at
+// depth D in floating point, and truncation commutes with the exact shift:
+// the key at depth d <= D is the depth-D key shifted right by 3 (D - d) bits,
+// the clamp to 2^d - 1 included.
+int bucket_max(const real3 *pos, int n, real L, std::vector<uint64_t> &ki)
 {
-  std::vector<int> levsta(d + 1), levend(d + 1);
-  std::vector<real> celeng(d + 1);
-  level_tables(d, L, levsta.data(), levend.data(), celeng.data());
+  constexpr int D = kMaxForestDepth;
+  std::vector<int> levsta(D + 1), levend(D + 1);
+  std::vector<real> celeng(D + 1);
+  level_tables(D, L, levsta.data(), levend.data(), celeng.data());
   const real half = celeng[0] / 2;
-  const real inv = 1 / celeng[d];
-  const int ncpd = 1 << d;
-  key.resize(n);
+  const real inv = 1 / celeng[D];
+  const int ncpd = 1 << D;
+  ki.resize(n);
   for (int i = 0; i < n; i++) {
     const real x = pos[i].x, y = pos[i].y, z = pos[i].z;
     int ix = (int)((x + half) * inv), iy = (int)((y + half) * inv), iz = (int)((z + half) * inv);
-    if (!(x >= -half && x <= half && y >= -half && y <= half && z >= -half && z <= half))
-      return set_error(BBFMM_E_RANGE, "a particle lies outside the root cell [-L/2, L/2]^3");
+    if (!(x >= -half && x <= half && y >= -half && y <= half && z >= -half && z <= half)) return BBFMM_E_RANGE;
     ix = std::min(ix, ncpd - 1);
     iy = std::min(iy, ncpd - 1);
     iz = std::min(iz, ncpd - 1);
-    key[i] = morton_key(ix, iy, iz, d);
+    // key in the high word, index in the low word: one sort gives the leaf
+    // order with ascending input index inside a leaf
+    ki[i] = ((uint64_t)morton_key(ix, iy, iz, D) << 32) | (uint32_t)i;
   }
   return BBFMM_OK;
+}
+
+void build_instance(Inst &I, const real3 *P, int n, int depth, int leafmax, real L, const std::vector<real> &celeng,
+                    const int *Ktab)
+{
+  constexpr int D = kMaxForestDepth;
+  I.n = n;
+  std::vector<uint64_t> ki;
+  if ((I.rc = bucket_max(P, n, L, ki)) != BBFMM_OK) return;
+  std::sort(ki.begin(), ki.end());
+  int d = depth > 0 ? depth : kMinLev;
+  if (depth <= 0) {
+    // the smallest depth whose leaves hold at most leafmax points
+    for (;; d++) {
+      const int sh = 32 + 3 * (D - d);
+      int run = 1, worst = 1;
+      for (int j = 1; j < n; j++) {
+        run = (ki[j] >> sh) == (ki[j - 1] >> sh) ? run + 1 : 1;
+        worst = std::max(worst, run);
+      }
+      if (worst <= leafmax || d == D) break;
+    }
+  }
+  I.d = d;
+  const int sh = 32 + 3 * (D - d);
+  if (d < D) {
+    // leaf order at depth d: ascending input index inside a leaf (the
+    // depth-D order is by sub-cell first)
+    for (auto &x : ki) x = ((x >> sh) << 32) | (x & 0xffffffffu);
+    std::sort(ki.begin(), ki.end());
+  }
+  const int shd = 32;  // the key now holds the depth-d leaf
+  // stored cells per level
+  std::vector<Level> lev(d + 1);
+  for (int j = 0; j < n; j++) {
+    const uint32_t kk = (uint32_t)(ki[j] >> shd);
+    if (lev[d].key.empty() || lev[d].key.back() != kk) lev[d].key.push_back(kk);
+  }
+  for (int l = d - 1; l >= 0; l--)
+    for (uint32_t kk : lev[l + 1].key) {
+      const uint32_t pk = kk >> 3;
+      if (lev[l].key.empty() || lev[l].key.back() != pk) lev[l].key.push_back(pk);
+    }
+  int run = 0;
+  for (int l = 0; l <= d; l++) {
+    lev[l].first = run;
+    run += (int)lev[l].key.size();
+  }
+  const int nc = run;
+  I.ncell = nc;
+  I.center.resize(nc);
+  I.celllev.resize(nc);
+  I.child.assign((size_t)nc * 8, -1);
+  I.cls.assign(nc, 0);
+  // centres by halving from the root (as build_tree), child table
+  I.center[0] = real4{0, 0, 0, 0};
+  I.celllev[0] = 0;
+  for (int l = 1; l <= d; l++) {
+    const real h = celeng[l] / 2;
+    const Level &up = lev[l - 1];
+    size_t pj = 0;  // parents are visited in key order
+    for (size_t j = 0; j < lev[l].key.size(); j++) {
+      const uint32_t kk = lev[l].key[j];
+      while (up.key[pj] != (kk >> 3)) pj++;
+      const int A = lev[l].first + (int)j;
+      const int Pc = up.first + (int)pj;
+      const int s = (int)(kk & 7);
+      const real4 pc = I.center[Pc];
+      I.center[A] = real4{(s & 4) ? pc.x + h : pc.x - h, (s & 2) ? pc.y + h : pc.y - h,
+                          (s & 1) ? pc.z + h : pc.z - h, 0};
+      I.celllev[A] = l;
+      I.cls[A] = s;
+      I.child[(size_t)8 * Pc + s] = A;
+    }
+  }
+  I.plev.resize(d);
+  for (int l = 0; l < d; l++)
+    for (size_t j = 0; j < lev[l].key.size(); j++) I.plev[l].push_back(lev[l].first + (int)j);
+  // leaves and particles (leaf order)
+  const int nl = (int)lev[d].key.size();
+  I.nleaf = nl;
+  I.perm.resize(n);
+  I.leafof.resize(n);
+  I.leafcell.resize(nl);
+  I.leafsta.resize(nl);
+  I.leafcnt.resize(nl);
+  {
+    int pj = 0;
+    for (int q = 0; q < nl; q++) {
+      const uint32_t kk = lev[d].key[q];
+      const int sta = pj;
+      while (pj < n && (uint32_t)(ki[pj] >> shd) == kk) {
+        I.perm[pj] = (int)(uint32_t)ki[pj];
+        I.leafof[pj] = q;
+        pj++;
+      }
+      I.leafcell[q] = lev[d].first + q;
+      I.leafsta[q] = sta;
+      I.leafcnt[q] = pj - sta;
+    }
+  }
+  // P2P neighbour leaves (self included)
+  I.nbr.assign((size_t)nl * 27, -1);
+  I.nnbr.assign(nl, 0);
+  const int ncpd = 1 << d;
+  for (int q = 0; q < nl; q++) {
+    int x, y, z;
+    morton_coords(lev[d].key[q], d, x, y, z);
+    int m = 0;
+    for (int dx = -1; dx <= 1; dx++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dz = -1; dz <= 1; dz++) {
+          const int X = x + dx, Y = y + dy, Z = z + dz;
+          if (X < 0 || Y < 0 || Z < 0 || X >= ncpd || Y >= ncpd || Z >= ncpd) continue;
+          const int B = lev[d].find(morton_key(X, Y, Z, d));
+          if (B < 0) continue;
+          const int bl = B - lev[d].first;
+          I.nbr[(size_t)q * 27 + m++] = bl;
+          I.pairs += (double)I.leafcnt[q] * (double)I.leafcnt[bl];
+        }
+    I.nnbr[q] = m;
+  }
+  // interaction lists: children of the parent's neighbours that are not neighbours
+  I.iinter.assign(nc, 0);
+  I.inter.assign((size_t)nc * kPitchInteraction, 0);
+  I.kidx.assign((size_t)nc * kPitchInteraction, 0);
+  I.vsrc.assign((size_t)nc * kM2LOffsetsPerClass, -1);
+  for (int l = kMinLev; l <= d; l++) {
+    const int np = 1 << (l - 1);
+    for (size_t j = 0; j < lev[l].key.size(); j++) {
+      const int F = lev[l].first + (int)j;
+      int fx, fy, fz;
+      morton_coords(lev[l].key[j], l, fx, fy, fz);
+      const int px = fx >> 1, py = fy >> 1, pz = fz >> 1;
+      int m = 0;
+      for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dz = -1; dz <= 1; dz++) {
+            const int X = px + dx, Y = py + dy, Z = pz + dz;
+            if (X < 0 || Y < 0 || Z < 0 || X >= np || Y >= np || Z >= np) continue;
+            const int U = lev[l - 1].find(morton_key(X, Y, Z, l - 1));
+            if (U < 0) continue;
+            for (int s = 0; s < 8; s++) {
+              const int C = I.child[(size_t)8 * U + s];
+              if (C < 0) continue;
+              const int vx = 2 * X + ((s >> 2) & 1) - fx, vy = 2 * Y + ((s >> 1) & 1) - fy,
+                        vz = 2 * Z + (s & 1) - fz;
+              if (std::abs(vx) <= 1 && std::abs(vy) <= 1 && std::abs(vz) <= 1) continue;
+              I.inter[(size_t)kPitchInteraction * F + m] = C;
+              I.kidx[(size_t)kPitchInteraction * F + m] = Ktab[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
+              m++;
+              const int slot = class_slot(I.cls[F], vx, vy, vz);
+              if (slot >= 0) I.vsrc[(size_t)kM2LOffsetsPerClass * F + slot] = C;
+            }
+          }
+      I.iinter[F] = m;
+      I.ninter += m;
+    }
+  }
+}
+
+// f(i) for i in [0, n) on up to `threads` threads, items taken in order
+template <class Fn>
+void parallel_items(int n, Fn f)
+{
+  int nt = (int)std::thread::hardware_concurrency();
+  if (const char *e = getenv("BBFMM_AMD_SETUP_THREADS")) nt = atoi(e);
+  nt = std::max(1, std::min(nt, std::min(n, 64)));
+  if (nt == 1) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; t++)
+    pool.emplace_back([&]() {
+      for (int i; (i = next.fetch_add(1)) < n;) f(i);
+    });
+  for (auto &th : pool) th.join();
 }
 }  // namespace
 
@@ -163,203 +365,132 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
   if (ninst < 1) return set_error(BBFMM_E_ARG, "no instances");
   int Ktab[343];
   ktable(Ktab);
+  (void)class_slot(0, 3, 3, 3);  // builds the slot table before the threads read it
   long long ptot = 0;
   for (int i = 0; i < ninst; i++) {
     if (Ns[i] < 1) return set_error(BBFMM_E_ARG, "instance %d has %d points", i, Ns[i]);
+    if (depth && depth[i] > 0 && (depth[i] < kMinLev || depth[i] > kMaxForestDepth))
+      return set_error(BBFMM_E_ARG, "instance %d: depth %d outside [%d, %d]", i, depth[i], kMinLev, kMaxForestDepth);
     ptot += Ns[i];
   }
   if (ptot > (1LL << 30)) return set_error(BBFMM_E_ARG, "too many points (%lld)", ptot);
   if (leafmax < 1) leafmax = 32;
   f.N = (int)ptot;
-  // choose the depths and count the cells
-  std::vector<int> dep(ninst);
-  std::vector<std::vector<uint32_t>> keys(ninst);
-  int p0 = 0;
-  for (int i = 0; i < ninst; i++) {
-    const real3 *P = pos + p0;
-    int d = (depth && depth[i] > 0) ? depth[i] : kMinLev;
-    if (d < kMinLev || d > kMaxForestDepth)
-      return set_error(BBFMM_E_ARG, "instance %d: depth %d outside [%d, %d]", i, d, kMinLev, kMaxForestDepth);
-    for (;;) {
-      BB_TRY(bucket(P, Ns[i], d, L, keys[i]));
-      if (depth && depth[i] > 0) break;
-      std::vector<uint32_t> s = keys[i];
-      std::sort(s.begin(), s.end());
-      int run = 1, worst = 1;
-      for (size_t j = 1; j < s.size(); j++) {
-        run = s[j] == s[j - 1] ? run + 1 : 1;
-        worst = std::max(worst, run);
-      }
-      if (worst <= leafmax || d == kMaxForestDepth) break;
-      d++;
-    }
-    dep[i] = d;
-    p0 += Ns[i];
-  }
-  int dmax = 0;
-  for (int i = 0; i < ninst; i++) dmax = std::max(dmax, dep[i]);
-  f.maxdepth = dmax;
+  std::vector<real> celeng(kMaxForestDepth + 1);
   {
-    std::vector<int> ls(dmax + 1), le(dmax + 1);
-    f.celeng.resize(dmax + 1);
-    level_tables(dmax, L, ls.data(), le.data(), f.celeng.data());
+    std::vector<int> ls(kMaxForestDepth + 1), le(kMaxForestDepth + 1);
+    level_tables(kMaxForestDepth, L, ls.data(), le.data(), celeng.data());
   }
-  f.ploff.assign(dmax + 1, 0);
-  std::vector<std::vector<int>> plev(dmax);  // parents per level across instances
-  std::vector<int> cellclass;                // sibling class of each cell
-  p0 = 0;
+  std::vector<long long> p0s(ninst + 1, 0);
+  for (int i = 0; i < ninst; i++) p0s[i + 1] = p0s[i] + Ns[i];
+  // instances on their own, largest first
+  std::vector<Inst> in(ninst);
+  std::vector<int> byz(ninst);
+  for (int i = 0; i < ninst; i++) byz[i] = i;
+  std::stable_sort(byz.begin(), byz.end(), [&](int a, int b) { return Ns[a] > Ns[b]; });
+  parallel_items(ninst, [&](int t) {
+    const int i = byz[t];
+    build_instance(in[i], pos + p0s[i], Ns[i], depth ? depth[i] : 0, leafmax, L, celeng, Ktab);
+  });
+  for (int i = 0; i < ninst; i++)
+    if (in[i].rc != BBFMM_OK) return set_error(in[i].rc, "a particle lies outside the root cell [-L/2, L/2]^3");
+  // global numbering: instances one after the other
+  std::vector<int> c0(ninst + 1, 0), lf0(ninst + 1, 0);
+  int dmax = 0;
   for (int i = 0; i < ninst; i++) {
-    const int d = dep[i], n = Ns[i];
-    const int c0 = f.ncell, lf0 = f.nleaf;
-    f.inst_N.push_back(n);
-    f.inst_depth.push_back(d);
-    f.inst_p0.push_back(p0);
-    f.inst_leaf0.push_back(lf0);
-    // particles in leaf order (stable: ascending input index within a leaf)
-    std::vector<int> order(n);
-    for (int j = 0; j < n; j++) order[j] = j;
-    const std::vector<uint32_t> &k = keys[i];
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return k[a] < k[b]; });
-    // stored cells per level
-    std::vector<Level> lev(d + 1);
-    for (int j = 0; j < n; j++) {
-      const uint32_t kk = k[order[j]];
-      if (lev[d].key.empty() || lev[d].key.back() != kk) lev[d].key.push_back(kk);
-    }
-    for (int l = d - 1; l >= 0; l--)
-      for (uint32_t kk : lev[l + 1].key) {
-        const uint32_t pk = kk >> 3;
-        if (lev[l].key.empty() || lev[l].key.back() != pk) lev[l].key.push_back(pk);
-      }
-    int run = c0;
-    for (int l = 0; l <= d; l++) {
-      lev[l].first = run;
-      run += (int)lev[l].key.size();
-    }
-    f.ncell = run;
-    f.center.resize(run);
-    f.celllev.resize(run);
-    f.child.resize((size_t)run * 8, -1);
-    // centres by halving from the root (as build_tree), child table
-    f.center[c0] = real4{0, 0, 0, 0};
-    f.celllev[c0] = 0;
-    for (int l = 1; l <= d; l++) {
-      const real h = f.celeng[l] / 2;
-      for (size_t j = 0; j < lev[l].key.size(); j++) {
-        const uint32_t kk = lev[l].key[j];
-        const int A = lev[l].first + (int)j;
-        const int P = lev[l - 1].find(kk >> 3);
-        const int s = (int)(kk & 7);
-        const real4 pc = f.center[P];
-        f.center[A] = real4{(s & 4) ? pc.x + h : pc.x - h, (s & 2) ? pc.y + h : pc.y - h,
-                            (s & 1) ? pc.z + h : pc.z - h, 0};
-        f.celllev[A] = l;
-        f.child[(size_t)8 * P + s] = A;
-      }
-    }
-    for (int l = 0; l < d; l++)
-      for (size_t j = 0; j < lev[l].key.size(); j++) plev[l].push_back(lev[l].first + (int)j);
-    // leaves
-    const int nl = (int)lev[d].key.size();
-    const real leaflen = L / (real)(1 << d);
-    const real ihalf = (real)1 / (leaflen / (real)2);
-    int pj = 0;
-    for (int q = 0; q < nl; q++) {
-      const uint32_t kk = lev[d].key[q];
-      const int sta = f.N > 0 ? p0 + pj : 0;
-      while (pj < n && k[order[pj]] == kk) {
-        f.perm.push_back(p0 + order[pj]);
-        f.leafof.push_back(lf0 + q);
-        pj++;
-      }
-      f.leafcell.push_back(lev[d].first + q);
-      f.leafsta.push_back(sta);
-      f.leafcnt.push_back(p0 + pj - sta);
-      f.leafihalf.push_back(ihalf);
-    }
-    f.nleaf = lf0 + nl;
-    // P2P neighbour leaves (self included)
-    f.nbr.resize((size_t)f.nleaf * 27, -1);
-    f.nnbr.resize(f.nleaf, 0);
-    const int ncpd = 1 << d;
-    for (int q = 0; q < nl; q++) {
-      int x, y, z;
-      morton_coords(lev[d].key[q], d, x, y, z);
-      int m = 0;
-      for (int dx = -1; dx <= 1; dx++)
-        for (int dy = -1; dy <= 1; dy++)
-          for (int dz = -1; dz <= 1; dz++) {
-            const int X = x + dx, Y = y + dy, Z = z + dz;
-            if (X < 0 || Y < 0 || Z < 0 || X >= ncpd || Y >= ncpd || Z >= ncpd) continue;
-            const int B = lev[d].find(morton_key(X, Y, Z, d));
-            if (B < 0) continue;
-            const int bl = B - lev[d].first + lf0;
-            f.nbr[(size_t)(lf0 + q) * 27 + m++] = bl;
-            f.pairs += (double)f.leafcnt[lf0 + q] * (double)f.leafcnt[bl];
-          }
-      f.nnbr[lf0 + q] = m;
-    }
-    // interaction lists: children of the parent's neighbours that are not neighbours
-    f.iinter.resize(run, 0);
-    f.inter.resize((size_t)run * kPitchInteraction, 0);
-    f.kidx.resize((size_t)run * kPitchInteraction, 0);
-    f.vsrc.resize((size_t)run * kM2LOffsetsPerClass, -1);
-    cellclass.resize(run, 0);
-    for (int l = 1; l <= d; l++)
-      for (size_t j = 0; j < lev[l].key.size(); j++) cellclass[lev[l].first + j] = (int)(lev[l].key[j] & 7);
-    for (int l = kMinLev; l <= d; l++) {
-      const int np = 1 << (l - 1);
-      for (size_t j = 0; j < lev[l].key.size(); j++) {
-        const int F = lev[l].first + (int)j;
-        int fx, fy, fz;
-        morton_coords(lev[l].key[j], l, fx, fy, fz);
-        const int px = fx >> 1, py = fy >> 1, pz = fz >> 1;
-        int m = 0;
-        for (int dx = -1; dx <= 1; dx++)
-          for (int dy = -1; dy <= 1; dy++)
-            for (int dz = -1; dz <= 1; dz++) {
-              const int X = px + dx, Y = py + dy, Z = pz + dz;
-              if (X < 0 || Y < 0 || Z < 0 || X >= np || Y >= np || Z >= np) continue;
-              const int U = lev[l - 1].find(morton_key(X, Y, Z, l - 1));
-              if (U < 0) continue;
-              for (int s = 0; s < 8; s++) {
-                const int C = f.child[(size_t)8 * U + s];
-                if (C < 0) continue;
-                const int vx = 2 * X + ((s >> 2) & 1) - fx, vy = 2 * Y + ((s >> 1) & 1) - fy,
-                          vz = 2 * Z + (s & 1) - fz;
-                if (std::abs(vx) <= 1 && std::abs(vy) <= 1 && std::abs(vz) <= 1) continue;
-                f.inter[(size_t)kPitchInteraction * F + m] = C;
-                f.kidx[(size_t)kPitchInteraction * F + m] = Ktab[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
-                m++;
-                const int slot = class_slot(cellclass[F], vx, vy, vz);
-                if (slot >= 0) f.vsrc[(size_t)kM2LOffsetsPerClass * F + slot] = C;
-              }
-            }
-        f.iinter[F] = m;
-        f.ninter += m;
-      }
-    }
-    p0 += n;
+    c0[i + 1] = c0[i] + in[i].ncell;
+    lf0[i + 1] = lf0[i] + in[i].nleaf;
+    dmax = std::max(dmax, in[i].d);
+    f.inst_N.push_back(Ns[i]);
+    f.inst_depth.push_back(in[i].d);
+    f.inst_p0.push_back((int)p0s[i]);
+    f.inst_leaf0.push_back(lf0[i]);
+    f.pairs += in[i].pairs;
+    f.ninter += in[i].ninter;
   }
+  const int ncell = c0[ninst], nleaf = lf0[ninst];
+  f.ncell = ncell;
+  f.nleaf = nleaf;
+  f.maxdepth = dmax;
+  f.celeng.assign(celeng.begin(), celeng.begin() + dmax + 1);
+  f.center.resize(ncell);
+  f.celllev.resize(ncell);
+  f.child.resize((size_t)ncell * 8);
+  f.perm.resize(f.N);
+  f.leafof.resize(f.N);
+  f.leafcell.resize(nleaf);
+  f.leafsta.resize(nleaf);
+  f.leafcnt.resize(nleaf);
+  f.leafihalf.resize(nleaf);
+  f.nbr.resize((size_t)nleaf * 27);
+  f.nnbr.resize(nleaf);
+  f.iinter.resize(ncell);
+  f.inter.resize((size_t)ncell * kPitchInteraction);
+  f.kidx.resize((size_t)ncell * kPitchInteraction);
+  f.vsrc.resize((size_t)ncell * kM2LOffsetsPerClass);
+  std::vector<int> cellclass(ncell);
+  parallel_items(ninst, [&](int i) {
+    Inst &I = in[i];
+    const int c = c0[i], lf = lf0[i];
+    const int p = (int)p0s[i];
+    auto shift = [](int v, int o) { return v < 0 ? v : v + o; };
+    for (int A = 0; A < I.ncell; A++) {
+      f.center[c + A] = I.center[A];
+      f.celllev[c + A] = I.celllev[A];
+      cellclass[c + A] = I.cls[A];
+      f.iinter[c + A] = I.iinter[A];
+    }
+    for (size_t k = 0; k < I.child.size(); k++) f.child[(size_t)8 * c + k] = shift(I.child[k], c);
+    for (int j = 0; j < I.n; j++) {
+      f.perm[p + j] = p + I.perm[j];
+      f.leafof[p + j] = lf + I.leafof[j];
+    }
+    const real ihalf = (real)1 / ((L / (real)(1 << I.d)) / (real)2);
+    for (int q = 0; q < I.nleaf; q++) {
+      f.leafcell[lf + q] = c + I.leafcell[q];
+      f.leafsta[lf + q] = p + I.leafsta[q];
+      f.leafcnt[lf + q] = I.leafcnt[q];
+      f.leafihalf[lf + q] = ihalf;
+      f.nnbr[lf + q] = I.nnbr[q];
+    }
+    for (size_t k = 0; k < I.nbr.size(); k++) f.nbr[(size_t)27 * lf + k] = shift(I.nbr[k], lf);
+    for (int A = 0; A < I.ncell; A++)
+      for (int k = 0; k < kPitchInteraction; k++) {
+        const size_t o = (size_t)kPitchInteraction * A + k;
+        f.inter[(size_t)kPitchInteraction * c + o] = k < I.iinter[A] ? I.inter[o] + c : 0;
+        f.kidx[(size_t)kPitchInteraction * c + o] = I.kidx[o];
+      }
+    // empty sources point at the zero cell
+    for (size_t k = 0; k < I.vsrc.size(); k++)
+      f.vsrc[(size_t)kM2LOffsetsPerClass * c + k] = I.vsrc[k] < 0 ? ncell : I.vsrc[k] + c;
+    std::vector<int>().swap(I.inter);
+    std::vector<int>().swap(I.kidx);
+    std::vector<int>().swap(I.vsrc);
+  });
+  // parents per level across instances
+  f.ploff.assign(dmax + 1, 0);
   for (int l = 0; l < dmax; l++) {
     f.ploff[l] = (int)f.plist.size();
-    f.plist.insert(f.plist.end(), plev[l].begin(), plev[l].end());
+    for (int i = 0; i < ninst; i++)
+      if (l < in[i].d)
+        for (int A : in[i].plev[l]) f.plist.push_back(A + c0[i]);
   }
   f.ploff[dmax] = (int)f.plist.size();
-  // class lists for the list MFMA M2L; empty sources point at the zero cell
-  for (auto &x : f.vsrc)
-    if (x < 0) x = f.ncell;
+  // class lists for the list MFMA M2L
   f.cls_off.assign(9, 0);
-  for (int A = 0; A < f.ncell; A++)
+  for (int A = 0; A < ncell; A++)
     if (f.celllev[A] >= kMinLev) f.cls_off[cellclass[A] + 1]++;
   for (int c = 0; c < 8; c++) f.cls_off[c + 1] += f.cls_off[c];
   f.cls_cells.assign(f.cls_off[8], 0);
   {
     std::vector<int> fill(f.cls_off.begin(), f.cls_off.end() - 1);
-    for (int A = 0; A < f.ncell; A++)
+    for (int A = 0; A < ncell; A++)
       if (f.celllev[A] >= kMinLev) f.cls_cells[fill[cellclass[A]]++] = A;
   }
-  if (depth_out) *depth_out = dep;
+  if (depth_out) {
+    depth_out->resize(ninst);
+    for (int i = 0; i < ninst; i++) (*depth_out)[i] = in[i].d;
+  }
   return BBFMM_OK;
 }
 
