@@ -7,26 +7,32 @@
 #include "common.h"
 #include "kernels.h"
 
+#include <mutex>
 #include <vector>
 
 namespace bb {
 
 constexpr int kDirectChunks = 64;
 
-__global__ void __launch_bounds__(256) k_direct(int N, const double4 *__restrict__ src, int nt,
-                                                const int *__restrict__ targets, double *__restrict__ part)
+// Partial sums of the direct sum: out_t needs (chunk, t) for chunk <
+// kDirectChunks; pos is N x 3, q is N (device), targets NULL: t itself.
+__global__ void __launch_bounds__(256) k_direct(int N, const double *__restrict__ pos, const double *__restrict__ q,
+                                                int nt, const int *__restrict__ targets, double *__restrict__ part)
 {
   __shared__ double4 tile[256];
   const int t = blockIdx.x * 256 + threadIdx.x;
   const int chunk = blockIdx.y;
   const int per = (N + kDirectChunks - 1) / kDirectChunks;
   const int j0 = chunk * per, j1 = min(N, j0 + per);
-  double4 x = make_double4(0, 0, 0, 0);
-  if (t < nt) x = src[targets[t]];
+  double3 x = make_double3(0, 0, 0);
+  if (t < nt) {
+    const int i = targets ? targets[t] : t;
+    x = make_double3(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]);
+  }
   double s = 0;
   for (int base = j0; base < j1; base += 256) {
     const int j = base + threadIdx.x;
-    if (j < j1) tile[threadIdx.x] = src[j];
+    if (j < j1) tile[threadIdx.x] = make_double4(pos[3 * (size_t)j], pos[3 * (size_t)j + 1], pos[3 * (size_t)j + 2], q[j]);
     __syncthreads();
     const int nk = min(256, j1 - base);
     for (int k = 0; k < nk; k++) {
@@ -41,14 +47,25 @@ __global__ void __launch_bounds__(256) k_direct(int N, const double4 *__restrict
   if (t < nt) part[(size_t)chunk * nt + t] = s;
 }
 
-__global__ void k_direct_reduce(int nt, const int *__restrict__ targets, const double4 *__restrict__ src,
+// out_t = q_t * (sum of the chunk partials in chunk order)
+__global__ void k_direct_reduce(int nt, const int *__restrict__ targets, const double *__restrict__ q,
                                 const double *__restrict__ part, double *__restrict__ out)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   double s = 0;
   for (int c = 0; c < kDirectChunks; c++) s += part[(size_t)c * nt + t];
-  out[t] = src[targets[t]].w * s;
+  out[t] = q[targets ? targets[t] : t] * s;
+}
+
+static int launch_direct(int N, const double *pos, const double *q, int nt, const int *targets, double *part,
+                         double *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_direct, dim3((nt + 255) / 256, kDirectChunks), dim3(256), 0, s, N, pos, q, nt, targets, part);
+  BB_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_direct_reduce, dim3((nt + 255) / 256), dim3(256), 0, s, nt, targets, q, part, out);
+  BB_HIP(hipGetLastError());
+  return BBFMM_OK;
 }
 
 } // namespace bb
@@ -66,33 +83,26 @@ extern "C" int bbfmm_amd_direct(int N, const double *pos, const double *q, int n
     tg[t] = targets ? targets[t] : t;
     if (tg[t] < 0 || tg[t] >= N) return set_error(BBFMM_E_ARG, "bbfmm_amd_direct: target %d out of range", tg[t]);
   }
-  std::vector<double4> h((size_t)N);
-  for (int i = 0; i < N; i++) h[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], q[i]);
-  double4 *d_src = nullptr;
+  BB_TRY(check_device());
+  double *d_pos = nullptr, *d_q = nullptr, *d_part = nullptr, *d_out = nullptr;
   int *d_tg = nullptr;
-  double *d_part = nullptr, *d_out = nullptr;
   int rc = BBFMM_OK;
   hipError_t e = hipSuccess;
   do {
-    if ((e = hipMalloc(&d_src, sizeof(double4) * N)) != hipSuccess) break;
+    if ((e = hipMalloc(&d_pos, sizeof(double) * 3 * (size_t)N)) != hipSuccess) break;
+    if ((e = hipMalloc(&d_q, sizeof(double) * (size_t)N)) != hipSuccess) break;
     if ((e = hipMalloc(&d_tg, sizeof(int) * nt)) != hipSuccess) break;
     if ((e = hipMalloc(&d_part, sizeof(double) * (size_t)nt * kDirectChunks)) != hipSuccess) break;
     if ((e = hipMalloc(&d_out, sizeof(double) * nt)) != hipSuccess) break;
-    if ((e = hipMemcpy(d_src, h.data(), sizeof(double4) * N, hipMemcpyHostToDevice)) != hipSuccess) break;
+    if ((e = hipMemcpy(d_pos, pos, sizeof(double) * 3 * (size_t)N, hipMemcpyHostToDevice)) != hipSuccess) break;
+    if ((e = hipMemcpy(d_q, q, sizeof(double) * (size_t)N, hipMemcpyHostToDevice)) != hipSuccess) break;
     if ((e = hipMemcpy(d_tg, tg.data(), sizeof(int) * nt, hipMemcpyHostToDevice)) != hipSuccess) break;
-    hipLaunchKernelGGL(k_direct, dim3((nt + 255) / 256, kDirectChunks), dim3(256), 0, nullptr, N, d_src, nt, d_tg,
-                       d_part);
-    if ((e = hipGetLastError()) != hipSuccess) break;
-    hipLaunchKernelGGL(k_direct_reduce, dim3((nt + 255) / 256), dim3(256), 0, nullptr, nt, d_tg, d_src, d_part,
-                       d_out);
-    if ((e = hipGetLastError()) != hipSuccess) break;
+    if ((rc = launch_direct(N, d_pos, d_q, nt, d_tg, d_part, d_out, nullptr)) != BBFMM_OK) break;
     if ((e = hipMemcpy(out, d_out, sizeof(double) * nt, hipMemcpyDeviceToHost)) != hipSuccess) break;
   } while (0);
   if (e != hipSuccess) rc = set_error(BBFMM_E_HIP, "bbfmm_amd_direct: %s", hipGetErrorString(e));
-  if (d_src) (void)hipFree(d_src);
-  if (d_tg) (void)hipFree(d_tg);
-  if (d_part) (void)hipFree(d_part);
-  if (d_out) (void)hipFree(d_out);
+  for (void *ptr : {(void *)d_pos, (void *)d_q, (void *)d_tg, (void *)d_part, (void *)d_out})
+    if (ptr) (void)hipFree(ptr);
   return rc;
 }
 
@@ -217,4 +227,57 @@ extern "C" int bbfmm_amd_direct_field(int N, const double *pos, int nvec, const 
   for (void *ptr : {(void *)d_xyz, (void *)d_part, (void *)d_q, (void *)d_pot, (void *)d_grad, (void *)d_tg})
     if (ptr) (void)hipFree(ptr);
   return rc;
+}
+
+// ---------------------------------------------------------------- without allocation
+// bbfmm_amd_direct for callers that cannot allocate device memory: the
+// chunk partial sums go to a file-scope device array (each GPU the library
+// runs on holds its own copy until the process ends).  Host code never
+// passes the array's name to a kernel: its device address is taken once per
+// device by hipGetSymbolAddress and kept in g_scratch_ptr.
+namespace bb {
+static __device__ __align__(256) double scratch[kDirectChunks * BBFMM_DIRECT_NOALLOC_MAX_TARGETS];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (resolved on the first
+// call per device)
+static int scratch_on_current_device(double **out)
+{
+  int dev = -1;
+  BB_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(BBFMM_E_ARG, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(BBFMM_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return BBFMM_OK;
+}
+} // namespace bb
+
+extern "C" int bbfmm_amd_direct_noalloc(int N, const double *pos, const double *q, int nt, const int *targets,
+                                        double *out, void *stream)
+{
+  using namespace bb;
+  clear_error();
+  if (N < 0 || nt < 0 || (N > 0 && (!pos || !q)) || (nt > 0 && !out))
+    return set_error(BBFMM_E_ARG, "bbfmm_amd_direct_noalloc: invalid argument");
+  if (nt > BBFMM_DIRECT_NOALLOC_MAX_TARGETS)
+    return set_error(BBFMM_E_ARG, "bbfmm_amd_direct_noalloc: %d targets exceed the maximum of %d", nt,
+                     BBFMM_DIRECT_NOALLOC_MAX_TARGETS);
+  if (!targets && nt > N) return set_error(BBFMM_E_ARG, "bbfmm_amd_direct_noalloc: %d targets > N = %d", nt, N);
+  if (nt == 0) return BBFMM_OK;
+  BB_TRY(check_device());
+  double *part = nullptr;
+  BB_TRY(scratch_on_current_device(&part));
+  // every partial sum the reduction reads is written by the same call, so
+  // the array needs no clearing
+  return launch_direct(N, pos, q, nt, targets, part, out, (hipStream_t)stream);
 }

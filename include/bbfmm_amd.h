@@ -285,6 +285,19 @@ int bbfmm_amd_stage_times(bbfmm_amd_plan *plan, double *out, int nout);
    (all particles when targets is NULL, nt = N).  pos is N x 3 doubles. */
 int bbfmm_amd_direct(int N, const double *pos, const double *q, int nt, const int *targets, double *out);
 
+/* No counterpart in the reference: bbfmm_amd_direct for callers that cannot
+   allocate device memory (code under stream capture, serving loops, harnesses
+   that forbid hipMalloc).  pos (N x 3), q (N), targets (nt indices, or NULL
+   for 0..nt-1) and out (nt) are DEVICE arrays; the call is asynchronous on
+   `stream` (hipStream_t, NULL = default) and allocates nothing: the partial
+   sums use a file-scope device array of 64 x BBFMM_DIRECT_NOALLOC_MAX_TARGETS
+   doubles (8 MiB) that each GPU holds until the process ends.  nt above the
+   maximum is rejected (BBFMM_E_ARG).  Calls that overlap on one GPU (other
+   streams or threads) race on that array.  Same kernels and results as
+   bbfmm_amd_direct. */
+#define BBFMM_DIRECT_NOALLOC_MAX_TARGETS 16384
+int bbfmm_amd_direct_noalloc(int N, const double *pos, const double *q, int nt, const int *targets,
+                             double *out, void *stream);
 /* FP64 direct-sum potentials and their gradients for nvec charge vectors
    (q: nvec x N): for target i = targets[t], pot[v*nt + t] =
    sum_{j: x_j != x_i} q_vj / r_ij and grad[(v*nt + t)*3 + c] = the c-th

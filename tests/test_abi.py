@@ -220,3 +220,19 @@ def test_device_operators_need_a_gpu():
         pass
     with pytest.raises(B.BBFMMError, match="error 4"):
         B.operators(4, 32, prec="d", device=True)
+
+
+def test_direct_noalloc_rejects_too_many_targets():
+    """bbfmm_amd_direct_noalloc: the partial sums live in a fixed device
+    array, so more targets than it holds are an argument error, raised
+    before any device call."""
+    lib = B.load("d")
+    lib.bbfmm_amd_direct_noalloc.restype = C.c_int
+    lib.bbfmm_amd_direct_noalloc.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                             C.c_void_p]
+    mx = 16384  # BBFMM_DIRECT_NOALLOC_MAX_TARGETS
+    dummy = C.c_void_p(16)  # never dereferenced: the size check comes first
+    rc = lib.bbfmm_amd_direct_noalloc(100000, dummy, dummy, mx + 1, None, dummy, None)
+    assert rc == 1 and b"maximum" in lib.bbfmm_amd_last_error_string()
+    rc = lib.bbfmm_amd_direct_noalloc(10, dummy, dummy, 20, None, dummy, None)  # identity targets beyond N
+    assert rc == 1
