@@ -110,6 +110,7 @@ struct bbfmm_amd_plan {
   // and the per-form operator layout (created on first use)
   int h_cls_off[9] = {};
   int *d_cls_cells = nullptr, *d_vsrc = nullptr;
+  int *d_cellcls = nullptr, *d_slotop = nullptr;  // class of each cell, operator of each class slot
   // downward pass of sparse forests on MFMA (n = 4, r = 32, fp32): the cells
   // of levels >= 2 grouped by (level, class) as (cell, parent), waves of up
   // to 32 of them per level, and the operators Kw .* U and the 8 L2L maps
@@ -197,7 +198,7 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad, p->d_phit, p->d_ipad, p->d_phin,
                   p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf, p->d_cls_cells,
                   p->d_vsrc, p->d_dcp, p->d_dwv, p->d_KU, p->d_T8, p->d_T8m, p->d_mplist, p->d_pq,
-                  p->d_qsh,   p->d_qsl};
+                  p->d_qsh,   p->d_qsl, p->d_cellcls, p->d_slotop};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
@@ -354,14 +355,20 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
   BB_TRY(upload(&p->d_nbr, f.nbr.data(), f.nbr.size(), stream));
   BB_TRY(upload(&p->d_nnbr, f.nnbr.data(), f.nnbr.size(), stream));
   BB_TRY(upload(&p->d_iinter, f.iinter.data(), f.iinter.size(), stream));
-  BB_TRY(upload(&p->d_inter, f.inter.data(), f.inter.size(), stream));
-  BB_TRY(upload(&p->d_kidx, f.kidx.data(), f.kidx.size(), stream));
+  if (f.complete) {
+    BB_TRY(upload(&p->d_inter, f.inter.data(), f.inter.size(), stream));
+    BB_TRY(upload(&p->d_kidx, f.kidx.data(), f.kidx.size(), stream));
+  }
   BB_TRY(upload(&p->d_Ktable, Kt, (size_t)343, stream));
   if (f.complete) BB_TRY(upload(&p->d_levsta, p->h_levsta.data(), p->h_levsta.size(), stream));
   if (!f.complete) {
     for (int c = 0; c < 9; c++) p->h_cls_off[c] = f.cls_off[c];
     BB_TRY(upload(&p->d_cls_cells, f.cls_cells.data(), f.cls_cells.size(), stream));
     BB_TRY(upload(&p->d_vsrc, f.vsrc.data(), f.vsrc.size(), stream));
+    BB_TRY(upload(&p->d_cellcls, f.cellcls.data(), f.cellcls.size(), stream));
+    std::vector<int> sop;
+    slot_operators(sop);
+    BB_TRY(upload(&p->d_slotop, sop.data(), sop.size(), stream));
   }
   if (k4)
     BB_TRY(upload(&p->d_K4, K4.data(), K4.size(), stream));
@@ -671,6 +678,11 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
   m.M = M;
   m.L = L;
   m.variant = p->scheme == M2L_SCHEME_BA ? p->m2l_variant : 0;
+  if (!p->complete) {
+    m.vsrc = p->d_vsrc;
+    m.cellcls = p->d_cellcls;
+    m.slotop = p->d_slotop;
+  }
   if (!p->complete && p->scheme == M2L_SCHEME_IJ) {
     BB_TRY(ensure_grid(p));
     return list_m2l_run(p->list, M, L, p->stream, mid);

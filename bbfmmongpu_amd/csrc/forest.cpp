@@ -143,7 +143,7 @@ struct Inst {
   std::vector<int> perm, leafof;                  // [n]
   std::vector<int> leafcell, leafsta, leafcnt;    // [nleaf]
   std::vector<int> nbr, nnbr;                     // [nleaf * 27], [nleaf]
-  std::vector<int> iinter, inter, kidx, vsrc;     // [ncell], [ncell * 192] x 2, [ncell * 189]
+  std::vector<int> iinter, vsrc;                  // [ncell], [ncell * 189]
   double pairs = 0, ninter = 0;
 };
 
@@ -177,8 +177,7 @@ int bucket_max(const real3 *pos, int n, real L, std::vector<uint64_t> &ki)
   return BBFMM_OK;
 }
 
-void build_instance(Inst &I, const real3 *P, int n, int depth, int leafmax, real L, const std::vector<real> &celeng,
-                    const int *Ktab)
+void build_instance(Inst &I, const real3 *P, int n, int depth, int leafmax, real L, const std::vector<real> &celeng)
 {
   constexpr int D = kMaxForestDepth;
   I.n = n;
@@ -297,10 +296,9 @@ void build_instance(Inst &I, const real3 *P, int n, int depth, int leafmax, real
         }
     I.nnbr[q] = m;
   }
-  // interaction lists: children of the parent's neighbours that are not neighbours
+  // interactions: children of the parent's neighbours that are not
+  // neighbours, stored by class slot (the order of the dense-grid offsets)
   I.iinter.assign(nc, 0);
-  I.inter.assign((size_t)nc * kPitchInteraction, 0);
-  I.kidx.assign((size_t)nc * kPitchInteraction, 0);
   I.vsrc.assign((size_t)nc * kM2LOffsetsPerClass, -1);
   for (int l = kMinLev; l <= d; l++) {
     const int np = 1 << (l - 1);
@@ -323,10 +321,8 @@ void build_instance(Inst &I, const real3 *P, int n, int depth, int leafmax, real
               const int vx = 2 * X + ((s >> 2) & 1) - fx, vy = 2 * Y + ((s >> 1) & 1) - fy,
                         vz = 2 * Z + (s & 1) - fz;
               if (std::abs(vx) <= 1 && std::abs(vy) <= 1 && std::abs(vz) <= 1) continue;
-              I.inter[(size_t)kPitchInteraction * F + m] = C;
-              I.kidx[(size_t)kPitchInteraction * F + m] = Ktab[49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
               m++;
-              const int slot = class_slot(I.cls[F], vx, vy, vz);
+              const int slot = class_slot(I.cls[F], vx, vy, vz);  // every far cousin has a slot
               if (slot >= 0) I.vsrc[(size_t)kM2LOffsetsPerClass * F + slot] = C;
             }
           }
@@ -363,8 +359,6 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
   f = HostForest();
   f.complete = false;
   if (ninst < 1) return set_error(BBFMM_E_ARG, "no instances");
-  int Ktab[343];
-  ktable(Ktab);
   (void)class_slot(0, 3, 3, 3);  // builds the slot table before the threads read it
   long long ptot = 0;
   for (int i = 0; i < ninst; i++) {
@@ -390,7 +384,7 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
   std::stable_sort(byz.begin(), byz.end(), [&](int a, int b) { return Ns[a] > Ns[b]; });
   parallel_items(ninst, [&](int t) {
     const int i = byz[t];
-    build_instance(in[i], pos + p0s[i], Ns[i], depth ? depth[i] : 0, leafmax, L, celeng, Ktab);
+    build_instance(in[i], pos + p0s[i], Ns[i], depth ? depth[i] : 0, leafmax, L, celeng);
   });
   for (int i = 0; i < ninst; i++)
     if (in[i].rc != BBFMM_OK) return set_error(in[i].rc, "a particle lies outside the root cell [-L/2, L/2]^3");
@@ -425,10 +419,9 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
   f.nbr.resize((size_t)nleaf * 27);
   f.nnbr.resize(nleaf);
   f.iinter.resize(ncell);
-  f.inter.resize((size_t)ncell * kPitchInteraction);
-  f.kidx.resize((size_t)ncell * kPitchInteraction);
   f.vsrc.resize((size_t)ncell * kM2LOffsetsPerClass);
-  std::vector<int> cellclass(ncell);
+  f.cellcls.resize(ncell);
+  std::vector<int> &cellclass = f.cellcls;
   parallel_items(ninst, [&](int i) {
     Inst &I = in[i];
     const int c = c0[i], lf = lf0[i];
@@ -454,17 +447,9 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
       f.nnbr[lf + q] = I.nnbr[q];
     }
     for (size_t k = 0; k < I.nbr.size(); k++) f.nbr[(size_t)27 * lf + k] = shift(I.nbr[k], lf);
-    for (int A = 0; A < I.ncell; A++)
-      for (int k = 0; k < kPitchInteraction; k++) {
-        const size_t o = (size_t)kPitchInteraction * A + k;
-        f.inter[(size_t)kPitchInteraction * c + o] = k < I.iinter[A] ? I.inter[o] + c : 0;
-        f.kidx[(size_t)kPitchInteraction * c + o] = I.kidx[o];
-      }
     // empty sources point at the zero cell
     for (size_t k = 0; k < I.vsrc.size(); k++)
       f.vsrc[(size_t)kM2LOffsetsPerClass * c + k] = I.vsrc[k] < 0 ? ncell : I.vsrc[k] + c;
-    std::vector<int>().swap(I.inter);
-    std::vector<int>().swap(I.kidx);
     std::vector<int>().swap(I.vsrc);
   });
   // parents per level across instances

@@ -25,12 +25,15 @@ struct HostForest {
   std::vector<int> perm;            // [N] leaf order -> input index
   std::vector<int> leafof;          // [N] leaf of each leaf-ordered particle
   std::vector<int> nbr, nnbr;       // [nleaf * 27] neighbour leaves (self included), [nleaf]
-  std::vector<int> iinter, inter, kidx;  // [ncell], [ncell * kPitchInteraction] x 2
+  // interaction counts [ncell]; lists and operator indices [ncell * kPitchInteraction]
+  // for a complete tree only (sparse forests read their sources from vsrc)
+  std::vector<int> iinter, inter, kidx;
   double pairs = 0, ninter = 0;     // P2P pairs and M2L interactions (levels >= 2), per vector
   std::vector<int> inst_N, inst_depth, inst_p0, inst_leaf0;  // batch instances
   // sparse forests: field cells of levels >= 2 grouped by sibling class, and
   // the source cell of each of the class's 189 M2L offsets (ncell: empty)
   std::vector<int> cls_off, cls_cells, vsrc;  // [9], [cells of levels >= 2], [ncell * 189]
+  std::vector<int> cellcls;                   // sparse forests: sibling class of each cell [ncell]
 };
 
 // The reference's complete tree (anotherTree after build_tree).

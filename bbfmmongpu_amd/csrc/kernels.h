@@ -102,6 +102,10 @@ struct M2LArgs {
   const real *M;           // [rhs][ncell][r]
   real *L;                 // [rhs][ncell][r]
   int variant = 0;         // basic scheme: 0 / 1 / 2 = 256 / 128 / 512 threads per block
+  // sparse forests (h_levsta NULL): sources by sibling-class slot instead of
+  // lists: vsrc [ncell][189] (ncell: empty slot), cellcls [ncell], slotop
+  // [8][189] operator of each slot (device)
+  const int *vsrc = nullptr, *cellcls = nullptr, *slotop = nullptr;
 };
 int launch_m2l_basic(const M2LArgs &a, hipStream_t s);
 int launch_m2l_sibling(const M2LArgs &a, hipStream_t s);
@@ -133,6 +137,8 @@ void list_m2l_set_flat(ListM2L *g, int flat);
 int list_m2l_run(ListM2L *g, const real *M, real *L, hipStream_t s, hipEvent_t mid);
 // slot of offset (vx, vy, vz) among the 189 M2L offsets of sibling class s, or -1
 int class_slot(int s, int vx, int vy, int vz);
+// operator index (Ktable order) of every slot: op[8 * 189], class-major
+void slot_operators(std::vector<int> &op);
 
 // ---- P2P ----
 struct P2PArgs {

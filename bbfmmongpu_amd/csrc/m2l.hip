@@ -15,12 +15,34 @@
 
 namespace bb {
 
-template <int R, int BLOCK>
+// The sources of field cell F come from its interaction list (inter, kidx),
+// or, on sparse forests (SLOTS), by sibling-class slot: vsrc[F][189] names
+// the source of each of the class's 189 offsets (ncell: empty) and
+// slotop[class][189] its operator; the empty slots are skipped.
+struct SlotArgs {
+  const int *vsrc = nullptr, *cellcls = nullptr, *slotop = nullptr;
+};
+
+template <bool SLOTS>
+__device__ __forceinline__ bool m2l_source(const SlotArgs &sa, int F, int t, int ncell, const int *il,
+                                           const int *kl, int &S, int &k)
+{
+  if (SLOTS) {
+    S = sa.vsrc[(size_t)kM2LOffsetsPerClass * F + t];
+    k = sa.slotop[sa.cellcls[F] * kM2LOffsetsPerClass + t];
+    return S < ncell;
+  }
+  S = il[t];
+  k = kl[t];
+  return true;
+}
+
+template <int R, int BLOCK, bool SLOTS = false>
 __global__ void __launch_bounds__(BLOCK) k_m2l_basic(int F0, int nF, int ncell, const int *__restrict__ iinter,
                                                    const int *__restrict__ inter, int ipitch,
                                                    const int *__restrict__ kidx,
                                                    const real4 *__restrict__ K4,
-                                                   const real *__restrict__ M, real *__restrict__ L)
+                                                   const real *__restrict__ M, real *__restrict__ L, SlotArgs sa)
 {
   constexpr int CPB = BLOCK / R;  // field cells per block
   const int cl = threadIdx.x / R;
@@ -28,13 +50,13 @@ __global__ void __launch_bounds__(BLOCK) k_m2l_basic(int F0, int nF, int ncell, 
   const int F = F0 + blockIdx.x * CPB + cl;
   if (F >= F0 + nF) return;  // no barrier below
   const size_t roff = (size_t)blockIdx.y * ncell * R;
-  const int ni = iinter[F];
-  const int *il = inter + (size_t)ipitch * F;
-  const int *kl = kidx + (size_t)kPitchInteraction * F;
+  const int ni = SLOTS ? kM2LOffsetsPerClass : iinter[F];
+  const int *il = SLOTS ? nullptr : inter + (size_t)ipitch * F;
+  const int *kl = SLOTS ? nullptr : kidx + (size_t)kPitchInteraction * F;
   real acc0 = 0, acc1 = 0;
   for (int t = 0; t < ni; t++) {
-    const int S = il[t];
-    const int k = kl[t];
+    int S, k;
+    if (!m2l_source<SLOTS>(sa, F, t, ncell, il, kl, S, k)) continue;
     const real4 *Kp = K4 + (size_t)k * (R / 4) * R + i;
     const real4 *Mp = reinterpret_cast<const real4 *>(M + roff + (size_t)S * R);
 #pragma unroll 8
@@ -164,13 +186,13 @@ __device__ __forceinline__ void any_rank_lane(int r, int &rt, int &cpb, int &cl,
   i0 = (int)threadIdx.x - cl * rt;
 }
 
-template <int BLOCK>
+template <int BLOCK, bool SLOTS = false>
 __global__ void __launch_bounds__(BLOCK) k_m2l_basic_any(int r, int F0, int nF, int ncell,
                                                        const int *__restrict__ iinter,
                                                        const int *__restrict__ inter, int ipitch,
                                                        const int *__restrict__ kidx,
                                                        const real *__restrict__ Kc,
-                                                       const real *__restrict__ M, real *__restrict__ L)
+                                                       const real *__restrict__ M, real *__restrict__ L, SlotArgs sa)
 {
   constexpr int RPT = (kMaxRank + BLOCK - 1) / BLOCK;  // rows per thread at most
   int rt, cpb, cl, i0;
@@ -178,15 +200,17 @@ __global__ void __launch_bounds__(BLOCK) k_m2l_basic_any(int r, int F0, int nF, 
   const int F = F0 + blockIdx.x * cpb + cl;
   if (cl >= cpb || F >= F0 + nF) return;  // no barrier below
   const size_t roff = (size_t)blockIdx.y * ncell * r;
-  const int ni = iinter[F];
-  const int *il = inter + (size_t)ipitch * F;
-  const int *kl = kidx + (size_t)kPitchInteraction * F;
+  const int ni = SLOTS ? kM2LOffsetsPerClass : iinter[F];
+  const int *il = SLOTS ? nullptr : inter + (size_t)ipitch * F;
+  const int *kl = SLOTS ? nullptr : kidx + (size_t)kPitchInteraction * F;
   real acc[RPT];
 #pragma unroll
   for (int u = 0; u < RPT; u++) acc[u] = 0;
   for (int t = 0; t < ni; t++) {
-    const real *Kk = Kc + (size_t)kl[t] * r * r + i0;
-    const real *Ms = M + roff + (size_t)il[t] * r;
+    int S, k;
+    if (!m2l_source<SLOTS>(sa, F, t, ncell, il, kl, S, k)) continue;
+    const real *Kk = Kc + (size_t)k * r * r + i0;
+    const real *Ms = M + roff + (size_t)S * r;
     for (int j = 0; j < r; j++) {
       const real m = Ms[j];
 #pragma unroll
@@ -274,16 +298,22 @@ template <int BLOCK>
 static int basic_any_t(const M2LArgs &a, hipStream_t s)
 {
   const int rt = a.r < BLOCK ? a.r : BLOCK, cpb = BLOCK / rt;
+  SlotArgs sa;
+  sa.vsrc = a.vsrc;
+  sa.cellcls = a.cellcls;
+  sa.slotop = a.slotop;
   if (!a.h_levsta) {
-    hipLaunchKernelGGL((k_m2l_basic_any<BLOCK>), dim3((a.ncell + cpb - 1) / cpb, a.nrhs), dim3(BLOCK), 0, s, a.r,
-                       0, a.ncell, a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.Kc, a.M, a.L);
+    // sparse forests: every cell, sources by class slot; cells of levels 0-1 have none (L = 0)
+    if (!a.vsrc) return set_error(BBFMM_E_ARG, "M2L basic: a sparse forest needs its slot sources");
+    hipLaunchKernelGGL((k_m2l_basic_any<BLOCK, true>), dim3((a.ncell + cpb - 1) / cpb, a.nrhs), dim3(BLOCK), 0, s,
+                       a.r, 0, a.ncell, a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.Kc, a.M, a.L, sa);
     BB_HIP(hipGetLastError());
     return BBFMM_OK;
   }
   for (int level = a.minlev; level <= a.maxlev; level++) {
     const int F0 = a.h_levsta[level], nF = a.h_levsta[level + 1] - F0;
     hipLaunchKernelGGL((k_m2l_basic_any<BLOCK>), dim3((nF + cpb - 1) / cpb, a.nrhs), dim3(BLOCK), 0, s, a.r, F0,
-                       nF, a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.Kc, a.M, a.L);
+                       nF, a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.Kc, a.M, a.L, sa);
     BB_HIP(hipGetLastError());
   }
   return BBFMM_OK;
@@ -318,18 +348,23 @@ template <int R, int BLOCK>
 static int basic_t(const M2LArgs &a, hipStream_t s)
 {
   constexpr int CPB = BLOCK / R;
+  SlotArgs sa;
+  sa.vsrc = a.vsrc;
+  sa.cellcls = a.cellcls;
+  sa.slotop = a.slotop;
   if (!a.h_levsta) {
     // no level table (a batch of sparse instances): one launch over every
-    // cell; cells without a list (levels 0-1) write L = 0
-    hipLaunchKernelGGL((k_m2l_basic<R, BLOCK>), dim3((a.ncell + CPB - 1) / CPB, a.nrhs), dim3(BLOCK), 0, s, 0,
-                       a.ncell, a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.K4, a.M, a.L);
+    // cell, sources by class slot; cells of levels 0-1 have none (L = 0)
+    if (!a.vsrc) return set_error(BBFMM_E_ARG, "M2L basic: a sparse forest needs its slot sources");
+    hipLaunchKernelGGL((k_m2l_basic<R, BLOCK, true>), dim3((a.ncell + CPB - 1) / CPB, a.nrhs), dim3(BLOCK), 0, s,
+                       0, a.ncell, a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.K4, a.M, a.L, sa);
     BB_HIP(hipGetLastError());
     return BBFMM_OK;
   }
   for (int level = a.minlev; level <= a.maxlev; level++) {
     const int F0 = a.h_levsta[level], nF = a.h_levsta[level + 1] - F0;
     hipLaunchKernelGGL((k_m2l_basic<R, BLOCK>), dim3((nF + CPB - 1) / CPB, a.nrhs), dim3(BLOCK), 0, s, F0, nF,
-                       a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.K4, a.M, a.L);
+                       a.ncell, a.iinter, a.inter, a.inter_pitch, a.kidx, a.K4, a.M, a.L, sa);
     BB_HIP(hipGetLastError());
   }
   return BBFMM_OK;

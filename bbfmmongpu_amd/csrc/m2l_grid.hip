@@ -1570,6 +1570,14 @@ static void slot_table(std::vector<int> &slot)
   }
 }
 
+void slot_operators(std::vector<int> &op)
+{
+  std::vector<int4> tab;
+  offset_table(tab);
+  op.resize(tab.size());
+  for (size_t i = 0; i < tab.size(); i++) op[i] = tab[i].x;
+}
+
 int class_slot(int s, int vx, int vy, int vz)
 {
   static std::vector<int> slot;
