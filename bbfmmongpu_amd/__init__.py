@@ -28,6 +28,7 @@ OPT_NEAR_STREAM = 8
 OPT_P2P_FORM = 9
 OPT_L2P_REFERENCE = 10
 OPT_ADDRESSING = 11
+OPT_M2L_VARIANT_UPPER = 12
 P2P_FP32, P2P_BF16X2 = 0, 1
 # "p2p_beside": P2P's own time when it runs on the side stream beside the far
 # field ("p2p" is then the wait for it after the far field)
@@ -261,10 +262,13 @@ class Plan:
     def set(self, scheme: int | None = None, form: int | None = None, timing: int | None = None,
             output: str | None = None, graph: bool | None = None, variant: int | None = None,
             level_switch: int | None = None, near_stream: bool | None = None, p2p_form: int | None = None,
-            l2p_reference: bool | None = None, flat_loads: bool | None = None):
+            l2p_reference: bool | None = None, flat_loads: bool | None = None,
+            variant_upper: int | None = None):
         """output: "q_phi" (q .* potential, the reference's output) or "potential";
         l2p_reference: orders n = 3, 7 evaluate L2P with the reference's z
         weights (anotherDownwardPassX.c:304) instead of the interpolant;
+        variant_upper: IJ split-BF16, the source-stationary variant of the
+        levels above the deepest (a launch of their own; 0: as the deepest);
         flat_loads: kernels with buffer-load and pointer forms take the
         pointer form (their form for arrays of 2 GB and more) at any size;
         graph: capture the whole solve into one hipGraph and replay it;
@@ -289,6 +293,8 @@ class Plan:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_NEAR_STREAM, int(bool(near_stream))))
         if p2p_form is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_P2P_FORM, int(p2p_form)))
+        if variant_upper is not None:
+            _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_M2L_VARIANT_UPPER, int(variant_upper)))
         if flat_loads is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_ADDRESSING, int(bool(flat_loads))))
         if l2p_reference is not None:
@@ -298,10 +304,10 @@ class Plan:
         return self
 
     def tune(self, reps: int = 3):
-        """M2L autotuner: times every valid (scheme, variant, level switch) on
-        this plan and keeps the fastest; returns the table of
-        (scheme, variant, level_switch, ms)."""
-        table = np.zeros((64, 4))
+        """M2L autotuner: times every valid (scheme, variant, upper-level
+        variant, level switch) on this plan and keeps the fastest; returns the
+        table of (scheme, variant, variant_upper, level_switch, ms)."""
+        table = np.zeros((64, 5))
         rows = C.c_int(0)
         _check(self.lib, self.lib.bbfmm_amd_plan_tune(self.h, reps, _p(table), 64, C.byref(rows)))
         return [tuple(r) for r in table[:min(rows.value, 64)].tolist()]

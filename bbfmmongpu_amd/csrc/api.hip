@@ -146,7 +146,8 @@ struct bbfmm_amd_plan {
   // scheme, and for the dense-grid schemes the first level they handle (the
   // basic scheme takes the levels below)
   int m2l_variant = 0, level_switch = bb::kMinLev;
-  int grid_minlev = bb::kMinLev, grid_variant = 0;
+  int m2l_variant_up = 0;  // source-stationary variant of the levels above the deepest (0: as the deepest)
+  int grid_minlev = bb::kMinLev, grid_variant = 0, grid_variant_up = 0;
   hipGraphExec_t gexec = nullptr;
   const real *gq = nullptr;
   real *gphi = nullptr;
@@ -621,13 +622,17 @@ static int ensure_grid(bbfmm_amd_plan *p)
   const int form = p->scheme == M2L_SCHEME_CL ? BBFMM_M2L_PLAIN : p->form;
   const int minlev = p->level_switch > kMinLev ? p->level_switch : kMinLev;
   const int variant = p->scheme == M2L_SCHEME_IJ ? p->m2l_variant : 0;
-  if (!p->grid || p->grid_form != form || p->grid_minlev != minlev || p->grid_variant != variant) {
+  const int variant_up = p->scheme == M2L_SCHEME_IJ ? p->m2l_variant_up : 0;
+  if (!p->grid || p->grid_form != form || p->grid_minlev != minlev || p->grid_variant != variant ||
+      p->grid_variant_up != variant_up) {
     if (p->grid) grid_m2l_destroy(p->grid);
     p->grid = nullptr;
-    BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, p->nrhs, form, p->h_K.data(), p->stream, minlev, variant));
+    BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, p->nrhs, form, p->h_K.data(), p->stream, minlev, variant,
+                           variant_up));
     p->grid_form = form;
     p->grid_minlev = minlev;
     p->grid_variant = variant;
+    p->grid_variant_up = variant_up;
   }
   grid_m2l_set_flat(p->grid, p->flat_loads);
   return BBFMM_OK;
@@ -1039,6 +1044,12 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     p->l2p_ref = value;
     drop_graph(p);
     return BBFMM_OK;
+  case BBFMM_OPT_M2L_VARIANT_UPPER:
+    if (value != 0 && value != 1 && value != 4 && value != 5)
+      return set_error(BBFMM_E_ARG, "M2L upper-level variant %d: 0, 1, 4 or 5", value);
+    p->m2l_variant_up = value;
+    drop_graph(p);
+    return BBFMM_OK;
   case BBFMM_OPT_M2L_VARIANT:
     if (value < 0 || value > 5) return set_error(BBFMM_E_ARG, "M2L variant %d out of range", value);
     p->m2l_variant = value;
@@ -1072,39 +1083,46 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
 
 // M2L autotuner: times every configuration valid for the plan's tree and
 // arithmetic form on the plan's own multipoles and keeps the fastest.
+constexpr int kTuneCols = 5;  // scheme, variant, upper-level variant, level switch, ms
+
 extern "C" int bbfmm_amd_plan_tune(bbfmm_amd_plan *p, int reps, double *table, int maxrows, int *nrows)
 {
   clear_error();
   if (!p) return set_error(BBFMM_E_ARG, "null plan");
   if (reps < 1) reps = 3;
   struct Cand {
-    int scheme, variant, sw;
+    int scheme, variant, sw, vup;
   };
   std::vector<Cand> cands;
   if (!p->complete) {
-    for (int v = 0; v < 3; v++) cands.push_back({M2L_SCHEME_BA, v, kMinLev});
+    for (int v = 0; v < 3; v++) cands.push_back({M2L_SCHEME_BA, v, kMinLev, 0});
 #ifdef SINGLE
     if ((p->form == BBFMM_M2L_BF16X2 || p->form == BBFMM_M2L_FP32MFMA) && p->r == 32)
-      cands.push_back({M2L_SCHEME_IJ, 0, kMinLev});
+      cands.push_back({M2L_SCHEME_IJ, 0, kMinLev, 0});
 #endif
   } else {
     std::vector<int> sws{kMinLev};
     if (p->maxlev > kMinLev) sws.push_back(kMinLev + 1);
     if (p->form == BBFMM_M2L_PLAIN) {
-      for (int v = 0; v < 3; v++) cands.push_back({M2L_SCHEME_BA, v, kMinLev});
-      cands.push_back({M2L_SCHEME_SI, 0, kMinLev});
-      for (int sw : sws) cands.push_back({M2L_SCHEME_IJ, 0, sw});
+      for (int v = 0; v < 3; v++) cands.push_back({M2L_SCHEME_BA, v, kMinLev, 0});
+      cands.push_back({M2L_SCHEME_SI, 0, kMinLev, 0});
+      for (int sw : sws) cands.push_back({M2L_SCHEME_IJ, 0, sw, 0});
     } else if (p->form == BBFMM_M2L_BF16X2) {
       for (int sw : sws)
-        for (int v = 1; v <= 5; v++) cands.push_back({M2L_SCHEME_IJ, v, sw});
+        for (int v = 1; v <= 5; v++) cands.push_back({M2L_SCHEME_IJ, v, sw, 0});
+      // source-stationary with a class-group size of its own above the deepest level
+      if (p->maxlev > kMinLev)
+        for (int v : {1, 4, 5})
+          for (int u : {1, 4, 5})
+            if (u != v) cands.push_back({M2L_SCHEME_IJ, v, kMinLev, u});
     } else if (p->r == 32) {
       for (int sw : sws)
-        for (int v = 2; v <= 3; v++) cands.push_back({M2L_SCHEME_IJ, v, sw});
+        for (int v = 2; v <= 3; v++) cands.push_back({M2L_SCHEME_IJ, v, sw, 0});
     } else {
-      for (int sw : sws) cands.push_back({M2L_SCHEME_IJ, 0, sw});
+      for (int sw : sws) cands.push_back({M2L_SCHEME_IJ, 0, sw, 0});
     }
   }
-  const int s0 = p->scheme, v0 = p->m2l_variant, w0 = p->level_switch;
+  const int s0 = p->scheme, v0 = p->m2l_variant, w0 = p->level_switch, u0 = p->m2l_variant_up;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   BB_HIP(hipEventCreate(&e0));
   BB_HIP(hipEventCreate(&e1));
@@ -1115,6 +1133,7 @@ extern "C" int bbfmm_amd_plan_tune(bbfmm_amd_plan *p, int reps, double *table, i
     p->scheme = cands[c].scheme;
     p->m2l_variant = cands[c].variant;
     p->level_switch = cands[c].sw;
+    p->m2l_variant_up = cands[c].vup;
     rc = run_m2l(p, p->d_M, p->d_L);  // warm-up (creates the grids)
     if (rc != BBFMM_OK) break;
     if (hipEventRecord(e0, p->stream) != hipSuccess) rc = set_error(BBFMM_E_HIP, "event record");
@@ -1128,10 +1147,12 @@ extern "C" int bbfmm_amd_plan_tune(bbfmm_amd_plan *p, int reps, double *table, i
     }
     const double per = ms / reps;
     if (table && rows < maxrows) {
-      table[4 * rows] = cands[c].scheme;
-      table[4 * rows + 1] = cands[c].variant;
-      table[4 * rows + 2] = cands[c].sw;
-      table[4 * rows + 3] = per;
+      double *row = table + (size_t)kTuneCols * rows;
+      row[0] = cands[c].scheme;
+      row[1] = cands[c].variant;
+      row[2] = cands[c].vup;
+      row[3] = cands[c].sw;
+      row[4] = per;
     }
     rows++;
     if (best < 0 || per < best_ms) {
@@ -1146,11 +1167,13 @@ extern "C" int bbfmm_amd_plan_tune(bbfmm_amd_plan *p, int reps, double *table, i
     p->scheme = s0;
     p->m2l_variant = v0;
     p->level_switch = w0;
+    p->m2l_variant_up = u0;
     return rc != BBFMM_OK ? rc : set_error(BBFMM_E_ARG, "no M2L configuration to tune");
   }
   p->scheme = cands[best].scheme;
   p->m2l_variant = cands[best].variant;
   p->level_switch = cands[best].sw;
+  p->m2l_variant_up = cands[best].vup;
   drop_graph(p);
   BB_TRY(ensure_grid(p));
   return BBFMM_OK;

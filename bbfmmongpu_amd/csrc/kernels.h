@@ -113,10 +113,12 @@ int launch_m2l_sibling(const M2LArgs &a, hipStream_t s);
 // ---- dense-grid M2L (ghost-padded per-sibling-class grids) ----
 struct GridM2L;  // defined in m2l_grid.hip
 // minlev: first level handled by the grids (levels below are left to the
-// caller); variant: 0 automatic, 1 source-stationary (split-BF16), 2 / 3 one
-// / two 32-column tiles per wave (r = 32 MFMA forms)
+// caller); variant: 0 automatic, 1 / 4 / 5 source-stationary (split-BF16)
+// with 8 / 4 / 2 field classes per wave, 2 / 3 one / two 32-column tiles per
+// wave (r = 32 MFMA forms); variant_upper: 0, or a source-stationary variant
+// for the levels above the deepest (then two launches)
 int grid_m2l_create(GridM2L **g, int r, int maxlev, int nrhs, int form, const real *K_colmajor_host,
-                    hipStream_t s, int minlev, int variant);
+                    hipStream_t s, int minlev, int variant, int variant_upper = 0);
 void grid_m2l_destroy(GridM2L *g);
 // flat = 1: the split-BF16 source-stationary kernel reads through pointers
 // (its form for grids of 2 GB and more) whatever the grid size

@@ -80,6 +80,12 @@ constexpr int kWaitVm0 = 0x0F70;
 struct GridM2L {
   int r = 0, maxlev = 0, nrhs = 1, form = 0, ncell_cap = 0, minlev = kMinLev, variant = 0;
   int flat = 0;  // 1: pointer-form loads even on grids below 2 GB
+  // split-BF16 source-stationary kernel with a class-group size of its own
+  // for the levels above the deepest: part[0] those levels, part[1] the
+  // deepest (one launch each); nparts = 1: one launch, args
+  int nparts = 1;
+  GridArgs part[2];
+  int part_blocks[2] = {0, 0};
   GridArgs args{};
   int nblocks = 0;
   size_t nslots = 0;
@@ -964,10 +970,19 @@ static void step_table(std::vector<int> &steps, int cls, int *nsteps)
   }
 }
 
+// field classes per wave of the source-stationary variants (1 / 4 / 5), 0 otherwise
+static int ss_classes(int variant)
+{
+  return variant == 1 ? 8 : variant == 4 ? 4 : variant == 5 ? 2 : 0;
+}
+
 int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const real *K, hipStream_t s,
-                    int minlev, int variant)
+                    int minlev, int variant, int variant_upper)
 {
   *out = nullptr;
+  if (variant_upper != 0 && !(ss_classes(variant_upper) && ss_classes(variant)))
+    return set_error(BBFMM_E_ARG, "dense-grid M2L: a variant for the upper levels (%d) needs source-stationary "
+                     "variants (1, 4, 5) for both", variant_upper);
   if (maxlev > kMaxLev) return set_error(BBFMM_E_ARG, "dense-grid M2L: maxlev %d > %d", maxlev, kMaxLev);
   if (minlev < kMinLev || minlev > maxlev)
     return set_error(BBFMM_E_ARG, "dense-grid M2L: first level %d outside [%d, %d]", minlev, kMinLev, maxlev);
@@ -1047,6 +1062,33 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   offset_table(tab);
   std::vector<int> steps;
   step_table(steps, a.ss ? a.ss : 8, a.nsteps);
+  // levels above the deepest with their own class-group size: two launches
+  const int ss_up = ss_classes(variant_upper);
+  size_t steps_up_off = 0;
+  if (a.ss && ss_up && ss_up != a.ss && maxlev > minlev) {
+    g->nparts = 2;
+    GridArgs &u = g->part[0], &d = g->part[1];
+    u = a;
+    d = a;
+    u.maxlev = maxlev - 1;
+    u.ss = ss_up;
+    int nbu = 0;
+    for (int l = minlev; l <= maxlev - 1; l++) {
+      u.blk0[l] = nbu;
+      const long long cols = (long long)(1 << (3 * (l - 1))) * nrhs;
+      nbu += (8 / ss_up) * (int)((cols + cpb - 1) / cpb);
+    }
+    u.blk0[maxlev] = nbu;
+    d.minlev = maxlev;
+    d.blk0[maxlev] = 0;
+    d.blk0[maxlev + 1] = a.blk0[maxlev + 1] - a.blk0[maxlev];
+    g->part_blocks[0] = nbu;
+    g->part_blocks[1] = d.blk0[maxlev + 1];
+    std::vector<int> su;
+    step_table(su, ss_up, u.nsteps);
+    steps_up_off = steps.size();
+    steps.insert(steps.end(), su.begin(), su.end());
+  }
   int rc = BBFMM_OK;
   auto fail = [&](hipError_t e, const char *what) {
     rc = set_error(BBFMM_E_HIP, "dense-grid M2L: %s: %s", what, hipGetErrorString(e));
@@ -1198,6 +1240,21 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   a.Mg1 = g->d_Mg1;
   a.Mg2 = g->d_Mg2;
   a.lvl_max = g->d_lvl_max;
+  if (g->nparts == 2) {
+    // everything from the final args (operator and grid pointers and sizes),
+    // then each part's own levels, blocks, class-group size and steps
+    for (int pi = 0; pi < 2; pi++) {
+      const GridArgs own = g->part[pi];
+      GridArgs &q = g->part[pi];
+      q = a;
+      q.minlev = own.minlev;
+      q.maxlev = own.maxlev;
+      for (int l = 0; l < kMaxLev + 2; l++) q.blk0[l] = own.blk0[l];
+      q.ss = own.ss;
+      for (int c = 0; c < 8; c++) q.nsteps[c] = own.nsteps[c];
+      q.steps = g->d_steps + (pi == 0 ? steps_up_off : 0);
+    }
+  }
   *out = g;
   return BBFMM_OK;
 }
@@ -1223,9 +1280,9 @@ void grid_m2l_destroy(GridM2L *g)
 #ifdef SINGLE
 // grids below 2 GB take the scalar-offset buffer-load form of the kernel
 template <int CLS>
-static void launch_ss(const GridM2L *g, const GridArgs &a, hipStream_t s)
+static void launch_ss(const GridM2L *g, const GridArgs &a, int nblocks, hipStream_t s)
 {
-  const dim3 grid((g->nblocks + 7) & ~7);
+  const dim3 grid((nblocks + 7) & ~7);
   if (a.gbytes > 0 && !g->flat)
     hipLaunchKernelGGL((k_m2l_mfma_ss<CLS, true>), grid, dim3(256), 0, s, a);
   else
@@ -1262,13 +1319,21 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_BF16X2>, dim3(sb), dim3(256), 0, s, a, M);
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
-    if (a.ss == 8)  // rounded up to whole rounds of the 8 XCDs; the extra blocks exit
-      launch_ss<8>(g, a, s);
-    else if (a.ss == 4)
-      launch_ss<4>(g, a, s);
-    else if (a.ss == 2)
-      launch_ss<2>(g, a, s);
-    else if (a.wave_cols == 64)
+    if (a.ss) {  // one launch, or the upper levels then the deepest
+      const int np = g->nparts;
+      for (int pi = 0; pi < np; pi++) {
+        GridArgs pa = np == 2 ? g->part[pi] : a;
+        pa.L = L;
+        const int nbk = np == 2 ? g->part_blocks[pi] : g->nblocks;
+        if (nbk == 0) continue;
+        if (pa.ss == 8)  // rounded up to whole rounds of the 8 XCDs; the extra blocks exit
+          launch_ss<8>(g, pa, nbk, s);
+        else if (pa.ss == 4)
+          launch_ss<4>(g, pa, nbk, s);
+        else
+          launch_ss<2>(g, pa, nbk, s);
+      }
+    } else if (a.wave_cols == 64)
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);
     else
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 1>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);

@@ -287,8 +287,8 @@ def run_c4(args):
     phi = torch.empty_like(q)
     plan.solve(q, phi)
     table = plan.tune(reps=3)
-    best = min(table, key=lambda t: t[3])
-    log(f"c4: tuner {[(int(a), int(b), int(c), round(d, 3)) for a, b, c, d in table]} -> {best}")
+    best = min(table, key=lambda t: t[4])
+    log(f"c4: tuner {[(int(a), int(b), int(u), int(c), round(d, 3)) for a, b, u, c, d in table]} -> {best}")
     ms_eager, st = timed_solves(plan, q, phi, args.steps, args.warmup, stream)
     plan.set(graph=True)
     ms_graph, _ = timed_solves(plan, q, phi, args.steps, args.warmup, stream)

@@ -189,6 +189,11 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                       Chebyshev interpolant.  The drop-in entry points
                                       (anotherFMMCompute, anotherDownwardPass) always take 1.
                                       Other orders: no effect */
+#define BBFMM_OPT_M2L_VARIANT_UPPER 12 /* IJ split-BF16 with a source-stationary variant
+                                          (1, 4, 5): the class-group size of the levels
+                                          above the deepest, run as a launch of their
+                                          own (1, 4, 5 = 8, 4, 2 classes per wave);
+                                          0 (default): the deepest level's */
 #define BBFMM_OPT_ADDRESSING 11 /* 0 (default): kernels with two forms read through
                                    scalar-offset buffer loads while their arrays are
                                    below 2 GB and through pointers beyond; 1: through
@@ -226,8 +231,9 @@ int bbfmm_amd_plan_set(bbfmm_amd_plan *plan, int option, int value);
    compile time): runs every M2L configuration valid for the plan's tree and
    arithmetic form (scheme basic/sibling/cluster/ij, tile variant, level
    switch) reps times on the plan's current multipoles, keeps the fastest
-   and writes up to maxrows rows (scheme, variant, level switch, ms per M2L)
-   to table.  The arithmetic form is never changed. */
+   and writes up to maxrows rows of 5 doubles (scheme, variant, upper-level
+   variant, level switch, ms per M2L) to table.  The arithmetic form is never
+   changed. */
 int bbfmm_amd_plan_tune(bbfmm_amd_plan *plan, int reps, double *table, int maxrows, int *nrows);
 
 /* Tree facts: out[0]=maxlev, out[1]=ncell, out[2]=N, out[3]=r, out[4]=n,

@@ -52,6 +52,41 @@ def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, nr
     plan.close()
 
 
+@pytest.mark.parametrize("variant,upper,nrhs", [(1, 5, 16), (1, 4, 16), (4, 5, 16), (5, 1, 16), (1, 5, 3)])
+def test_m2l_upper_levels_own_variant(variant, upper, nrhs, ops4, cuda):
+    """Split-BF16 source-stationary M2L with the levels above the deepest in a
+    launch of their own (BBFMM_OPT_M2L_VARIANT_UPPER): the same L as the
+    oracle, and the same bits as the single launch of the deepest level's
+    variant on that level."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, _q = O.uniform_inputs(65536, seed=0)
+    tree = O.Tree(pos, 0, "d")
+    rng = np.random.default_rng(17)
+    M = rng.standard_normal((nrhs, tree.ncell * 32)).astype(np.float32)
+    Md = torch.from_numpy(M).to(cuda)
+    out = {}
+    for up in (0, upper):
+        plan = B.Plan(pos, 4, U, K, prec="s", nrhs=nrhs).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2,
+                                                             variant=variant, variant_upper=up)
+        Ld = torch.zeros_like(Md)
+        plan.m2l(Md, Ld)
+        torch.cuda.synchronize()
+        out[up] = Ld.cpu().numpy()
+        plan.close()
+    ls = O.levsta(tree.maxlev)
+    deep = ls[tree.maxlev] * 32  # the deepest level's cells: same kernel, same launch shape
+    assert np.array_equal(out[0][:, deep:], out[upper][:, deep:])
+    s = ls[2] * 32
+    Kd = K.astype(np.float32).astype(np.float64)
+    for j in (0, nrhs - 1):
+        ref = O.m2l(tree, Kd, M[j].astype(np.float64), 32, "basic")
+        err = O.rel_l2(ref[s:], out[upper][j, s:])
+        print(f"variant={variant} upper={upper} nrhs={nrhs} rhs={j}: {err:.2e}", flush=True)
+        assert err < 3e-5
+
+
 def test_fp16x3_scales_each_level(ops4, cuda):
     """The split-FP16 form scales K and every level's M by powers of two:
     multipoles whose size differs by 1e10 between levels (beyond fp16's
@@ -91,10 +126,10 @@ def test_tuned_plan_solves_like_oracle(form, nrhs, ops4, cuda):
     plan = B.Plan(pos, 4, U, K, prec="s", nrhs=nrhs).set(scheme=B.SCHEME_IJ, form=form)
     plan.solve(torch.from_numpy(Q).to(cuda))
     table = plan.tune(reps=2)
-    print("tuning table:", [(int(a), int(b), int(c), round(d, 4)) for a, b, c, d in table], flush=True)
+    print("tuning table:", [(int(a), int(b), int(u), int(c), round(d, 4)) for a, b, u, c, d in table], flush=True)
     assert len(table) >= 2
-    best = min(table, key=lambda t: t[3])
-    assert all(t[3] > 0 for t in table)
+    best = min(table, key=lambda t: t[4])
+    assert all(t[4] > 0 for t in table)
     phi = plan.solve(torch.from_numpy(Q).to(cuda)).cpu().numpy().reshape(nrhs, -1)
     tree = O.Tree(pos, 0, "d")
     ref = O.fmm(tree, Q[0].astype(np.float64), 4, U.astype(np.float32).astype(np.float64),
