@@ -404,6 +404,57 @@ def m2l_drv(r, Lv, K, M, minlev, maxlev, L0, center, ineigh, neighbors, iinter, 
 
 
 # ---------------------------------------------------------------- autograd
+class _PlanCache:
+    """Plans of the autograd path kept across calls.  An entry is reused while
+    the order, operators (by content), precision, level modifier, vector
+    count, particle count and stream are the same; new positions are moved
+    into it by bbfmm_amd_plan_update_positions while every particle stays in
+    its leaf, and the plan is rebuilt when one leaves.  At most `size` plans
+    are kept (least recently used first out)."""
+
+    def __init__(self, size: int = 4):
+        import collections
+        self.size = size
+        self.entries = collections.OrderedDict()
+        self.builds = 0  # plans built (a test counts them)
+
+    @staticmethod
+    def _fingerprint(a) -> tuple:
+        import zlib
+        a = np.ascontiguousarray(a)
+        return (a.shape, a.dtype.str, zlib.crc32(a.view(np.uint8)))
+
+    def get(self, pos_host, n, U, K, prec, lmod, nrhs, stream):
+        key = (n, self._fingerprint(U), self._fingerprint(K), prec, lmod, nrhs, pos_host.shape[0], stream)
+        plan = self.entries.get(key)
+        if plan is not None:
+            p = np.ascontiguousarray(pos_host, dtype=dtype(prec)).reshape(-1, 3)
+            if not np.array_equal(plan.pos, p):
+                try:
+                    plan.update_positions(p)
+                except BBFMMError:
+                    plan.close()
+                    del self.entries[key]
+                    plan = None
+        if plan is None:
+            plan = Plan(pos_host, n, U, K, nrhs=nrhs, prec=prec, lmod=lmod, stream=stream)
+            self.builds += 1
+            self.entries[key] = plan
+            while len(self.entries) > self.size:
+                _, old = self.entries.popitem(last=False)
+                old.close()
+        self.entries.move_to_end(key)
+        return plan
+
+    def clear(self):
+        for plan in self.entries.values():
+            plan.close()
+        self.entries.clear()
+
+
+plan_cache = _PlanCache()
+
+
 def _make_autograd():
     torch = _torch()
 
@@ -413,19 +464,18 @@ def _make_autograd():
         dLoss/dpos from the adjoint solve (bbfmm_amd_gradient).  The
         backward recomputes the expansions (a solve of the 2m vectors
         [q; dLoss/dphi]); nothing but the inputs is kept from the forward.
+        Plans come from plan_cache: the tree and lists are built once per
+        geometry, and positions that stay in their leaves are only uploaded.
         cfg = (n, U, K, prec, lmod, scheme, form)."""
 
         @staticmethod
         def forward(ctx, pos, q, cfg):
             n, U, K, prec, lmod, scheme, form = cfg
             m = q.shape[0]
-            plan = Plan(pos.detach().cpu().numpy(), n, U, K, nrhs=m, prec=prec, lmod=lmod,
-                        stream=torch.cuda.current_stream().cuda_stream)
-            try:
-                plan.set(scheme=scheme, form=form, output="potential")
-                phi = plan.solve(q.detach().contiguous())
-            finally:
-                plan.close()
+            plan = plan_cache.get(pos.detach().cpu().numpy(), n, U, K, prec, lmod, m,
+                                  torch.cuda.current_stream().cuda_stream)
+            plan.set(scheme=scheme, form=form, output="potential")
+            phi = plan.solve(q.detach().contiguous())
             ctx.save_for_backward(pos, q)
             ctx.cfg = cfg
             return phi
@@ -435,14 +485,10 @@ def _make_autograd():
             pos, q = ctx.saved_tensors
             n, U, K, prec, lmod, scheme, form = ctx.cfg
             m = q.shape[0]
-            plan = Plan(pos.detach().cpu().numpy(), n, U, K, nrhs=2 * m, prec=prec, lmod=lmod,
-                        stream=torch.cuda.current_stream().cuda_stream)
-            try:
-                plan.set(scheme=scheme, form=form)
-                dq, dx = plan.gradient(q.detach().contiguous(), gphi.detach().contiguous())
-                torch.cuda.current_stream().synchronize()
-            finally:
-                plan.close()
+            plan = plan_cache.get(pos.detach().cpu().numpy(), n, U, K, prec, lmod, 2 * m,
+                                  torch.cuda.current_stream().cuda_stream)
+            plan.set(scheme=scheme, form=form)
+            dq, dx = plan.gradient(q.detach().contiguous(), gphi.detach().contiguous())
             return dx.to(pos.dtype), dq, None
 
     return FMMPotential

@@ -76,7 +76,8 @@ struct bbfmm_amd_plan {
   double m2l_interactions = 0, p2p_pairs = 0;
   // device
   real4 *d_pos = nullptr, *d_center = nullptr, *d_K4 = nullptr;
-  real *d_Kc = nullptr;  // column-major operators (ranks other than 32 and 256)
+  real *d_Kc = nullptr;  // column-major operators (ranks other than 32 and 256; the cluster scheme)
+  int *d_opoff = nullptr;  // offset of each operator (cluster scheme)
   real4 *d_pq = nullptr;  // sparse forests: positions with one vector's charge in w (lane-per-particle P2P)
   int *d_perm = nullptr, *d_leafsta = nullptr, *d_leafcnt = nullptr, *d_leafof = nullptr;
   int *d_iperm = nullptr;  // original index -> leaf order
@@ -192,7 +193,7 @@ int check_device()
 
 static void plan_free_device(bbfmm_amd_plan *p)
 {
-  void *ptrs[] = {p->d_pos, p->d_center, p->d_K4, p->d_Kc, p->d_perm, p->d_leafsta, p->d_leafcnt, p->d_leafof,
+  void *ptrs[] = {p->d_pos, p->d_center, p->d_K4, p->d_Kc, p->d_opoff, p->d_perm, p->d_leafsta, p->d_leafcnt, p->d_leafof,
                   p->d_nbr, p->d_nnbr, p->d_iinter, p->d_inter, p->d_kidx, p->d_Ktable, p->d_levsta,
                   p->d_Ucm, p->d_Urm, p->d_Tkz, p->d_Kw, p->d_Cw, p->d_scale, p->d_q, p->d_phi,
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
@@ -371,10 +372,14 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
     slot_operators(sop);
     BB_TRY(upload(&p->d_slotop, sop.data(), sop.size(), stream));
   }
-  if (k4)
-    BB_TRY(upload(&p->d_K4, K4.data(), K4.size(), stream));
-  else
-    BB_TRY(upload(&p->d_Kc, K, (size_t)kNumOps * r * r, stream));
+  if (k4) BB_TRY(upload(&p->d_K4, K4.data(), K4.size(), stream));
+  // column-major K: the any-rank kernels and the cluster scheme
+  if (!k4 || r <= kClusterMaxRank) BB_TRY(upload(&p->d_Kc, K, (size_t)kNumOps * r * r, stream));
+  {
+    int op[kNumOps];
+    operator_offsets(op);
+    BB_TRY(upload(&p->d_opoff, op, (size_t)kNumOps, stream));
+  }
   BB_TRY(upload(&p->d_Ucm, U, (size_t)n3 * r, stream));
   BB_TRY(upload(&p->d_Urm, Urm.data(), Urm.size(), stream));
   BB_TRY(upload(&p->d_Tkz, Tkz, (size_t)n * n, stream));
@@ -606,6 +611,7 @@ static PassArgs pass_args(const bbfmm_amd_plan *p, bool rows = false)
 static int ensure_grid(bbfmm_amd_plan *p)
 {
   if (p->scheme != M2L_SCHEME_CL && p->scheme != M2L_SCHEME_IJ) return BBFMM_OK;
+  if (p->scheme == M2L_SCHEME_CL && p->complete && p->r <= kClusterMaxRank) return BBFMM_OK;  // cluster kernel
   if (!p->complete) {
     // sparse forest: the list MFMA form of the IJ blocking
     if (p->scheme != M2L_SCHEME_IJ) return set_error(BBFMM_E_ARG, "scheme CL needs a complete tree");
@@ -679,6 +685,7 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
   m.kidx = p->d_kidx;
   m.K4 = p->d_K4;
   m.Kc = p->d_Kc;
+  m.opoff = p->d_opoff;
   m.Ktable = p->d_Ktable;
   m.M = M;
   m.L = L;
@@ -702,6 +709,20 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
     if (mid) BB_HIP(hipEventRecord(mid, p->stream));
     return launch_m2l_sibling(m, p->stream);
   case M2L_SCHEME_CL:
+    if (p->r <= kClusterMaxRank) {
+      // levels below the switch on the basic scheme, as for IJ
+      if (p->level_switch > kMinLev) {
+        M2LArgs b = m;
+        b.maxlev = p->level_switch - 1;
+        b.variant = 0;
+        BB_TRY(launch_m2l_basic(b, p->stream));
+        m.minlev = p->level_switch;
+      }
+      if (mid) BB_HIP(hipEventRecord(mid, p->stream));
+      return launch_m2l_cluster(m, p->stream);
+    }
+    // larger ranks: the operators do not fit LDS whole; the dense grids (real precision)
+    [[fallthrough]];
   case M2L_SCHEME_IJ:
     BB_TRY(ensure_grid(p));
     if (p->level_switch > kMinLev) {
@@ -1106,6 +1127,7 @@ extern "C" int bbfmm_amd_plan_tune(bbfmm_amd_plan *p, int reps, double *table, i
     if (p->form == BBFMM_M2L_PLAIN) {
       for (int v = 0; v < 3; v++) cands.push_back({M2L_SCHEME_BA, v, kMinLev, 0});
       cands.push_back({M2L_SCHEME_SI, 0, kMinLev, 0});
+      if (p->r <= kClusterMaxRank) cands.push_back({M2L_SCHEME_CL, 0, kMinLev, 0});
       for (int sw : sws) cands.push_back({M2L_SCHEME_IJ, 0, sw, 0});
     } else if (p->form == BBFMM_M2L_BF16X2) {
       for (int sw : sws)
@@ -1729,7 +1751,13 @@ extern "C" int m2l_drv(int r, real *L, real *K, real *M, int minlev, int maxlev,
     if ((rc = upload(&p.d_kidx, kidx.data(), kidx.size(), nullptr)) != BBFMM_OK) break;
     if ((rc = upload(&p.d_Ktable, Kt, (size_t)343, nullptr)) != BBFMM_OK) break;
     if (k4 && (rc = upload(&p.d_K4, K4.data(), K4.size(), nullptr)) != BBFMM_OK) break;
-    if (!k4 && (rc = upload(&p.d_Kc, K, (size_t)kNumOps * r * r, nullptr)) != BBFMM_OK) break;
+    if ((!k4 || r <= kClusterMaxRank) && (rc = upload(&p.d_Kc, K, (size_t)kNumOps * r * r, nullptr)) != BBFMM_OK)
+      break;
+    {
+      int op[kNumOps];
+      operator_offsets(op);
+      if ((rc = upload(&p.d_opoff, op, (size_t)kNumOps, nullptr)) != BBFMM_OK) break;
+    }
     if ((rc = upload(&p.d_M, M, (size_t)ncell * r, nullptr)) != BBFMM_OK) break;
     if ((rc = dalloc(&dL, (size_t)ncell * r)) != BBFMM_OK) break;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -1749,11 +1777,13 @@ extern "C" int m2l_drv(int r, real *L, real *K, real *M, int minlev, int maxlev,
     m.kidx = p.d_kidx;
     m.K4 = p.d_K4;
     m.Kc = p.d_Kc;
+    m.opoff = p.d_opoff;
     m.Ktable = p.d_Ktable;
     m.M = p.d_M;
     m.L = dL;
     if (scheme == M2L_SCHEME_BA) rc = launch_m2l_basic(m, nullptr);
     else if (scheme == M2L_SCHEME_SI) rc = launch_m2l_sibling(m, nullptr);
+    else if (scheme == M2L_SCHEME_CL && r <= kClusterMaxRank) rc = launch_m2l_cluster(m, nullptr);
     else {
       rc = grid_m2l_create(&p.grid, r, maxlev, 1, BBFMM_M2L_PLAIN, p.h_K.data(), nullptr, minlev, 0);
       if (rc == BBFMM_OK) rc = grid_m2l_run(p.grid, p.d_M, dL, ncell, nullptr, nullptr);

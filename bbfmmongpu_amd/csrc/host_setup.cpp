@@ -13,6 +13,7 @@
 //  - lists: uncles in the parent's neighbour order x children 0..7, a cousin
 //    is a neighbour when every |dc|^2 <= (1.5 L_l)^2 (anotherInteractionList.c:11-111).
 #include "common.h"
+#include "kernels.h"
 
 #include <algorithm>
 #include <cmath>
@@ -198,6 +199,15 @@ void ktable(int *Kt)
     for (int vy = -3; vy <= 3; vy++)
       for (int vz = -3; vz <= 3; vz++, idx++)
         Kt[idx] = (std::abs(vx) > 1 || std::abs(vy) > 1 || std::abs(vz) > 1) ? op++ : -1;
+}
+
+// Inverse of the Ktable: the offset of operator k as (vx + 3) 49 + (vy + 3) 7 + (vz + 3).
+void operator_offsets(int *op)
+{
+  int Kt[343];
+  ktable(Kt);
+  for (int idx = 0; idx < 343; idx++)
+    if (Kt[idx] >= 0) op[Kt[idx]] = idx;
 }
 
 // Operator index of every interaction (m2l_aux.c:1135-1191): offset in units

@@ -106,9 +106,16 @@ struct M2LArgs {
   // lists: vsrc [ncell][189] (ncell: empty slot), cellcls [ncell], slotop
   // [8][189] operator of each slot (device)
   const int *vsrc = nullptr, *cellcls = nullptr, *slotop = nullptr;
+  // cluster scheme: offset of each operator, (vx + 3) 49 + (vy + 3) 7 + (vz + 3) [316] (device)
+  const int *opoff = nullptr;
 };
+constexpr int kClusterMaxRank = 64;  // the cluster kernel stages whole r x r operators in LDS
 int launch_m2l_basic(const M2LArgs &a, hipStream_t s);
 int launch_m2l_sibling(const M2LArgs &a, hipStream_t s);
+// cluster blocking, r <= kClusterMaxRank, complete trees (a.Kc, a.opoff set)
+int launch_m2l_cluster(const M2LArgs &a, hipStream_t s);
+// operator offsets in Ktable order: op[k] = (vx + 3) 49 + (vy + 3) 7 + (vz + 3)
+void operator_offsets(int *op);
 
 // ---- dense-grid M2L (ghost-padded per-sibling-class grids) ----
 struct GridM2L;  // defined in m2l_grid.hip

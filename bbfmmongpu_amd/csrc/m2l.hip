@@ -3,7 +3,10 @@
 //            interaction list (the semantics of m2l_kern_basic.cu:218-384);
 //   sibling (M2L_SCHEME_SI): one block per 8-sibling field cluster, the 8
 //            source siblings of each neighbouring cluster staged in LDS
-//            (the blocking of m2l_kern_sibling_blocking.cu / CPU7A).
+//            (the blocking of m2l_kern_sibling_blocking.cu / CPU7A);
+//   cluster (M2L_SCHEME_CL): one block per chunk of 2^3 field clusters, the
+//            operators staged in LDS and shared by its 64 cells (the
+//            blocking of m2l_host_cluster_blocking.cu).
 // Both write L (not accumulate) for every cell of levels minlev..maxlev.
 // r = 32 and 256 have kernels with the rank fixed at compile time (K in the
 // K4 layout below); every other rank runs the *_any kernels (K column-major).
@@ -292,6 +295,179 @@ __global__ void __launch_bounds__(256) k_m2l_sibling_any(int r, int level, int p
       for (int u = 0; u < RPT; u++)
         if (i0 + u * rt < r) L[roff + (size_t)(A0 + f) * r + i0 + u * rt] = acc[f][u];
   }
+}
+
+// ---------------------------------------------------------------- cluster blocking
+// M2L_SCHEME_CL (the blocking of m2l_host_cluster_blocking.cu:29-166): a
+// block is a chunk of 2 x 2 x 2 field clusters, the 64 cells of a 4 x 4 x 4
+// cube of the level.  The 316 operators pass through LDS in batches of
+// opb = 4096 / r^2 (double-buffered: the next batch is loaded into registers
+// while the current one is applied) and are shared by the chunk's 64 cells;
+// each cell applies those whose offset is in its interaction list (the
+// source inside the domain and its parent a neighbour of the cell's parent,
+// anotherInteractionList.c:11-111).  Four threads per cell, each holding
+// RPT consecutive rows; the rows a thread reads from LDS are the same
+// address for the 16 cells of a wave.  K column-major (K(i, j) at
+// k r^2 + j r + i), r <= 4 RPT.
+constexpr int kClusterBatch = 4096;  // reals of operators per LDS buffer
+
+template <int RPT>
+__global__ void __launch_bounds__(256) k_m2l_cluster(int r, int level, int lev0, int ncell,
+                                                    const int *__restrict__ opoff,
+                                                    const real *__restrict__ Kc,
+                                                    const real *__restrict__ M, real *__restrict__ L)
+{
+  constexpr int PER_THREAD = kClusterBatch / 256;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  real *sk = reinterpret_cast<real *>(smem_raw);  // [2][kClusterBatch]
+  const int c = threadIdx.x >> 2, rg = threadIdx.x & 3;
+  const int G = 1 << level;   // cells per axis
+  const int nch = G >> 2;     // chunks per axis
+  const int ch = blockIdx.x;
+  const int cx = ch / (nch * nch), cy = (ch / nch) % nch, cz = ch % nch;
+  const int fx = 4 * cx + (c >> 4), fy = 4 * cy + ((c >> 2) & 3), fz = 4 * cz + (c & 3);
+  const size_t roff = (size_t)blockIdx.y * ncell * r;
+  const int i0 = rg * RPT;
+  const int r2 = r * r;
+  const int opb = kClusterBatch / r2;  // operators per batch (>= 1 for r <= 64)
+  const int nbt = (kNumOps + opb - 1) / opb;
+  const size_t ktot = (size_t)kNumOps * r2;
+  real acc[RPT];
+#pragma unroll
+  for (int u = 0; u < RPT; u++) acc[u] = 0;
+  real pre[PER_THREAD];
+  auto fetch = [&](int bt) {
+    const size_t base = (size_t)bt * opb * r2;
+#pragma unroll
+    for (int v = 0; v < PER_THREAD; v++) {
+      const int o = threadIdx.x + 256 * v;
+      pre[v] = (o < opb * r2 && base + o < ktot) ? Kc[base + o] : (real)0;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int v = 0; v < PER_THREAD; v++) sk[buf * kClusterBatch + threadIdx.x + 256 * v] = pre[v];
+  };
+  fetch(0);
+  store(0);
+  __syncthreads();
+  for (int bt = 0; bt < nbt; bt++) {
+    if (bt + 1 < nbt) fetch(bt + 1);
+    const real *skb = sk + (bt & 1) * kClusterBatch;
+    for (int q = 0; q < opb; q++) {
+      const int k = bt * opb + q;
+      if (k >= kNumOps) break;
+      const int pv = opoff[k];  // (vx + 3) 49 + (vy + 3) 7 + (vz + 3)
+      const int vx = pv / 49 - 3, vy = (pv / 7) % 7 - 3, vz = pv % 7 - 3;
+      const int sx = fx + vx, sy = fy + vy, sz = fz + vz;
+      if (sx < 0 || sy < 0 || sz < 0 || sx >= G || sy >= G || sz >= G) continue;
+      const int dpx = (sx >> 1) - (fx >> 1), dpy = (sy >> 1) - (fy >> 1), dpz = (sz >> 1) - (fz >> 1);
+      if (dpx < -1 || dpx > 1 || dpy < -1 || dpy > 1 || dpz < -1 || dpz > 1) continue;
+      const real *Ms = M + roff + (size_t)(lev0 + morton_encode(sx, sy, sz, level)) * r;
+      const real *kq = skb + q * r2 + i0;
+      // the source's coefficients 16 at a time: the loads are issued together
+      for (int j0 = 0; j0 < r; j0 += 16) {
+        real m[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) m[t] = j0 + t < r ? Ms[j0 + t] : (real)0;
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+          if (j0 + t >= r) break;
+          const real *kc = kq + (j0 + t) * r;
+#pragma unroll
+          for (int u = 0; u < RPT; u++)
+            if (i0 + u < r) acc[u] += kc[u] * m[t];
+        }
+      }
+    }
+    // the other buffer was last read in batch bt - 1, before the barrier that ended it
+    if (bt + 1 < nbt) store((bt + 1) & 1);
+    __syncthreads();
+  }
+  real *Lp = L + roff + (size_t)(lev0 + morton_encode(fx, fy, fz, level)) * r;
+#pragma unroll
+  for (int u = 0; u < RPT; u++)
+    if (i0 + u < r) Lp[i0 + u] = acc[u];
+}
+
+// r = 32: the same blocking with the chunk's operators shared through the
+// scalar unit instead of LDS.  Lane l of each wave is cell l of the chunk
+// (all 32 rows); wave w applies the operators w, w + 4, ...; K_k is the same
+// for the 64 lanes, so its entries come by scalar loads into SGPRs and each
+// FMA reads one SGPR and one VGPR (the lane's source coefficient).  The four
+// waves' partial sums are added through LDS in a fixed order.
+__global__ void __launch_bounds__(256) k_m2l_cluster32(int level, int lev0, int ncell,
+                                                      const int *__restrict__ opoff,
+                                                      const real *__restrict__ Kc,
+                                                      const real *__restrict__ M, real *__restrict__ L)
+{
+  __shared__ real red[3][32][64];  // partial sums of waves 1..3, [row][cell]
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int G = 1 << level, nch = G >> 2;
+  const int ch = blockIdx.x;
+  const int cx = ch / (nch * nch), cy = (ch / nch) % nch, cz = ch % nch;
+  const int fx = 4 * cx + (lane >> 4), fy = 4 * cy + ((lane >> 2) & 3), fz = 4 * cz + (lane & 3);
+  const size_t roff = (size_t)blockIdx.y * ncell * 32;
+  real acc[32];
+#pragma unroll
+  for (int i = 0; i < 32; i++) acc[i] = 0;
+  for (int k = w; k < kNumOps; k += 4) {
+    const int pv = opoff[k];
+    const int vx = pv / 49 - 3, vy = (pv / 7) % 7 - 3, vz = pv % 7 - 3;
+    const int sx = fx + vx, sy = fy + vy, sz = fz + vz;
+    if (sx < 0 || sy < 0 || sz < 0 || sx >= G || sy >= G || sz >= G) continue;
+    const int dpx = (sx >> 1) - (fx >> 1), dpy = (sy >> 1) - (fy >> 1), dpz = (sz >> 1) - (fz >> 1);
+    if (dpx < -1 || dpx > 1 || dpy < -1 || dpy > 1 || dpz < -1 || dpz > 1) continue;
+    const real4 *Ms = reinterpret_cast<const real4 *>(M + roff + (size_t)(lev0 + morton_encode(sx, sy, sz, level)) * 32);
+    real4 m[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) m[q] = Ms[q];
+    const real *__restrict__ Kk = Kc + (size_t)k * 1024;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const real mj[4] = {m[q].x, m[q].y, m[q].z, m[q].w};
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const real *__restrict__ kc = Kk + (4 * q + t) * 32;  // column j
+#pragma unroll
+        for (int i = 0; i < 32; i++) acc[i] += kc[i] * mj[t];
+      }
+    }
+  }
+  if (w > 0)
+#pragma unroll
+    for (int i = 0; i < 32; i++) red[w - 1][i][lane] = acc[i];
+  __syncthreads();
+  if (w != 0) return;
+  real *Lp = L + roff + (size_t)(lev0 + morton_encode(fx, fy, fz, level)) * 32;
+#pragma unroll
+  for (int i = 0; i < 32; i++) Lp[i] = ((acc[i] + red[0][i][lane]) + red[1][i][lane]) + red[2][i][lane];
+}
+
+int launch_m2l_cluster(const M2LArgs &a, hipStream_t s)
+{
+  if (a.minlev < 2) return set_error(BBFMM_E_ARG, "M2L cluster scheme needs minlev >= 2");
+  if (!a.h_levsta) return set_error(BBFMM_E_ARG, "M2L cluster scheme needs a complete tree");
+  if (a.r < 1 || a.r > kClusterMaxRank)
+    return set_error(BBFMM_E_ARG, "M2L cluster scheme: r=%d outside 1..%d", a.r, kClusterMaxRank);
+  if (!a.Kc || !a.opoff) return set_error(BBFMM_E_ARG, "M2L cluster scheme needs the column-major operators");
+  static_assert(kClusterMaxRank * kClusterMaxRank <= kClusterBatch, "one operator per LDS buffer at least");
+  const size_t sh = sizeof(real) * 2 * kClusterBatch;
+  for (int level = a.minlev; level <= a.maxlev; level++) {
+    const int nch = 1 << (3 * (level - 2));
+    if (a.r == 32)
+      hipLaunchKernelGGL(k_m2l_cluster32, dim3(nch, a.nrhs), dim3(256), 0, s, level, a.h_levsta[level], a.ncell,
+                         a.opoff, a.Kc, a.M, a.L);
+    else if (a.r <= 32)
+      hipLaunchKernelGGL(k_m2l_cluster<8>, dim3(nch, a.nrhs), dim3(256), sh, s, a.r, level, a.h_levsta[level],
+                         a.ncell, a.opoff, a.Kc, a.M, a.L);
+    else
+      hipLaunchKernelGGL(k_m2l_cluster<16>, dim3(nch, a.nrhs), dim3(256), sh, s, a.r, level, a.h_levsta[level],
+                         a.ncell, a.opoff, a.Kc, a.M, a.L);
+    BB_HIP(hipGetLastError());
+  }
+  return BBFMM_OK;
 }
 
 template <int BLOCK>

@@ -150,3 +150,64 @@ def test_gradient_argument_checks(ops4, cuda):
     with pytest.raises(B.BBFMMError, match="another leaf"):
         odd.update_positions(p2)
     odd.close()
+
+
+def test_autograd_plan_cache(ops4, cuda):
+    """potential() keeps its plans: a second call on the same positions
+    builds nothing, positions moved inside their leaves are uploaded into the
+    cached plans, a particle moved to another leaf makes a new plan; in every
+    case the potentials and gradients equal those of fresh plans."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    U, K = U.astype(np.float32), K.astype(np.float32)
+    N, m = 20000, 8
+    rng = np.random.default_rng(11)
+    pos = rng.random((N, 3)) - 0.5
+    Q = torch.from_numpy(rng.uniform(-1, 1, (m, N)).astype(np.float32)).to(cuda)
+    T = torch.from_numpy(rng.uniform(-1, 1, (m, N)).astype(np.float32)).to(cuda)
+
+    def grads(p):
+        p_t = torch.tensor(p, dtype=torch.float32, device=cuda, requires_grad=True)
+        q_t = Q.clone().requires_grad_(True)
+        ph = B.potential(p_t, q_t, 4, U, K, prec="s")
+        (0.5 * ((ph - T) ** 2).sum()).backward()
+        return ph.detach().cpu().numpy(), q_t.grad.cpu().numpy(), p_t.grad.cpu().numpy()
+
+    def fresh(p):
+        B.plan_cache.clear()
+        return grads(p)
+
+    B.plan_cache.clear()
+    b0 = B.plan_cache.builds
+    first = grads(pos)
+    assert B.plan_cache.builds == b0 + 2  # forward (m vectors) and backward (2m)
+    again = grads(pos)
+    assert B.plan_cache.builds == b0 + 2
+    for a, b in zip(first, again):
+        assert np.array_equal(a, b)
+    # inside the leaves (depth 3 here: leaves of 1/8; a shift of 1e-6 keeps
+    # every particle in its leaf unless it lies within 1e-6 of a face)
+    leaf = 1.0 / 8
+    frac = (pos + 0.5) / leaf
+    keep = np.all((frac - np.floor(frac) > 1e-4) & (np.ceil(frac) - frac > 1e-4), axis=1)
+    moved = pos + np.where(keep[:, None], 1e-6, 0.0)
+    b1 = B.plan_cache.builds
+    cached = grads(moved)
+    assert B.plan_cache.builds == b1
+    ref = fresh(moved)
+    for a, b in zip(cached, ref):
+        assert np.array_equal(a, b)
+    # one particle to another leaf: new plans
+    across = moved.copy()
+    across[0] = -across[0]
+    if np.array_equal(np.floor((across[0] + 0.5) / leaf), np.floor((moved[0] + 0.5) / leaf)):
+        across[0] = np.where(across[0] > 0, across[0] - leaf, across[0] + leaf)
+    grads(moved)
+    b2 = B.plan_cache.builds
+    crossed = grads(across)
+    assert B.plan_cache.builds == b2 + 2
+    ref = fresh(across)
+    for a, b in zip(crossed, ref):
+        assert np.array_equal(a, b)
+    B.plan_cache.clear()
