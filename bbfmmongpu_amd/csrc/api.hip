@@ -111,6 +111,12 @@ struct bbfmm_amd_plan {
   // and the per-form operator layout (created on first use)
   int h_cls_off[9] = {};
   int *d_cls_cells = nullptr, *d_vsrc = nullptr;
+  // the list M2L above a level switch: the class lists of the cells of
+  // levels >= list_sw (host copies of every cell's level and class to build them)
+  std::vector<int> h_celllev, h_cellcls;
+  int h_cls_off_sw[9] = {};
+  int *d_cls_cells_sw = nullptr;
+  int list_sw = -1;
   int *d_cellcls = nullptr, *d_slotop = nullptr;  // class of each cell, operator of each class slot
   // downward pass of sparse forests on MFMA (n = 4, r = 32, fp32): the cells
   // of levels >= 2 grouped by (level, class) as (cell, parent), waves of up
@@ -199,6 +205,7 @@ static void plan_free_device(bbfmm_amd_plan *p)
                   p->d_S, p->d_M, p->d_L, p->d_F, p->d_qin, p->d_phiout, p->d_farfield, p->d_qt,
                   p->d_ppos, p->d_seg, p->d_nseg, p->d_pad_of, p->d_iperm, p->d_grad, p->d_phit, p->d_ipad, p->d_phin,
                   p->d_celllev, p->d_child, p->d_plist, p->d_leafcell, p->d_leafihalf, p->d_cls_cells,
+                  p->d_cls_cells_sw,
                   p->d_vsrc, p->d_dcp, p->d_dwv, p->d_KU, p->d_T8, p->d_T8m, p->d_mplist, p->d_pq,
                   p->d_qsh,   p->d_qsl, p->d_cellcls, p->d_slotop};
   for (void *q : ptrs)
@@ -366,6 +373,8 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
   if (!f.complete) {
     for (int c = 0; c < 9; c++) p->h_cls_off[c] = f.cls_off[c];
     BB_TRY(upload(&p->d_cls_cells, f.cls_cells.data(), f.cls_cells.size(), stream));
+    p->h_celllev = f.celllev;
+    p->h_cellcls = f.cellcls;
     BB_TRY(upload(&p->d_vsrc, f.vsrc.data(), f.vsrc.size(), stream));
     BB_TRY(upload(&p->d_cellcls, f.cellcls.data(), f.cellcls.size(), stream));
     std::vector<int> sop;
@@ -615,12 +624,33 @@ static int ensure_grid(bbfmm_amd_plan *p)
   if (!p->complete) {
     // sparse forest: the list MFMA form of the IJ blocking
     if (p->scheme != M2L_SCHEME_IJ) return set_error(BBFMM_E_ARG, "scheme CL needs a complete tree");
-    if (!p->list || p->list_form != p->form) {
+    const int sw = p->level_switch > kMinLev ? p->level_switch : kMinLev;
+    if (!p->list || p->list_form != p->form || p->list_sw != sw) {
       if (p->list) list_m2l_destroy(p->list);
       p->list = nullptr;
-      BB_TRY(list_m2l_create(&p->list, p->r, p->nrhs, p->form, p->h_K.data(), p->ncell, p->h_cls_off,
-                             p->d_cls_cells, p->d_vsrc, p->stream));
+      const int *off = p->h_cls_off, *cells = p->d_cls_cells;
+      if (sw > kMinLev) {
+        // the class lists restricted to levels >= sw (same order)
+        std::vector<int> cc;
+        int o[9] = {};
+        for (int c = 0; c < 8; c++) {
+          o[c] = (int)cc.size();
+          for (int A = 0; A < p->ncell; A++)
+            if (p->h_celllev[A] >= sw && p->h_cellcls[A] == c) cc.push_back(A);
+        }
+        o[8] = (int)cc.size();
+        if (p->d_cls_cells_sw) (void)hipFree(p->d_cls_cells_sw);
+        p->d_cls_cells_sw = nullptr;
+        BB_TRY(upload(&p->d_cls_cells_sw, cc.data(), cc.size(), p->stream));
+        BB_HIP(hipStreamSynchronize(p->stream));  // the host list must outlive the copy
+        for (int c = 0; c < 9; c++) p->h_cls_off_sw[c] = o[c];
+        off = p->h_cls_off_sw;
+        cells = p->d_cls_cells_sw;
+      }
+      BB_TRY(list_m2l_create(&p->list, p->r, p->nrhs, p->form, p->h_K.data(), p->ncell, off, cells, p->d_vsrc,
+                             p->stream));
       p->list_form = p->form;
+      p->list_sw = sw;
     }
     list_m2l_set_flat(p->list, p->flat_loads);
     return BBFMM_OK;
@@ -697,6 +727,14 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
   }
   if (!p->complete && p->scheme == M2L_SCHEME_IJ) {
     BB_TRY(ensure_grid(p));
+    if (p->level_switch > kMinLev) {
+      // the levels below the switch on the basic scheme (sources by class slot)
+      M2LArgs b = m;
+      b.variant = 0;
+      b.celllev = p->d_celllev;
+      b.lev_hi = p->level_switch - 1;
+      BB_TRY(launch_m2l_basic(b, p->stream));
+    }
     return list_m2l_run(p->list, M, L, p->stream, mid);
   }
   if (!p->complete && p->scheme != M2L_SCHEME_BA)
@@ -1119,7 +1157,8 @@ extern "C" int bbfmm_amd_plan_tune(bbfmm_amd_plan *p, int reps, double *table, i
     for (int v = 0; v < 3; v++) cands.push_back({M2L_SCHEME_BA, v, kMinLev, 0});
 #ifdef SINGLE
     if ((p->form == BBFMM_M2L_BF16X2 || p->form == BBFMM_M2L_FP32MFMA) && p->r == 32)
-      cands.push_back({M2L_SCHEME_IJ, 0, kMinLev, 0});
+      for (int sw = kMinLev; sw <= std::min(p->maxlev, kMinLev + 2); sw++)  // list M2L from level sw, basic below
+        cands.push_back({M2L_SCHEME_IJ, 0, sw, 0});
 #endif
   } else {
     std::vector<int> sws{kMinLev};

@@ -142,8 +142,7 @@ struct Inst {
   std::vector<std::vector<int>> plev;             // cells with children per level 0..d-1
   std::vector<int> perm, leafof;                  // [n]
   std::vector<int> leafcell, leafsta, leafcnt;    // [nleaf]
-  std::vector<int> nbr, nnbr;                     // [nleaf * 27], [nleaf]
-  std::vector<int> iinter, vsrc;                  // [ncell], [ncell * 189]
+  std::vector<Level> lev;                         // stored cells per level (sorted keys)
   double pairs = 0, ninter = 0;
 };
 
@@ -177,7 +176,48 @@ int bucket_max(const real3 *pos, int n, real L, std::vector<uint64_t> &ki)
   return BBFMM_OK;
 }
 
-void build_instance(Inst &I, const real3 *P, int n, int depth, int leafmax, real L, const std::vector<real> &celeng)
+// Offset tables of the list construction, by the sibling class sf of the
+// field cell: far_slot[(sf * 27 + u) * 8 + sc] = the class slot of the
+// offset to child sc of uncle u (u = 9 (dx + 1) + 3 (dy + 1) + dz + 1), -1
+// when near; near_uc[sf * 27 + e] = (uncle, child) of the neighbour at the
+// offset e of the same numbering.
+struct ListTabs {
+  std::vector<int> far_slot;
+  std::vector<int2> near_uc;
+  ListTabs() : far_slot(8 * 27 * 8, -1), near_uc(8 * 27)
+  {
+    for (int sf = 0; sf < 8; sf++) {
+      const int fb[3] = {(sf >> 2) & 1, (sf >> 1) & 1, sf & 1};
+      for (int u = 0; u < 27; u++) {
+        const int du[3] = {u / 9 - 1, (u / 3) % 3 - 1, u % 3 - 1};
+        for (int sc = 0; sc < 8; sc++) {
+          const int cb[3] = {(sc >> 2) & 1, (sc >> 1) & 1, sc & 1};
+          int v[3];
+          bool near = true;
+          for (int a = 0; a < 3; a++) {
+            v[a] = 2 * du[a] + cb[a] - fb[a];
+            near = near && v[a] >= -1 && v[a] <= 1;
+          }
+          far_slot[((size_t)sf * 27 + u) * 8 + sc] = near ? -1 : class_slot(sf, v[0], v[1], v[2]);
+        }
+      }
+      for (int e = 0; e < 27; e++) {
+        const int de[3] = {e / 9 - 1, (e / 3) % 3 - 1, e % 3 - 1};
+        int u = 0, sc = 0;
+        for (int a = 0; a < 3; a++) {
+          const int t = fb[a] + de[a];               // in [-1, 2]
+          const int dd = t >= 0 ? t / 2 : -1;        // floor(t / 2)
+          u = 3 * u + (dd + 1);
+          sc = 2 * sc + (t - 2 * dd);
+        }
+        near_uc[sf * 27 + e] = make_int2(u, sc);
+      }
+    }
+  }
+};
+
+void build_instance(Inst &I, const real3 *P, int n, int depth, int leafmax, real L, const std::vector<real> &celeng,
+                    const ListTabs &tabs)
 {
   constexpr int D = kMaxForestDepth;
   I.n = n;
@@ -275,62 +315,73 @@ void build_instance(Inst &I, const real3 *P, int n, int depth, int leafmax, real
       I.leafcnt[q] = pj - sta;
     }
   }
-  // P2P neighbour leaves (self included)
-  I.nbr.assign((size_t)nl * 27, -1);
-  I.nnbr.assign(nl, 0);
-  const int ncpd = 1 << d;
-  for (int q = 0; q < nl; q++) {
-    int x, y, z;
-    morton_coords(lev[d].key[q], d, x, y, z);
-    int m = 0;
-    for (int dx = -1; dx <= 1; dx++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dz = -1; dz <= 1; dz++) {
-          const int X = x + dx, Y = y + dy, Z = z + dz;
-          if (X < 0 || Y < 0 || Z < 0 || X >= ncpd || Y >= ncpd || Z >= ncpd) continue;
-          const int B = lev[d].find(morton_key(X, Y, Z, d));
-          if (B < 0) continue;
-          const int bl = B - lev[d].first;
-          I.nbr[(size_t)q * 27 + m++] = bl;
-          I.pairs += (double)I.leafcnt[q] * (double)I.leafcnt[bl];
-        }
-    I.nnbr[q] = m;
-  }
-  // interactions: children of the parent's neighbours that are not
-  // neighbours, stored by class slot (the order of the dense-grid offsets)
-  I.iinter.assign(nc, 0);
-  I.vsrc.assign((size_t)nc * kM2LOffsetsPerClass, -1);
+  I.lev = std::move(lev);
+}
+
+// Lists of instance I, written straight into the global tables of f (its
+// cells from c0, its leaves from lf0).  The 27 neighbours ("uncles") of a
+// parent P are looked up once for its up to 8 children; a child f = 2p + s
+// (per axis) and a child c = 2(p + u) + s' of uncle u are at offset
+// v = 2u + s' - s: far offsets (|v|_inf >= 2) are the child's interactions
+// (stored by class slot), near ones at the deepest level its P2P neighbours
+// (in the order of the leaf's own offsets e in [-1, 1]^3:
+// e -> uncle floor((s + e) / 2), child (s + e) mod 2).
+void build_lists(Inst &I, HostForest &f, int c0, int lf0, const ListTabs &tabs)
+{
+  const std::vector<Level> &lev = I.lev;
+  const int d = I.d;
+  // this instance's part of the global tables: empty slots name the zero cell
+  std::fill(f.vsrc.begin() + (size_t)kM2LOffsetsPerClass * c0, f.vsrc.begin() + (size_t)kM2LOffsetsPerClass * (c0 + I.ncell),
+            f.ncell);
   for (int l = kMinLev; l <= d; l++) {
     const int np = 1 << (l - 1);
-    for (size_t j = 0; j < lev[l].key.size(); j++) {
-      const int F = lev[l].first + (int)j;
-      int fx, fy, fz;
-      morton_coords(lev[l].key[j], l, fx, fy, fz);
-      const int px = fx >> 1, py = fy >> 1, pz = fz >> 1;
-      int m = 0;
-      for (int dx = -1; dx <= 1; dx++)
-        for (int dy = -1; dy <= 1; dy++)
-          for (int dz = -1; dz <= 1; dz++) {
-            const int X = px + dx, Y = py + dy, Z = pz + dz;
-            if (X < 0 || Y < 0 || Z < 0 || X >= np || Y >= np || Z >= np) continue;
-            const int U = lev[l - 1].find(morton_key(X, Y, Z, l - 1));
-            if (U < 0) continue;
-            for (int s = 0; s < 8; s++) {
-              const int C = I.child[(size_t)8 * U + s];
-              if (C < 0) continue;
-              const int vx = 2 * X + ((s >> 2) & 1) - fx, vy = 2 * Y + ((s >> 1) & 1) - fy,
-                        vz = 2 * Z + (s & 1) - fz;
-              if (std::abs(vx) <= 1 && std::abs(vy) <= 1 && std::abs(vz) <= 1) continue;
-              m++;
-              const int slot = class_slot(I.cls[F], vx, vy, vz);  // every far cousin has a slot
-              if (slot >= 0) I.vsrc[(size_t)kM2LOffsetsPerClass * F + slot] = C;
-            }
+    const Level &up = lev[l - 1];
+    for (size_t j = 0; j < up.key.size(); j++) {
+      const int Pc = up.first + (int)j;
+      int px, py, pz;
+      morton_coords(up.key[j], l - 1, px, py, pz);
+      int unc[27];
+      for (int u = 0; u < 27; u++) {
+        const int X = px + u / 9 - 1, Y = py + (u / 3) % 3 - 1, Z = pz + u % 3 - 1;
+        unc[u] = (X < 0 || Y < 0 || Z < 0 || X >= np || Y >= np || Z >= np) ? -1
+                                                                          : up.find(morton_key(X, Y, Z, l - 1));
+      }
+      for (int sf = 0; sf < 8; sf++) {
+        const int F = I.child[(size_t)8 * Pc + sf];
+        if (F < 0) continue;
+        const int *far = &tabs.far_slot[(size_t)sf * 27 * 8];
+        int m = 0;
+        for (int u = 0; u < 27; u++) {
+          if (unc[u] < 0) continue;
+          for (int sc = 0; sc < 8; sc++) {
+            const int C = I.child[(size_t)8 * unc[u] + sc];
+            const int slot = far[u * 8 + sc];
+            if (C < 0 || slot < 0) continue;
+            m++;
+            f.vsrc[(size_t)kM2LOffsetsPerClass * (c0 + F) + slot] = c0 + C;
           }
-      I.iinter[F] = m;
-      I.ninter += m;
+        }
+        f.iinter[c0 + F] = m;
+        I.ninter += m;
+        if (l == d) {  // F is a leaf: its P2P neighbours (self included)
+          const int q = F - lev[d].first;
+          int nm = 0;
+          for (int e = 0; e < 27; e++) {
+            const int2 uc = tabs.near_uc[sf * 27 + e];
+            if (unc[uc.x] < 0) continue;
+            const int B = I.child[(size_t)8 * unc[uc.x] + uc.y];
+            if (B < 0) continue;
+            const int bl = B - lev[d].first;
+            f.nbr[(size_t)(lf0 + q) * 27 + nm++] = lf0 + bl;
+            I.pairs += (double)I.leafcnt[q] * (double)I.leafcnt[bl];
+          }
+          f.nnbr[lf0 + q] = nm;
+        }
+      }
     }
   }
 }
+
 
 // f(i) for i in [0, n) on up to `threads` threads, items taken in order
 template <class Fn>
@@ -359,7 +410,7 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
   f = HostForest();
   f.complete = false;
   if (ninst < 1) return set_error(BBFMM_E_ARG, "no instances");
-  (void)class_slot(0, 3, 3, 3);  // builds the slot table before the threads read it
+  const ListTabs tabs;  // (also builds class_slot's table before the threads run)
   long long ptot = 0;
   for (int i = 0; i < ninst; i++) {
     if (Ns[i] < 1) return set_error(BBFMM_E_ARG, "instance %d has %d points", i, Ns[i]);
@@ -384,7 +435,7 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
   std::stable_sort(byz.begin(), byz.end(), [&](int a, int b) { return Ns[a] > Ns[b]; });
   parallel_items(ninst, [&](int t) {
     const int i = byz[t];
-    build_instance(in[i], pos + p0s[i], Ns[i], depth ? depth[i] : 0, leafmax, L, celeng);
+    build_instance(in[i], pos + p0s[i], Ns[i], depth ? depth[i] : 0, leafmax, L, celeng, tabs);
   });
   for (int i = 0; i < ninst; i++)
     if (in[i].rc != BBFMM_OK) return set_error(in[i].rc, "a particle lies outside the root cell [-L/2, L/2]^3");
@@ -399,8 +450,6 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
     f.inst_depth.push_back(in[i].d);
     f.inst_p0.push_back((int)p0s[i]);
     f.inst_leaf0.push_back(lf0[i]);
-    f.pairs += in[i].pairs;
-    f.ninter += in[i].ninter;
   }
   const int ncell = c0[ninst], nleaf = lf0[ninst];
   f.ncell = ncell;
@@ -416,10 +465,10 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
   f.leafsta.resize(nleaf);
   f.leafcnt.resize(nleaf);
   f.leafihalf.resize(nleaf);
-  f.nbr.resize((size_t)nleaf * 27);
-  f.nnbr.resize(nleaf);
-  f.iinter.resize(ncell);
-  f.vsrc.resize((size_t)ncell * kM2LOffsetsPerClass);
+  f.nbr.assign((size_t)nleaf * 27, -1);
+  f.nnbr.assign(nleaf, 0);
+  f.iinter.assign(ncell, 0);
+  f.vsrc.resize((size_t)ncell * kM2LOffsetsPerClass);  // filled by build_lists
   f.cellcls.resize(ncell);
   std::vector<int> &cellclass = f.cellcls;
   parallel_items(ninst, [&](int i) {
@@ -431,7 +480,6 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
       f.center[c + A] = I.center[A];
       f.celllev[c + A] = I.celllev[A];
       cellclass[c + A] = I.cls[A];
-      f.iinter[c + A] = I.iinter[A];
     }
     for (size_t k = 0; k < I.child.size(); k++) f.child[(size_t)8 * c + k] = shift(I.child[k], c);
     for (int j = 0; j < I.n; j++) {
@@ -444,14 +492,13 @@ int forest_from_points(HostForest &f, int ninst, const int *Ns, const real3 *pos
       f.leafsta[lf + q] = p + I.leafsta[q];
       f.leafcnt[lf + q] = I.leafcnt[q];
       f.leafihalf[lf + q] = ihalf;
-      f.nnbr[lf + q] = I.nnbr[q];
     }
-    for (size_t k = 0; k < I.nbr.size(); k++) f.nbr[(size_t)27 * lf + k] = shift(I.nbr[k], lf);
-    // empty sources point at the zero cell
-    for (size_t k = 0; k < I.vsrc.size(); k++)
-      f.vsrc[(size_t)kM2LOffsetsPerClass * c + k] = I.vsrc[k] < 0 ? ncell : I.vsrc[k] + c;
-    std::vector<int>().swap(I.vsrc);
+    build_lists(I, f, c, lf, tabs);
   });
+  for (int i = 0; i < ninst; i++) {
+    f.pairs += in[i].pairs;
+    f.ninter += in[i].ninter;
+  }
   // parents per level across instances
   f.ploff.assign(dmax + 1, 0);
   for (int l = 0; l < dmax; l++) {

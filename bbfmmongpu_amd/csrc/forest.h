@@ -7,9 +7,34 @@
 
 #include "common.h"
 
+#include <memory>
+#include <utility>
 #include <vector>
 
 namespace bb {
+
+// std::allocator that leaves new elements uninitialised: for large tables
+// that are filled right after they are sized
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+  template <class U>
+  void construct(U *p) noexcept
+  {
+    ::new ((void *)p) U;
+  }
+  template <class U, class... A>
+  void construct(U *p, A &&...a)
+  {
+    ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
 
 struct HostForest {
   int N = 0, ncell = 0, nleaf = 0, maxdepth = 0;
@@ -32,7 +57,8 @@ struct HostForest {
   std::vector<int> inst_N, inst_depth, inst_p0, inst_leaf0;  // batch instances
   // sparse forests: field cells of levels >= 2 grouped by sibling class, and
   // the source cell of each of the class's 189 M2L offsets (ncell: empty)
-  std::vector<int> cls_off, cls_cells, vsrc;  // [9], [cells of levels >= 2], [ncell * 189]
+  std::vector<int> cls_off, cls_cells;        // [9], [cells of levels >= 2]
+  std::vector<int, NoInitAlloc<int>> vsrc;    // [ncell * 189]
   std::vector<int> cellcls;                   // sparse forests: sibling class of each cell [ncell]
 };
 

@@ -108,6 +108,9 @@ struct M2LArgs {
   const int *vsrc = nullptr, *cellcls = nullptr, *slotop = nullptr;
   // cluster scheme: offset of each operator, (vx + 3) 49 + (vy + 3) 7 + (vz + 3) [316] (device)
   const int *opoff = nullptr;
+  // sparse forests: only the cells of levels lev_lo..lev_hi (celllev [ncell], device; NULL: all)
+  const int *celllev = nullptr;
+  int lev_lo = 0, lev_hi = 1 << 30;
 };
 constexpr int kClusterMaxRank = 64;  // the cluster kernel stages whole r x r operators in LDS
 int launch_m2l_basic(const M2LArgs &a, hipStream_t s);

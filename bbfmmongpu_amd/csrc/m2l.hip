@@ -24,7 +24,14 @@ namespace bb {
 // slotop[class][189] its operator; the empty slots are skipped.
 struct SlotArgs {
   const int *vsrc = nullptr, *cellcls = nullptr, *slotop = nullptr;
+  const int *celllev = nullptr;  // not null: only the cells of levels lev_lo..lev_hi
+  int lev_lo = 0, lev_hi = 1 << 30;
 };
+
+__device__ __forceinline__ bool slot_cell_skipped(const SlotArgs &sa, int F)
+{
+  return sa.celllev && (sa.celllev[F] < sa.lev_lo || sa.celllev[F] > sa.lev_hi);
+}
 
 template <bool SLOTS>
 __device__ __forceinline__ bool m2l_source(const SlotArgs &sa, int F, int t, int ncell, const int *il,
@@ -52,6 +59,7 @@ __global__ void __launch_bounds__(BLOCK) k_m2l_basic(int F0, int nF, int ncell, 
   const int i = threadIdx.x % R;
   const int F = F0 + blockIdx.x * CPB + cl;
   if (F >= F0 + nF) return;  // no barrier below
+  if (SLOTS && slot_cell_skipped(sa, F)) return;
   const size_t roff = (size_t)blockIdx.y * ncell * R;
   const int ni = SLOTS ? kM2LOffsetsPerClass : iinter[F];
   const int *il = SLOTS ? nullptr : inter + (size_t)ipitch * F;
@@ -202,6 +210,7 @@ __global__ void __launch_bounds__(BLOCK) k_m2l_basic_any(int r, int F0, int nF, 
   any_rank_lane<BLOCK>(r, rt, cpb, cl, i0);
   const int F = F0 + blockIdx.x * cpb + cl;
   if (cl >= cpb || F >= F0 + nF) return;  // no barrier below
+  if (SLOTS && slot_cell_skipped(sa, F)) return;
   const size_t roff = (size_t)blockIdx.y * ncell * r;
   const int ni = SLOTS ? kM2LOffsetsPerClass : iinter[F];
   const int *il = SLOTS ? nullptr : inter + (size_t)ipitch * F;
@@ -478,6 +487,9 @@ static int basic_any_t(const M2LArgs &a, hipStream_t s)
   sa.vsrc = a.vsrc;
   sa.cellcls = a.cellcls;
   sa.slotop = a.slotop;
+  sa.celllev = a.celllev;
+  sa.lev_lo = a.lev_lo;
+  sa.lev_hi = a.lev_hi;
   if (!a.h_levsta) {
     // sparse forests: every cell, sources by class slot; cells of levels 0-1 have none (L = 0)
     if (!a.vsrc) return set_error(BBFMM_E_ARG, "M2L basic: a sparse forest needs its slot sources");
@@ -528,6 +540,9 @@ static int basic_t(const M2LArgs &a, hipStream_t s)
   sa.vsrc = a.vsrc;
   sa.cellcls = a.cellcls;
   sa.slotop = a.slotop;
+  sa.celllev = a.celllev;
+  sa.lev_lo = a.lev_lo;
+  sa.lev_hi = a.lev_hi;
   if (!a.h_levsta) {
     // no level table (a batch of sparse instances): one launch over every
     // cell, sources by class slot; cells of levels 0-1 have none (L = 0)

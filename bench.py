@@ -208,7 +208,8 @@ def run_c3(args):
     Q = rng.uniform(-1.0, 1.0, size=(nrhs, N))
     log(f"c3: N={N} nrhs={nrhs}: operator precompute n=8 r=256 (FP64, GPU)")
     t0 = time.time()
-    U, K, sigma = B.operators(8, 256, prec="s", device=True)
+    U64, K64, sigma = B.operators(8, 256, prec="d", device=True)
+    U, K = U64.astype(np.float32), K64.astype(np.float32)
     t_ops = time.time() - t0
     t0 = time.time()
     plan = B.Plan(pos, 8, U, K, nrhs=nrhs, prec="s", stream=stream.cuda_stream).set(
@@ -227,6 +228,20 @@ def run_c3(args):
         d = B.direct(pos, Q[j], tg)
         errs.append(rel_l2(d, phi[j, tg]))
     log(f"c3: errors vs FP64 direct {errs}")
+    # the fp64 plain solve of vector 0 (dense-grid M2L in fp64 FMAs), kept for
+    # error comparison: its own error against the direct sum and the chosen
+    # form's distance from it
+    p64 = B.Plan(pos, 8, U64, K64, nrhs=1, prec="d", stream=stream.cuda_stream).set(
+        scheme=B.SCHEME_IJ, form=B.FORM_PLAIN)
+    q64 = torch.from_numpy(Q[0].astype(np.float64)).to(dev)
+    phi64d = torch.empty_like(q64)
+    ms64, st64 = timed_solves(p64, q64, phi64d, 1, 1, stream)
+    phi64 = phi64d.cpu().numpy()
+    p64.close()
+    err64 = rel_l2(B.direct(pos, Q[0], tg), phi64[tg])
+    chosen_vs_fp64 = rel_l2(phi64, phi[0])
+    log(f"c3: fp64 plain solve {ms64:.1f} ms (M2L {st64['m2l']:.1f} ms), its error vs FP64 direct {err64:.3e}, "
+        f"chosen vs fp64 plain {chosen_vs_fp64:.3e}")
     r = 256
     inter = plan.m2l_interactions
     flop_m2l = r * (2 * r - 1) * inter * nrhs
@@ -246,7 +261,9 @@ def run_c3(args):
         "s_per_solve": ms * 1e-3 / nrhs,
         "m2l_gflops": flop_m2l / (st["m2l"] * 1e-3) / 1e9,
         "p2p_ginteractions_per_s": plan.p2p_pairs * nrhs / (st["p2p"] * 1e-3) / 1e9,
-        "rel_l2_err": {"q_phi": errs, "targets": int(tg.size)},
+        "rel_l2_err": {"q_phi": errs, "targets": int(tg.size), "fp64_plain_q_phi": err64,
+                       "chosen_vs_fp64_plain": chosen_vs_fp64},
+        "fp64_plain": {"ms_per_solve": ms64, "stage_ms": st64},
         "stage_ms": st, "setup_s": {"operators": t_ops, "plan": t_setup},
         "roofline": roof,
     }

@@ -180,7 +180,8 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                  hi*lo + lo*hi) with fp32 accumulation, relative error of a
                                  term below 3 * 2^-16.  No counterpart in the reference
                                  (its P2P is plain fp32) */
-#define BBFMM_OPT_LEVEL_SWITCH 7 /* CL/IJ schemes: first level on the dense grids, the
+#define BBFMM_OPT_LEVEL_SWITCH 7 /* CL/IJ schemes: first level on the dense grids (on a
+                                    batch of sparse instances: on the list M2L), the
                                     levels below on the basic scheme (default 2) */
 #define BBFMM_OPT_L2P_REFERENCE 10 /* orders n % 4 == 3 (n = 3, 7): 1 evaluates L2P as the
                                       reference does, weighting the last z node with the

@@ -161,3 +161,31 @@ def test_batch_list_mfma_against_oracle(form, nrhs, tol, ops4, cuda):
             assert err < tol
         o += n
     plan.close()
+
+
+@pytest.mark.parametrize("form,switch,tol", [(1, 3, 3e-5), (1, 4, 3e-5), (2, 3, 1e-5)])
+def test_batch_list_m2l_level_switch(form, switch, tol, ops4, cuda):
+    """Sparse forests with a level switch: the basic scheme (sources by class
+    slot, fp32 FMAs) on levels 2 .. switch - 1 and the list MFMA kernel from
+    the switch on, per instance against the oracle."""
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    inst = _instances()
+    pos_list = [p for p, _ in inst]
+    depths = [d for _, d in inst]
+    Ntot = sum(p.shape[0] for p in pos_list)
+    Q = np.random.default_rng(18).uniform(-1.0, 1.0, size=(1, Ntot)).astype(np.float32)
+    plan = B.Plan.batch(pos_list, 4, U, K, nrhs=1, depths=depths, prec="s").set(
+        form=form, scheme=B.SCHEME_IJ, level_switch=switch)
+    phi = plan.solve_host(Q).reshape(Ntot)
+    plan.close()
+    Ud, Kd = U.astype(np.float32).astype(np.float64), K.astype(np.float32).astype(np.float64)
+    o = 0
+    for (p, d) in inst:
+        n = p.shape[0]
+        tree = O.Tree(p.astype(np.float32).astype(np.float64), 0, "d", maxlev=d)
+        ref = O.fmm(tree, Q[0, o:o + n].astype(np.float64), 4, Ud, Kd, scheme="basic")
+        err = O.rel_l2(ref["phi"], phi[o:o + n])
+        print(f"level switch {switch} form={form} instance N={n} depth={d}: {err:.2e}", flush=True)
+        assert err < tol
+        o += n

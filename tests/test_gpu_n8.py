@@ -85,3 +85,44 @@ def test_solve_n8_against_oracle(prec, form, nrhs, check, ops8, cuda):
         err = O.rel_l2(ref["phi"], phi[j])
         print(f"n=8 solve prec={prec} form={form} rhs={j}: rel_l2={err:.3e}", flush=True)
         assert err < tol
+
+
+@pytest.fixture(scope="module")
+def real_ops8(cuda):
+    """The kernel's own n = 8, r = 256 operators (FP64 precompute with the
+    sampling, Gram matrix and E blocks on the GPU)."""
+    import bbfmmongpu_amd as B
+    U, K, sigma = B.operators(8, R8, prec="d", device=True)
+    return U, K, sigma
+
+
+@pytest.mark.parametrize("prec,form", [("d", 0), ("s", 0), ("s", 2)])
+def test_solve_n8_real_operators(prec, form, real_ops8, cuda):
+    """n = 8 with the real FP64 operators: the GPU solve against the oracle's
+    solve on the same operators (1e-12 fp64, 1e-5 fp32), and the oracle
+    against the FP64 direct sum (the accuracy n = 8 buys: below 2e-5 on
+    these points, against about 6e-4 at n = 4)."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, sigma = real_ops8
+    assert sigma[0] > sigma[R8 - 1] > 0
+    N = 32768
+    pos, q = O.uniform_inputs(N, seed=9)
+    dt = _dt(prec)
+    q = q.astype(dt)
+    plan = B.Plan(pos, 8, U, K, nrhs=1, prec=prec).set(scheme=B.SCHEME_IJ, form=form)
+    assert plan.maxlev == 3
+    phi = plan.solve(torch.from_numpy(q).to(cuda)).cpu().numpy()
+    plan.close()
+    tree = O.Tree(pos, 0, prec)
+    ref = O.fmm(tree, q, 8, U.astype(dt), K.astype(dt), scheme="sibling")
+    err = O.rel_l2(ref["phi"], phi)
+    tol = 1e-5 if prec == "s" else 1e-12
+    print(f"n=8 real operators prec={prec} form={form}: vs oracle {err:.3e}", flush=True)
+    assert err < tol
+    if prec == "d":
+        tg = np.arange(0, N, 37, dtype=np.int32)
+        d = O.direct(pos, q, tg) * q[tg]
+        acc = O.rel_l2(d, ref["phi"][tg])
+        print(f"n=8 oracle vs FP64 direct sum: {acc:.3e}", flush=True)
+        assert acc < 2e-5
