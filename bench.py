@@ -95,6 +95,70 @@ def measured_traffic(kernel: str):
     return None, src
 
 
+def short_kernel_name(name: str) -> str:
+    """rocprofv3 kernel name -> the short form the summaries use, e.g.
+    'void bb::k_m2l_mfma_ss<4, true>(bb::GridArgs)' -> 'k_m2l_mfma_ss<4,true>'."""
+    n = name.strip()
+    for pre in ("void ", "bb::"):
+        if n.startswith(pre):
+            n = n[len(pre):]
+    n = n.replace("bb::", "")
+    depth, out = 0, []
+    for ch in n:  # drop the argument list (the first '(' outside template brackets)
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            break
+        out.append(ch)
+    return "".join(out).replace(" ", "")
+
+
+def summarize_pmc(fetch_csv: str, write_csv: str, out_path: str, command: str):
+    """Writes the HBM-traffic summary bench.py reads (profiles/pmc_traffic_c2.txt)
+    from two rocprofv3 --pmc passes of the same run (FETCH_SIZE, WRITE_SIZE;
+    KiB per dispatch).  Both counters are scaled by what they read for the
+    256 MiB calibration copy (--copy-calibration: the largest
+    __amd_rocclr_copyBuffer dispatch), and every kernel of KERNEL_SOURCE gets
+    the SHA-256 of its source file, so that a later change marks it stale."""
+    import csv
+    import collections
+
+    def load(path, counter):
+        per = collections.defaultdict(list)
+        copy = 0.0
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] != counter:
+                    continue
+                v = float(row["Counter_Value"])
+                name = short_kernel_name(row["Kernel_Name"])
+                if name == "__amd_rocclr_copyBuffer":
+                    copy = max(copy, v)
+                per[name].append(v)
+        return per, copy
+
+    fetch, fcopy = load(fetch_csv, "FETCH_SIZE")
+    write, wcopy = load(write_csv, "WRITE_SIZE")
+    nbytes = 256 << 20
+    ffac, wfac = nbytes / (fcopy * 1024), nbytes / (wcopy * 1024)
+    lines = ["# rocprofv3 FETCH_SIZE and WRITE_SIZE passes (KiB per dispatch, mean over dispatches)",
+             f"# command: {command}",
+             f"# calibration: 256 MiB device copy reads FETCH {fcopy:.0f} KiB, WRITE {wcopy:.0f} KiB; "
+             f"factors fetch {ffac:.4f} write {wfac:.4f}",
+             f"copy_kernel=__amd_rocclr_copyBuffer bytes={nbytes} fetch_kib={fcopy:.0f} write_kib={wcopy:.0f}"]
+    for name in sorted(set(fetch) | set(write)):
+        fk = float(np.mean(fetch[name])) if fetch.get(name) else 0.0
+        wk = float(np.mean(write[name])) if write.get(name) else 0.0
+        traffic = (fk * ffac + wk * wfac) * 1024
+        sha = source_sha(name)
+        lines.append(f"kernel={name} dispatches={len(fetch.get(name, write.get(name, [])))} fetch_kib={fk:.0f} "
+                     f"write_kib={wk:.0f} traffic_bytes={traffic:.0f}" + (f" src_sha={sha}" if sha else ""))
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def host_cores():
     """Physical cores this process may run on: the (socket, core) pairs that
     lscpu lists for the CPUs of the affinity mask, capped by the cgroup CPU
@@ -342,7 +406,8 @@ def run_c4(args):
                                f"a=0.1, half sphere surface r=0.4, leaves <= 32 points, n=4, r=32",
                    "instances": len(pos_list), "N_total": Ntot, "depth_min": min(depths),
                    "depth_max": max(depths), "m2l_form": "split-BF16 (3 products, FP32 accumulation)",
-                   "m2l_config": {"scheme": int(best[0]), "variant": int(best[1]), "level_switch": int(best[2])}},
+                   "m2l_config": {"scheme": int(best[0]), "variant": int(best[1]), "variant_upper": int(best[2]),
+                                  "level_switch": int(best[3])}},
         "particles_per_s": Ntot / (ms_graph * 1e-3),
         "ms_eager": ms_eager, "ms_graph": ms_graph,
         "m2l_gflops": flop_m2l / (st["m2l"] * 1e-3) / 1e9,
@@ -494,11 +559,19 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU baseline solves (median; after 1 warm-up)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for HBM byte counters)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the HBM-traffic summary (profiles/pmc_traffic_c2.txt) from the "
+                         "pmc_counter_collection.csv files of a FETCH_SIZE and a WRITE_SIZE rocprofv3 pass "
+                         "of this bench run with --copy-calibration, then exit")
+    ap.add_argument("--pmc-command", default="", help="the profiling command, recorded in the summary")
     ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: N=1M n=4 chosen vs plain (default); c3: N=4M n=8 r=256 depth 6, "
                          "16 vectors, FP32 MFMA M2L; c4: 64 uneven instances in one plan and one hipGraph; "
                          "c5: N=262K forward + adjoint (gradients) of 8 vectors")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], TRAFFIC_FILE, args.pmc_command)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note only "
               "(run with --gpus 1)", file=sys.stderr)
