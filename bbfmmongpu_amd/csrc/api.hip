@@ -1104,13 +1104,13 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_M2L_VARIANT_UPPER:
-    if (value != 0 && value != 1 && value != 4 && value != 5)
-      return set_error(BBFMM_E_ARG, "M2L upper-level variant %d: 0, 1, 4 or 5", value);
+    if (value != 0 && value != 1 && (value < 4 || value > 7))
+      return set_error(BBFMM_E_ARG, "M2L upper-level variant %d: 0, 1 or 4 to 7", value);
     p->m2l_variant_up = value;
     drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_M2L_VARIANT:
-    if (value < 0 || value > 5) return set_error(BBFMM_E_ARG, "M2L variant %d out of range", value);
+    if (value < 0 || value > 7) return set_error(BBFMM_E_ARG, "M2L variant %d out of range", value);
     p->m2l_variant = value;
     drop_graph(p);
     return BBFMM_OK;

@@ -179,12 +179,14 @@ int bucket_max(const real3 *pos, int n, real L, std::vector<uint64_t> &ki)
 // Offset tables of the list construction, by the sibling class sf of the
 // field cell: far_slot[(sf * 27 + u) * 8 + sc] = the class slot of the
 // offset to child sc of uncle u (u = 9 (dx + 1) + 3 (dy + 1) + dz + 1), -1
-// when near; near_uc[sf * 27 + e] = (uncle, child) of the neighbour at the
-// offset e of the same numbering.
+// when near, and slot_uc its inverse (each of the 189 slots of a class is one
+// (uncle, child)); near_uc[sf * 27 + e] = (uncle, child) of the neighbour at
+// the offset e of the same numbering.
 struct ListTabs {
   std::vector<int> far_slot;
   std::vector<int2> near_uc;
-  ListTabs() : far_slot(8 * 27 * 8, -1), near_uc(8 * 27)
+  std::vector<int2> slot_uc;  // [sf * 189 + slot]: the (uncle, child) of each class slot
+  ListTabs() : far_slot(8 * 27 * 8, -1), near_uc(8 * 27), slot_uc(8 * kM2LOffsetsPerClass, make_int2(-1, -1))
   {
     for (int sf = 0; sf < 8; sf++) {
       const int fb[3] = {(sf >> 2) & 1, (sf >> 1) & 1, sf & 1};
@@ -198,7 +200,9 @@ struct ListTabs {
             v[a] = 2 * du[a] + cb[a] - fb[a];
             near = near && v[a] >= -1 && v[a] <= 1;
           }
-          far_slot[((size_t)sf * 27 + u) * 8 + sc] = near ? -1 : class_slot(sf, v[0], v[1], v[2]);
+          const int slot = near ? -1 : class_slot(sf, v[0], v[1], v[2]);
+          far_slot[((size_t)sf * 27 + u) * 8 + sc] = slot;
+          if (slot >= 0) slot_uc[(size_t)sf * kM2LOffsetsPerClass + slot] = make_int2(u, sc);
         }
       }
       for (int e = 0; e < 27; e++) {
@@ -319,57 +323,62 @@ void build_instance(Inst &I, const real3 *P, int n, int depth, int leafmax, real
 }
 
 // Lists of instance I, written straight into the global tables of f (its
-// cells from c0, its leaves from lf0).  The 27 neighbours ("uncles") of a
-// parent P are looked up once for its up to 8 children; a child f = 2p + s
-// (per axis) and a child c = 2(p + u) + s' of uncle u are at offset
-// v = 2u + s' - s: far offsets (|v|_inf >= 2) are the child's interactions
-// (stored by class slot), near ones at the deepest level its P2P neighbours
-// (in the order of the leaf's own offsets e in [-1, 1]^3:
-// e -> uncle floor((s + e) / 2), child (s + e) mod 2).
+// cells from c0, its leaves from lf0).  Level by level from the root, every
+// cell's 27 neighbours (same level, -1 where absent; offset
+// e = 9 (dx + 1) + 3 (dy + 1) + dz + 1) follow from its parent's: the
+// neighbour at offset e of child s of P is child (s + e) mod 2 of P's
+// neighbour floor((s + e) / 2) (per axis).  For a child f = 2p + s of P, the
+// 27 neighbours ("uncles") of P give its interaction list: the child c =
+// 2(p + u) + s' of uncle u is at offset v = 2u + s' - s, and the far offsets
+// (|v|_inf >= 2) are stored by class slot, every slot written once (the zero
+// cell where empty); at the deepest level the near ones are the leaf's P2P
+// neighbours, in the order of the leaf's own offsets.
 void build_lists(Inst &I, HostForest &f, int c0, int lf0, const ListTabs &tabs)
 {
   const std::vector<Level> &lev = I.lev;
   const int d = I.d;
-  // this instance's part of the global tables: empty slots name the zero cell
-  std::fill(f.vsrc.begin() + (size_t)kM2LOffsetsPerClass * c0, f.vsrc.begin() + (size_t)kM2LOffsetsPerClass * (c0 + I.ncell),
-            f.ncell);
-  for (int l = kMinLev; l <= d; l++) {
-    const int np = 1 << (l - 1);
+  const int zero = f.ncell;
+  std::vector<int> nb_up(27, -1), nb_cur;
+  nb_up[13] = 0;  // the root is its own neighbour at offset 0
+  std::fill(f.vsrc.begin() + (size_t)kM2LOffsetsPerClass * c0, f.vsrc.begin() + (size_t)kM2LOffsetsPerClass * (c0 + 1),
+            zero);
+  for (int l = 1; l <= d; l++) {
     const Level &up = lev[l - 1];
+    const bool more = l < d;  // the cells of level l are parents of the next level
+    if (more) nb_cur.assign((size_t)27 * lev[l].key.size(), -1);
     for (size_t j = 0; j < up.key.size(); j++) {
       const int Pc = up.first + (int)j;
-      int px, py, pz;
-      morton_coords(up.key[j], l - 1, px, py, pz);
-      int unc[27];
-      for (int u = 0; u < 27; u++) {
-        const int X = px + u / 9 - 1, Y = py + (u / 3) % 3 - 1, Z = pz + u % 3 - 1;
-        unc[u] = (X < 0 || Y < 0 || Z < 0 || X >= np || Y >= np || Z >= np) ? -1
-                                                                          : up.find(morton_key(X, Y, Z, l - 1));
-      }
+      const int *unc = &nb_up[27 * j];
       for (int sf = 0; sf < 8; sf++) {
         const int F = I.child[(size_t)8 * Pc + sf];
         if (F < 0) continue;
-        const int *far = &tabs.far_slot[(size_t)sf * 27 * 8];
+        const int q = F - lev[l].first;
+        int nbF[27];
+        for (int e = 0; e < 27; e++) {
+          const int2 uc = tabs.near_uc[sf * 27 + e];
+          const int U = unc[uc.x];
+          nbF[e] = U < 0 ? -1 : I.child[(size_t)8 * U + uc.y];
+        }
+        if (more) std::copy(nbF, nbF + 27, &nb_cur[(size_t)27 * q]);
+        int *vs = &f.vsrc[(size_t)kM2LOffsetsPerClass * (c0 + F)];
+        if (l < kMinLev) {  // no list above level 2: all slots empty
+          std::fill(vs, vs + kM2LOffsetsPerClass, zero);
+          continue;
+        }
+        const int2 *suc = &tabs.slot_uc[(size_t)sf * kM2LOffsetsPerClass];
         int m = 0;
-        for (int u = 0; u < 27; u++) {
-          if (unc[u] < 0) continue;
-          for (int sc = 0; sc < 8; sc++) {
-            const int C = I.child[(size_t)8 * unc[u] + sc];
-            const int slot = far[u * 8 + sc];
-            if (C < 0 || slot < 0) continue;
-            m++;
-            f.vsrc[(size_t)kM2LOffsetsPerClass * (c0 + F) + slot] = c0 + C;
-          }
+        for (int k = 0; k < kM2LOffsetsPerClass; k++) {
+          const int U = unc[suc[k].x];
+          const int C = U < 0 ? -1 : I.child[(size_t)8 * U + suc[k].y];
+          vs[k] = C < 0 ? zero : c0 + C;
+          m += C >= 0;
         }
         f.iinter[c0 + F] = m;
         I.ninter += m;
         if (l == d) {  // F is a leaf: its P2P neighbours (self included)
-          const int q = F - lev[d].first;
           int nm = 0;
           for (int e = 0; e < 27; e++) {
-            const int2 uc = tabs.near_uc[sf * 27 + e];
-            if (unc[uc.x] < 0) continue;
-            const int B = I.child[(size_t)8 * unc[uc.x] + uc.y];
+            const int B = nbF[e];
             if (B < 0) continue;
             const int bl = B - lev[d].first;
             f.nbr[(size_t)(lf0 + q) * 27 + nm++] = lf0 + bl;
@@ -379,6 +388,7 @@ void build_lists(Inst &I, HostForest &f, int c0, int lf0, const ListTabs &tabs)
         }
       }
     }
+    if (more) nb_up.swap(nb_cur);
   }
 }
 

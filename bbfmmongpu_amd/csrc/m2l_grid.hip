@@ -67,6 +67,7 @@ struct GridArgs {
   const int *steps;  // [class group][208][kStepWords] step records (step_table)
   int nsteps[8];     // steps of each class group
   int ss;            // source-stationary kernel: field classes per wave (8, 4 or 2); 0: not used
+  int ss_tpw;        // source-stationary kernel: 32-column tiles per wave (1 or 2)
   int gbytes;        // bytes of one grid array when below 2 GB (scalar-offset buffer loads), else 0
   int kbytes;        // bytes of the operator array
 };
@@ -395,19 +396,66 @@ __global__ void __launch_bounds__(256) k_m2l_mfma(GridArgs g)
   }
 }
 
+// ---------------------------------------------------------------- ghost steps of a wave
+// The parents of a wave's span columns (columns are (parent, vector) pairs,
+// vector fastest, then pz, py, px) lie in the box [lo, hi] per axis (the
+// whole axis where the columns span more than one row).  A step of shift d
+// whose shifted box lies outside [0, P) in some axis reads only the ghost
+// layer (zeros) for every column of the wave: its products add nothing, and
+// the wave skips them.  A wave without valid columns skips every step.
+struct WaveBox {
+  int lo[3], hi[3];
+};
+
+__device__ __forceinline__ WaveBox wave_box(int col0, int ncols, int nrhs, int P, int span)
+{
+  WaveBox b;
+  if (col0 >= ncols) {
+    for (int a = 0; a < 3; a++) {
+      b.lo[a] = 2 * P;
+      b.hi[a] = -2 * P;
+    }
+    return b;
+  }
+  const int f = col0 / nrhs, t = min(col0 + span - 1, ncols - 1) / nrhs;
+  const int fx = f / (P * P), fy = (f / P) % P, fz = f % P;
+  const int tx = t / (P * P), ty = (t / P) % P, tz = t % P;
+  b.lo[0] = fx;
+  b.hi[0] = tx;
+  const bool row = fx == tx, line = row && fy == ty;
+  b.lo[1] = row ? fy : 0;
+  b.hi[1] = row ? ty : P - 1;
+  b.lo[2] = line ? fz : 0;
+  b.hi[2] = line ? tz : P - 1;
+  return b;
+}
+
+// the step record's first word: source class | packed shift << 3
+__device__ __forceinline__ bool ghost_step(const WaveBox &b, int e0, int P)
+{
+  const int pk = e0 >> 3;
+  const int d[3] = {((pk >> 4) & 3) - 1, ((pk >> 2) & 3) - 1, (pk & 3) - 1};
+  bool g = false;
+#pragma unroll
+  for (int a = 0; a < 3; a++) g |= (b.hi[a] + d[a] < 0) | (b.lo[a] + d[a] >= P);
+  return g;
+}
+
 // ---------------------------------------------------------------- source-stationary MFMA kernel
 // Same products, loop order inverted: a field cell of class s at parent p and
 // a source of class s' at parent p + d interact through the offset
 // v = 2d + s' - s, so for one (s', d) the shifted source tile M_{s'}[p + d]
-// serves every field class.  A wave owns 32 columns (parent, vector) of CLS
-// field classes (CLS accumulators); per step (s', d) it loads the source tile
-// once and multiplies it by the CLS operators K_v, which the block stages in
-// LDS (double-buffered) for all its waves.  CLS = 8 takes all classes of a
-// column group in one block; CLS = 4 or 2 splits a column group over 2 or 4
-// blocks (class groups), which evens out the work per CU when the column
-// groups do not fill whole rounds of the chip, and drops the steps in which
-// every class of the group has a near-field offset.  Split-BF16 (fp32 build).
-template <int CLS, bool BUF>
+// serves every field class.  A wave owns TPW tiles of 32 columns (parent,
+// vector) of CLS field classes (TPW x CLS accumulators); per step (s', d) it
+// loads its source tiles once and multiplies them by the CLS operators K_v,
+// which the block stages in LDS (double-buffered) for all its waves.  CLS = 8
+// takes all classes of a column group in one block; CLS = 4 or 2 splits a
+// column group over 2 or 4 blocks (class groups), which evens out the work
+// per CU when the column groups do not fill whole rounds of the chip, and
+// drops the steps in which every class of the group has a near-field offset.
+// With TPW = 2 every operator fragment read from LDS serves two tiles.
+// Split-BF16 (fp32 build).
+template <int CLS, int TPW, bool BUF>
 __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
 {
   constexpr int WAVES = 4;  // one k-part (k-step x hi/lo) of the operators per wave
@@ -430,25 +478,41 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
   const int PP = P + 2;
   const int ncols = P * P * P * g.nrhs;
   const int half = lane >> 5;
-  // columns past the end compute a copy of column 0 and are not stored
-  const int col = (grp * WAVES + wave) * 32 + (lane & 31);
-  const bool valid = col < ncols;
-  const int cc = valid ? col : 0;
-  const int pl = cc / g.nrhs;
-  const int rh = cc - pl * g.nrhs;
-  const int pz = pl % P, py = (pl / P) % P, px = pl / (P * P);
-  const int base = ((px + 1) * PP + py + 1) * PP + pz + 1;
+  const int sh0 = PP * PP + PP + 1;
+  const int col0 = (grp * WAVES + wave) * 32 * TPW;
+  // per tile: columns past the end compute a copy of column 0 and are not stored
+  bool valid[TPW];
+  int rh[TPW], px[TPW], py[TPW], pz[TPW], base[TPW], voff[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; t++) {
+    const int col = col0 + 32 * t + (lane & 31);
+    valid[t] = col < ncols;
+    const int cc = valid[t] ? col : 0;
+    const int pl = cc / g.nrhs;
+    rh[t] = cc - pl * g.nrhs;
+    pz[t] = pl % P;
+    py[t] = (pl / P) % P;
+    px[t] = pl / (P * P);
+    base[t] = ((px[t] + 1) * PP + py[t] + 1) * PP + pz[t] + 1;
+    // the scalar offset of a buffer load is unsigned: the shift's most
+    // negative value, -(PP^2 + PP + 1), moves to the lane part (base >= PP^2 + PP + 1)
+    voff[t] = (((base[t] - sh0) * g.nrhs + rh[t]) * 32 + half * 8) * 2;
+  }
   const int slot0 = g.lvl_slot0[l];
   const int cls = PP * PP * PP;
   const bf16x8 *Kl = (const bf16x8 *)g.Ka + wave * 64 + lane;
   const __bf16 *Mh = (const __bf16 *)g.Mg0;
   const __bf16 *Ml = (const __bf16 *)g.Mg1;
+  const WaveBox box =
+      wave_box((grp * WAVES + __builtin_amdgcn_readfirstlane(wave)) * 32 * TPW, ncols, g.nrhs, P, 32 * TPW);
 
-  f32x16 acc[CLS];
+  f32x16 acc[TPW][CLS];
 #pragma unroll
-  for (int c = 0; c < CLS; c++)
+  for (int t = 0; t < TPW; t++)
 #pragma unroll
-    for (int e = 0; e < 16; e++) acc[c][e] = 0.0f;
+    for (int c = 0; c < CLS; c++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) acc[t][c][e] = 0.0f;
 
   // Step record of the class group: e[0] = source class | packed shift << 3,
   // e[1 + i/2] holds the operator of the group's classes i (low 16 bits) and
@@ -456,7 +520,7 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
   // runs all CLS classes.  Each wave moves its k-part of the step's operators
   // (1 KB each) from global memory straight into the other LDS buffer
   // (global_load_lds, 16 bytes per lane at lane-contiguous addresses) while
-  // the MFMAs of the current step run; the source tile of the next step is
+  // the MFMAs of the current step run; the source tiles of the next step are
   // loaded into registers at the same time.  Records are read two steps ahead.
   typedef __attribute__((address_space(3))) void lds_void;
   // BUF: buffer loads whose per-step part is a scalar offset (the operator's
@@ -477,40 +541,36 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
                                          0, 0);
     }
   };
-  auto src_off = [&](const int *e) {
+  // source tiles of step e: hi and lo of k-steps 0 and 1
+  struct Src {
+    bf16x8 h0, l0, h1, l1;
+  };
+  auto load_src = [&](const int *e, Src *d) {
     const int sc = e[0] & 7, pk = e[0] >> 3;
     const int dx = ((pk >> 4) & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = (pk & 3) - 1;
-    return ((size_t)(slot0 + sc * cls + (dx * PP + dy) * PP + dz + base) * g.nrhs + rh) * 32 + half * 8;
-  };
-  // the scalar offset is unsigned: the shift's most negative value,
-  // -(PP^2 + PP + 1), moves to the lane part (base >= PP^2 + PP + 1)
-  const int sh0 = PP * PP + PP + 1;
-  auto src_soff = [&](const int *e) -> int {
-    const int sc = e[0] & 7, pk = e[0] >> 3;
-    const int dx = ((pk >> 4) & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = (pk & 3) - 1;
-    return (slot0 + sc * cls + (dx * PP + dy) * PP + dz + sh0) * g.nrhs * 64;
-  };
-  const int voff = (((base - sh0) * g.nrhs + rh) * 32 + half * 8) * 2;
-  // source tile of step e: hi and lo of k-steps 0 and 1
-  auto load_src = [&](const int *e, bf16x8 &h0, bf16x8 &l0, bf16x8 &h1, bf16x8 &l1) {
-    if (BUF) {
-      const int so = src_soff(e);
-      h0 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMh, voff, so, 0));
-      l0 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMl, voff, so, 0));
-      h1 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMh, voff + 32, so, 0));
-      l1 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMl, voff + 32, so, 0));
-    } else {
-      const size_t o = src_off(e);
-      h0 = *(const bf16x8 *)(Mh + o);
-      l0 = *(const bf16x8 *)(Ml + o);
-      h1 = *(const bf16x8 *)(Mh + o + 16);
-      l1 = *(const bf16x8 *)(Ml + o + 16);
+    const int sft = slot0 + sc * cls + (dx * PP + dy) * PP + dz;
+#pragma unroll
+    for (int t = 0; t < TPW; t++) {
+      if (BUF) {
+        const int so = (sft + sh0) * g.nrhs * 64;
+        d[t].h0 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMh, voff[t], so, 0));
+        d[t].l0 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMl, voff[t], so, 0));
+        d[t].h1 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMh, voff[t] + 32, so, 0));
+        d[t].l1 = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rMl, voff[t] + 32, so, 0));
+      } else {
+        const size_t o = ((size_t)(sft + base[t]) * g.nrhs + rh[t]) * 32 + half * 8;
+        d[t].h0 = *(const bf16x8 *)(Mh + o);
+        d[t].l0 = *(const bf16x8 *)(Ml + o);
+        d[t].h1 = *(const bf16x8 *)(Mh + o + 16);
+        d[t].l1 = *(const bf16x8 *)(Ml + o + 16);
+      }
     }
   };
   const int nst = g.nsteps[cg];
   const int *S = g.steps + (size_t)cg * kSteps * kStepWords;
   int e1[5];
-  bf16x8 bh0, bl0, bh1, bl1;
+  int ec = S[0];  // first word of the current step's record
+  Src cur[TPW];
   {
     int e0[5];
 #pragma unroll
@@ -519,7 +579,7 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
 #pragma unroll
     for (int i = 0; i < 5; i++) e1[i] = S[s1 * kStepWords + i];
     load_k(e0, 0);
-    load_src(e0, bh0, bl0, bh1, bl1);
+    load_src(e0, cur);
   }
   __builtin_amdgcn_s_waitcnt(kWaitVm0);
   __syncthreads();
@@ -531,63 +591,77 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
     // loads here); moved to scalar registers after the end-of-step wait
     const int4 r2a = *reinterpret_cast<const int4 *>(S + n2 * kStepWords);
     const int r2b = CLS > 6 ? S[n2 * kStepWords + 4] : 0;
+    const bool skip = ghost_step(box, ec, P);
     // k-step 0 of all classes, then the k-step-1 fragments; only then the
     // loads into the other buffer (the compiler orders LDS reads after any
     // earlier global_load_lds), then k-step 1
     bf16x8 ah[CLS], al[CLS];
+    if (!skip) {
 #pragma unroll
-    for (int c = 0; c < CLS; c++) {
-      ah[c] = sK[b][c][0][lane];
-      al[c] = sK[b][c][1][lane];
-    }
+      for (int c = 0; c < CLS; c++) {
+        ah[c] = sK[b][c][0][lane];
+        al[c] = sK[b][c][1][lane];
+      }
+      // all reads issued before the first product (otherwise the scheduler
+      // puts each read just before its MFMA, which then waits on it)
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int c = 0; c < CLS; c++) {
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh0, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl0, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh0, acc[c], 0, 0, 0);
-    }
+      for (int c = 0; c < CLS; c++)
 #pragma unroll
-    for (int c = 0; c < CLS; c++) {
-      ah[c] = sK[b][c][2][lane];
-      al[c] = sK[b][c][3][lane];
+        for (int t = 0; t < TPW; t++) {
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], cur[t].h0, acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], cur[t].l0, acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], cur[t].h0, acc[t][c], 0, 0, 0);
+        }
+#pragma unroll
+      for (int c = 0; c < CLS; c++) {
+        ah[c] = sK[b][c][2][lane];
+        al[c] = sK[b][c][3][lane];
+      }
     }
     __builtin_amdgcn_sched_barrier(0);
     load_k(e1, b ^ 1);
-    bf16x8 nh0, nl0, nh1, nl1;
-    load_src(e1, nh0, nl0, nh1, nl1);
+    Src nxt[TPW];
+    load_src(e1, nxt);
     __builtin_amdgcn_sched_barrier(0);
+    if (!skip) {
 #pragma unroll
-    for (int c = 0; c < CLS; c++) {
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh1, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl1, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh1, acc[c], 0, 0, 0);
+      for (int c = 0; c < CLS; c++)
+#pragma unroll
+        for (int t = 0; t < TPW; t++) {
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], cur[t].h1, acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], cur[t].l1, acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], cur[t].h1, acc[t][c], 0, 0, 0);
+        }
     }
-    bh0 = nh0;
-    bl0 = nl0;
-    bh1 = nh1;
-    bl1 = nl1;
+#pragma unroll
+    for (int t = 0; t < TPW; t++) cur[t] = nxt[t];
     // the next step's operators are in LDS for every wave after this (the
     // scheduling barrier keeps the MFMAs above the wait)
     __builtin_amdgcn_sched_barrier(0);
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
     __syncthreads();
+    ec = e1[0];
     e1[0] = __builtin_amdgcn_readfirstlane(r2a.x);
     e1[1] = __builtin_amdgcn_readfirstlane(r2a.y);
     e1[2] = __builtin_amdgcn_readfirstlane(r2a.z);
     e1[3] = __builtin_amdgcn_readfirstlane(r2a.w);
     e1[4] = __builtin_amdgcn_readfirstlane(r2b);
   }
-  if (!valid) return;
 #pragma unroll
-  for (int i = 0; i < CLS; i++) {
-    const int c = cg * CLS + i;
-    const int fx = 2 * px + ((c >> 2) & 1), fy = 2 * py + ((c >> 1) & 1), fz = 2 * pz + (c & 1);
-    const int A = g.levsta[l] + morton3(fx, fy, fz, l);
-    float *Lp = (float *)g.L + ((size_t)rh * g.ncell + A) * 32 + 4 * half;
+  for (int t = 0; t < TPW; t++) {
+    if (!valid[t]) continue;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      f32x4 v = {acc[i][4 * q], acc[i][4 * q + 1], acc[i][4 * q + 2], acc[i][4 * q + 3]};
-      *reinterpret_cast<f32x4 *>(Lp + 8 * q) = v;
+    for (int i = 0; i < CLS; i++) {
+      const int c = cg * CLS + i;
+      const int fx = 2 * px[t] + ((c >> 2) & 1), fy = 2 * py[t] + ((c >> 1) & 1), fz = 2 * pz[t] + (c & 1);
+      const int A = g.levsta[l] + morton3(fx, fy, fz, l);
+      float *Lp = (float *)g.L + ((size_t)rh[t] * g.ncell + A) * 32 + 4 * half;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        f32x4 v = {acc[t][i][4 * q], acc[t][i][4 * q + 1], acc[t][i][4 * q + 2], acc[t][i][4 * q + 3]};
+        *reinterpret_cast<f32x4 *>(Lp + 8 * q) = v;
+      }
     }
   }
 }
@@ -970,11 +1044,14 @@ static void step_table(std::vector<int> &steps, int cls, int *nsteps)
   }
 }
 
-// field classes per wave of the source-stationary variants (1 / 4 / 5), 0 otherwise
+// field classes per wave of the source-stationary variants (1 / 4, 7 / 5, 6),
+// 0 otherwise, and their 32-column tiles per wave
 static int ss_classes(int variant)
 {
-  return variant == 1 ? 8 : variant == 4 ? 4 : variant == 5 ? 2 : 0;
+  return variant == 1 ? 8 : (variant == 4 || variant == 7) ? 4 : (variant == 5 || variant == 6) ? 2 : 0;
 }
+
+static int ss_tiles(int variant) { return variant == 6 || variant == 7 ? 2 : 1; }
 
 int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const real *K, hipStream_t s,
                     int minlev, int variant, int variant_upper)
@@ -982,11 +1059,11 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   *out = nullptr;
   if (variant_upper != 0 && !(ss_classes(variant_upper) && ss_classes(variant)))
     return set_error(BBFMM_E_ARG, "dense-grid M2L: a variant for the upper levels (%d) needs source-stationary "
-                     "variants (1, 4, 5) for both", variant_upper);
+                     "variants (1, 4 to 7) for both", variant_upper);
   if (maxlev > kMaxLev) return set_error(BBFMM_E_ARG, "dense-grid M2L: maxlev %d > %d", maxlev, kMaxLev);
   if (minlev < kMinLev || minlev > maxlev)
     return set_error(BBFMM_E_ARG, "dense-grid M2L: first level %d outside [%d, %d]", minlev, kMinLev, maxlev);
-  if (variant < 0 || variant > 5) return set_error(BBFMM_E_ARG, "dense-grid M2L: variant %d", variant);
+  if (variant < 0 || variant > 7) return set_error(BBFMM_E_ARG, "dense-grid M2L: variant %d", variant);
   if ((variant == 1 || variant >= 4) && form != BBFMM_M2L_BF16X2)
     return set_error(BBFMM_E_ARG, "dense-grid M2L: the source-stationary variant is split-BF16 only");
   if (variant >= 2 && (form == BBFMM_M2L_PLAIN || r != 32))
@@ -1030,19 +1107,19 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   int tpw = 1, cpb;
   // variant 0 (automatic): source-stationary blocking once the deepest level
   // has enough columns (parents x vectors) to give every SIMD at least a wave;
-  // 1 / 4 / 5: source-stationary with 8 / 4 / 2 field classes per wave;
-  // 2 / 3: one / two 32-column tiles per wave
+  // 1 / 4 / 5: source-stationary with 8 / 4 / 2 field classes per wave and
+  // one 32-column tile, 6 / 7: 2 / 4 classes and two tiles per wave;
+  // 2 / 3: class-stationary, one / two 32-column tiles per wave
   const long long deep = (long long)(1 << (3 * (maxlev - 1))) * nrhs;
   a.ss = 0;
   if (form == BBFMM_M2L_BF16X2) {
-    if (variant == 1) a.ss = 8;
-    if (variant == 4 || (variant == 0 && deep >= 32 * 1024)) a.ss = 4;
-    if (variant == 5) a.ss = 2;
+    a.ss = variant == 0 && deep >= 32 * 1024 ? 4 : ss_classes(variant);
+    a.ss_tpw = ss_tiles(variant);
   }
   if (form == BBFMM_M2L_PLAIN) {
     cpb = r == 32 ? 64 : 256 / (r < 256 ? r : 256);  // k_m2l_grid_plain32: 64 columns per block
   } else if (a.ss) {
-    cpb = 4 * 32;  // 4 waves x 32 columns, one class group per block
+    cpb = 4 * 32 * a.ss_tpw;  // 4 waves x 32 or 64 columns, one class group per block
   } else {
     // two tiles per wave once there are enough columns to fill the chip
     tpw = variant == 3 ? 2 : variant == 2 ? 1 : (r == 32 && deep >= 8 * 1024 ? 2 : 1);  // r = 256: one tile
@@ -1063,20 +1140,23 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   std::vector<int> steps;
   step_table(steps, a.ss ? a.ss : 8, a.nsteps);
   // levels above the deepest with their own class-group size: two launches
-  const int ss_up = ss_classes(variant_upper);
+  const int ss_up = ss_classes(variant_upper), tpw_up = ss_tiles(variant_upper);
   size_t steps_up_off = 0;
-  if (a.ss && ss_up && ss_up != a.ss && maxlev > minlev) {
+  if (a.ss && ss_up && (ss_up != a.ss || tpw_up != a.ss_tpw) && maxlev > minlev) {
     g->nparts = 2;
     GridArgs &u = g->part[0], &d = g->part[1];
     u = a;
     d = a;
     u.maxlev = maxlev - 1;
     u.ss = ss_up;
+    u.ss_tpw = tpw_up;
+    u.wave_cols = 32 * tpw_up;
+    const int cpb_up = 4 * 32 * tpw_up;
     int nbu = 0;
     for (int l = minlev; l <= maxlev - 1; l++) {
       u.blk0[l] = nbu;
       const long long cols = (long long)(1 << (3 * (l - 1))) * nrhs;
-      nbu += (8 / ss_up) * (int)((cols + cpb - 1) / cpb);
+      nbu += (8 / ss_up) * (int)((cols + cpb_up - 1) / cpb_up);
     }
     u.blk0[maxlev] = nbu;
     d.minlev = maxlev;
@@ -1242,7 +1322,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   a.lvl_max = g->d_lvl_max;
   if (g->nparts == 2) {
     // everything from the final args (operator and grid pointers and sizes),
-    // then each part's own levels, blocks, class-group size and steps
+    // then each part's own levels, blocks, class-group size, tiles and steps
     for (int pi = 0; pi < 2; pi++) {
       const GridArgs own = g->part[pi];
       GridArgs &q = g->part[pi];
@@ -1251,6 +1331,8 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
       q.maxlev = own.maxlev;
       for (int l = 0; l < kMaxLev + 2; l++) q.blk0[l] = own.blk0[l];
       q.ss = own.ss;
+      q.ss_tpw = own.ss_tpw;
+      q.wave_cols = own.wave_cols;
       for (int c = 0; c < 8; c++) q.nsteps[c] = own.nsteps[c];
       q.steps = g->d_steps + (pi == 0 ? steps_up_off : 0);
     }
@@ -1283,10 +1365,17 @@ template <int CLS>
 static void launch_ss(const GridM2L *g, const GridArgs &a, int nblocks, hipStream_t s)
 {
   const dim3 grid((nblocks + 7) & ~7);
-  if (a.gbytes > 0 && !g->flat)
-    hipLaunchKernelGGL((k_m2l_mfma_ss<CLS, true>), grid, dim3(256), 0, s, a);
+  const bool buf = a.gbytes > 0 && !g->flat;
+  if (CLS <= 4 && a.ss_tpw == 2) {  // two tiles per wave: 2 or 4 classes
+    constexpr int C = CLS <= 4 ? CLS : 4;
+    if (buf)
+      hipLaunchKernelGGL((k_m2l_mfma_ss<C, 2, true>), grid, dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_m2l_mfma_ss<C, 2, false>), grid, dim3(256), 0, s, a);
+  } else if (buf)
+    hipLaunchKernelGGL((k_m2l_mfma_ss<CLS, 1, true>), grid, dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((k_m2l_mfma_ss<CLS, false>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_m2l_mfma_ss<CLS, 1, false>), grid, dim3(256), 0, s, a);
 }
 #endif
 
@@ -1645,8 +1734,11 @@ void slot_operators(std::vector<int> &op)
 
 int class_slot(int s, int vx, int vy, int vz)
 {
-  static std::vector<int> slot;
-  if (slot.empty()) slot_table(slot);
+  static const std::vector<int> slot = [] {  // built once, thread-safe
+    std::vector<int> t;
+    slot_table(t);
+    return t;
+  }();
   if (vx < -3 || vx > 3 || vy < -3 || vy > 3 || vz < -3 || vz > 3) return -1;
   return slot[s * 343 + 49 * (vx + 3) + 7 * (vy + 3) + (vz + 3)];
 }

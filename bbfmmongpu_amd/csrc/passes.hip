@@ -445,6 +445,13 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int l
     wb[i * LP + v] = a.L[((size_t)v * a.ncell + ch) * R + i];
   }
   __syncthreads();
+  // U fragments of the post-M2L product (A = U rows: lane (i, k) holds
+  // U[node 16 t + i][coefficient 4 kk + k]), in flight during the x and y passes
+  float uc[4][R / 4];
+#pragma unroll
+  for (int kk = 0; kk < R / 4; kk++)
+#pragma unroll
+    for (int t = 0; t < 4; t++) uc[t][kk] = a.Ucm[(16 * t + l16) + (size_t)N3 * (4 * kk + kq)];
   for (int o = tid; o < 16 * 2 * N3; o += 512) {  // x
     const int v = o / (2 * N3), r = o % (2 * N3);
     const int xb = r / N3, rem = r % N3, cx = rem / N2, yz = rem % N2;
@@ -473,9 +480,7 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int l
   for (int kk = 0; kk < R / 4; kk++) {
     const float b = wb[(4 * kk + kq) * LP + l16];
 #pragma unroll
-    for (int t = 0; t < 4; t++)
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.Ucm[(16 * t + l16) + (size_t)N3 * (4 * kk + kq)], b, acc[t], 0, 0,
-                                                    0);
+    for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(uc[t][kk], b, acc[t], 0, 0, 0);
   }
   // + z pass of the L2L for child w: node = 16t + 4kq + e = (l1 = 4t + kq, cz = e)
   const float sc = a.scale[a.maxlev];
@@ -509,6 +514,13 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int l
     cheb_row<4>(a.Tkz, ih * (x.y - c.y), sy);
     cheb_row<4>(a.Tkz, ih * (x.z - c.z), sz);
     const float zq = pick4(sz, kq);
+    // vector 0's charges of the far-field output, loaded before the products
+    float qf[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int i = tb + 4 * kq + e;
+      qf[e] = l16 == 0 && i < cnt && !a.potential ? a.qt[(size_t)(psta + i) * 16] : 1.0f;
+    }
     f32x4 ph = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) {
@@ -522,7 +534,7 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int l
         const size_t o = (size_t)(psta + i) * 16 + l16;
         const float v = a.prefac3 * ph[e];
         a.phit[o] = v;
-        if (l16 == 0) a.farfield[sta + i] = (a.potential ? 1.0f : a.qt[o]) * v;
+        if (l16 == 0) a.farfield[sta + i] = qf[e] * v;
       }
     }
   }
@@ -534,7 +546,7 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int l
 // writes the leaf's M; the leaves' S stay in LDS, where the block forms P's S
 // by the M2M interpolation (z, y, x passes over the 16 vectors) and P's M.
 // The leaves' S never go to memory.
-__global__ void __launch_bounds__(512) k_up_leaf_v16(PassArgs a, int p0, int leaf0)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) k_up_leaf_v16(PassArgs a, int p0, int leaf0)
 {
   constexpr int NN = 4, N3 = 64, N2 = 16, R = 32, LP = 17, CP = 13, CH = 32;
   constexpr int WB = N3 * LP;  // per wave: staging (CH * (CP + LP) floats), then the leaf's S
@@ -560,8 +572,15 @@ __global__ void __launch_bounds__(512) k_up_leaf_v16(PassArgs a, int p0, int lea
     const real4 c = a.center[ch];
     for (int base = 0; base < cnt; base += CH) {
       const int nk = min(CH, cnt - base);
+      // the chunk's positions and charge rows: every load issued before any use
+      const real4 x = a.pos[sta + base + (lane < nk ? lane : 0)];
+      const float *qr = a.qt + (size_t)(psta + base) * 16;
+#pragma unroll
+      for (int j = 0; j < CH * 16 / 64; j++) {
+        const int o = lane + 64 * j;
+        if (o < nk * 16) sq[(o >> 4) * LP + (o & 15)] = qr[o];
+      }
       if (lane < nk) {
-        const real4 x = a.pos[sta + base + lane];
         float t[4];
         cheb_row<4>(a.Tkz, ih * (x.x - c.x), t);
 #pragma unroll
@@ -573,8 +592,6 @@ __global__ void __launch_bounds__(512) k_up_leaf_v16(PassArgs a, int p0, int lea
 #pragma unroll
         for (int m = 0; m < 4; m++) sc[lane * CP + 8 + m] = t[m];
       }
-      const float *qr = a.qt + (size_t)(psta + base) * 16;
-      for (int o = lane; o < nk * 16; o += 64) sq[(o >> 4) * LP + (o & 15)] = qr[o];
       wave_sync();
       for (int k0 = 0; k0 < nk; k0 += 4) {
         const int p = k0 + kq;
@@ -597,16 +614,22 @@ __global__ void __launch_bounds__(512) k_up_leaf_v16(PassArgs a, int p0, int lea
   for (int t = 0; t < 4; t++)
 #pragma unroll
     for (int e = 0; e < 4; e++) wb[(16 * t + 4 * kq + e) * LP + l16] = a.prefac3 * acc[t][e];
+  // U fragments of the M products (A = U^T rows: lane (i, k) holds
+  // U[node 4 kk + k][coefficient 16 t + i]), all loads issued at once
+  float uf[2][16];
+#pragma unroll
+  for (int kk = 0; kk < 16; kk++)
+#pragma unroll
+    for (int t = 0; t < 2; t++) uf[t][kk] = a.Urm[(size_t)(4 * kk + kq) * R + 16 * t + l16];
   wave_sync();
   {  // the leaf's M = U^T (Kw .* S)
     f32x4 m[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
-#pragma unroll 4
+#pragma unroll
     for (int kk = 0; kk < 16; kk++) {
       const int node = 4 * kk + kq;
       const float b = wb[node * LP + l16] * a.Kw[node];
 #pragma unroll
-      for (int t = 0; t < 2; t++)
-        m[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.Urm[(size_t)node * R + 16 * t + l16], b, m[t], 0, 0, 0);
+      for (int t = 0; t < 2; t++) m[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(uf[t][kk], b, m[t], 0, 0, 0);
     }
 #pragma unroll
     for (int t = 0; t < 2; t++)
@@ -638,6 +661,13 @@ __global__ void __launch_bounds__(512) k_up_leaf_v16(PassArgs a, int p0, int lea
     syy[v * 2 * N3 + r] = t;
   }
   __syncthreads();
+  // U fragments of P's M product (waves 0 and 1, one 16-coefficient tile
+  // each), loaded while the x pass runs
+  float up[16];
+  if (w < 2) {
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) up[kk] = a.Urm[(size_t)(4 * kk + kq) * R + 16 * w + l16];
+  }
   for (int o = tid; o < 16 * N3; o += 512) {  // x
     const int v = o / N3, l = o % N3;
     const int px = l / N2, rem = l % N2;
@@ -651,12 +681,11 @@ __global__ void __launch_bounds__(512) k_up_leaf_v16(PassArgs a, int p0, int lea
     sw[v * N3 + l] = t * a.Kw[l];
   }
   __syncthreads();
-  {  // P's M: a thread per (vector, coefficient)
-    const int v = tid / R, i = tid % R;
-    float m = 0;
-#pragma unroll 8
-    for (int l = 0; l < N3; l++) m += a.Urm[(size_t)l * R + i] * sw[v * N3 + l];
-    a.M[((size_t)v * a.ncell + P) * R + i] = m;
+  if (w < 2) {  // P's M = U^T sw on MFMA, coefficients 16 w .. 16 w + 15
+    f32x4 m = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) m = __builtin_amdgcn_mfma_f32_16x16x4f32(up[kk], sw[l16 * N3 + 4 * kk + kq], m, 0, 0, 0);
+    *reinterpret_cast<f32x4 *>(a.M + ((size_t)l16 * a.ncell + P) * R + 16 * w + 4 * kq) = m;
   }
 }
 

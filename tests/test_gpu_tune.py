@@ -15,13 +15,15 @@ CONFIGS = [
     (0, 3, 0, 2, 1e-5), (0, 3, 0, 3, 1e-5), (0, 2, 0, 3, 1e-5),
     (1, 3, 1, 2, 3e-5), (1, 3, 2, 2, 3e-5), (1, 3, 3, 2, 3e-5), (1, 3, 1, 3, 3e-5), (1, 3, 3, 4, 3e-5),
     (1, 3, 4, 2, 3e-5), (1, 3, 5, 2, 3e-5), (1, 3, 4, 3, 3e-5), (1, 3, 5, 4, 3e-5),
+    (1, 3, 6, 2, 3e-5), (1, 3, 7, 2, 3e-5), (1, 3, 6, 3, 3e-5), (1, 3, 7, 4, 3e-5),
     (2, 3, 2, 2, 1e-5), (2, 3, 3, 3, 1e-5),
     (3, 3, 2, 2, 1e-6), (3, 3, 3, 3, 1e-6), (3, 3, 0, 2, 1e-6),
 ]
 
 
 # source-stationary class groups with a vector count that leaves partial column tiles
-RAGGED = [(1, 3, 1, 2, 3e-5, 3), (1, 3, 4, 2, 3e-5, 3), (1, 3, 5, 2, 3e-5, 5)]
+RAGGED = [(1, 3, 1, 2, 3e-5, 3), (1, 3, 4, 2, 3e-5, 3), (1, 3, 5, 2, 3e-5, 5), (1, 3, 6, 2, 3e-5, 3),
+          (1, 3, 7, 2, 3e-5, 5)]
 
 
 @pytest.mark.parametrize("form,scheme,variant,switch,tol,nrhs",
@@ -52,7 +54,8 @@ def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, nr
     plan.close()
 
 
-@pytest.mark.parametrize("variant,upper,nrhs", [(1, 5, 16), (1, 4, 16), (4, 5, 16), (5, 1, 16), (1, 5, 3)])
+@pytest.mark.parametrize("variant,upper,nrhs", [(1, 5, 16), (1, 4, 16), (4, 5, 16), (5, 1, 16), (1, 5, 3), (6, 7, 16),
+                                                 (1, 7, 16)])
 def test_m2l_upper_levels_own_variant(variant, upper, nrhs, ops4, cuda):
     """Split-BF16 source-stationary M2L with the levels above the deepest in a
     launch of their own (BBFMM_OPT_M2L_VARIANT_UPPER): the same L as the
@@ -153,3 +156,29 @@ def test_tuned_batch_plan(ops4, cuda):
     phi1 = plan.solve_host(q)
     assert O.rel_l2(phi0, phi1) < 1e-6
     plan.close()
+
+
+@pytest.mark.parametrize("two,one,nrhs", [(6, 5, 16), (7, 4, 16), (6, 5, 3), (7, 4, 5)])
+def test_m2l_two_tiles_per_wave_same_bits(two, one, nrhs, ops4, cuda):
+    """Variants 6 and 7 give every wave of the source-stationary kernel two
+    32-column tiles (2 and 4 field classes); each tile's products and their
+    order are those of variants 5 and 4 (one tile per wave), so L is the same
+    to the bit, also with vector counts that leave partial tiles."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, _q = O.uniform_inputs(65536, seed=3)
+    rng = np.random.default_rng(17)
+    out = {}
+    for v in (two, one):
+        plan = B.Plan(pos, 4, U, K, prec="s", nrhs=nrhs).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2, variant=v)
+        if v == two:
+            M = rng.standard_normal((nrhs, plan.ncell * 32)).astype(np.float32)
+            Md = torch.from_numpy(M).to(cuda)
+        Ld = torch.zeros_like(Md)
+        plan.m2l(Md, Ld)
+        torch.cuda.synchronize()
+        out[v] = Ld.cpu().numpy()
+        plan.close()
+    assert np.abs(out[one]).max() > 0
+    assert np.array_equal(out[two], out[one])
