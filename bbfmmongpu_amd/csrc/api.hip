@@ -431,20 +431,22 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
       x.w = 0;
     }
     for (int i = 0; i < N; i++) ppos[pad_of[i]] = hpos[i];
-    std::vector<int2> seg((size_t)nleaf * 27, make_int2(0, 0));
-    std::vector<int> nseg((size_t)nleaf, 0);
-    for (int k = 0; k < nleaf; k++) {
-      int m = 0;
-      if (pcnt[k] > 0) seg[(size_t)k * 27 + m++] = make_int2(psta[k], pcnt[k]);
-      for (int b = 0; b < nnbr[k]; b++) {
-        const int B = nbr[(size_t)k * 27 + b];
-        if (B != k && pcnt[B] > 0) seg[(size_t)k * 27 + m++] = make_int2(psta[B], pcnt[B]);
-      }
-      nseg[k] = m;
-    }
     BB_TRY(upload(&p->d_ppos, ppos.data(), ppos.size(), stream));
-    BB_TRY(upload(&p->d_seg, seg.data(), seg.size(), stream));
-    BB_TRY(upload(&p->d_nseg, nseg.data(), nseg.size(), stream));
+    if (nrhs >= 16) {  // segment tables of the 16-vector P2P kernels
+      std::vector<int2> seg((size_t)nleaf * 27, make_int2(0, 0));
+      std::vector<int> nseg((size_t)nleaf, 0);
+      for (int k = 0; k < nleaf; k++) {
+        int m = 0;
+        if (pcnt[k] > 0) seg[(size_t)k * 27 + m++] = make_int2(psta[k], pcnt[k]);
+        for (int b = 0; b < nnbr[k]; b++) {
+          const int B = nbr[(size_t)k * 27 + b];
+          if (B != k && pcnt[B] > 0) seg[(size_t)k * 27 + m++] = make_int2(psta[B], pcnt[B]);
+        }
+        nseg[k] = m;
+      }
+      BB_TRY(upload(&p->d_seg, seg.data(), seg.size(), stream));
+      BB_TRY(upload(&p->d_nseg, nseg.data(), nseg.size(), stream));
+    }
     BB_TRY(upload(&p->d_pad_of, pad_of.data(), pad_of.size(), stream));
     std::vector<int> ipad((size_t)N);
     for (int i = 0; i < N; i++) ipad[p->h_perm[i]] = pad_of[i];
@@ -509,15 +511,22 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
         for (int cn = 0; cn < n3; cn++)
           T8m[((size_t)c8 * n3 + cn) * n3 + pn] = T8[((size_t)c8 * n3 + pn) * n3 + cn];
     BB_TRY(upload(&p->d_T8m, T8m.data(), T8m.size(), stream));
-    mplist = f.plist;
-    auto cmask = [&](int P) {
-      int m = 0;
-      for (int c = 0; c < 8; c++) m |= (f.child[(size_t)8 * P + c] >= 0) << c;
-      return m;
-    };
-    for (size_t l = 0; l + 1 < f.ploff.size(); l++)
-      std::stable_sort(mplist.begin() + f.ploff[l], mplist.begin() + f.ploff[l + 1],
-                       [&](int x, int y) { return cmask(x) < cmask(y); });
+    // (a stable counting sort by the 8-bit child mask, per level)
+    mplist.resize(f.plist.size());
+    for (size_t l = 0; l + 1 < f.ploff.size(); l++) {
+      const int b = f.ploff[l], e = f.ploff[l + 1];
+      std::vector<unsigned char> mk((size_t)(e - b));
+      int cnt[257] = {0};
+      for (int i = b; i < e; i++) {
+        const int P = f.plist[i];
+        int m = 0;
+        for (int c = 0; c < 8; c++) m |= (f.child[(size_t)8 * P + c] >= 0) << c;
+        mk[i - b] = (unsigned char)m;
+        cnt[m + 1]++;
+      }
+      for (int m = 0; m < 256; m++) cnt[m + 1] += cnt[m];
+      for (int i = b; i < e; i++) mplist[b + cnt[mk[i - b]]++] = f.plist[i];
+    }
     if (!mplist.empty()) BB_TRY(upload(&p->d_mplist, mplist.data(), mplist.size(), stream));
   }
 #endif
