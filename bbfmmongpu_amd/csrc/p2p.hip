@@ -230,7 +230,50 @@ __device__ __forceinline__ void p2p16_step(const float4 &z, float bq, const floa
     g[t] = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
   }
 #pragma unroll
-  for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(g[t], bq, acc[t], 0, 0, 0);
+  for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bq, g[t], acc[t], 0, 0, 0);
+}
+
+// results of NT target tiles of the transposed product (lane l holds the
+// vectors 4 (l / 16) .. 4 (l / 16) + 3 of target tb + 16 t + l % 16) added
+// into phi (q * potential, or the potential), the particle rows (one
+// 16-byte piece of a row per lane), or the near rows
+template <int NT>
+__device__ __forceinline__ void p2p16_store_t(const P2PArgs &a, int rhs0, int sta, int cnt, int psta, int tb,
+                                              const f32x4 *acc)
+{
+  const int l = threadIdx.x;
+  const int ti = l & 15, kq = l >> 4;
+  if (a.phin) {
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int i = tb + 16 * t + ti;
+      if (i < cnt) *reinterpret_cast<f32x4 *>(a.phin + (size_t)(psta + i) * 16 + 4 * kq) = acc[t];
+    }
+    return;
+  }
+  if (a.phit) {
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int i = tb + 16 * t + ti;
+      if (i < cnt) {
+        const size_t o = (size_t)(psta + i) * 16 + 4 * kq;
+        const f32x4 far = *reinterpret_cast<const f32x4 *>(a.phit + o);
+        const f32x4 q = a.potential ? f32x4{1.f, 1.f, 1.f, 1.f} : *reinterpret_cast<const f32x4 *>(a.qt + o);
+        *reinterpret_cast<f32x4 *>(a.phit + o) = q * (far + acc[t]);
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int i = tb + 16 * t + ti;
+    if (i < cnt)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const size_t o = (size_t)(rhs0 + 4 * kq + e) * a.N + sta + i;
+        a.phi[o] += (a.potential ? 1.0f : a.q[o]) * acc[t][e];
+      }
+  }
 }
 
 // results of NT target tiles (accumulator layout of the 16x16 MFMA: lane l
@@ -318,7 +361,7 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
       g[t] = rr != 0.f ? __builtin_amdgcn_rsqf(rr) : 0.f;
     }
 #pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(g[t], bq, acc[t], 0, 0, 0);
+    for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(bq, g[t], acc[t], 0, 0, 0);
   };
   if (pself > 0) {
     float4 zA = src_pos(psta);
@@ -375,17 +418,19 @@ __device__ __forceinline__ void p2p16_pass(const P2PArgs &a, int rhs0, int2 sg, 
       p2p16_step<NT>(y3, q3, x, acc);
     }
   }
-  p2p16_store<NT>(a, rhs0, sta, cnt, psta, tb, acc);
+  p2p16_store_t<NT>(a, rhs0, sta, cnt, psta, tb, acc);
 }
 
 // Sixteen vectors on MFMA; qt is [padded N][16].  Lane l computes
-// G[target l%16 of each 16-target tile][source 4j + l/16] and feeds the
-// charges of that source for vector l%16 as the B operand.  The self leaf is
+// G[target l%16 of each 16-target tile][source 4j + l/16] and feeds it as the
+// B operand, the charges of that source for vector l%16 as the A operand
+// (the transposed product: a lane ends with four consecutive vectors of one
+// target, stored as one 16-byte piece of its row).  The self leaf is
 // summed first with the r != 0 exclusion; the other neighbours form one
 // flattened sequence of 4-source groups, loaded three groups ahead into a
 // ring of four register sets.  Up to four target tiles share a pass.
 template <bool BUF>
-__global__ void __launch_bounds__(64) k_p2p_seg16(P2PArgs a, int rhs0)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) k_p2p_seg16(P2PArgs a, int rhs0)
 {
   const int leaf = xcd_leaf(blockIdx.x, a.nleaf);
   const int l = threadIdx.x;

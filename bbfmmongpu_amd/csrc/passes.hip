@@ -352,9 +352,10 @@ __global__ void __launch_bounds__(256) k_p2m_rows_mfma4(PassArgs a)
 // L2P of the particle-row form, n = 4, on MFMA: one wave per leaf, four
 // leaves per block.  The leaf's F (64 nodes x 16 vectors) is staged in LDS;
 // per tile of 16 particles and k-step of 4 nodes, lane l supplies
-// A = W[particle l%16][node 4kk + l/16] and B = F[node 4kk + l/16][vector
-// l%16].  The far-field potentials go to the particle rows (no q factor)
-// and vector 0's far field, with the output's factor, to farfield.
+// A = F^T[vector l%16][node 4kk + l/16] and B = W^T[node 4kk + l/16][particle
+// l%16], so that a lane holds four consecutive vectors of one particle.  The
+// far-field potentials go to the particle rows (no q factor) and vector 0's
+// far field, with the output's factor, to farfield.
 __global__ void __launch_bounds__(256) k_l2p_rows_mfma4(PassArgs a)
 {
   constexpr int N3 = 64, LP = 17;
@@ -393,18 +394,15 @@ __global__ void __launch_bounds__(256) k_l2p_rows_mfma4(PassArgs a)
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) {
       const float w = sx[kk >> 2] * sy[kk & 3] * zq;
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(w, sf[(4 * kk + kq) * LP + l16], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sf[(4 * kk + kq) * LP + l16], w, acc, 0, 0, 0);
     }
-    // acc[v] = phi[particle tb + 4 kq + v][vector l16]
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int i = tb + 4 * kq + e;
-      if (i < cnt) {
-        const size_t o = (size_t)(psta + i) * 16 + l16;
-        const float ph = a.prefac3 * acc[e];
-        a.phit[o] = ph;
-        if (l16 == 0) a.farfield[sta + i] = (a.potential ? 1.0f : a.qt[o]) * ph;
-      }
+    // acc[e] = phi[particle tb + l16][vector 4 kq + e]: one 16-byte piece of
+    // the particle's row per lane
+    if (ok) {
+      const size_t o = (size_t)(psta + p) * 16 + 4 * kq;
+      const f32x4 ph = a.prefac3 * acc;
+      *reinterpret_cast<f32x4 *>(a.phit + o) = ph;
+      if (kq == 0) a.farfield[sta + p] = (a.potential ? 1.0f : a.qt[o]) * ph[0];
     }
   }
 }
@@ -416,7 +414,7 @@ __global__ void __launch_bounds__(256) k_l2p_rows_mfma4(PassArgs a)
 // shared by the block, the z pass per wave in the accumulator layout); then
 // the L2P of the leaf's particles as in k_l2p_rows_mfma4.  The leaves' F
 // never goes to memory.
-__global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int leaf0)
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) k_down_leaf_v16(PassArgs a, int p0, int leaf0)
 {
   constexpr int NN = 4, N3 = 64, N2 = 16, R = 32, LP = 17;
   // per wave: the leaf's L in rows 0..31 ([coefficient][vector]), later its F
@@ -514,28 +512,21 @@ __global__ void __launch_bounds__(512) k_down_leaf_v16(PassArgs a, int p0, int l
     cheb_row<4>(a.Tkz, ih * (x.y - c.y), sy);
     cheb_row<4>(a.Tkz, ih * (x.z - c.z), sz);
     const float zq = pick4(sz, kq);
-    // vector 0's charges of the far-field output, loaded before the products
-    float qf[4];
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int i = tb + 4 * kq + e;
-      qf[e] = l16 == 0 && i < cnt && !a.potential ? a.qt[(size_t)(psta + i) * 16] : 1.0f;
-    }
+    // vector 0's charge of the far-field output, loaded before the products
+    const size_t o = (size_t)(psta + p) * 16 + 4 * kq;
+    const float qf = ok && kq == 0 && !a.potential ? a.qt[o] : 1.0f;
+    // transposed product (A = F^T, B = W^T): ph[e] = phi[particle tb + l16]
+    // [vector 4 kq + e], one 16-byte piece of the particle's row per lane
     f32x4 ph = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kk = 0; kk < 16; kk++) {
       const float wt = sx[kk >> 2] * sy[kk & 3] * zq;
-      ph = __builtin_amdgcn_mfma_f32_16x16x4f32(wt, wb[(4 * kk + kq) * LP + l16], ph, 0, 0, 0);
+      ph = __builtin_amdgcn_mfma_f32_16x16x4f32(wb[(4 * kk + kq) * LP + l16], wt, ph, 0, 0, 0);
     }
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int i = tb + 4 * kq + e;
-      if (i < cnt) {
-        const size_t o = (size_t)(psta + i) * 16 + l16;
-        const float v = a.prefac3 * ph[e];
-        a.phit[o] = v;
-        if (l16 == 0) a.farfield[sta + i] = qf[e] * v;
-      }
+    if (ok) {
+      const f32x4 v = a.prefac3 * ph;
+      *reinterpret_cast<f32x4 *>(a.phit + o) = v;
+      if (kq == 0) a.farfield[sta + p] = qf * v[0];
     }
   }
 }

@@ -41,28 +41,46 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c2.txt")
 CSRC = os.path.join(ROOT, "bbfmmongpu_amd", "csrc")
 # source file of each kernel whose HBM traffic the bench reports
-KERNEL_SOURCE = {"k_m2l_mfma_ss<4,true>": "m2l_grid.hip", "k_p2p_seg16<true>": "p2p.hip",
+KERNEL_SOURCE = {"k_m2l_mfma_ss<4,1,true>": "m2l_grid.hip", "k_p2p_seg16<true>": "p2p.hip",
                  "k_p2p_seg16x<3>": "p2p.hip"}
 
 
-def ss_group_steps(cls: int) -> list[int]:
-    """Steps (source class s', parent shift d != 0) of each group of cls field
-    classes in the source-stationary M2L: those in which some class of the
-    group has a far offset v = 2d + s' - s (|v|_inf >= 2)."""
-    out = []
-    for g in range(8 // cls):
-        n = 0
-        for sc in range(8):
-            for d in itertools.product((-1, 0, 1), repeat=3):
-                if d == (0, 0, 0):
-                    continue
-                far = False
-                for c in range(g * cls, (g + 1) * cls):
-                    v = [2 * d[a] + ((sc >> (2 - a)) & 1) - ((c >> (2 - a)) & 1) for a in range(3)]
-                    far |= max(abs(x) for x in v) >= 2
-                n += far
-        out.append(n)
-    return out
+def ss_issued_wave_steps(cls: int, maxlev: int, nrhs: int) -> int:
+    """(wave, step) pairs the source-stationary M2L runs its products on: a
+    wave (32 columns = (parent, vector) pairs, vector fastest, then pz, py,
+    px) skips the steps whose parent shift d moves its whole parent box
+    outside the level's [0, P)^3 (ghost sources only); the box is exact in
+    px and conservative (the whole axis) in py and pz when the columns span
+    more than one row."""
+    total = 0
+    for lev in range(2, maxlev + 1):
+        P = 1 << (lev - 1)
+        ncols = P ** 3 * nrhs
+        nwave = -(-ncols // 128) * 4
+        col0 = np.arange(nwave) * 32
+        ok = col0 < ncols
+        f = np.minimum(col0, ncols - 1) // nrhs
+        t = np.minimum(col0 + 31, ncols - 1) // nrhs
+        fx, fy, fz = f // (P * P), (f // P) % P, f % P
+        tx, ty, tz = t // (P * P), (t // P) % P, t % P
+        row = fx == tx
+        line = row & (fy == ty)
+        lo = np.stack([fx, np.where(row, fy, 0), np.where(line, fz, 0)], 1)
+        hi = np.stack([tx, np.where(row, ty, P - 1), np.where(line, tz, P - 1)], 1)
+        for g in range(8 // cls):
+            ds = []
+            for sc in range(8):
+                for d in itertools.product((-1, 0, 1), repeat=3):
+                    if d == (0, 0, 0):
+                        continue
+                    far = any(max(abs(2 * d[a] + ((sc >> (2 - a)) & 1) - ((c >> (2 - a)) & 1)) for a in range(3)) >= 2
+                              for c in range(g * cls, (g + 1) * cls))
+                    if far:
+                        ds.append(d)
+            D = np.array(ds)[None, :, :]
+            ghost = ((hi[:, None, :] + D < 0) | (lo[:, None, :] + D >= P)).any(axis=2)
+            total += int((~ghost[ok]).sum())
+    return total
 
 
 def source_sha(kernel: str):
@@ -97,7 +115,7 @@ def measured_traffic(kernel: str):
 
 def short_kernel_name(name: str) -> str:
     """rocprofv3 kernel name -> the short form the summaries use, e.g.
-    'void bb::k_m2l_mfma_ss<4, true>(bb::GridArgs)' -> 'k_m2l_mfma_ss<4,true>'."""
+    'void bb::k_m2l_mfma_ss<4, 1, true>(bb::GridArgs)' -> 'k_m2l_mfma_ss<4,1,true>'."""
     n = name.strip()
     for pre in ("void ", "bb::"):
         if n.startswith(pre):
@@ -683,11 +701,10 @@ def main():
     p2p_flop = pairs * (18 + 2 * nrhs)  # geometry (20-flop convention) shared by the vectors
     p2p_valu_flop = pairs * 18          # the 1/r part alone (the sum over sources runs on MFMA)
     # issued MFMA flops of the source-stationary kernel (4 field classes per
-    # wave): per 32-column tile and class group, the group's steps x 4 classes
-    # x 6 products of 32x32x16 (zero operator included)
-    groups = sum(((1 << (3 * (lev - 1))) * nrhs + 127) // 128 for lev in range(2, chosen.maxlev + 1))
-    issued = sum(ss_group_steps(4)) * 4 * 6 * (2 * 32 * 32 * 16) * groups * 4
-    m2l_kernel = "k_m2l_mfma_ss<4,true>"
+    # wave): per wave and step it runs, 4 classes x 6 products of 32x32x16
+    # (zero operator included, ghost steps skipped)
+    issued = ss_issued_wave_steps(4, chosen.maxlev, nrhs) * 4 * 6 * (2 * 32 * 32 * 16)
+    m2l_kernel = "k_m2l_mfma_ss<4,1,true>"
     tr_m2l, tr_m2l_src = measured_traffic(m2l_kernel)
     roof_m2l = {"kernel": m2l_kernel, "bound": "mfma", "unit": "TFLOP/s",
                 "achieved": flop_m2l * nrhs / (st_chosen["m2l"] * 1e-3) / 1e12, "peak": PEAK_BF16_TFLOPS,
