@@ -85,9 +85,10 @@ __global__ void k_gather_rows16(int N, const int *__restrict__ ipad, const real 
   real v[16];
 #pragma unroll
   for (int k = 0; k < 16; k++) v[k] = q[(size_t)k * N + j];
-  real *row = qt + (size_t)ipad[j] * 16;
+  // the row as four 16-byte stores (rows are 64-byte aligned)
+  real4 *row = reinterpret_cast<real4 *>(qt + (size_t)ipad[j] * 16);
 #pragma unroll
-  for (int k = 0; k < 16; k++) row[k] = v[k];
+  for (int k = 0; k < 4; k++) row[k] = real4{v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]};
 }
 
 // phin (not null): the near field arrived in rows of its own (P2P ran
@@ -102,7 +103,13 @@ __global__ void k_scatter_rows16(int N, const int *__restrict__ ipad, const real
   const size_t r0 = (size_t)ipad[j] * 16;
   real v[16];
 #pragma unroll
-  for (int k = 0; k < 16; k++) v[k] = phit[r0 + k];
+  for (int k = 0; k < 4; k++) {  // four 16-byte loads (rows are 64-byte aligned)
+    const real4 t = reinterpret_cast<const real4 *>(phit + r0)[k];
+    v[4 * k] = t.x;
+    v[4 * k + 1] = t.y;
+    v[4 * k + 2] = t.z;
+    v[4 * k + 3] = t.w;
+  }
   if (phin) {
 #pragma unroll
     for (int k = 0; k < 16; k++) {
