@@ -4,7 +4,8 @@
 // anotherNearField.c:137-206; the r != 0 test is applied in the self leaf).
 //
 // One 64-lane wavefront per leaf; the neighbour particles are staged in LDS
-// 64 at a time and every lane (target) sweeps them.  NR right-hand sides are
+// 64 at a time and every lane (target) sweeps them (or a 1/2, 1/4 share of
+// them on leaves of at most 32, 16 particles).  NR right-hand sides are
 // processed together so that the geometry (dx, dy, dz, 1/r) is computed once
 // per pair and shared by NR charge vectors.
 // 1/r: fp32 uses the hardware v_rsq_f32 (AMD's own reciprocal square root,
@@ -23,6 +24,10 @@ __device__ __forceinline__ real rsqrt_r(real x)
 #endif
 }
 
+// A leaf of at most 16 (32) targets takes 16 (32) lanes per target and
+// splits every source chunk 4 (2) ways over the lane groups (source k to
+// group k mod S), the groups' sums added across lanes at the end, so that
+// the wavefront's 64 lanes stay busy on leaves of a few dozen particles.
 template <int NR>
 __global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
 {
@@ -42,8 +47,11 @@ __global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
   }
   __syncthreads();
   const size_t N = a.N;
-  for (int tb = 0; tb < cnt; tb += CH) {
-    const int i = tb + t;
+  const int S = cnt <= 16 ? 4 : cnt <= 32 ? 2 : 1;  // source groups (uniform in the block)
+  const int T = CH / S;                              // targets per pass
+  const int tl = t % T, h = t / T;
+  for (int tb = 0; tb < cnt; tb += T) {
+    const int i = tb + tl;
     const bool act = i < cnt;
     real4 xf;
     if (act) {
@@ -66,9 +74,9 @@ __global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
         }
         __syncthreads();
         if (!self) {
-          int k = 0;
-          for (; k + 1 < nk; k += 2) {
-            const real4 s0 = sp[k], s1 = sp[k + 1];
+          int k = h;
+          for (; k + S < nk; k += 2 * S) {
+            const real4 s0 = sp[k], s1 = sp[k + S];
             const real dx0 = s0.x - xf.x, dy0 = s0.y - xf.y, dz0 = s0.z - xf.z;
             const real dx1 = s1.x - xf.x, dy1 = s1.y - xf.y, dz1 = s1.z - xf.z;
             const real w0 = rsqrt_r(dx0 * dx0 + dy0 * dy0 + dz0 * dz0);
@@ -76,7 +84,7 @@ __global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
 #pragma unroll
             for (int r = 0; r < NR; r++) {
               acc[r] += w0 * sq[r][k];
-              acc2[r] += w1 * sq[r][k + 1];
+              acc2[r] += w1 * sq[r][k + S];
             }
           }
           if (k < nk) {
@@ -87,7 +95,7 @@ __global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
             for (int r = 0; r < NR; r++) acc[r] += w0 * sq[r][k];
           }
         } else {
-          for (int k = 0; k < nk; k++) {
+          for (int k = h; k < nk; k += S) {
             const real4 s0 = sp[k];
             const real dx0 = s0.x - xf.x, dy0 = s0.y - xf.y, dz0 = s0.z - xf.z;
             const real rr = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
@@ -99,11 +107,18 @@ __global__ void __launch_bounds__(64) k_p2p(P2PArgs a, int rhs0)
         __syncthreads();
       }
     }
-    if (act) {
+    real tot[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      tot[r] = acc[r] + acc2[r];
+      if (S == 4) tot[r] += __shfl_xor(tot[r], 16);
+      if (S >= 2) tot[r] += __shfl_xor(tot[r], 32);
+    }
+    if (act && h == 0) {
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         const size_t o = (size_t)(rhs0 + r) * N + sta + i;
-        a.phi[o] += (a.potential ? (real)1 : a.q[o]) * (acc[r] + acc2[r]);
+        a.phi[o] += (a.potential ? (real)1 : a.q[o]) * tot[r];
       }
     }
   }
