@@ -65,6 +65,11 @@ constexpr int kEvSideA = ST_TOTAL + 1, kEvSideB = ST_TOTAL + 2;
 
 struct bbfmm_amd_plan {
   int N = 0, n = 0, n3 = 0, r = 0, nrhs = 1, maxlev = 0, ncell = 0, nleaf = 0, leaf0 = 0;
+  // vectors of the cell arrays (S, F, M, L), the dense-grid M2L of the MFMA
+  // forms and the particle-row form: 16 for plans of 8 to 15 vectors on a
+  // complete tree (fp32 build), whose solves then run the 16-vector
+  // particle-row kernels with the vectors past nrhs zero; nrhs otherwise
+  int width = 1;
   bool complete = true;  // one complete octree (reference layout); false: a batch of sparse instances
   int ninst = 1;
   std::vector<int> h_ploff;  // [maxlev+1] offsets of each level's parents in d_plist
@@ -300,6 +305,10 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
   p->n3 = n * n * n;
   p->r = r;
   p->nrhs = nrhs;
+  p->width = nrhs;
+#ifdef SINGLE
+  if (f.complete && nrhs >= 8 && nrhs < 16) p->width = 16;
+#endif
   p->maxlev = f.maxdepth;
   p->ncell = f.ncell;
   p->nleaf = f.nleaf;
@@ -432,7 +441,7 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
     }
     for (int i = 0; i < N; i++) ppos[pad_of[i]] = hpos[i];
     BB_TRY(upload(&p->d_ppos, ppos.data(), ppos.size(), stream));
-    if (nrhs >= 16) {  // segment tables of the 16-vector P2P kernels
+    if (p->width >= 16) {  // segment tables of the 16-vector P2P kernels
       std::vector<int2> seg((size_t)nleaf * 27, make_int2(0, 0));
       std::vector<int> nseg((size_t)nleaf, 0);
       for (int k = 0; k < nleaf; k++) {
@@ -535,14 +544,16 @@ static int plan_init_forest(bbfmm_amd_plan *p, const HostForest &f, const real3 
   BB_TRY(dalloc(&p->d_q, NR));
   BB_TRY(dalloc(&p->d_phi, NR));
   BB_TRY(dalloc(&p->d_farfield, (size_t)N));
-  BB_TRY(dalloc(&p->d_S, (size_t)ncell * n3 * nrhs));
-  BB_TRY(dalloc(&p->d_F, (size_t)ncell * n3 * nrhs));
-  BB_TRY(dalloc(&p->d_M, (size_t)ncell * r * nrhs));
-  BB_TRY(dalloc(&p->d_L, (size_t)ncell * r * nrhs));
-  BB_HIP(hipMemsetAsync(p->d_M, 0, sizeof(real) * (size_t)ncell * r * nrhs, stream));
-  BB_HIP(hipMemsetAsync(p->d_L, 0, sizeof(real) * (size_t)ncell * r * nrhs, stream));
-  BB_HIP(hipMemsetAsync(p->d_S, 0, sizeof(real) * (size_t)ncell * n3 * nrhs, stream));
-  BB_HIP(hipMemsetAsync(p->d_F, 0, sizeof(real) * (size_t)ncell * n3 * nrhs, stream));
+  const int W = p->width;
+  BB_TRY(dalloc(&p->d_S, (size_t)ncell * n3 * W));
+  BB_TRY(dalloc(&p->d_F, (size_t)ncell * n3 * W));
+  BB_TRY(dalloc(&p->d_M, (size_t)ncell * r * W));
+  BB_TRY(dalloc(&p->d_L, (size_t)ncell * r * W));
+  // zero: the vectors past nrhs stay zero in the forms that compute nrhs
+  BB_HIP(hipMemsetAsync(p->d_M, 0, sizeof(real) * (size_t)ncell * r * W, stream));
+  BB_HIP(hipMemsetAsync(p->d_L, 0, sizeof(real) * (size_t)ncell * r * W, stream));
+  BB_HIP(hipMemsetAsync(p->d_S, 0, sizeof(real) * (size_t)ncell * n3 * W, stream));
+  BB_HIP(hipMemsetAsync(p->d_F, 0, sizeof(real) * (size_t)ncell * n3 * W, stream));
   BB_HIP(hipStreamSynchronize(stream));
   return BBFMM_OK;
 }
@@ -563,7 +574,7 @@ static int plan_init(bbfmm_amd_plan *p, const anotherTree *t, const real3 *pos, 
 static bool rows16(const bbfmm_amd_plan *p)
 {
 #ifdef SINGLE
-  return p->form != BBFMM_M2L_PLAIN && p->nrhs == 16 && p->npad > 0;
+  return p->form != BBFMM_M2L_PLAIN && p->width == 16 && p->npad > 0;
 #else
   (void)p;
   return false;
@@ -672,7 +683,9 @@ static int ensure_grid(bbfmm_amd_plan *p)
       p->grid_variant_up != variant_up) {
     if (p->grid) grid_m2l_destroy(p->grid);
     p->grid = nullptr;
-    BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, p->nrhs, form, p->h_K.data(), p->stream, minlev, variant,
+    // the MFMA forms on the plan's width (the particle-row form's vectors)
+    const int gv = form != BBFMM_M2L_PLAIN ? p->width : p->nrhs;
+    BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, gv, form, p->h_K.data(), p->stream, minlev, variant,
                            variant_up));
     p->grid_form = form;
     p->grid_minlev = minlev;
@@ -689,13 +702,13 @@ static int prepare_solve(bbfmm_amd_plan *p)
 {
   BB_TRY(ensure_grid(p));
 #ifdef SINGLE
-  if (p->form != BBFMM_M2L_PLAIN && p->nrhs >= 16 && !p->d_qt) {
+  if (p->form != BBFMM_M2L_PLAIN && p->width >= 16 && !p->d_qt) {
     const size_t cnt = (size_t)(p->npad > 0 ? p->npad : 1) * 16;
     BB_TRY(dalloc(&p->d_qt, cnt));
     BB_HIP(hipMemsetAsync(p->d_qt, 0, sizeof(real) * cnt, p->stream));
   }
   if (rows16(p) && !p->d_phit) BB_TRY(dalloc(&p->d_phit, (size_t)p->npad * 16));
-  if (p->p2p_split && p->form != BBFMM_M2L_PLAIN && p->nrhs >= 16 && p->npad > 0 && !p->d_qsh) {
+  if (p->p2p_split && p->form != BBFMM_M2L_PLAIN && p->width >= 16 && p->npad > 0 && !p->d_qsh) {
     const size_t bytes = sizeof(__bf16) * (size_t)p->npad * 16;
     BB_HIP(hipMalloc(&p->d_qsh, bytes));
     BB_HIP(hipMalloc(&p->d_qsl, bytes));
@@ -832,8 +845,10 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
   hipStream_t s = p->stream;
   PassArgs a = pass_args(p, rows);
   P2PArgs pp = p2p_args(p);
+  const int nv = rows ? p->width : p->nrhs;  // vectors this solve computes
+  pp.nrhs = nv;
 #ifdef SINGLE
-  if (pp.use_mfma && p->nrhs >= 16) {
+  if (pp.use_mfma && nv >= 16) {
     if (!p->d_qt) return set_error(BBFMM_E_ARG, "internal: solve not prepared");
     pp.qt = p->d_qt;
     if (rows) pp.phit = p->d_phit;
@@ -872,27 +887,27 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
   if (fuse)  // P2M and the first M2M level in one kernel
     BB_TRY(launch_up_leaf(a, p->h_ploff.data(), s));
   else
-    BB_TRY(launch_p2m(a, s, p->nrhs));
+    BB_TRY(launch_p2m(a, s, nv));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2M), s));
-  BB_TRY(launch_m2m(a, p->h_ploff.data(), s, p->nrhs));
+  BB_TRY(launch_m2m(a, p->h_ploff.data(), s, nv));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2L_PREP), s));
   BB_TRY(run_m2l(p, p->d_M, p->d_L, tm ? stage_event(p, ST_M2L) : nullptr));
   BB_TRY(fork_near());
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_POST), s));
   if (a.dcp) {
     // sparse forests: post-M2L and L2L level by level in one product per cell
-    BB_TRY(launch_down_cols(a, p->h_dwv_off.data(), s, p->nrhs));
+    BB_TRY(launch_down_cols(a, p->h_dwv_off.data(), s, nv));
     if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2L), s));
   } else {
-    BB_TRY(launch_postm2l(a, s, p->nrhs));
+    BB_TRY(launch_postm2l(a, s, nv));
     if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2L), s));
-    BB_TRY(launch_l2l(a, p->h_ploff.data(), s, p->nrhs));
+    BB_TRY(launch_l2l(a, p->h_ploff.data(), s, nv));
   }
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_L2P), s));
   if (fuse)
     BB_TRY(launch_down_leaf(a, p->h_ploff.data(), s));
   else
-    BB_TRY(launch_l2p(a, s, p->nrhs));
+    BB_TRY(launch_l2p(a, s, nv));
   if (!rows)  // the particle-row L2P writes the far field itself
     BB_HIP(hipMemcpyAsync(p->d_farfield, p->d_phi, sizeof(real) * p->N, hipMemcpyDeviceToDevice, s));
   // "p2p" stage: P2P itself, or with the near field beside the far field the
@@ -976,7 +991,7 @@ static int solve_eager(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev, bool
   const bool rows = rows16(p);
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_GATHER), s));
   if (rows)
-    BB_TRY(launch_gather_rows16(p->d_ipad, q_dev, p->d_qt, p->N, s));
+    BB_TRY(launch_gather_rows16(p->d_ipad, q_dev, p->nrhs, p->d_qt, p->N, s));
   else
     BB_TRY(launch_gather(p->d_perm, q_dev, p->d_q, p->N, p->nrhs, s));
   BB_TRY(solve_sorted(p, tm, rows));
@@ -984,7 +999,7 @@ static int solve_eager(bbfmm_amd_plan *p, const real *q_dev, real *phi_dev, bool
   if (rows) {
     const bool beside = near_beside(p, rows);
     BB_TRY(launch_scatter_rows16(p->d_ipad, p->d_phit, beside ? p->d_phin : nullptr, p->d_qt, p->potential, phi_dev,
-                                 p->N, s));
+                                 p->nrhs, p->N, s));
   }
   else
     BB_TRY(launch_scatter(p->d_iperm, p->d_phi, phi_dev, p->N, p->nrhs, s));
@@ -1373,7 +1388,14 @@ extern "C" int bbfmm_amd_m2l(bbfmm_amd_plan *p, const real *M_dev, real *L_dev)
 {
   clear_error();
   if (!p || !M_dev || !L_dev) return set_error(BBFMM_E_ARG, "null argument");
-  return run_m2l(p, M_dev, L_dev);
+  if (p->width == p->nrhs) return run_m2l(p, M_dev, L_dev);
+  // a plan wider than its vectors: the nrhs vectors through the plan's own
+  // M and L (the vectors past nrhs are zero there)
+  const size_t cnt = (size_t)p->nrhs * p->ncell * p->r;
+  BB_HIP(hipMemcpyAsync(p->d_M, M_dev, sizeof(real) * cnt, hipMemcpyDeviceToDevice, p->stream));
+  BB_TRY(run_m2l(p, p->d_M, p->d_L));
+  BB_HIP(hipMemcpyAsync(L_dev, p->d_L, sizeof(real) * cnt, hipMemcpyDeviceToDevice, p->stream));
+  return BBFMM_OK;
 }
 
 extern "C" int bbfmm_amd_fetch(bbfmm_amd_plan *p, int which, real *dst)

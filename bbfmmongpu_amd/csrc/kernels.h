@@ -62,13 +62,14 @@ struct PassArgs {
 int launch_gather(const int *perm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
 // dst[j] = src[iperm[j]] per vector (iperm: user index -> leaf-order index)
 int launch_scatter(const int *iperm, const real *src, real *dst, int N, int nrhs, hipStream_t s);
-// particle-row form of 16 vectors: qt[ipad[j]][v] = q[v][j] and
-// phi[v][j] = phit[ipad[j]][v] (ipad: user index -> padded leaf-order index)
-int launch_gather_rows16(const int *ipad, const real *q, real *qt, int N, hipStream_t s);
+// particle-row form of 16 vectors: qt[ipad[j]][v] = q[v][j] for the nq
+// (1..16) vectors of q, 0 for v >= nq, and phi[v][j] = phit[ipad[j]][v] for
+// v < nq (ipad: user index -> padded leaf-order index)
+int launch_gather_rows16(const int *ipad, const real *q, int nq, real *qt, int N, hipStream_t s);
 // phin (may be null): near-field rows to add, then the output's q factor
 // (qt rows, unless potential); without phin the rows of phit are final
 int launch_scatter_rows16(const int *ipad, const real *phit, const real *phin, const real *qt, int potential,
-                          real *phi, int N, hipStream_t s);
+                          real *phi, int nq, int N, hipStream_t s);
 int launch_p2m(const PassArgs &a, hipStream_t s, int nrhs);
 // h_ploff[l]: offset of level l's parents in a.plist (levels 0..maxlev)
 int launch_m2m(const PassArgs &a, const int *h_ploff, hipStream_t s, int nrhs);

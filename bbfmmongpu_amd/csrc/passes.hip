@@ -77,14 +77,14 @@ __global__ void k_scatter(int N, const int *__restrict__ iperm, const real *__re
 // 64-byte row at its padded leaf-order slot; the scatter reads the row of
 // potentials back and writes 16 coalesced values.  Random accesses are whole
 // rows, not single words.
-__global__ void k_gather_rows16(int N, const int *__restrict__ ipad, const real *__restrict__ q,
+__global__ void k_gather_rows16(int N, const int *__restrict__ ipad, const real *__restrict__ q, int nq,
                                 real *__restrict__ qt)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
   real v[16];
 #pragma unroll
-  for (int k = 0; k < 16; k++) v[k] = q[(size_t)k * N + j];
+  for (int k = 0; k < 16; k++) v[k] = k < nq ? q[(size_t)k * N + j] : (real)0;
   // the row as four 16-byte stores (rows are 64-byte aligned)
   real4 *row = reinterpret_cast<real4 *>(qt + (size_t)ipad[j] * 16);
 #pragma unroll
@@ -96,7 +96,7 @@ __global__ void k_gather_rows16(int N, const int *__restrict__ ipad, const real 
 // q factor (qt, unless potential).
 __global__ void k_scatter_rows16(int N, const int *__restrict__ ipad, const real *__restrict__ phit,
                                  const real *__restrict__ phin, const real *__restrict__ qt, int potential,
-                                 real *__restrict__ phi)
+                                 real *__restrict__ phi, int nq)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
@@ -118,7 +118,8 @@ __global__ void k_scatter_rows16(int N, const int *__restrict__ ipad, const real
     }
   }
 #pragma unroll
-  for (int k = 0; k < 16; k++) phi[(size_t)k * N + j] = v[k];
+  for (int k = 0; k < 16; k++)
+    if (k < nq) phi[(size_t)k * N + j] = v[k];
 }
 
 // M = U^T (Kw .* S) for one cell; sw holds Kw .* S in LDS.
@@ -1129,20 +1130,22 @@ int launch_scatter(const int *iperm, const real *src, real *dst, int N, int nrhs
   return BBFMM_OK;
 }
 
-int launch_gather_rows16(const int *ipad, const real *q, real *qt, int N, hipStream_t s)
+int launch_gather_rows16(const int *ipad, const real *q, int nq, real *qt, int N, hipStream_t s)
 {
   if (N <= 0) return BBFMM_OK;
-  hipLaunchKernelGGL(k_gather_rows16, dim3((N + 255) / 256), dim3(256), 0, s, N, ipad, q, qt);
+  if (nq < 1 || nq > 16) return set_error(BBFMM_E_ARG, "particle rows: %d vectors (1..16)", nq);
+  hipLaunchKernelGGL(k_gather_rows16, dim3((N + 255) / 256), dim3(256), 0, s, N, ipad, q, nq, qt);
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }
 
 int launch_scatter_rows16(const int *ipad, const real *phit, const real *phin, const real *qt, int potential,
-                          real *phi, int N, hipStream_t s)
+                          real *phi, int nq, int N, hipStream_t s)
 {
   if (N <= 0) return BBFMM_OK;
+  if (nq < 1 || nq > 16) return set_error(BBFMM_E_ARG, "particle rows: %d vectors (1..16)", nq);
   hipLaunchKernelGGL(k_scatter_rows16, dim3((N + 255) / 256), dim3(256), 0, s, N, ipad, phit, phin, qt, potential,
-                     phi);
+                     phi, nq);
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }

@@ -226,3 +226,55 @@ def test_near_field_beside_far_field_is_bit_identical(ops4, cuda):
         assert np.array_equal(beside, after)
         assert st["p2p_beside"] > 0
     plan.close()
+
+
+@pytest.mark.parametrize("nrhs", [8, 11, 15])
+def test_plans_of_8_to_15_vectors_run_16_wide(nrhs, ops4, cuda):
+    """A plan of 8 to 15 vectors on a complete tree (fp32, an MFMA form) runs
+    the 16-vector particle-row kernels with the vectors past nrhs zero: each
+    of its vectors equals, to the bit, the same vector of a 16-vector plan,
+    and the oracle's solve within the split-BF16 bound.  The plan's M2L call
+    (staged through its own 16-vector arrays) matches the 16-vector plan's,
+    and its gradient (2m = nrhs vectors, non-row form) the plain fp32
+    form's."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, Q = O.uniform_inputs(65536, seed=44, nrhs=16)
+    Q = Q.astype(np.float32)
+    q16 = torch.from_numpy(Q).cuda()
+    p16 = B.Plan(pos, 4, U, K, nrhs=16, prec="s", lmod=1).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2)
+    ref16 = p16.solve(q16).cpu().numpy()
+    rng = np.random.default_rng(nrhs)
+    M = rng.standard_normal((16, p16.ncell * 32)).astype(np.float32)
+    M[nrhs:] = 0
+    L16 = torch.zeros(16, p16.ncell * 32, device=cuda)
+    p16.m2l(torch.from_numpy(M).cuda(), L16)
+    p16.close()
+    p = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", lmod=1).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2)
+    phi = p.solve(q16[:nrhs].contiguous()).cpu().numpy()
+    assert phi.shape == (nrhs, 65536)
+    assert np.array_equal(phi, ref16[:nrhs])
+    L = torch.zeros(nrhs, p.ncell * 32, device=cuda)
+    p.m2l(torch.from_numpy(M[:nrhs]).cuda(), L)
+    assert np.array_equal(L.cpu().numpy(), L16.cpu().numpy()[:nrhs])
+    tree = O.Tree(pos, 1, "d")
+    Kd, Ud = K.astype(np.float32).astype(np.float64), U.astype(np.float32).astype(np.float64)
+    for j in (0, nrhs - 1):
+        ref = O.fmm(tree, Q[j].astype(np.float64), 4, Ud, Kd, scheme="sibling")
+        err = O.rel_l2(ref["phi"], phi[j])
+        print(f"{nrhs}-vector plan, vector {j}: {err:.2e}", flush=True)
+        assert err < 3e-5
+    if nrhs % 2 == 0:
+        m = nrhs // 2
+        g = torch.from_numpy(np.random.default_rng(3).standard_normal((m, 65536)).astype(np.float32)).cuda()
+        qm = q16[:m].contiguous()
+        dq, dx = p.gradient(qm, g)
+        pp = B.Plan(pos, 4, U, K, nrhs=nrhs, prec="s", lmod=1).set(scheme=B.SCHEME_IJ, form=B.FORM_PLAIN)
+        dq_p, dx_p = pp.gradient(qm, g)
+        pp.close()
+        e_q = float((dq - dq_p).norm() / dq_p.norm())
+        e_x = float((dx - dx_p).norm() / dx_p.norm())
+        print(f"{nrhs}-vector plan gradient vs plain fp32: dq {e_q:.2e} dx {e_x:.2e}", flush=True)
+        assert e_q < 3e-5 and e_x < 3e-5
+    p.close()
