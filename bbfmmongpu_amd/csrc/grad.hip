@@ -162,18 +162,29 @@ __global__ void __launch_bounds__(64) k_p2p_grad(GradArgs a)
           }
         }
         __syncthreads();
-        for (int jj = 0; jj < nj; jj++) {
+        // two sources per iteration, each weight one chain of FMAs (the
+        // LDS reads of the second overlap the arithmetic of the first)
+        auto pair = [&](int jj) {
           const real dx = sp[jj].x - xk.x, dy = sp[jj].y - xk.y, dz = sp[jj].z - xk.z;
           const real rr = dx * dx + dy * dy + dz * dz;
           real w = 0;
 #pragma unroll
-          for (int v = 0; v < MMAX; v++) w += gk[v] * sqg[jj][v] + qk[v] * sqg[jj][MMAX + v];
+          for (int v = 0; v < MMAX; v++) {
+            w += gk[v] * sqg[jj][v];
+            w += qk[v] * sqg[jj][MMAX + v];
+          }
           const real ri = rr != (real)0 ? rsqrt_g(rr) : (real)0;
-          const real s = w * ri * ri * ri;
+          const real s = w * (ri * ri * ri);
           ax += s * dx;
           ay += s * dy;
           az += s * dz;
+        };
+        int jj = 0;
+        for (; jj + 1 < nj; jj += 2) {
+          pair(jj);
+          pair(jj + 1);
         }
+        if (jj < nj) pair(jj);
       }
     }
     if (valid) {
