@@ -1105,15 +1105,17 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     a.gbytes = gb < ((size_t)1 << 31) ? (int)gb : 0;
   }
   int tpw = 1, cpb;
-  // variant 0 (automatic): source-stationary blocking once the deepest level
-  // has enough columns (parents x vectors) to give every SIMD at least a wave;
+  // variant 0 (automatic): source-stationary blocking (4 classes per wave)
+  // once the deepest level has 8192 columns (parents x vectors) or more (at
+  // depth 4 with 16 vectors: 0.177 ms against 0.228 ms for two class-stationary
+  // tiles per wave);
   // 1 / 4 / 5: source-stationary with 8 / 4 / 2 field classes per wave and
   // one 32-column tile, 6 / 7: 2 / 4 classes and two tiles per wave;
   // 2 / 3: class-stationary, one / two 32-column tiles per wave
   const long long deep = (long long)(1 << (3 * (maxlev - 1))) * nrhs;
   a.ss = 0;
   if (form == BBFMM_M2L_BF16X2) {
-    a.ss = variant == 0 && deep >= 32 * 1024 ? 4 : ss_classes(variant);
+    a.ss = variant == 0 && deep >= 8 * 1024 ? 4 : ss_classes(variant);
     a.ss_tpw = ss_tiles(variant);
   }
   if (form == BBFMM_M2L_PLAIN) {
