@@ -406,49 +406,83 @@ def m2l_drv(r, Lv, K, M, minlev, maxlev, L0, center, ineigh, neighbors, iinter, 
 # ---------------------------------------------------------------- autograd
 class _PlanCache:
     """Plans of the autograd path kept across calls.  An entry is reused while
-    the order, operators (by content), precision, level modifier, vector
-    count, particle count and stream are the same; new positions are moved
-    into it by bbfmm_amd_plan_update_positions while every particle stays in
-    its leaf, and the plan is rebuilt when one leaves.  At most `size` plans
-    are kept (least recently used first out)."""
+    the order, precision, level modifier, vector count, particle count and
+    stream are the same and the operators are equal: the same array objects
+    as the entry's last call (taken as unmodified), or equal by value to the
+    copies the entry keeps.  Positions need no transfer when they are the
+    same GPU tensor object at the same version (torch counts in-place
+    changes) or equal on the device; new positions are moved into the plan by
+    bbfmm_amd_plan_update_positions while every particle stays in its leaf,
+    and the plan is rebuilt when one leaves.  At most `size` plans are kept
+    (least recently used first out)."""
 
     def __init__(self, size: int = 4):
-        import collections
         self.size = size
-        self.entries = collections.OrderedDict()
+        # [key, U copy, K copy, (U, K) of the last call, device positions, tensor stamp, plan]
+        self.entries = []
         self.builds = 0  # plans built (a test counts them)
 
     @staticmethod
-    def _fingerprint(a) -> tuple:
-        import zlib
-        a = np.ascontiguousarray(a)
-        return (a.shape, a.dtype.str, zlib.crc32(a.view(np.uint8)))
+    def _stamp(t):
+        import weakref
+        return (weakref.ref(t), t._version)
 
-    def get(self, pos_host, n, U, K, prec, lmod, nrhs, stream):
-        key = (n, self._fingerprint(U), self._fingerprint(K), prec, lmod, nrhs, pos_host.shape[0], stream)
-        plan = self.entries.get(key)
-        if plan is not None:
-            p = np.ascontiguousarray(pos_host, dtype=dtype(prec)).reshape(-1, 3)
+    @staticmethod
+    def _same(stamp, t):
+        return stamp is not None and stamp[0]() is t and stamp[1] == t._version
+
+    def get(self, pos, n, U, K, prec, lmod, nrhs, stream):
+        torch = _torch() if not isinstance(pos, np.ndarray) else None
+        dev_pos = pos.detach() if torch is not None and pos.is_cuda else None
+        N = pos.shape[0]
+        key = (n, prec, lmod, nrhs, N, stream)
+        hit = None
+        for e in self.entries:
+            if e[0] != key:
+                continue
+            if e[3][0] is U and e[3][1] is K:
+                hit = e
+                break
+            Ua, Ka = np.asarray(U), np.asarray(K)
+            if e[1].shape == Ua.shape and e[2].shape == Ka.shape and np.array_equal(e[1], Ua) \
+                    and np.array_equal(e[2], Ka):
+                hit = e
+                break
+        if hit is not None:
+            self.entries.remove(hit)
+            self.entries.append(hit)
+            hit[3] = (U, K)
+            plan = hit[6]
+            if dev_pos is not None and hit[4] is not None:
+                if self._same(hit[5], pos) or (hit[4].dtype == dev_pos.dtype and torch.equal(hit[4], dev_pos)):
+                    hit[5] = self._stamp(pos)
+                    return plan
+            p = np.ascontiguousarray(pos.detach().cpu().numpy() if torch is not None else pos,
+                                     dtype=dtype(prec)).reshape(-1, 3)
             if not np.array_equal(plan.pos, p):
                 try:
                     plan.update_positions(p)
                 except BBFMMError:
                     plan.close()
-                    del self.entries[key]
-                    plan = None
-        if plan is None:
-            plan = Plan(pos_host, n, U, K, nrhs=nrhs, prec=prec, lmod=lmod, stream=stream)
-            self.builds += 1
-            self.entries[key] = plan
-            while len(self.entries) > self.size:
-                _, old = self.entries.popitem(last=False)
-                old.close()
-        self.entries.move_to_end(key)
+                    self.entries.remove(hit)
+                    hit = None
+            if hit is not None:
+                hit[4] = dev_pos.clone() if dev_pos is not None else None
+                hit[5] = self._stamp(pos) if dev_pos is not None else None
+                return plan
+        p = pos.detach().cpu().numpy() if torch is not None else pos
+        plan = Plan(p, n, U, K, nrhs=nrhs, prec=prec, lmod=lmod, stream=stream)
+        self.builds += 1
+        self.entries.append([key, np.array(U, copy=True), np.array(K, copy=True), (U, K),
+                             dev_pos.clone() if dev_pos is not None else None,
+                             self._stamp(pos) if dev_pos is not None else None, plan])
+        while len(self.entries) > self.size:
+            self.entries.pop(0)[6].close()
         return plan
 
     def clear(self):
-        for plan in self.entries.values():
-            plan.close()
+        for e in self.entries:
+            e[6].close()
         self.entries.clear()
 
 
@@ -472,8 +506,7 @@ def _make_autograd():
         def forward(ctx, pos, q, cfg):
             n, U, K, prec, lmod, scheme, form = cfg
             m = q.shape[0]
-            plan = plan_cache.get(pos.detach().cpu().numpy(), n, U, K, prec, lmod, m,
-                                  torch.cuda.current_stream().cuda_stream)
+            plan = plan_cache.get(pos, n, U, K, prec, lmod, m, torch.cuda.current_stream().cuda_stream)
             plan.set(scheme=scheme, form=form, output="potential")
             phi = plan.solve(q.detach().contiguous())
             ctx.save_for_backward(pos, q)
@@ -485,8 +518,7 @@ def _make_autograd():
             pos, q = ctx.saved_tensors
             n, U, K, prec, lmod, scheme, form = ctx.cfg
             m = q.shape[0]
-            plan = plan_cache.get(pos.detach().cpu().numpy(), n, U, K, prec, lmod, 2 * m,
-                                  torch.cuda.current_stream().cuda_stream)
+            plan = plan_cache.get(pos, n, U, K, prec, lmod, 2 * m, torch.cuda.current_stream().cuda_stream)
             plan.set(scheme=scheme, form=form)
             dq, dx = plan.gradient(q.detach().contiguous(), gphi.detach().contiguous())
             return dx.to(pos.dtype), dq, None

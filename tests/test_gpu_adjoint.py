@@ -211,3 +211,43 @@ def test_autograd_plan_cache(ops4, cuda):
     for a, b in zip(crossed, ref):
         assert np.array_equal(a, b)
     B.plan_cache.clear()
+
+
+def test_autograd_plan_cache_in_place_moves(ops4, cuda):
+    """The same position tensor moved in place (an optimiser step): the cache
+    sees the new version and moves the positions into its plans; the
+    results equal those of fresh plans."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    U, K = U.astype(np.float32), K.astype(np.float32)
+    N, m = 20000, 8
+    rng = np.random.default_rng(12)
+    pos = rng.random((N, 3)) - 0.5
+    Q = torch.from_numpy(rng.uniform(-1, 1, (m, N)).astype(np.float32)).to(cuda)
+    T = torch.from_numpy(rng.uniform(-1, 1, (m, N)).astype(np.float32)).to(cuda)
+    p_t = torch.tensor(pos, dtype=torch.float32, device=cuda, requires_grad=True)
+
+    def step():
+        p_t.grad = None
+        q_t = Q.clone().requires_grad_(True)
+        ph = B.potential(p_t, q_t, 4, U, K, prec="s")
+        (0.5 * ((ph - T) ** 2).sum()).backward()
+        return ph.detach().cpu().numpy(), q_t.grad.cpu().numpy(), p_t.grad.cpu().numpy()
+
+    B.plan_cache.clear()
+    step()
+    builds = B.plan_cache.builds
+    leaf = 1.0 / 8
+    frac = (pos + 0.5) / leaf
+    keep = np.all((frac - np.floor(frac) > 1e-4) & (np.ceil(frac) - frac > 1e-4), axis=1)
+    shift = torch.from_numpy(np.where(keep[:, None], 1e-6, 0.0).astype(np.float32)).to(cuda)
+    with torch.no_grad():
+        p_t.add_(shift)
+    moved = step()
+    assert B.plan_cache.builds == builds  # uploaded into the cached plans, not rebuilt
+    B.plan_cache.clear()
+    ref = step()
+    for a, b in zip(moved, ref):
+        assert np.array_equal(a, b)
+    B.plan_cache.clear()
