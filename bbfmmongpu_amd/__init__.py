@@ -29,6 +29,7 @@ OPT_P2P_FORM = 9
 OPT_L2P_REFERENCE = 10
 OPT_ADDRESSING = 11
 OPT_M2L_VARIANT_UPPER = 12
+OPT_M2L_UPPER_SPLIT = 13
 P2P_FP32, P2P_BF16X2 = 0, 1
 # "p2p_beside": P2P's own time when it runs on the side stream beside the far
 # field ("p2p" is then the wait for it after the far field)
@@ -263,12 +264,14 @@ class Plan:
             output: str | None = None, graph: bool | None = None, variant: int | None = None,
             level_switch: int | None = None, near_stream: bool | None = None, p2p_form: int | None = None,
             l2p_reference: bool | None = None, flat_loads: bool | None = None,
-            variant_upper: int | None = None):
+            variant_upper: int | None = None, upper_split: int | None = None):
         """output: "q_phi" (q .* potential, the reference's output) or "potential";
         l2p_reference: orders n = 3, 7 evaluate L2P with the reference's z
         weights (anotherDownwardPassX.c:304) instead of the interpolant;
         variant_upper: IJ split-BF16, the source-stationary variant of the
         levels above the deepest (a launch of their own; 0: as the deepest);
+        upper_split: that launch takes each class group's steps in this many
+        parts (1..16), their partial sums added to L afterwards;
         flat_loads: kernels with buffer-load and pointer forms take the
         pointer form (their form for arrays of 2 GB and more) at any size;
         graph: capture the whole solve into one hipGraph and replay it;
@@ -295,6 +298,8 @@ class Plan:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_P2P_FORM, int(p2p_form)))
         if variant_upper is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_M2L_VARIANT_UPPER, int(variant_upper)))
+        if upper_split is not None:
+            _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_M2L_UPPER_SPLIT, int(upper_split)))
         if flat_loads is not None:
             _check(self.lib, self.lib.bbfmm_amd_plan_set(self.h, OPT_ADDRESSING, int(bool(flat_loads))))
         if l2p_reference is not None:

@@ -159,7 +159,8 @@ struct bbfmm_amd_plan {
   // basic scheme takes the levels below)
   int m2l_variant = 0, level_switch = bb::kMinLev;
   int m2l_variant_up = 0;  // source-stationary variant of the levels above the deepest (0: as the deepest)
-  int grid_minlev = bb::kMinLev, grid_variant = 0, grid_variant_up = 0;
+  int m2l_upper_split = 1;  // parts of the upper levels' step lists (one block each)
+  int grid_minlev = bb::kMinLev, grid_variant = 0, grid_variant_up = 0, grid_upper_split = 1;
   hipGraphExec_t gexec = nullptr;
   const real *gq = nullptr;
   real *gphi = nullptr;
@@ -679,18 +680,20 @@ static int ensure_grid(bbfmm_amd_plan *p)
   const int minlev = p->level_switch > kMinLev ? p->level_switch : kMinLev;
   const int variant = p->scheme == M2L_SCHEME_IJ ? p->m2l_variant : 0;
   const int variant_up = p->scheme == M2L_SCHEME_IJ ? p->m2l_variant_up : 0;
+  const int upper_split = variant_up ? p->m2l_upper_split : 1;
   if (!p->grid || p->grid_form != form || p->grid_minlev != minlev || p->grid_variant != variant ||
-      p->grid_variant_up != variant_up) {
+      p->grid_variant_up != variant_up || p->grid_upper_split != upper_split) {
     if (p->grid) grid_m2l_destroy(p->grid);
     p->grid = nullptr;
     // the MFMA forms on the plan's width (the particle-row form's vectors)
     const int gv = form != BBFMM_M2L_PLAIN ? p->width : p->nrhs;
     BB_TRY(grid_m2l_create(&p->grid, p->r, p->maxlev, gv, form, p->h_K.data(), p->stream, minlev, variant,
-                           variant_up));
+                           variant_up, upper_split));
     p->grid_form = form;
     p->grid_minlev = minlev;
     p->grid_variant = variant;
     p->grid_variant_up = variant_up;
+    p->grid_upper_split = upper_split;
   }
   grid_m2l_set_flat(p->grid, p->flat_loads);
   return BBFMM_OK;
@@ -1131,6 +1134,11 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     if (value != 0 && value != 1 && (value < 4 || value > 7))
       return set_error(BBFMM_E_ARG, "M2L upper-level variant %d: 0, 1 or 4 to 7", value);
     p->m2l_variant_up = value;
+    drop_graph(p);
+    return BBFMM_OK;
+  case BBFMM_OPT_M2L_UPPER_SPLIT:
+    if (value < 1 || value > 16) return set_error(BBFMM_E_ARG, "M2L upper-level step parts %d: 1 to 16", value);
+    p->m2l_upper_split = value;
     drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_M2L_VARIANT:

@@ -127,9 +127,11 @@ struct GridM2L;  // defined in m2l_grid.hip
 // caller); variant: 0 automatic, 1 / 4 / 5 source-stationary (split-BF16)
 // with 8 / 4 / 2 field classes per wave, 2 / 3 one / two 32-column tiles per
 // wave (r = 32 MFMA forms); variant_upper: 0, or a source-stationary variant
-// for the levels above the deepest (then two launches)
+// for the levels above the deepest (then two launches); upper_split: that
+// launch takes each class group's steps in this many parts (1..16), one
+// block each, whose partial sums are added afterwards
 int grid_m2l_create(GridM2L **g, int r, int maxlev, int nrhs, int form, const real *K_colmajor_host,
-                    hipStream_t s, int minlev, int variant, int variant_upper = 0);
+                    hipStream_t s, int minlev, int variant, int variant_upper = 0, int upper_split = 1);
 void grid_m2l_destroy(GridM2L *g);
 // flat = 1: the split-BF16 source-stationary kernel reads through pointers
 // (its form for grids of 2 GB and more) whatever the grid size

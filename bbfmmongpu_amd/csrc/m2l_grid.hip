@@ -70,6 +70,12 @@ struct GridArgs {
   int ss_tpw;        // source-stationary kernel: 32-column tiles per wave (1 or 2)
   int gbytes;        // bytes of one grid array when below 2 GB (scalar-offset buffer loads), else 0
   int kbytes;        // bytes of the operator array
+  // source-stationary kernel: each class group's step list in nsplit parts
+  // (one block each); part 0 stores into L, part k >= 1 into Lx
+  // [k - 1][rhs][up_ncell][32] (cells up_cell0 ...), added to L afterwards
+  int nsplit;
+  int up_cell0, up_ncell;
+  float *Lx;
 };
 
 constexpr int kSteps = 208;  // 8 source classes x 26 non-zero parent shifts
@@ -87,6 +93,7 @@ struct GridM2L {
   int nparts = 1;
   GridArgs part[2];
   int part_blocks[2] = {0, 0};
+  float *d_Lx = nullptr;  // partial sums of the upper part's steps beyond the first part
   GridArgs args{};
   int nblocks = 0;
   size_t nslots = 0;
@@ -472,8 +479,10 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
   if (lb >= nb) return;
   int l = g.minlev;
   while (l < g.maxlev && lb >= g.blk0[l + 1]) l++;
+  const int nsp = g.nsplit > 1 ? g.nsplit : 1;
   const int rel = lb - g.blk0[l];
-  const int grp = rel / NCG, cg = rel - grp * NCG;
+  const int part = rel % nsp, rq = rel / nsp;
+  const int grp = rq / NCG, cg = rq - grp * NCG;
   const int P = 1 << (l - 1);
   const int PP = P + 2;
   const int ncols = P * P * P * g.nrhs;
@@ -566,8 +575,10 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
       }
     }
   };
-  const int nst = g.nsteps[cg];
-  const int *S = g.steps + (size_t)cg * kSteps * kStepWords;
+  // this block's part of the class group's steps (all of them when nsplit = 1)
+  const int s_lo = part * g.nsteps[cg] / nsp;
+  const int nst = (part + 1) * g.nsteps[cg] / nsp - s_lo;
+  const int *S = g.steps + ((size_t)cg * kSteps + s_lo) * kStepWords;
   int e1[5];
   int ec = S[0];  // first word of the current step's record
   Src cur[TPW];
@@ -656,7 +667,9 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
       const int c = cg * CLS + i;
       const int fx = 2 * px[t] + ((c >> 2) & 1), fy = 2 * py[t] + ((c >> 1) & 1), fz = 2 * pz[t] + (c & 1);
       const int A = g.levsta[l] + morton3(fx, fy, fz, l);
-      float *Lp = (float *)g.L + ((size_t)rh[t] * g.ncell + A) * 32 + 4 * half;
+      float *Lp = part == 0 ? (float *)g.L + ((size_t)rh[t] * g.ncell + A) * 32 + 4 * half
+                            : g.Lx + (((size_t)(part - 1) * g.nrhs + rh[t]) * g.up_ncell + (A - g.up_cell0)) * 32 +
+                                  4 * half;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         f32x4 v = {acc[t][i][4 * q], acc[t][i][4 * q + 1], acc[t][i][4 * q + 2], acc[t][i][4 * q + 3]};
@@ -664,6 +677,23 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
       }
     }
   }
+}
+
+// L of the cells up_cell0 .. up_cell0 + up_ncell - 1 += the partial sums of
+// parts 1 .. nsplit - 1 (in part order), one float4 per thread
+__global__ void __launch_bounds__(256) k_m2l_add_parts(GridArgs g)
+{
+  const size_t n4 = (size_t)g.nrhs * g.up_ncell * 8;
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n4) return;
+  const size_t rc = i >> 3;  // rhs * up_ncell + cell
+  const int q = (int)(i & 7);
+  const size_t rh = rc / g.up_ncell, c = rc - rh * g.up_ncell;
+  f32x4 *Lp = reinterpret_cast<f32x4 *>(g.L + (rh * g.ncell + g.up_cell0 + c) * 32) + q;
+  f32x4 v = *Lp;
+  const f32x4 *X = reinterpret_cast<const f32x4 *>(g.Lx);
+  for (int k = 0; k + 1 < g.nsplit; k++) v += X[((size_t)k * g.nrhs * g.up_ncell + rc) * 8 + q];
+  *Lp = v;
 }
 
 // ---------------------------------------------------------------- r = 256 FP32 MFMA kernel
@@ -1054,9 +1084,27 @@ static int ss_classes(int variant)
 static int ss_tiles(int variant) { return variant == 6 || variant == 7 ? 2 : 1; }
 
 int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const real *K, hipStream_t s,
-                    int minlev, int variant, int variant_upper)
+                    int minlev, int variant, int variant_upper, int upper_split)
 {
   *out = nullptr;
+  // variant 0 on a deepest level of 64 K columns (parents x vectors) or more,
+  // two blocks of eight 32-column tiles per CU on 256 CUs: that level with
+  // two tiles per wave (4 classes; fewer operator bytes per product), the
+  // levels above in a launch of one tile per wave, their steps in 4 parts
+  // while they have fewer blocks than 3 per CU (at C2, N = 1M, depth 5, 16
+  // vectors: 0.585 against 0.601 ms for one launch of one tile per wave)
+  if (variant == 0 && variant_upper == 0 && upper_split == 1 && form == BBFMM_M2L_BF16X2 && r == 32 &&
+      maxlev > minlev && (long long)(1 << (3 * (maxlev - 1))) * nrhs >= 64 * 1024) {
+    long long up_blocks = 0;
+    for (int l = minlev; l < maxlev; l++) up_blocks += 2 * (((long long)(1 << (3 * (l - 1))) * nrhs + 127) / 128);
+    variant = 7;
+    variant_upper = 4;
+    upper_split = up_blocks * 4 <= 3 * 256 ? 4 : 1;
+  }
+  if (upper_split < 1 || upper_split > 16)
+    return set_error(BBFMM_E_ARG, "dense-grid M2L: upper-level step parts %d outside 1..16", upper_split);
+  if (upper_split > 1 && !ss_classes(variant_upper))
+    return set_error(BBFMM_E_ARG, "dense-grid M2L: upper-level step parts need a source-stationary upper variant");
   if (variant_upper != 0 && !(ss_classes(variant_upper) && ss_classes(variant)))
     return set_error(BBFMM_E_ARG, "dense-grid M2L: a variant for the upper levels (%d) needs source-stationary "
                      "variants (1, 4 to 7) for both", variant_upper);
@@ -1089,6 +1137,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   a.nrhs = nrhs;
   a.minlev = minlev;
   a.maxlev = maxlev;
+  a.nsplit = 1;
   g->minlev = minlev;
   g->variant = variant;
   for (int l = 0; l <= maxlev + 1; l++) a.levsta[l] = ((1 << (3 * l)) - 1) / 7;
@@ -1144,7 +1193,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   // levels above the deepest with their own class-group size: two launches
   const int ss_up = ss_classes(variant_upper), tpw_up = ss_tiles(variant_upper);
   size_t steps_up_off = 0;
-  if (a.ss && ss_up && (ss_up != a.ss || tpw_up != a.ss_tpw) && maxlev > minlev) {
+  if (a.ss && ss_up && (ss_up != a.ss || tpw_up != a.ss_tpw || upper_split > 1) && maxlev > minlev) {
     g->nparts = 2;
     GridArgs &u = g->part[0], &d = g->part[1];
     u = a;
@@ -1153,12 +1202,15 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     u.ss = ss_up;
     u.ss_tpw = tpw_up;
     u.wave_cols = 32 * tpw_up;
+    u.nsplit = upper_split;
+    u.up_cell0 = a.levsta[minlev];
+    u.up_ncell = a.levsta[maxlev] - a.levsta[minlev];
     const int cpb_up = 4 * 32 * tpw_up;
     int nbu = 0;
     for (int l = minlev; l <= maxlev - 1; l++) {
       u.blk0[l] = nbu;
       const long long cols = (long long)(1 << (3 * (l - 1))) * nrhs;
-      nbu += (8 / ss_up) * (int)((cols + cpb_up - 1) / cpb_up);
+      nbu += upper_split * (8 / ss_up) * (int)((cols + cpb_up - 1) / cpb_up);
     }
     u.blk0[maxlev] = nbu;
     d.minlev = maxlev;
@@ -1310,6 +1362,10 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     }
 #endif
   }
+  if (rc == BBFMM_OK && g->nparts == 2 && g->part[0].nsplit > 1) {
+    const size_t nx = (size_t)(g->part[0].nsplit - 1) * nrhs * g->part[0].up_ncell * 32;
+    if ((e = hipMalloc(&g->d_Lx, sizeof(float) * nx)) != hipSuccess) fail(e, "hipMalloc");
+  }
   if (rc == BBFMM_OK && (e = hipStreamSynchronize(s)) != hipSuccess) fail(e, "sync");
   if (rc != BBFMM_OK) {
     grid_m2l_destroy(g);
@@ -1337,6 +1393,10 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
       q.wave_cols = own.wave_cols;
       for (int c = 0; c < 8; c++) q.nsteps[c] = own.nsteps[c];
       q.steps = g->d_steps + (pi == 0 ? steps_up_off : 0);
+      q.nsplit = own.nsplit;
+      q.up_cell0 = own.up_cell0;
+      q.up_ncell = own.up_ncell;
+      q.Lx = pi == 0 ? g->d_Lx : nullptr;
     }
   }
   *out = g;
@@ -1358,6 +1418,7 @@ void grid_m2l_destroy(GridM2L *g)
   if (g->d_Mg1) (void)hipFree(g->d_Mg1);
   if (g->d_Mg2) (void)hipFree(g->d_Mg2);
   if (g->d_lvl_max) (void)hipFree(g->d_lvl_max);
+  if (g->d_Lx) (void)hipFree(g->d_Lx);
   delete g;
 }
 
@@ -1423,6 +1484,10 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
           launch_ss<4>(g, pa, nbk, s);
         else
           launch_ss<2>(g, pa, nbk, s);
+        if (pa.nsplit > 1) {  // the upper part's steps in parts: add parts 1.. to L
+          const size_t n4 = (size_t)pa.nrhs * pa.up_ncell * 8;
+          hipLaunchKernelGGL(k_m2l_add_parts, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, s, pa);
+        }
       }
     } else if (a.wave_cols == 64)
       hipLaunchKernelGGL((k_m2l_mfma<BBFMM_M2L_BF16X2, 2>), dim3((g->nblocks + 7) & ~7), dim3(256), 0, s, a);

@@ -41,26 +41,34 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c2.txt")
 CSRC = os.path.join(ROOT, "bbfmmongpu_amd", "csrc")
 # source file of each kernel whose HBM traffic the bench reports
-KERNEL_SOURCE = {"k_m2l_mfma_ss<4,1,true>": "m2l_grid.hip", "k_p2p_seg16<true>": "p2p.hip",
-                 "k_p2p_seg16x<3>": "p2p.hip"}
+KERNEL_SOURCE = {"k_m2l_mfma_ss<4,1,true>": "m2l_grid.hip", "k_m2l_mfma_ss<4,2,true>": "m2l_grid.hip",
+                 "k_m2l_add_parts": "m2l_grid.hip", "k_p2p_seg16<true>": "p2p.hip", "k_p2p_seg16x<3>": "p2p.hip"}
+# the launches of the split-BF16 M2L stage at C2 (automatic configuration):
+# the deepest level with two 32-column tiles per wave, the levels above with
+# one, their steps in 4 parts, then the partial sums added
+M2L_KERNELS = ("k_m2l_mfma_ss<4,2,true>", "k_m2l_mfma_ss<4,1,true>", "k_m2l_add_parts")
 
 
-def ss_issued_wave_steps(cls: int, maxlev: int, nrhs: int) -> int:
-    """(wave, step) pairs the source-stationary M2L runs its products on: a
-    wave (32 columns = (parent, vector) pairs, vector fastest, then pz, py,
-    px) skips the steps whose parent shift d moves its whole parent box
-    outside the level's [0, P)^3 (ghost sources only); the box is exact in
-    px and conservative (the whole axis) in py and pz when the columns span
-    more than one row."""
+def ss_issued_mfma(cls: int, maxlev: int, nrhs: int, deep_tiles: int = 1) -> int:
+    """MFMAs (32x32x16) the source-stationary M2L issues: per (wave, step) it
+    runs its products on, cls classes x tiles x 3 split products x 2 k-steps.
+    A wave (32 columns = (parent, vector) pairs per tile, vector fastest,
+    then pz, py, px) skips the steps whose parent shift d moves its whole
+    parent box outside the level's [0, P)^3 (ghost sources only); the box is
+    exact in px and conservative (the whole axis) in py and pz when the
+    columns span more than one row.  deep_tiles: tiles per wave on the
+    deepest level (one on the others)."""
     total = 0
     for lev in range(2, maxlev + 1):
         P = 1 << (lev - 1)
         ncols = P ** 3 * nrhs
-        nwave = -(-ncols // 128) * 4
-        col0 = np.arange(nwave) * 32
+        tiles = deep_tiles if lev == maxlev else 1
+        span = 32 * tiles
+        nwave = -(-ncols // (4 * span)) * 4
+        col0 = np.arange(nwave) * span
         ok = col0 < ncols
         f = np.minimum(col0, ncols - 1) // nrhs
-        t = np.minimum(col0 + 31, ncols - 1) // nrhs
+        t = np.minimum(col0 + span - 1, ncols - 1) // nrhs
         fx, fy, fz = f // (P * P), (f // P) % P, f % P
         tx, ty, tz = t // (P * P), (t // P) % P, t % P
         row = fx == tx
@@ -79,7 +87,7 @@ def ss_issued_wave_steps(cls: int, maxlev: int, nrhs: int) -> int:
                         ds.append(d)
             D = np.array(ds)[None, :, :]
             ghost = ((hi[:, None, :] + D < 0) | (lo[:, None, :] + D >= P)).any(axis=2)
-            total += int((~ghost[ok]).sum())
+            total += int((~ghost[ok]).sum()) * cls * tiles * 6
     return total
 
 
@@ -701,11 +709,16 @@ def main():
     p2p_flop = pairs * (18 + 2 * nrhs)  # geometry (20-flop convention) shared by the vectors
     p2p_valu_flop = pairs * 18          # the 1/r part alone (the sum over sources runs on MFMA)
     # issued MFMA flops of the source-stationary kernel (4 field classes per
-    # wave): per wave and step it runs, 4 classes x 6 products of 32x32x16
-    # (zero operator included, ghost steps skipped)
-    issued = ss_issued_wave_steps(4, chosen.maxlev, nrhs) * 4 * 6 * (2 * 32 * 32 * 16)
-    m2l_kernel = "k_m2l_mfma_ss<4,1,true>"
-    tr_m2l, tr_m2l_src = measured_traffic(m2l_kernel)
+    # wave, two tiles per wave on the deepest level): per wave and step it
+    # runs, 4 classes x tiles x 6 products of 32x32x16 (zero operator
+    # included, ghost steps skipped)
+    issued = ss_issued_mfma(4, chosen.maxlev, nrhs, deep_tiles=2) * (2 * 32 * 32 * 16)
+    # the M2L stage is three launches; its roofline takes their sum (time
+    # between the stage's events, HBM bytes of the three)
+    m2l_kernel = " + ".join(M2L_KERNELS)
+    tr_parts = [measured_traffic(k) for k in M2L_KERNELS]
+    tr_m2l = sum(t for t, _ in tr_parts) if all(t is not None for t, _ in tr_parts) else None
+    tr_m2l_src = {k: src for k, (_, src) in zip(M2L_KERNELS, tr_parts)}
     roof_m2l = {"kernel": m2l_kernel, "bound": "mfma", "unit": "TFLOP/s",
                 "achieved": flop_m2l * nrhs / (st_chosen["m2l"] * 1e-3) / 1e12, "peak": PEAK_BF16_TFLOPS,
                 "launch_ms": st_chosen["m2l"], "algorithmic_flop_per_launch": flop_m2l * nrhs,

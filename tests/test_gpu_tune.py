@@ -54,6 +54,57 @@ def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, nr
     plan.close()
 
 
+@pytest.mark.parametrize("variant,upper,split,nrhs", [(7, 4, 4, 16), (4, 4, 2, 16), (4, 5, 3, 3), (1, 4, 16, 16),
+                                                       (6, 7, 5, 16)])
+def test_m2l_upper_levels_in_step_parts(variant, upper, split, nrhs, ops4, cuda):
+    """BBFMM_OPT_M2L_UPPER_SPLIT: the launch of the levels above the deepest
+    takes each class group's steps in `split` parts whose partial sums are
+    added afterwards.  The deepest level keeps the bits of the single launch
+    of its variant; every level matches the oracle's M2L; the upper levels
+    match the unsplit upper launch to fp32 rounding (the sum order differs)."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, _q = O.uniform_inputs(65536, seed=0)
+    tree = O.Tree(pos, 0, "d")
+    rng = np.random.default_rng(23)
+    M = rng.standard_normal((nrhs, tree.ncell * 32)).astype(np.float32)
+    Md = torch.from_numpy(M).to(cuda)
+    out = {}
+    for up, sp in ((0, 1), (upper, 1), (upper, split)):
+        plan = B.Plan(pos, 4, U, K, prec="s", nrhs=nrhs).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2,
+                                                             variant=variant, variant_upper=up, upper_split=sp)
+        Ld = torch.zeros_like(Md)
+        plan.m2l(Md, Ld)
+        torch.cuda.synchronize()
+        out[(up, sp)] = Ld.cpu().numpy()
+        plan.close()
+    ls = O.levsta(tree.maxlev)
+    deep, s = ls[tree.maxlev] * 32, ls[2] * 32
+    got = out[(upper, split)]
+    assert np.array_equal(out[(0, 1)][:, deep:], got[:, deep:])
+    d = O.rel_l2(out[(upper, 1)][:, s:deep].astype(np.float64), got[:, s:deep])
+    print(f"variant={variant} upper={upper} split={split} nrhs={nrhs}: upper levels vs unsplit {d:.2e}", flush=True)
+    assert d < 1e-6
+    Kd = K.astype(np.float32).astype(np.float64)
+    for j in (0, nrhs - 1):
+        ref = O.m2l(tree, Kd, M[j].astype(np.float64), 32, "basic")
+        err = O.rel_l2(ref[s:], got[j, s:])
+        print(f"variant={variant} upper={upper} split={split} nrhs={nrhs} rhs={j}: {err:.2e}", flush=True)
+        assert err < 3e-5
+
+
+def test_m2l_upper_split_rejected_without_upper_variant(ops4, cuda):
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, _q = O.uniform_inputs(4096, seed=0)
+    plan = B.Plan(pos, 4, U, K, prec="s", nrhs=16).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2, variant=4)
+    with pytest.raises(B.BBFMMError):
+        plan.set(upper_split=17)
+    plan.set(upper_split=4)  # no effect without an upper-level variant
+    plan.close()
+
+
 @pytest.mark.parametrize("variant,upper,nrhs", [(1, 5, 16), (1, 4, 16), (4, 5, 16), (5, 1, 16), (1, 5, 3), (6, 7, 16),
                                                  (1, 7, 16)])
 def test_m2l_upper_levels_own_variant(variant, upper, nrhs, ops4, cuda):
