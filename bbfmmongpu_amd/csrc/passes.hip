@@ -72,23 +72,23 @@ __global__ void k_scatter(int N, const int *__restrict__ iperm, const real *__re
   if (j < N) dst[off + j] = src[off + iperm[j]];
 }
 
-// Particle-row form of 16 vectors.  A thread per user-order particle j reads
-// its 16 charges (one coalesced load per vector) and writes them as one
-// 64-byte row at its padded leaf-order slot; the scatter reads the row of
-// potentials back and writes 16 coalesced values.  Random accesses are whole
-// rows, not single words.
+// Particle-row form of 16 vectors.  Four lanes per user-order particle j:
+// lane part k4 reads vectors 4 k4 .. 4 k4 + 3 of its charges (each load of a
+// wave covers 16 consecutive particles of 4 vectors) and writes them as one
+// 16-byte piece of the particle's 64-byte row at its padded leaf-order slot,
+// so every store of a wave writes 16 whole rows; the scatter reads the rows
+// of potentials back the same way and writes coalesced values.  Random
+// accesses are whole rows, not single words.
 __global__ void k_gather_rows16(int N, const int *__restrict__ ipad, const real *__restrict__ q, int nq,
                                 real *__restrict__ qt)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = (int)(t >> 2), k0 = 4 * (int)(t & 3);
   if (j >= N) return;
-  real v[16];
+  real v[4];
 #pragma unroll
-  for (int k = 0; k < 16; k++) v[k] = k < nq ? q[(size_t)k * N + j] : (real)0;
-  // the row as four 16-byte stores (rows are 64-byte aligned)
-  real4 *row = reinterpret_cast<real4 *>(qt + (size_t)ipad[j] * 16);
-#pragma unroll
-  for (int k = 0; k < 4; k++) row[k] = real4{v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]};
+  for (int c = 0; c < 4; c++) v[c] = k0 + c < nq ? q[(size_t)(k0 + c) * N + j] : (real)0;
+  reinterpret_cast<real4 *>(qt + (size_t)ipad[j] * 16)[k0 >> 2] = real4{v[0], v[1], v[2], v[3]};
 }
 
 // phin (not null): the near field arrived in rows of its own (P2P ran
@@ -98,28 +98,25 @@ __global__ void k_scatter_rows16(int N, const int *__restrict__ ipad, const real
                                  const real *__restrict__ phin, const real *__restrict__ qt, int potential,
                                  real *__restrict__ phi, int nq)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
-  const size_t r0 = (size_t)ipad[j] * 16;
-  real v[16];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {  // four 16-byte loads (rows are 64-byte aligned)
-    const real4 t = reinterpret_cast<const real4 *>(phit + r0)[k];
-    v[4 * k] = t.x;
-    v[4 * k + 1] = t.y;
-    v[4 * k + 2] = t.z;
-    v[4 * k + 3] = t.w;
-  }
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = (int)(t >> 2), k0 = 4 * (int)(t & 3);
+  if (j >= N || k0 >= nq) return;
+  const size_t r0 = (size_t)ipad[j] * 16 + k0;
+  const real4 a = *reinterpret_cast<const real4 *>(phit + r0);
+  real v[4] = {a.x, a.y, a.z, a.w};
   if (phin) {
+    const real4 b = *reinterpret_cast<const real4 *>(phin + r0);
+    const real4 w = *reinterpret_cast<const real4 *>(qt + r0);
+    const real nb[4] = {b.x, b.y, b.z, b.w}, qw[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const real t = v[k] + phin[r0 + k];
-      v[k] = potential ? t : qt[r0 + k] * t;
+    for (int c = 0; c < 4; c++) {
+      const real s = v[c] + nb[c];
+      v[c] = potential ? s : qw[c] * s;
     }
   }
 #pragma unroll
-  for (int k = 0; k < 16; k++)
-    if (k < nq) phi[(size_t)k * N + j] = v[k];
+  for (int c = 0; c < 4; c++)
+    if (k0 + c < nq) phi[(size_t)(k0 + c) * N + j] = v[c];
 }
 
 // M = U^T (Kw .* S) for one cell; sw holds Kw .* S in LDS.
@@ -1134,7 +1131,7 @@ int launch_gather_rows16(const int *ipad, const real *q, int nq, real *qt, int N
 {
   if (N <= 0) return BBFMM_OK;
   if (nq < 1 || nq > 16) return set_error(BBFMM_E_ARG, "particle rows: %d vectors (1..16)", nq);
-  hipLaunchKernelGGL(k_gather_rows16, dim3((N + 255) / 256), dim3(256), 0, s, N, ipad, q, nq, qt);
+  hipLaunchKernelGGL(k_gather_rows16, dim3((unsigned)((4LL * N + 255) / 256)), dim3(256), 0, s, N, ipad, q, nq, qt);
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }
@@ -1144,8 +1141,8 @@ int launch_scatter_rows16(const int *ipad, const real *phit, const real *phin, c
 {
   if (N <= 0) return BBFMM_OK;
   if (nq < 1 || nq > 16) return set_error(BBFMM_E_ARG, "particle rows: %d vectors (1..16)", nq);
-  hipLaunchKernelGGL(k_scatter_rows16, dim3((N + 255) / 256), dim3(256), 0, s, N, ipad, phit, phin, qt, potential,
-                     phi, nq);
+  hipLaunchKernelGGL(k_scatter_rows16, dim3((unsigned)((4LL * N + 255) / 256)), dim3(256), 0, s, N, ipad, phit, phin,
+                     qt, potential, phi, nq);
   BB_HIP(hipGetLastError());
   return BBFMM_OK;
 }
