@@ -168,7 +168,9 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
 #define BBFMM_OPT_M2L_VARIANT 6  /* tile shape of the M2L scheme: basic 0/1/2 = 256/128/512
                                     threads per block; IJ MFMA forms 0 automatic,
                                     1/4/5 source-stationary (split-BF16) with 8/4/2 field
-                                    classes per wave, 2/3 one/two 32-column tiles per wave */
+                                    classes per wave, 6/7 the same with 2/4 classes and
+                                    two 32-column tiles per wave, 2/3 class-stationary
+                                    with one/two 32-column tiles per wave */
 #define BBFMM_OPT_NEAR_STREAM 8  /* 16-vector MFMA solves: 1 runs P2P on a side stream
                                     beside the downward pass (forked after the M2L, joined
                                     before the scatter), 0 (default) after it; the results
@@ -191,10 +193,12 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                       (anotherFMMCompute, anotherDownwardPass) always take 1.
                                       Other orders: no effect */
 #define BBFMM_OPT_M2L_VARIANT_UPPER 12 /* IJ split-BF16 with a source-stationary variant
-                                          (1, 4, 5): the class-group size of the levels
-                                          above the deepest, run as a launch of their
-                                          own (1, 4, 5 = 8, 4, 2 classes per wave);
-                                          0 (default): the deepest level's */
+                                          (1, 4 to 7): the class-group size and tiles
+                                          per wave of the levels above the deepest, run
+                                          as a launch of their own (1, 4, 5 = 8, 4, 2
+                                          classes per wave, one tile; 6, 7 = 2, 4
+                                          classes, two tiles); 0 (default): the deepest
+                                          level's, or the automatic configuration */
 #define BBFMM_OPT_M2L_UPPER_SPLIT 13 /* with BBFMM_OPT_M2L_VARIANT_UPPER set: the launch
                                         of the upper levels takes each class group's
                                         steps in this many parts (1 to 16, default 1),
