@@ -1142,7 +1142,7 @@ extern "C" int bbfmm_amd_plan_set(bbfmm_amd_plan *p, int option, int value)
     drop_graph(p);
     return BBFMM_OK;
   case BBFMM_OPT_M2L_VARIANT:
-    if (value < 0 || value > 7) return set_error(BBFMM_E_ARG, "M2L variant %d out of range", value);
+    if (value < 0 || value > 8) return set_error(BBFMM_E_ARG, "M2L variant %d out of range", value);
     p->m2l_variant = value;
     drop_graph(p);
     return BBFMM_OK;

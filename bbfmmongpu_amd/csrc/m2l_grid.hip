@@ -68,6 +68,8 @@ struct GridArgs {
   int nsteps[8];     // steps of each class group
   int ss;            // source-stationary kernel: field classes per wave (8, 4 or 2); 0: not used
   int ss_tpw;        // source-stationary kernel: 32-column tiles per wave (1 or 2)
+  int ss_pair;       // 1: both class groups of a column group in one block sharing source tiles
+                     // (k_m2l_mfma_ss_pair: 4 classes, 2 tiles per wave, the 8-class step table)
   int gbytes;        // bytes of one grid array when below 2 GB (scalar-offset buffer loads), else 0
   int kbytes;        // bytes of the operator array
   // source-stationary kernel: each class group's step list in nsplit parts
@@ -679,6 +681,207 @@ __global__ void __launch_bounds__(256) k_m2l_mfma_ss(GridArgs g)
   }
 }
 
+// ---------------------------------------------------------------- class-group pairs sharing source tiles
+// k_m2l_mfma_ss with 4 classes and two 32-column tiles per wave, and both
+// class groups of a column group in one block of 8 waves (waves 0-3 classes
+// 0-3, waves 4-7 classes 4-7, wave w on the columns of tile pair w % 4).
+// The block walks the union of the two groups' step lists (the 8-class
+// table: 208 steps; a group's classes that are near in a step take the zero
+// operator), so the shifted source tiles of a step are the same for both
+// groups: the block moves them into LDS once (wave w moves tile w / 4 of its
+// own columns, global_load_lds, double-buffered like the operators) and
+// every wave reads its B fragments from there.  Per 48 MFMAs a wave moves
+// 4 KB of operators and 4 KB of sources from memory (8 + 4 KB when each
+// group loads its own tiles).  LDS: 64 KB of operators + 64 KB of source
+// tiles, one block per CU.
+template <bool BUF>
+__global__ void __launch_bounds__(512) k_m2l_mfma_ss_pair(GridArgs g)
+{
+  constexpr int CLS = 4, TPW = 2;
+  __shared__ bf16x8 sK[2][8][4][64];         // [buffer][class][kstep*2 + hi/lo][lane]
+  __shared__ bf16x8 sS[2][4][TPW][4][64];    // [buffer][tile pair][tile][h0, l0, h1, l1][lane]
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int tp = wave & 3, cg = wave >> 2;
+  const int nb = g.blk0[g.maxlev + 1];
+  const int lb = (blockIdx.x & 7) * ((nb + 7) >> 3) + (blockIdx.x >> 3);
+  if (lb >= nb) return;
+  int l = g.minlev;
+  while (l < g.maxlev && lb >= g.blk0[l + 1]) l++;
+  const int grp = lb - g.blk0[l];
+  const int P = 1 << (l - 1);
+  const int PP = P + 2;
+  const int ncols = P * P * P * g.nrhs;
+  const int half = lane >> 5;
+  const int sh0 = PP * PP + PP + 1;
+  const int col0 = (grp * 4 + tp) * 32 * TPW;
+  bool valid[TPW];
+  int rh[TPW], px[TPW], py[TPW], pz[TPW], base[TPW], voff[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; t++) {
+    const int col = col0 + 32 * t + (lane & 31);
+    valid[t] = col < ncols;
+    const int cc = valid[t] ? col : 0;
+    const int pl = cc / g.nrhs;
+    rh[t] = cc - pl * g.nrhs;
+    pz[t] = pl % P;
+    py[t] = (pl / P) % P;
+    px[t] = pl / (P * P);
+    base[t] = ((px[t] + 1) * PP + py[t] + 1) * PP + pz[t] + 1;
+    voff[t] = (((base[t] - sh0) * g.nrhs + rh[t]) * 32 + half * 8) * 2;
+  }
+  const int slot0 = g.lvl_slot0[l];
+  const int cls = PP * PP * PP;
+  const WaveBox box = wave_box((grp * 4 + __builtin_amdgcn_readfirstlane(tp)) * 32 * TPW, ncols, g.nrhs, P, 32 * TPW);
+  f32x16 acc[TPW][CLS];
+#pragma unroll
+  for (int t = 0; t < TPW; t++)
+#pragma unroll
+    for (int c = 0; c < CLS; c++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) acc[t][c][e] = 0.0f;
+
+  typedef __attribute__((address_space(3))) void lds_void;
+  const __amdgpu_buffer_rsrc_t rK = __builtin_amdgcn_make_buffer_rsrc((void *)g.Ka, 0, g.kbytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rMh = __builtin_amdgcn_make_buffer_rsrc((void *)g.Mg0, 0, g.gbytes, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rMl = __builtin_amdgcn_make_buffer_rsrc((void *)g.Mg1, 0, g.gbytes, 0x00020000);
+  const bf16x8 *Kl = (const bf16x8 *)g.Ka + tp * 64 + lane;
+  const __bf16 *Mh = (const __bf16 *)g.Mg0;
+  const __bf16 *Ml = (const __bf16 *)g.Mg1;
+  const int my_t = cg;  // the tile of its own columns this wave moves into LDS
+  // operators: wave w moves k-part w % 4 of its group's 4 classes
+  auto load_k = [&](const int *e, int buf) {
+#pragma unroll
+    for (int i = 0; i < CLS; i++) {
+      const int k = (e[1 + 2 * cg + (i >> 1)] >> (16 * (i & 1))) & 0xffff;
+      if (BUF)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rK, (lds_void *)&sK[buf][4 * cg + i][tp][0], 16,
+                                                 (tp * 64 + lane) * 16, k * 256 * 16, 0, 0);
+      else
+        __builtin_amdgcn_global_load_lds((const void *)(Kl + (size_t)k * 256), (lds_void *)&sK[buf][4 * cg + i][tp][0],
+                                         16, 0, 0);
+    }
+  };
+  // source tile my_t of this wave's columns for step e: h0, l0, h1, l1
+  auto load_s = [&](const int *e, int buf) {
+    const int sc = e[0] & 7, pk = e[0] >> 3;
+    const int dx = ((pk >> 4) & 3) - 1, dy = ((pk >> 2) & 3) - 1, dz = (pk & 3) - 1;
+    const int sft = slot0 + sc * cls + (dx * PP + dy) * PP + dz;
+    lds_void *d = (lds_void *)&sS[buf][tp][my_t][0][0];
+    if (BUF) {
+      const int so = (sft + sh0) * g.nrhs * 64;
+      const int vo = my_t ? voff[1] : voff[0];
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rMh, d, 16, vo, so, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rMl, (lds_void *)&sS[buf][tp][my_t][1][0], 16, vo, so, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rMh, (lds_void *)&sS[buf][tp][my_t][2][0], 16, vo + 32, so, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rMl, (lds_void *)&sS[buf][tp][my_t][3][0], 16, vo + 32, so, 0, 0);
+    } else {
+      const int bs = my_t ? base[1] : base[0];
+      const int rr = my_t ? rh[1] : rh[0];
+      const size_t o = ((size_t)(sft + bs) * g.nrhs + rr) * 32 + half * 8;
+      __builtin_amdgcn_global_load_lds((const void *)(Mh + o), d, 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(Ml + o), (lds_void *)&sS[buf][tp][my_t][1][0], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(Mh + o + 16), (lds_void *)&sS[buf][tp][my_t][2][0], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(Ml + o + 16), (lds_void *)&sS[buf][tp][my_t][3][0], 16, 0, 0);
+    }
+  };
+  const int nst = g.nsteps[0];
+  const int *S = g.steps;
+  int e1[5];
+  int ec = S[0];
+  {
+    int e0[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) e0[i] = S[i];
+    const int s1 = nst > 1 ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) e1[i] = S[s1 * kStepWords + i];
+    load_k(e0, 0);
+    load_s(e0, 0);
+  }
+  __builtin_amdgcn_s_waitcnt(kWaitVm0);
+  __syncthreads();
+  for (int st = 0; st < nst; st++) {
+    const int b = st & 1;
+    const int n2 = st + 2 < nst ? st + 2 : nst - 1;
+    const int4 r2a = *reinterpret_cast<const int4 *>(S + n2 * kStepWords);
+    const int r2b = S[n2 * kStepWords + 4];
+    const bool skip = ghost_step(box, ec, P);
+    // every LDS read of the step comes before the DMA into the other buffer
+    bf16x8 ah[CLS], al[CLS], bh[TPW], bl[TPW];
+    if (!skip) {
+#pragma unroll
+      for (int c = 0; c < CLS; c++) {
+        ah[c] = sK[b][4 * cg + c][0][lane];
+        al[c] = sK[b][4 * cg + c][1][lane];
+      }
+#pragma unroll
+      for (int t = 0; t < TPW; t++) {
+        bh[t] = sS[b][tp][t][0][lane];
+        bl[t] = sS[b][tp][t][1][lane];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int c = 0; c < CLS; c++)
+#pragma unroll
+        for (int t = 0; t < TPW; t++) {
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh[t], acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl[t], acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh[t], acc[t][c], 0, 0, 0);
+        }
+#pragma unroll
+      for (int c = 0; c < CLS; c++) {
+        ah[c] = sK[b][4 * cg + c][2][lane];
+        al[c] = sK[b][4 * cg + c][3][lane];
+      }
+#pragma unroll
+      for (int t = 0; t < TPW; t++) {
+        bh[t] = sS[b][tp][t][2][lane];
+        bl[t] = sS[b][tp][t][3][lane];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    load_k(e1, b ^ 1);
+    load_s(e1, b ^ 1);
+    __builtin_amdgcn_sched_barrier(0);
+    if (!skip) {
+#pragma unroll
+      for (int c = 0; c < CLS; c++)
+#pragma unroll
+        for (int t = 0; t < TPW; t++) {
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh[t], acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl[t], acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh[t], acc[t][c], 0, 0, 0);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    __syncthreads();
+    ec = e1[0];
+    e1[0] = __builtin_amdgcn_readfirstlane(r2a.x);
+    e1[1] = __builtin_amdgcn_readfirstlane(r2a.y);
+    e1[2] = __builtin_amdgcn_readfirstlane(r2a.z);
+    e1[3] = __builtin_amdgcn_readfirstlane(r2a.w);
+    e1[4] = __builtin_amdgcn_readfirstlane(r2b);
+  }
+#pragma unroll
+  for (int t = 0; t < TPW; t++) {
+    if (!valid[t]) continue;
+#pragma unroll
+    for (int i = 0; i < CLS; i++) {
+      const int c = 4 * cg + i;
+      const int fx = 2 * px[t] + ((c >> 2) & 1), fy = 2 * py[t] + ((c >> 1) & 1), fz = 2 * pz[t] + (c & 1);
+      const int A = g.levsta[l] + morton3(fx, fy, fz, l);
+      float *Lp = (float *)g.L + ((size_t)rh[t] * g.ncell + A) * 32 + 4 * half;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        f32x4 v = {acc[t][i][4 * q], acc[t][i][4 * q + 1], acc[t][i][4 * q + 2], acc[t][i][4 * q + 3]};
+        *reinterpret_cast<f32x4 *>(Lp + 8 * q) = v;
+      }
+    }
+  }
+}
+
 // L of the cells up_cell0 .. up_cell0 + up_ncell - 1 += the partial sums of
 // parts 1 .. nsplit - 1 (in part order), one float4 per thread
 __global__ void __launch_bounds__(256) k_m2l_add_parts(GridArgs g)
@@ -1078,26 +1281,28 @@ static void step_table(std::vector<int> &steps, int cls, int *nsteps)
 // 0 otherwise, and their 32-column tiles per wave
 static int ss_classes(int variant)
 {
-  return variant == 1 ? 8 : (variant == 4 || variant == 7) ? 4 : (variant == 5 || variant == 6) ? 2 : 0;
+  return variant == 1 ? 8 : (variant == 4 || variant == 7 || variant == 8) ? 4 : (variant == 5 || variant == 6) ? 2 : 0;
 }
 
-static int ss_tiles(int variant) { return variant == 6 || variant == 7 ? 2 : 1; }
+static int ss_tiles(int variant) { return variant == 6 || variant == 7 || variant == 8 ? 2 : 1; }
 
 int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const real *K, hipStream_t s,
                     int minlev, int variant, int variant_upper, int upper_split)
 {
   *out = nullptr;
   // variant 0 on a deepest level of 64 K columns (parents x vectors) or more,
-  // two blocks of eight 32-column tiles per CU on 256 CUs: that level with
-  // two tiles per wave (4 classes; fewer operator bytes per product), the
-  // levels above in a launch of one tile per wave, their steps in 4 parts
-  // while they have fewer blocks than 3 per CU (at C2, N = 1M, depth 5, 16
-  // vectors: 0.585 against 0.601 ms for one launch of one tile per wave)
+  // one block of 256 columns and both class groups per CU on 256 CUs: that
+  // level with two tiles per wave and the class groups sharing the source
+  // tiles (variant 8; fewer bytes moved per product), the levels above in a
+  // launch of one tile per wave, their steps in 4 parts while they have
+  // fewer blocks than 3 per CU (at C2, N = 1M, depth 5, 16 vectors: 0.579
+  // against 0.595 ms with variant 7 on the deepest level and 0.601 ms for
+  // one launch of one tile per wave)
   if (variant == 0 && variant_upper == 0 && upper_split == 1 && form == BBFMM_M2L_BF16X2 && r == 32 &&
       maxlev > minlev && (long long)(1 << (3 * (maxlev - 1))) * nrhs >= 64 * 1024) {
     long long up_blocks = 0;
     for (int l = minlev; l < maxlev; l++) up_blocks += 2 * (((long long)(1 << (3 * (l - 1))) * nrhs + 127) / 128);
-    variant = 7;
+    variant = 8;
     variant_upper = 4;
     upper_split = up_blocks * 4 <= 3 * 256 ? 4 : 1;
   }
@@ -1105,13 +1310,13 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     return set_error(BBFMM_E_ARG, "dense-grid M2L: upper-level step parts %d outside 1..16", upper_split);
   if (upper_split > 1 && !ss_classes(variant_upper))
     return set_error(BBFMM_E_ARG, "dense-grid M2L: upper-level step parts need a source-stationary upper variant");
-  if (variant_upper != 0 && !(ss_classes(variant_upper) && ss_classes(variant)))
+  if (variant_upper != 0 && (!(ss_classes(variant_upper) && ss_classes(variant)) || variant_upper == 8))
     return set_error(BBFMM_E_ARG, "dense-grid M2L: a variant for the upper levels (%d) needs source-stationary "
-                     "variants (1, 4 to 7) for both", variant_upper);
+                     "variants for both (1, 4 to 7 above the deepest level)", variant_upper);
   if (maxlev > kMaxLev) return set_error(BBFMM_E_ARG, "dense-grid M2L: maxlev %d > %d", maxlev, kMaxLev);
   if (minlev < kMinLev || minlev > maxlev)
     return set_error(BBFMM_E_ARG, "dense-grid M2L: first level %d outside [%d, %d]", minlev, kMinLev, maxlev);
-  if (variant < 0 || variant > 7) return set_error(BBFMM_E_ARG, "dense-grid M2L: variant %d", variant);
+  if (variant < 0 || variant > 8) return set_error(BBFMM_E_ARG, "dense-grid M2L: variant %d", variant);
   if ((variant == 1 || variant >= 4) && form != BBFMM_M2L_BF16X2)
     return set_error(BBFMM_E_ARG, "dense-grid M2L: the source-stationary variant is split-BF16 only");
   if (variant >= 2 && (form == BBFMM_M2L_PLAIN || r != 32))
@@ -1166,6 +1371,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   if (form == BBFMM_M2L_BF16X2) {
     a.ss = variant == 0 && deep >= 8 * 1024 ? 4 : ss_classes(variant);
     a.ss_tpw = ss_tiles(variant);
+    a.ss_pair = variant == 8;
   }
   if (form == BBFMM_M2L_PLAIN) {
     cpb = r == 32 ? 64 : 256 / (r < 256 ? r : 256);  // k_m2l_grid_plain32: 64 columns per block
@@ -1181,7 +1387,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   for (int l = minlev; l <= maxlev; l++) {
     a.blk0[l] = nb;
     const long long cols = (long long)(1 << (3 * (l - 1))) * nrhs;
-    nb += (a.ss ? 8 / a.ss : 8) * (int)((cols + cpb - 1) / cpb);
+    nb += (a.ss_pair ? 1 : a.ss ? 8 / a.ss : 8) * (int)((cols + cpb - 1) / cpb);
   }
   a.blk0[maxlev + 1] = nb;
   g->nblocks = nb;
@@ -1189,7 +1395,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   std::vector<int4> tab;
   offset_table(tab);
   std::vector<int> steps;
-  step_table(steps, a.ss ? a.ss : 8, a.nsteps);
+  step_table(steps, a.ss && !a.ss_pair ? a.ss : 8, a.nsteps);
   // levels above the deepest with their own class-group size: two launches
   const int ss_up = ss_classes(variant_upper), tpw_up = ss_tiles(variant_upper);
   size_t steps_up_off = 0;
@@ -1201,6 +1407,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     u.maxlev = maxlev - 1;
     u.ss = ss_up;
     u.ss_tpw = tpw_up;
+    u.ss_pair = 0;
     u.wave_cols = 32 * tpw_up;
     u.nsplit = upper_split;
     u.up_cell0 = a.levsta[minlev];
@@ -1390,6 +1597,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
       for (int l = 0; l < kMaxLev + 2; l++) q.blk0[l] = own.blk0[l];
       q.ss = own.ss;
       q.ss_tpw = own.ss_tpw;
+      q.ss_pair = own.ss_pair;
       q.wave_cols = own.wave_cols;
       for (int c = 0; c < 8; c++) q.nsteps[c] = own.nsteps[c];
       q.steps = g->d_steps + (pi == 0 ? steps_up_off : 0);
@@ -1429,7 +1637,12 @@ static void launch_ss(const GridM2L *g, const GridArgs &a, int nblocks, hipStrea
 {
   const dim3 grid((nblocks + 7) & ~7);
   const bool buf = a.gbytes > 0 && !g->flat;
-  if (CLS <= 4 && a.ss_tpw == 2) {  // two tiles per wave: 2 or 4 classes
+  if (a.ss_pair) {
+    if (buf)
+      hipLaunchKernelGGL(k_m2l_mfma_ss_pair<true>, grid, dim3(512), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_m2l_mfma_ss_pair<false>, grid, dim3(512), 0, s, a);
+  } else if (CLS <= 4 && a.ss_tpw == 2) {  // two tiles per wave: 2 or 4 classes
     constexpr int C = CLS <= 4 ? CLS : 4;
     if (buf)
       hipLaunchKernelGGL((k_m2l_mfma_ss<C, 2, true>), grid, dim3(256), 0, s, a);

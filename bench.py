@@ -41,15 +41,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_c2.txt")
 CSRC = os.path.join(ROOT, "bbfmmongpu_amd", "csrc")
 # source file of each kernel whose HBM traffic the bench reports
-KERNEL_SOURCE = {"k_m2l_mfma_ss<4,1,true>": "m2l_grid.hip", "k_m2l_mfma_ss<4,2,true>": "m2l_grid.hip",
+KERNEL_SOURCE = {"k_m2l_mfma_ss<4,1,true>": "m2l_grid.hip", "k_m2l_mfma_ss_pair<true>": "m2l_grid.hip",
                  "k_m2l_add_parts": "m2l_grid.hip", "k_p2p_seg16<true>": "p2p.hip", "k_p2p_seg16x<3>": "p2p.hip"}
 # the launches of the split-BF16 M2L stage at C2 (automatic configuration):
-# the deepest level with two 32-column tiles per wave, the levels above with
-# one, their steps in 4 parts, then the partial sums added
-M2L_KERNELS = ("k_m2l_mfma_ss<4,2,true>", "k_m2l_mfma_ss<4,1,true>", "k_m2l_add_parts")
+# the deepest level with two 32-column tiles per wave and both class groups
+# in one block, the levels above with one tile per wave and their steps in 4
+# parts, then the partial sums added
+M2L_KERNELS = ("k_m2l_mfma_ss_pair<true>", "k_m2l_mfma_ss<4,1,true>", "k_m2l_add_parts")
 
 
-def ss_issued_mfma(cls: int, maxlev: int, nrhs: int, deep_tiles: int = 1) -> int:
+def ss_issued_mfma(cls: int, maxlev: int, nrhs: int, deep_tiles: int = 1, deep_union: bool = False) -> int:
     """MFMAs (32x32x16) the source-stationary M2L issues: per (wave, step) it
     runs its products on, cls classes x tiles x 3 split products x 2 k-steps.
     A wave (32 columns = (parent, vector) pairs per tile, vector fastest,
@@ -57,7 +58,8 @@ def ss_issued_mfma(cls: int, maxlev: int, nrhs: int, deep_tiles: int = 1) -> int
     parent box outside the level's [0, P)^3 (ghost sources only); the box is
     exact in px and conservative (the whole axis) in py and pz when the
     columns span more than one row.  deep_tiles: tiles per wave on the
-    deepest level (one on the others)."""
+    deepest level (one on the others); deep_union: there both class groups
+    walk the union of their step lists (one block for both)."""
     total = 0
     for lev in range(2, maxlev + 1):
         P = 1 << (lev - 1)
@@ -81,8 +83,9 @@ def ss_issued_mfma(cls: int, maxlev: int, nrhs: int, deep_tiles: int = 1) -> int
                 for d in itertools.product((-1, 0, 1), repeat=3):
                     if d == (0, 0, 0):
                         continue
+                    group = range(8) if deep_union and lev == maxlev else range(g * cls, (g + 1) * cls)
                     far = any(max(abs(2 * d[a] + ((sc >> (2 - a)) & 1) - ((c >> (2 - a)) & 1)) for a in range(3)) >= 2
-                              for c in range(g * cls, (g + 1) * cls))
+                              for c in group)
                     if far:
                         ds.append(d)
             D = np.array(ds)[None, :, :]
@@ -709,10 +712,11 @@ def main():
     p2p_flop = pairs * (18 + 2 * nrhs)  # geometry (20-flop convention) shared by the vectors
     p2p_valu_flop = pairs * 18          # the 1/r part alone (the sum over sources runs on MFMA)
     # issued MFMA flops of the source-stationary kernel (4 field classes per
-    # wave, two tiles per wave on the deepest level): per wave and step it
-    # runs, 4 classes x tiles x 6 products of 32x32x16 (zero operator
-    # included, ghost steps skipped)
-    issued = ss_issued_mfma(4, chosen.maxlev, nrhs, deep_tiles=2) * (2 * 32 * 32 * 16)
+    # wave, two tiles per wave on the deepest level, where both class groups
+    # walk the union of their steps): per wave and step it runs, 4 classes x
+    # tiles x 6 products of 32x32x16 (zero operator included, ghost steps
+    # skipped)
+    issued = ss_issued_mfma(4, chosen.maxlev, nrhs, deep_tiles=2, deep_union=True) * (2 * 32 * 32 * 16)
     # the M2L stage is three launches; its roofline takes their sum (time
     # between the stage's events, HBM bytes of the three)
     m2l_kernel = " + ".join(M2L_KERNELS)

@@ -169,7 +169,9 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                     threads per block; IJ MFMA forms 0 automatic,
                                     1/4/5 source-stationary (split-BF16) with 8/4/2 field
                                     classes per wave, 6/7 the same with 2/4 classes and
-                                    two 32-column tiles per wave, 2/3 class-stationary
+                                    two 32-column tiles per wave, 8 as 7 with both class
+                                    groups in one block sharing the source tiles through
+                                    LDS (the 8-class step list), 2/3 class-stationary
                                     with one/two 32-column tiles per wave */
 #define BBFMM_OPT_NEAR_STREAM 8  /* 16-vector MFMA solves: 1 runs P2P on a side stream
                                     beside the downward pass (forked after the M2L, joined

@@ -55,7 +55,7 @@ def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, nr
 
 
 @pytest.mark.parametrize("variant,upper,split,nrhs", [(7, 4, 4, 16), (4, 4, 2, 16), (4, 5, 3, 3), (1, 4, 16, 16),
-                                                       (6, 7, 5, 16)])
+                                                       (6, 7, 5, 16), (8, 4, 4, 16), (8, 5, 2, 3)])
 def test_m2l_upper_levels_in_step_parts(variant, upper, split, nrhs, ops4, cuda):
     """BBFMM_OPT_M2L_UPPER_SPLIT: the launch of the levels above the deepest
     takes each class group's steps in `split` parts whose partial sums are
@@ -209,12 +209,15 @@ def test_tuned_batch_plan(ops4, cuda):
     plan.close()
 
 
-@pytest.mark.parametrize("two,one,nrhs", [(6, 5, 16), (7, 4, 16), (6, 5, 3), (7, 4, 5)])
+@pytest.mark.parametrize("two,one,nrhs", [(6, 5, 16), (7, 4, 16), (6, 5, 3), (7, 4, 5), (8, 4, 16), (8, 7, 5)])
 def test_m2l_two_tiles_per_wave_same_bits(two, one, nrhs, ops4, cuda):
     """Variants 6 and 7 give every wave of the source-stationary kernel two
     32-column tiles (2 and 4 field classes); each tile's products and their
     order are those of variants 5 and 4 (one tile per wave), so L is the same
-    to the bit, also with vector counts that leave partial tiles."""
+    to the bit, also with vector counts that leave partial tiles.  Variant 8
+    (both class groups in one block, sharing the source tiles through LDS)
+    walks the union of the groups' step lists in the same order, the extra
+    steps with zero operators, so it too gives the bits of variant 4."""
     import torch
     import bbfmmongpu_amd as B
     U, K, _ = ops4
