@@ -726,7 +726,9 @@ static int prepare_solve(bbfmm_amd_plan *p)
   return BBFMM_OK;
 }
 
-static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = nullptr)
+// deep_in_grid: the deepest level's multipoles were written into the
+// split-BF16 grids by the fused first upward level
+static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = nullptr, bool deep_in_grid = false)
 {
   M2LArgs m;
   m.r = p->r;
@@ -796,7 +798,7 @@ static int run_m2l(bbfmm_amd_plan *p, const real *M, real *L, hipEvent_t mid = n
       b.variant = 0;
       BB_TRY(launch_m2l_basic(b, p->stream));
     }
-    return grid_m2l_run(p->grid, M, L, p->ncell, p->stream, mid);
+    return grid_m2l_run(p->grid, M, L, p->ncell, p->stream, mid, deep_in_grid ? 1 : 0);
   default: return set_error(BBFMM_E_ARG, "unknown M2L scheme %d", p->scheme);
   }
 }
@@ -886,6 +888,13 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
     a.l2l_end = p->maxlev - 1;
     a.m2m_start = p->maxlev - 2;
   }
+  // the split-BF16 dense-grid M2L: the fused first upward level writes the
+  // leaves' multipoles into the grids too, and the M2L scatters only the
+  // levels above
+  bool leaves_in_grid = false;
+  if (fuse && p->scheme == M2L_SCHEME_IJ && p->grid &&
+      grid_m2l_leaf_grids(p->grid, &a.grid_hi, &a.grid_lo, &a.grid_slot0))
+    leaves_in_grid = true;
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_P2M), s));
   if (fuse)  // P2M and the first M2M level in one kernel
     BB_TRY(launch_up_leaf(a, p->h_ploff.data(), s));
@@ -894,7 +903,7 @@ static int solve_sorted(bbfmm_amd_plan *p, bool tm, bool rows)
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2M), s));
   BB_TRY(launch_m2m(a, p->h_ploff.data(), s, nv));
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_M2L_PREP), s));
-  BB_TRY(run_m2l(p, p->d_M, p->d_L, tm ? stage_event(p, ST_M2L) : nullptr));
+  BB_TRY(run_m2l(p, p->d_M, p->d_L, tm ? stage_event(p, ST_M2L) : nullptr, leaves_in_grid));
   BB_TRY(fork_near());
   if (tm) BB_HIP(hipEventRecord(stage_event(p, ST_POST), s));
   if (a.dcp) {

@@ -45,6 +45,12 @@ struct PassArgs {
   int l2l_end = 0x7fffffff;
   // fused first upward level (launch_up_leaf): M2M starts at parents of this level
   int m2m_start = 0x7fffffff;
+  // fused first upward level with the split-BF16 dense-grid M2L: the leaves'
+  // M also go straight into the grids (hi and lo parts, [slot][16][32] bf16
+  // from slot grid_slot0 of the deepest level); null: not written
+  void *grid_hi = nullptr;
+  void *grid_lo = nullptr;
+  int grid_slot0 = 0;
   // sparse forests, n = 4, r = 32, fp32: P2M and post-M2L take 32 cells
   // (leaves) per wave as the columns of MFMA products
   int cols = 0;
@@ -137,7 +143,14 @@ void grid_m2l_destroy(GridM2L *g);
 // (its form for grids of 2 GB and more) whatever the grid size
 void grid_m2l_set_flat(GridM2L *g, int flat);
 // records `mid` (if not null) between the scatter into the grids and the M2L kernel
-int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, hipEvent_t mid);
+// deep_in_grid: the deepest level's multipoles are already in the grids
+// (written by the fused first upward level), so only the levels above are
+// scattered from M
+int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, hipEvent_t mid, int deep_in_grid = 0);
+// the split-BF16 grids of the deepest level, for the fused first upward
+// level to write into: returns 1 and sets hi, lo and the level's first slot
+// when the grid is split-BF16 with 16 vectors and covers that level, else 0
+int grid_m2l_leaf_grids(const GridM2L *g, void **hi, void **lo, int *slot0);
 
 // ---- list MFMA M2L (trees storing only occupied cells) ----
 struct ListM2L;  // defined in m2l_grid.hip

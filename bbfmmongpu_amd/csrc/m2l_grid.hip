@@ -1655,7 +1655,24 @@ static void launch_ss(const GridM2L *g, const GridArgs &a, int nblocks, hipStrea
 }
 #endif
 
-int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, hipEvent_t mid)
+int grid_m2l_leaf_grids(const GridM2L *g, void **hi, void **lo, int *slot0)
+{
+#ifdef SINGLE
+  if (g && g->form == BBFMM_M2L_BF16X2 && g->nrhs == 16 && g->args.minlev <= g->args.maxlev) {
+    *hi = g->d_Mg0;
+    *lo = g->d_Mg1;
+    *slot0 = g->args.lvl_slot0[g->args.maxlev];
+    return 1;
+  }
+#endif
+  (void)g;
+  (void)hi;
+  (void)lo;
+  (void)slot0;
+  return 0;
+}
+
+int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, hipEvent_t mid, int deep_in_grid)
 {
   GridArgs a = g->args;
   if (ncell != a.ncell) return set_error(BBFMM_E_ARG, "dense-grid M2L: ncell %d != %d", ncell, a.ncell);
@@ -1681,7 +1698,17 @@ int grid_m2l_run(GridM2L *g, const real *M, real *L, int ncell, hipStream_t s, h
     return BBFMM_OK;
 #ifdef SINGLE
   case BBFMM_M2L_BF16X2:
-    hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_BF16X2>, dim3(sb), dim3(256), 0, s, a, M);
+    if (deep_in_grid) {  // the levels above the deepest only
+      if (a.maxlev > a.minlev) {
+        GridArgs u = a;
+        u.maxlev = a.maxlev - 1;
+        const long long ucells = (long long)(a.levsta[a.maxlev] - a.levsta[a.minlev]) * a.nrhs;
+        hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_BF16X2>, dim3((int)((ucells * (a.r / 8) + 255) / 256)), dim3(256),
+                           0, s, u, M);
+      }
+    } else {
+      hipLaunchKernelGGL(k_grid_scatter<BBFMM_M2L_BF16X2>, dim3(sb), dim3(256), 0, s, a, M);
+    }
     BB_HIP(hipGetLastError());
     if (mid) BB_HIP(hipEventRecord(mid, s));
     if (a.ss) {  // one launch, or the upper levels then the deepest

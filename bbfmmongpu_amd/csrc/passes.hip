@@ -536,6 +536,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) k
   }
 }
 
+typedef __bf16 bf16x4g __attribute__((ext_vector_type(4)));
+
 // First upward level fused (particle rows, n = 4, complete tree): one block
 // per parent P on level maxlev - 1 and a wave per child leaf.  Each wave runs
 // the leaf's P2M as k_p2m_rows_mfma4 (32 particles staged at a time) and
@@ -630,6 +632,36 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) k
 #pragma unroll
     for (int t = 0; t < 2; t++)
       *reinterpret_cast<f32x4 *>(a.M + ((size_t)l16 * a.ncell + ch) * R + 16 * t + 4 * kq) = m[t];
+    if (a.grid_hi) {
+      // and into the split-BF16 grid of the deepest level: the leaf is child
+      // (sibling class) w of P; slot of class w at P's grid coordinates
+      // (ghost layer of width 1), then [vector][coefficient]; hi = the
+      // nearest bf16, lo = bf16 of the remainder (as the grid scatter)
+      const int lp = a.maxlev - 1;  // P's level
+      const int mp = P - ((1 << (3 * lp)) - 1) / 7;
+      int gx = 0, gy = 0, gz = 0;
+      for (int bt = 0; bt < lp; bt++) {
+        const int sb = (mp >> (3 * bt)) & 7;
+        gx |= ((sb >> 2) & 1) << bt;
+        gy |= ((sb >> 1) & 1) << bt;
+        gz |= (sb & 1) << bt;
+      }
+      const int PP = (1 << lp) + 2;
+      const size_t slot = (size_t)a.grid_slot0 + ((w * PP + gx + 1) * PP + gy + 1) * PP + gz + 1;
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        bf16x4g hi, lo;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const __bf16 h = (__bf16)m[t][e];
+          hi[e] = h;
+          lo[e] = (__bf16)(m[t][e] - (float)h);
+        }
+        const size_t o = (slot * 16 + l16) * R + 16 * t + 4 * kq;
+        *reinterpret_cast<bf16x4g *>((__bf16 *)a.grid_hi + o) = hi;
+        *reinterpret_cast<bf16x4g *>((__bf16 *)a.grid_lo + o) = lo;
+      }
+    }
   }
   __syncthreads();
   // M2M of P (as k_m2m): z pass over the 8 leaves' S
