@@ -13,7 +13,7 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7, 8])
 def test_source_stationary_m2l_pointer_form(variant, ops4, cuda):
     import torch
     import bbfmmongpu_amd as B
