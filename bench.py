@@ -45,26 +45,26 @@ KERNEL_SOURCE = {"k_m2l_mfma_ss<4,1,true>": "m2l_grid.hip", "k_m2l_mfma_ss_pair<
                  "k_m2l_add_parts": "m2l_grid.hip", "k_p2p_seg16<true>": "p2p.hip", "k_p2p_seg16x<3>": "p2p.hip"}
 # the launches of the split-BF16 M2L stage at C2 (automatic configuration):
 # the deepest level with two 32-column tiles per wave and both class groups
-# in one block, the levels above with one tile per wave and their steps in 4
-# parts, then the partial sums added
+# in one block, the levels above with one tile and 4 classes per wave and
+# their steps in 4 parts, then the partial sums added
 M2L_KERNELS = ("k_m2l_mfma_ss_pair<true>", "k_m2l_mfma_ss<4,1,true>", "k_m2l_add_parts")
 
 
-def ss_issued_mfma(cls: int, maxlev: int, nrhs: int, deep_tiles: int = 1, deep_union: bool = False) -> int:
+def ss_issued_mfma(maxlev: int, nrhs: int, deep=(4, 2, True), upper=(4, 1)) -> int:
     """MFMAs (32x32x16) the source-stationary M2L issues: per (wave, step) it
-    runs its products on, cls classes x tiles x 3 split products x 2 k-steps.
+    runs its products on, classes x tiles x 3 split products x 2 k-steps.
     A wave (32 columns = (parent, vector) pairs per tile, vector fastest,
     then pz, py, px) skips the steps whose parent shift d moves its whole
     parent box outside the level's [0, P)^3 (ghost sources only); the box is
     exact in px and conservative (the whole axis) in py and pz when the
-    columns span more than one row.  deep_tiles: tiles per wave on the
-    deepest level (one on the others); deep_union: there both class groups
-    walk the union of their step lists (one block for both)."""
+    columns span more than one row.  deep = (classes per wave, tiles per
+    wave, both class groups on the union of their step lists) of the
+    deepest level, upper = (classes per wave, tiles per wave) of the others."""
     total = 0
     for lev in range(2, maxlev + 1):
+        cls, tiles, union = deep if lev == maxlev else (*upper, False)
         P = 1 << (lev - 1)
         ncols = P ** 3 * nrhs
-        tiles = deep_tiles if lev == maxlev else 1
         span = 32 * tiles
         nwave = -(-ncols // (4 * span)) * 4
         col0 = np.arange(nwave) * span
@@ -78,12 +78,12 @@ def ss_issued_mfma(cls: int, maxlev: int, nrhs: int, deep_tiles: int = 1, deep_u
         lo = np.stack([fx, np.where(row, fy, 0), np.where(line, fz, 0)], 1)
         hi = np.stack([tx, np.where(row, ty, P - 1), np.where(line, tz, P - 1)], 1)
         for g in range(8 // cls):
+            group = range(8) if union else range(g * cls, (g + 1) * cls)
             ds = []
             for sc in range(8):
                 for d in itertools.product((-1, 0, 1), repeat=3):
                     if d == (0, 0, 0):
                         continue
-                    group = range(8) if deep_union and lev == maxlev else range(g * cls, (g + 1) * cls)
                     far = any(max(abs(2 * d[a] + ((sc >> (2 - a)) & 1) - ((c >> (2 - a)) & 1)) for a in range(3)) >= 2
                               for c in group)
                     if far:
@@ -711,12 +711,12 @@ def main():
     # roofline of the dominant kernel of the chosen form
     p2p_flop = pairs * (18 + 2 * nrhs)  # geometry (20-flop convention) shared by the vectors
     p2p_valu_flop = pairs * 18          # the 1/r part alone (the sum over sources runs on MFMA)
-    # issued MFMA flops of the source-stationary kernel (4 field classes per
-    # wave, two tiles per wave on the deepest level, where both class groups
-    # walk the union of their steps): per wave and step it runs, 4 classes x
-    # tiles x 6 products of 32x32x16 (zero operator included, ghost steps
-    # skipped)
-    issued = ss_issued_mfma(4, chosen.maxlev, nrhs, deep_tiles=2, deep_union=True) * (2 * 32 * 32 * 16)
+    # issued MFMA flops of the source-stationary kernel (deepest level: 4
+    # field classes and two tiles per wave, both class groups on the union of
+    # their steps; above: 4 classes, one tile): per wave and step it runs,
+    # classes x tiles x 6 products of 32x32x16 (zero operator included, ghost
+    # steps skipped)
+    issued = ss_issued_mfma(chosen.maxlev, nrhs) * (2 * 32 * 32 * 16)
     # the M2L stage is three launches; its roofline takes their sum (time
     # between the stage's events, HBM bytes of the three)
     m2l_kernel = " + ".join(M2L_KERNELS)

@@ -55,7 +55,7 @@ def test_m2l_configuration_against_oracle(form, scheme, variant, switch, tol, nr
 
 
 @pytest.mark.parametrize("variant,upper,split,nrhs", [(7, 4, 4, 16), (4, 4, 2, 16), (4, 5, 3, 3), (1, 4, 16, 16),
-                                                       (6, 7, 5, 16), (8, 4, 4, 16), (8, 5, 2, 3)])
+                                                       (6, 7, 5, 16), (8, 4, 4, 16), (8, 5, 2, 3), (8, 1, 7, 16)])
 def test_m2l_upper_levels_in_step_parts(variant, upper, split, nrhs, ops4, cuda):
     """BBFMM_OPT_M2L_UPPER_SPLIT: the launch of the levels above the deepest
     takes each class group's steps in `split` parts whose partial sums are
