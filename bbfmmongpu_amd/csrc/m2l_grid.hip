@@ -1295,7 +1295,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   // level with two tiles per wave and the class groups sharing the source
   // tiles (variant 8; fewer bytes moved per product), the levels above in a
   // launch of one tile per wave, their steps in 4 parts while they have
-  // fewer blocks than 3 per CU (at C2, N = 1M, depth 5, 16 vectors: 0.579
+  // fewer blocks than 3 per CU (N = 1M uniform, depth 5, 16 vectors: 0.579
   // against 0.595 ms with variant 7 on the deepest level and 0.601 ms for
   // one launch of one tile per wave)
   if (variant == 0 && variant_upper == 0 && upper_split == 1 && form == BBFMM_M2L_BF16X2 && r == 32 &&
