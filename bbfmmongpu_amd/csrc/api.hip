@@ -680,7 +680,7 @@ static int ensure_grid(bbfmm_amd_plan *p)
   const int minlev = p->level_switch > kMinLev ? p->level_switch : kMinLev;
   const int variant = p->scheme == M2L_SCHEME_IJ ? p->m2l_variant : 0;
   const int variant_up = p->scheme == M2L_SCHEME_IJ ? p->m2l_variant_up : 0;
-  const int upper_split = variant_up ? p->m2l_upper_split : 1;
+  const int upper_split = p->m2l_upper_split;
   if (!p->grid || p->grid_form != form || p->grid_minlev != minlev || p->grid_variant != variant ||
       p->grid_variant_up != variant_up || p->grid_upper_split != upper_split) {
     if (p->grid) grid_m2l_destroy(p->grid);

@@ -1307,9 +1307,13 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     upper_split = up_blocks * 4 <= 3 * 256 ? 4 : 1;
   }
   if (upper_split < 1 || upper_split > 16)
-    return set_error(BBFMM_E_ARG, "dense-grid M2L: upper-level step parts %d outside 1..16", upper_split);
-  if (upper_split > 1 && !ss_classes(variant_upper))
-    return set_error(BBFMM_E_ARG, "dense-grid M2L: upper-level step parts need a source-stationary upper variant");
+    return set_error(BBFMM_E_ARG, "dense-grid M2L: step parts %d outside 1..16", upper_split);
+  // step parts without an upper-level variant: the whole (single) launch
+  int whole_split = 1;
+  if (variant_upper == 0 && upper_split > 1) {
+    whole_split = upper_split;
+    upper_split = 1;
+  }
   if (variant_upper != 0 && (!(ss_classes(variant_upper) && ss_classes(variant)) || variant_upper == 8))
     return set_error(BBFMM_E_ARG, "dense-grid M2L: a variant for the upper levels (%d) needs source-stationary "
                      "variants for both (1, 4 to 7 above the deepest level)", variant_upper);
@@ -1383,11 +1387,27 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     cpb = 4 * tpw * 32;
   }
   a.wave_cols = tpw * 32;
+  if (whole_split > 1 && !a.ss) {
+    delete g;
+    return set_error(BBFMM_E_ARG, "dense-grid M2L: step parts need the split-BF16 source-stationary kernel");
+  }
+  // variant 0 on a deepest level of 8 K to 64 K columns (one launch, 4
+  // classes and one tile per wave): each class group's step list in as many
+  // parts as keep the blocks to one round at 3 per CU on 256 CUs (N = 262144
+  // uniform, depth 4, 16 vectors: 146 column-group blocks, 5 parts)
+  if (variant == 0 && variant_upper == 0 && whole_split == 1 && a.ss == 4 && a.ss_tpw == 1 && !a.ss_pair) {
+    long long nb1 = 0;
+    for (int l = minlev; l <= maxlev; l++) nb1 += 2 * (((long long)(1 << (3 * (l - 1))) * nrhs + cpb - 1) / cpb);
+    whole_split = (int)std::max(1LL, std::min(16LL, 3 * 256 / nb1));
+  }
+  a.nsplit = whole_split;
+  a.up_cell0 = a.levsta[minlev];
+  a.up_ncell = a.levsta[maxlev + 1] - a.levsta[minlev];
   int nb = 0;
   for (int l = minlev; l <= maxlev; l++) {
     a.blk0[l] = nb;
     const long long cols = (long long)(1 << (3 * (l - 1))) * nrhs;
-    nb += (a.ss_pair ? 1 : a.ss ? 8 / a.ss : 8) * (int)((cols + cpb - 1) / cpb);
+    nb += whole_split * (a.ss_pair ? 1 : a.ss ? 8 / a.ss : 8) * (int)((cols + cpb - 1) / cpb);
   }
   a.blk0[maxlev + 1] = nb;
   g->nblocks = nb;
@@ -1573,6 +1593,10 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
     const size_t nx = (size_t)(g->part[0].nsplit - 1) * nrhs * g->part[0].up_ncell * 32;
     if ((e = hipMalloc(&g->d_Lx, sizeof(float) * nx)) != hipSuccess) fail(e, "hipMalloc");
   }
+  if (rc == BBFMM_OK && g->nparts == 1 && a.nsplit > 1) {  // the whole launch in step parts
+    const size_t nx = (size_t)(a.nsplit - 1) * nrhs * a.up_ncell * 32;
+    if ((e = hipMalloc(&g->d_Lx, sizeof(float) * nx)) != hipSuccess) fail(e, "hipMalloc");
+  }
   if (rc == BBFMM_OK && (e = hipStreamSynchronize(s)) != hipSuccess) fail(e, "sync");
   if (rc != BBFMM_OK) {
     grid_m2l_destroy(g);
@@ -1585,6 +1609,7 @@ int grid_m2l_create(GridM2L **out, int r, int maxlev, int nrhs, int form, const 
   a.Mg1 = g->d_Mg1;
   a.Mg2 = g->d_Mg2;
   a.lvl_max = g->d_lvl_max;
+  a.Lx = g->nparts == 1 ? g->d_Lx : nullptr;
   if (g->nparts == 2) {
     // everything from the final args (operator and grid pointers and sizes),
     // then each part's own levels, blocks, class-group size, tiles and steps

@@ -201,12 +201,14 @@ typedef struct bbfmm_amd_plan bbfmm_amd_plan;
                                           classes per wave, one tile; 6, 7 = 2, 4
                                           classes, two tiles); 0 (default): the deepest
                                           level's, or the automatic configuration */
-#define BBFMM_OPT_M2L_UPPER_SPLIT 13 /* with BBFMM_OPT_M2L_VARIANT_UPPER set: the launch
-                                        of the upper levels takes each class group's
-                                        steps in this many parts (1 to 16, default 1),
-                                        one block each, their partial sums added to L
-                                        after it (a shorter critical path for the few
-                                        columns of those levels) */
+#define BBFMM_OPT_M2L_UPPER_SPLIT 13 /* IJ split-BF16 source-stationary M2L: with
+                                        BBFMM_OPT_M2L_VARIANT_UPPER set, the launch of
+                                        the upper levels, else the single launch, takes
+                                        each class group's steps in this many parts (1
+                                        to 16, default 1), one block each, their partial
+                                        sums added to L after it (a shorter critical
+                                        path when the columns are few); variant 0
+                                        chooses the parts itself */
 #define BBFMM_OPT_ADDRESSING 11 /* 0 (default): kernels with two forms read through
                                    scalar-offset buffer loads while their arrays are
                                    below 2 GB and through pointers beyond; 1: through

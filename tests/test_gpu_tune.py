@@ -94,14 +94,57 @@ def test_m2l_upper_levels_in_step_parts(variant, upper, split, nrhs, ops4, cuda)
         assert err < 3e-5
 
 
-def test_m2l_upper_split_rejected_without_upper_variant(ops4, cuda):
+@pytest.mark.parametrize("variant,split,nrhs", [(4, 3, 16), (0, 1, 16), (5, 2, 3), (7, 4, 16)])
+def test_m2l_whole_launch_in_step_parts(variant, split, nrhs, ops4, cuda):
+    """BBFMM_OPT_M2L_UPPER_SPLIT without an upper-level variant: the single
+    launch takes each class group's steps in `split` parts, their partial
+    sums added afterwards; variant 0 does so by itself on deepest levels of
+    8 K to 64 K columns (here 8192: 5 parts).  L matches the unsplit launch
+    of the same tile shape to fp32 rounding and the oracle's M2L."""
+    import torch
+    import bbfmmongpu_amd as B
+    U, K, _ = ops4
+    pos, _q = O.uniform_inputs(65536, seed=0)
+    tree = O.Tree(pos, 0, "d")
+    assert tree.maxlev == 4
+    rng = np.random.default_rng(29)
+    M = rng.standard_normal((nrhs, tree.ncell * 32)).astype(np.float32)
+    Md = torch.from_numpy(M).to(cuda)
+    out = {}
+    one = variant if variant else 4
+    for v, sp in ((one, 1), (variant, split)):
+        plan = B.Plan(pos, 4, U, K, prec="s", nrhs=nrhs).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2, variant=v,
+                                                             upper_split=sp)
+        Ld = torch.zeros_like(Md)
+        plan.m2l(Md, Ld)
+        torch.cuda.synchronize()
+        out[(v, sp)] = Ld.cpu().numpy()
+        plan.close()
+    s = O.levsta(tree.maxlev)[2] * 32
+    got = out[(variant, split)]
+    d = O.rel_l2(out[(one, 1)][:, s:].astype(np.float64), got[:, s:])
+    print(f"variant={variant} split={split} nrhs={nrhs}: vs unsplit {d:.2e}", flush=True)
+    assert 0 < d < 1e-6 if (variant, split) != (one, 1) else d == 0
+    Kd = K.astype(np.float32).astype(np.float64)
+    for j in (0, nrhs - 1):
+        ref = O.m2l(tree, Kd, M[j].astype(np.float64), 32, "basic")
+        err = O.rel_l2(ref[s:], got[j, s:])
+        assert err < 3e-5
+
+
+def test_m2l_step_parts_option_range(ops4, cuda):
     import bbfmmongpu_amd as B
     U, K, _ = ops4
     pos, _q = O.uniform_inputs(4096, seed=0)
     plan = B.Plan(pos, 4, U, K, prec="s", nrhs=16).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2, variant=4)
-    with pytest.raises(B.BBFMMError):
-        plan.set(upper_split=17)
-    plan.set(upper_split=4)  # no effect without an upper-level variant
+    for bad in (0, 17):
+        with pytest.raises(B.BBFMMError):
+            plan.set(upper_split=bad)
+    plan.close()
+    plan = B.Plan(pos, 4, U, K, prec="s", nrhs=16).set(scheme=B.SCHEME_IJ, form=B.FORM_BF16X2, variant=2,
+                                                       upper_split=2)
+    with pytest.raises(B.BBFMMError):  # step parts need the source-stationary kernel
+        plan.solve_host(np.ones((16, 4096), np.float32))
     plan.close()
 
 
