@@ -146,22 +146,48 @@ __global__ void __launch_bounds__(64) k_p2p_grad(GradArgs a)
       gk[v] = v < a.mb ? a.qg[(size_t)(a.m + a.v0 + v) * a.N + i] : (real)0;
     }
     real ax = 0, ay = 0, az = 0;
-    for (int b = 0; b < nn; b++) {
-      const int B = nb[b];
+    // chunks of up to 64 sources, neighbour by neighbour; the next chunk is
+    // loaded into registers while the current one is summed from LDS
+    int b = 0, jb = 0;
+    real4 rp = real4{0, 0, 0, 0};
+    real rq[2 * MMAX];
+    auto fetch = [&](int bb, int jj0) {  // lane's source of chunk (bb, jj0), if any
+      if (bb >= nn) return;
+      const int B = nb[bb];
       const int bs = a.leafsta[B], bc = a.leafcnt[B];
-      for (int jb = 0; jb < bc; jb += CH) {
-        const int nj = min(CH, bc - jb);
-        __syncthreads();
-        if ((int)threadIdx.x < nj) {
-          const int j = bs + jb + threadIdx.x;
-          sp[threadIdx.x] = a.pos[j];
+      if ((int)threadIdx.x < min(CH, bc - jj0)) {
+        const int j = bs + jj0 + threadIdx.x;
+        rp = a.pos[j];
 #pragma unroll
-          for (int v = 0; v < MMAX; v++) {
-            sqg[threadIdx.x][v] = v < a.mb ? a.qg[(size_t)(a.v0 + v) * a.N + j] : (real)0;
-            sqg[threadIdx.x][MMAX + v] = v < a.mb ? a.qg[(size_t)(a.m + a.v0 + v) * a.N + j] : (real)0;
-          }
+        for (int v = 0; v < MMAX; v++) {
+          rq[v] = v < a.mb ? a.qg[(size_t)(a.v0 + v) * a.N + j] : (real)0;
+          rq[MMAX + v] = v < a.mb ? a.qg[(size_t)(a.m + a.v0 + v) * a.N + j] : (real)0;
         }
-        __syncthreads();
+      }
+    };
+    // skip neighbours without particles (none in practice: lists hold non-empty leaves)
+    while (b < nn && a.leafcnt[nb[b]] == 0) b++;
+    fetch(b, 0);
+    while (b < nn) {
+      const int bc = a.leafcnt[nb[b]];
+      const int nj = min(CH, bc - jb);
+      __syncthreads();  // the previous chunk has been summed
+      if ((int)threadIdx.x < nj) {
+        sp[threadIdx.x] = rp;
+#pragma unroll
+        for (int v = 0; v < 2 * MMAX; v++) sqg[threadIdx.x][v] = rq[v];
+      }
+      __syncthreads();
+      // the next chunk's indices, and its loads in flight during the sums
+      if (jb + CH < bc) {
+        jb += CH;
+      } else {
+        jb = 0;
+        b++;
+        while (b < nn && a.leafcnt[nb[b]] == 0) b++;
+      }
+      fetch(b, jb);
+      {
         // two sources per iteration, each weight one chain of FMAs (the
         // LDS reads of the second overlap the arithmetic of the first)
         auto pair = [&](int jj) {
