@@ -206,11 +206,13 @@ __global__ void __launch_bounds__(64) k_p2p_grad(GradArgs a)
           az += s * dz;
         };
         int jj = 0;
-        for (; jj + 1 < nj; jj += 2) {
+        for (; jj + 3 < nj; jj += 4) {
           pair(jj);
           pair(jj + 1);
+          pair(jj + 2);
+          pair(jj + 3);
         }
-        if (jj < nj) pair(jj);
+        for (; jj < nj; jj++) pair(jj);
       }
     }
     if (valid) {
