@@ -188,8 +188,8 @@ __global__ void __launch_bounds__(64) k_p2p_grad(GradArgs a)
       }
       fetch(b, jb);
       {
-        // two sources per iteration, each weight one chain of FMAs (the
-        // LDS reads of the second overlap the arithmetic of the first)
+        // four sources per iteration, each weight one chain of FMAs (the
+        // LDS reads of the next overlap the arithmetic of the previous)
         auto pair = [&](int jj) {
           const real dx = sp[jj].x - xk.x, dy = sp[jj].y - xk.y, dz = sp[jj].z - xk.z;
           const real rr = dx * dx + dy * dy + dz * dz;
